@@ -1,0 +1,100 @@
+"""ctypes binding of libdonofhip.so (C ABI declared in include/donof_hip.h)."""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdonofhip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "donof_hip.h")
+
+DN_OK, DN_ERR_ARG, DN_ERR_HIP, DN_ERR_NOMEM, DN_ERR_BUDGET, DN_ERR_NONFINITE, DN_ERR_STATE = range(7)
+DN_FP64 = 0
+ERROR_NAMES = {1: "DN_ERR_ARG", 2: "DN_ERR_HIP", 3: "DN_ERR_NOMEM", 4: "DN_ERR_BUDGET",
+               5: "DN_ERR_NONFINITE", 6: "DN_ERR_STATE"}
+
+
+class DnError(RuntimeError):
+    """A libdonofhip call returned a nonzero status."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERROR_NAMES.get(code, code), msg))
+        self.code = code
+
+
+class dn_dims(ct.Structure):
+    _fields_ = [(f, ct.c_int32) for f in
+                ("nbf", "nbfaux", "nbf5", "no1", "ndoc", "nsoc", "nbeta", "nalpha", "ncwo", "nv")]
+
+
+_P = ct.c_void_p
+_D = ct.POINTER(ct.c_double)
+_I = ct.c_int
+_SIGS = {
+    "dn_last_error": (ct.c_char_p, []),
+    "dn_device_count": (_I, [ct.POINTER(ct.c_int)]),
+    "dn_ctx_create": (_I, [_I, ct.c_uint64, ct.POINTER(_P)]),
+    "dn_ctx_destroy": (_I, [_P]),
+    "dn_ctx_hbm_used": (_I, [_P, ct.POINTER(ct.c_uint64)]),
+    "dn_sys_create": (_I, [_P, ct.POINTER(dn_dims), _I, ct.POINTER(_P)]),
+    "dn_sys_destroy": (_I, [_P]),
+    "dn_sys_footprint": (_I, [ct.POINTER(dn_dims), _I, ct.POINTER(ct.c_uint64)]),
+    "dn_sys_upload_b": (_I, [_P, _D]),
+    "dn_sys_download_b": (_I, [_P, _D]),
+    "dn_sys_generate_b": (_I, [_P, ct.c_uint64]),
+    "dn_jkh_mo": (_I, [_P, _D, _D, _D, _D, _D]),
+    "dn_lagrange2": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _D, _D, _D]),
+    "dn_sys_set_orbitals": (_I, [_P, _D]),
+    "dn_sys_set_hcore": (_I, [_P, _D]),
+    "dn_jkh_mo_resident": (_I, [_P]),
+    "dn_sys_get_jkh": (_I, [_P, _D, _D, _D]),
+    "dn_sys_sync": (_I, [_P]),
+    "dn_sys_set_timing": (_I, [_P, _I]),
+    "dn_sys_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),
+}
+
+_lib = None
+
+
+def library_symbols(header=HEADER):
+    """Names of the functions declared in include/donof_hip.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dn_[a-z0-9_]+)\s*\(", text)))
+
+
+def load_library(path=LIB_PATH):
+    """Load libdonofhip.so and attach signatures.  Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DnError(DN_ERR_STATE, "libdonofhip.so not built (run __graft_entry__.build())")
+    lib = ct.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != DN_OK:
+        msg = _lib.dn_last_error().decode(errors="replace") if _lib else ""
+        raise DnError(rc, msg)
+
+
+def dptr(a):
+    """Pointer to a float64 array's data (must be contiguous in the ABI's order)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_D)
+
+
+def as_f64(a, order="F"):
+    """Column-major float64 view/copy of a host array, as the ABI expects."""
+    return np.require(np.asarray(a, dtype=np.float64), requirements=[order])

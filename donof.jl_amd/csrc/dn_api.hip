@@ -1,0 +1,495 @@
+// C ABI of libdonofhip: contexts, resident systems, the J/K-MO build and the
+// Lagrangian.  Declarations and reference citations are in include/donof_hip.h.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "dn_internal.h"
+
+namespace dn {
+
+// kernels of dn_jk.hip / dn_lag.hip
+int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux, double* part,
+                size_t part_elems, double* J, double* K);
+int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
+               double* Hcore);
+int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
+int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
+          const double* n, int nalpha, int nbeta, double* X, double* w);
+int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf5, int naux,
+             const double* ck12, const double* w, double* bmat);
+int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nbf5,
+             double* elag);
+int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
+                int nbf, int nbf5, double* out);
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_ctx_mutex;
+
+bool dims_valid(const dn_dims& d, std::string* why) {
+  auto bad = [&](const char* m) { *why = m; return false; };
+  if (d.nbf <= 0 || d.nbfaux <= 0 || d.nbf5 <= 0) return bad("nbf, nbfaux, nbf5 must be > 0");
+  if (d.nbf5 > d.nbf) return bad("nbf5 must be <= nbf");
+  if (d.nbf > 46340) return bad("nbf too large");
+  if (d.no1 < 0 || d.ndoc < 0 || d.nsoc < 0 || d.ncwo < 0 || d.nv < 0)
+    return bad("negative occupation sizes");
+  if (d.nbeta < 0 || d.nalpha < d.nbeta || d.nalpha > d.nbf5)
+    return bad("need 0 <= nbeta <= nalpha <= nbf5");
+  if (d.nv != d.ncwo * d.ndoc) return bad("nv must equal ncwo*ndoc");
+  if (d.no1 + d.ndoc * (1 + d.ncwo) + d.nsoc != d.nbf5)
+    return bad("nbf5 must equal no1 + ndoc*(1+ncwo) + nsoc");
+  return true;
+}
+
+// Element counts of every device buffer of a system.
+struct Layout {
+  size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bmat, xw, part, scal;
+  size_t total_bytes() const {
+    return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + 2 * big + bmat + 2 * xw +
+                part + scal);
+  }
+};
+
+size_t lag_splits(const dn_dims& d) {
+  // batch-reduce splits over Q for the Lagrangian GEMM: aim at ~1024 workgroups
+  const int tm = (d.nbf + 63) / 64, tn = (d.nbf5 + 63) / 64;
+  long want = 1024 / std::max(1, tm * tn);
+  if (want < 1) want = 1;
+  if (want > d.nbfaux) want = d.nbfaux;
+  const long bps = (d.nbfaux + want - 1) / want;
+  return (size_t)((d.nbfaux + bps - 1) / bps);
+}
+
+Layout layout_of(const dn_dims& d) {
+  Layout L{};
+  const size_t nbf = d.nbf, nbf5 = d.nbf5, naux = d.nbfaux;
+  L.b = nbf * nbf * naux;
+  L.C = nbf * nbf;
+  L.H = nbf * nbf;
+  L.n = nbf5;
+  L.nn = nbf5 * nbf5;
+  L.nbf2 = nbf * nbf;
+  L.nbf_nbf5 = nbf * nbf5;
+  L.big = nbf5 * nbf * naux;
+  L.bmat = nbf5 * nbf5 * naux;
+  L.xw = nbf5 * naux;
+  const size_t jk_part = 2 * 64 * L.nn;
+  const size_t lag_part = lag_splits(d) * L.nbf_nbf5;
+  L.part = std::max(jk_part, lag_part);
+  L.scal = 64;
+  return L;
+}
+
+int alloc(dn_sys* s, size_t elems, double** out) {
+  const size_t bytes = std::max<size_t>(elems, 1) * sizeof(double);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(DN_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  s->owned.push_back(DevBuf{p, bytes});
+  *out = static_cast<double*>(p);
+  return DN_OK;
+}
+
+void release(dn_sys* s) {
+  for (auto& b : s->owned) (void)hipFree(b.p);
+  s->owned.clear();
+}
+
+int check_sys(dn_sys* s) {
+  if (!s) return fail(DN_ERR_ARG, "null dn_sys");
+  return DN_OK;
+}
+
+// b_pn[p,(n,Q)] = sum_m C5[m,p] b[m,(n,Q)]: one GEMM with N = nbf*naux (integrals.jl:159).
+int transform1(dn_sys* s) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf5;
+  g.N = d.nbf * d.nbfaux;
+  g.K = d.nbf;
+  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
+  g.B = s->b; g.ldb = d.nbf; g.b_kcontig = true;
+  g.C = s->bpn; g.ldc = d.nbf5;
+  if (s->timing) {
+    if (s->ev_used == s->ev_start.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      DN_HIP(hipEventCreate(&a));
+      DN_HIP(hipEventCreate(&b));
+      s->ev_start.push_back(a);
+      s->ev_stop.push_back(b);
+    }
+    DN_HIP(hipEventRecord(s->ev_start[s->ev_used], s->stream));
+  }
+  DN_TRY(gemm_f64(s->stream, g));
+  if (s->timing) {
+    DN_HIP(hipEventRecord(s->ev_stop[s->ev_used], s->stream));
+    s->ev_used += 1;
+  }
+  return DN_OK;
+}
+
+// out[:, :, Q] = b_pn[:, :, Q] * C[:, 0:ncols]  (integrals.jl:160 for ncols = nbf5; the
+// transpose of b_MO of utils.jl:19-20 for ncols = nbf).  Slice stride nbf5*ncols.
+int transform2(dn_sys* s, int ncols, double* out) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf5;
+  g.N = ncols;
+  g.K = d.nbf;
+  g.A = s->bpn; g.lda = d.nbf5; g.sA = (long)d.nbf5 * d.nbf; g.a_kcontig = false;
+  g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
+  g.C = out; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * ncols;
+  g.nbatch = d.nbfaux;
+  g.bps = 1;
+  return gemm_f64(s->stream, g);
+}
+
+// HC = H C5 ; optionally Hmat = C^T HC
+int hc_products(dn_sys* s, bool want_hmat) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
+  g.A = s->H; g.lda = d.nbf; g.a_kcontig = false;
+  g.B = s->C; g.ldb = d.nbf; g.b_kcontig = true;
+  g.C = s->HC; g.ldc = d.nbf;
+  DN_TRY(gemm_f64(s->stream, g));
+  if (want_hmat) {
+    Gemm h;
+    h.M = d.nbf; h.N = d.nbf5; h.K = d.nbf;
+    h.A = s->C; h.lda = d.nbf; h.a_kcontig = true;
+    h.B = s->HC; h.ldb = d.nbf; h.b_kcontig = true;
+    h.C = s->Hmat; h.ldc = d.nbf;
+    DN_TRY(gemm_f64(s->stream, h));
+  }
+  return DN_OK;
+}
+
+int jkh_resident(dn_sys* s) {
+  const dn_dims& d = s->d;
+  if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
+  DN_TRY(transform1(s));
+  DN_TRY(transform2(s, d.nbf5, s->bt));
+  DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->part,
+                     s->part_elems, s->J, s->K));
+  DN_TRY(hc_products(s, false));
+  DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
+  return DN_OK;
+}
+
+int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
+  const dn_dims& d = s->d;
+  if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
+  if (nbeta < 0 || nalpha < 0 || nbeta > d.nbf5 || nalpha > d.nbf5)
+    return fail(DN_ERR_ARG, "nalpha/nbeta out of range");
+  const long slice = (long)d.nbf5 * d.nbf;
+  DN_TRY(transform1(s));
+  DN_TRY(transform2(s, d.nbf, s->bt));
+  DN_TRY(hc_products(s, true));
+  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta, s->X,
+               s->w));
+  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
+  DN_TRY(lag_init(s->stream, s->Hmat, s->n, d.nbf, d.nbf5, s->elag));
+  // elag[a,b] += sum_Q sum_q bt[q,a,Q] bmat[q,b,Q]
+  const int nsplit = (int)lag_splits(d);
+  const int bps = (d.nbfaux + nsplit - 1) / nsplit;
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf5;
+  g.A = s->bt; g.lda = d.nbf5; g.sA = slice; g.a_kcontig = true;
+  g.B = s->bmat; g.ldb = d.nbf5; g.sB = (long)d.nbf5 * d.nbf5; g.b_kcontig = true;
+  g.C = s->part; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
+  g.nbatch = d.nbfaux;
+  g.bps = bps;
+  const int used = (d.nbfaux + bps - 1) / bps;
+  if ((size_t)used * d.nbf * d.nbf5 > s->part_elems) return fail(DN_ERR_ARG, "partials overflow");
+  DN_TRY(gemm_f64(s->stream, g));
+  DN_TRY(reduce_splits(s->stream, s->part, (long)d.nbf * d.nbf5, used, d.nbf, d.nbf5, d.nbf,
+                       s->elag, d.nbf, 1.0));
+  DN_TRY(energy_elec(s->stream, s->elag, s->Hmat, s->n, d.nbf, d.nbf5, s->scal));
+  return DN_OK;
+}
+
+int h2d(dn_sys* s, double* dst, const double* src, size_t elems) {
+  if (!src) return fail(DN_ERR_ARG, "null host pointer");
+  DN_HIP(hipMemcpyAsync(dst, src, elems * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  return DN_OK;
+}
+
+int d2h(dn_sys* s, double* dst, const double* src, size_t elems) {
+  if (!dst) return DN_OK;
+  DN_HIP(hipMemcpyAsync(dst, src, elems * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  return DN_OK;
+}
+
+bool all_finite(const double* x, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(x[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+}  // namespace dn
+
+using namespace dn;
+
+extern "C" {
+
+const char* dn_last_error(void) { return g_last_error.c_str(); }
+
+int dn_device_count(int* count) {
+  if (!count) return fail(DN_ERR_ARG, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return DN_OK;
+}
+
+int dn_ctx_create(int device, uint64_t hbm_budget_bytes, dn_ctx** out) {
+  if (!out) return fail(DN_ERR_ARG, "null out");
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(DN_ERR_HIP, "no HIP device visible");
+  if (device < 0 || device >= count) return fail(DN_ERR_ARG, "device index out of range");
+  DN_HIP(hipSetDevice(device));
+  size_t free_b = 0, total_b = 0;
+  DN_HIP(hipMemGetInfo(&free_b, &total_b));
+  auto* c = new dn_ctx;
+  c->device = device;
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)free_b;
+  c->used = 0;
+  *out = c;
+  return DN_OK;
+}
+
+int dn_ctx_destroy(dn_ctx* ctx) {
+  if (!ctx) return DN_OK;
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  delete ctx;
+  return DN_OK;
+}
+
+int dn_ctx_hbm_used(const dn_ctx* ctx, uint64_t* used) {
+  if (!ctx || !used) return fail(DN_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  *used = ctx->used;
+  return DN_OK;
+}
+
+int dn_sys_footprint(const dn_dims* dims, int precision, uint64_t* bytes) {
+  if (!dims || !bytes) return fail(DN_ERR_ARG, "null argument");
+  std::string why;
+  if (!dims_valid(*dims, &why)) return fail(DN_ERR_ARG, why);
+  if (precision != DN_FP64) return fail(DN_ERR_ARG, "unknown precision");
+  *bytes = layout_of(*dims).total_bytes();
+  return DN_OK;
+}
+
+int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out) {
+  if (!ctx || !dims || !out) return fail(DN_ERR_ARG, "null argument");
+  std::string why;
+  if (!dims_valid(*dims, &why)) return fail(DN_ERR_ARG, why);
+  if (precision != DN_FP64) return fail(DN_ERR_ARG, "unknown precision");
+  const Layout L = layout_of(*dims);
+  const uint64_t need = L.total_bytes();
+  {
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    if (ctx->used + need > ctx->budget)
+      return fail(DN_ERR_BUDGET, "system needs " + std::to_string(need) +
+                                     " bytes; budget left " +
+                                     std::to_string(ctx->budget - ctx->used));
+    ctx->used += need;
+  }
+  DN_HIP(hipSetDevice(ctx->device));
+  auto* s = new dn_sys;
+  s->ctx = ctx;
+  s->d = *dims;
+  s->precision = precision;
+  s->bytes = need;
+  int rc = DN_OK;
+  auto A = [&](size_t n, double** p) {
+    if (rc == DN_OK) rc = alloc(s, n, p);
+  };
+  A(L.b, &s->b);
+  A(L.C, &s->C);
+  A(L.H, &s->H);
+  A(L.n, &s->n);
+  A(L.nn, &s->cj12);
+  A(L.nn, &s->ck12);
+  A(L.nn, &s->J);
+  A(L.nn, &s->K);
+  A(L.n, &s->Hcore);
+  A(L.nbf2, &s->elag);
+  A(L.nbf_nbf5, &s->Hmat);
+  A(L.nbf_nbf5, &s->HC);
+  A(L.big, &s->bpn);
+  A(L.big, &s->bt);
+  A(L.bmat, &s->bmat);
+  A(L.xw, &s->X);
+  A(L.xw, &s->w);
+  A(L.part, &s->part);
+  A(L.scal, &s->scal);
+  s->part_elems = L.part;
+  hipError_t e = hipSuccess;
+  if (rc == DN_OK) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (rc == DN_OK && e != hipSuccess) rc = fail(DN_ERR_HIP, hipGetErrorString(e));
+  if (rc != DN_OK) {
+    std::string msg = g_last_error;
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    release(s);
+    {
+      std::lock_guard<std::mutex> lock(g_ctx_mutex);
+      ctx->used -= need;
+    }
+    delete s;
+    return fail(rc, msg);
+  }
+  *out = s;
+  return DN_OK;
+}
+
+int dn_sys_destroy(dn_sys* s) {
+  if (!s) return DN_OK;
+  (void)hipSetDevice(s->ctx->device);
+  if (s->stream) {
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamDestroy(s->stream);
+  }
+  for (auto ev : s->ev_start) (void)hipEventDestroy(ev);
+  for (auto ev : s->ev_stop) (void)hipEventDestroy(ev);
+  release(s);
+  {
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    s->ctx->used -= s->bytes;
+  }
+  delete s;
+  return DN_OK;
+}
+
+int dn_sys_upload_b(dn_sys* s, const double* b) {
+  DN_TRY(check_sys(s));
+  const size_t n = (size_t)s->d.nbf * s->d.nbf * s->d.nbfaux;
+  DN_TRY(h2d(s, s->b, b, n));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  s->b_ready = true;
+  return DN_OK;
+}
+
+int dn_sys_download_b(dn_sys* s, double* b) {
+  DN_TRY(check_sys(s));
+  if (!b) return fail(DN_ERR_ARG, "null host pointer");
+  const size_t n = (size_t)s->d.nbf * s->d.nbf * s->d.nbfaux;
+  DN_TRY(d2h(s, b, s->b, n));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  return DN_OK;
+}
+
+int dn_sys_generate_b(dn_sys* s, uint64_t seed) {
+  DN_TRY(check_sys(s));
+  DN_TRY(synth_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, seed));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  s->b_ready = true;
+  return DN_OK;
+}
+
+int dn_sys_set_orbitals(dn_sys* s, const double* C) {
+  DN_TRY(check_sys(s));
+  return h2d(s, s->C, C, (size_t)s->d.nbf * s->d.nbf);
+}
+
+int dn_sys_set_hcore(dn_sys* s, const double* H) {
+  DN_TRY(check_sys(s));
+  return h2d(s, s->H, H, (size_t)s->d.nbf * s->d.nbf);
+}
+
+int dn_jkh_mo_resident(dn_sys* s) {
+  DN_TRY(check_sys(s));
+  return jkh_resident(s);
+}
+
+int dn_sys_get_jkh(dn_sys* s, double* J, double* K, double* Hcore) {
+  DN_TRY(check_sys(s));
+  const size_t nn = (size_t)s->d.nbf5 * s->d.nbf5;
+  DN_TRY(d2h(s, J, s->J, nn));
+  DN_TRY(d2h(s, K, s->K, nn));
+  DN_TRY(d2h(s, Hcore, s->Hcore, s->d.nbf5));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  if ((J && !all_finite(J, nn)) || (K && !all_finite(K, nn)) ||
+      (Hcore && !all_finite(Hcore, s->d.nbf5)))
+    return fail(DN_ERR_NONFINITE, "non-finite value in J/K/H_core");
+  return DN_OK;
+}
+
+int dn_sys_sync(dn_sys* s) {
+  DN_TRY(check_sys(s));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  return DN_OK;
+}
+
+int dn_jkh_mo(dn_sys* s, const double* C, const double* H, double* J, double* K,
+              double* Hcore) {
+  DN_TRY(check_sys(s));
+  DN_TRY(dn_sys_set_orbitals(s, C));
+  DN_TRY(dn_sys_set_hcore(s, H));
+  DN_TRY(jkh_resident(s));
+  return dn_sys_get_jkh(s, J, K, Hcore);
+}
+
+int dn_lagrange2(dn_sys* s, const double* C, const double* n, const double* H,
+                 const double* cj12, const double* ck12, int nalpha, int nbeta, double* elag,
+                 double* Hmat, double* E) {
+  DN_TRY(check_sys(s));
+  const dn_dims& d = s->d;
+  const size_t nn = (size_t)d.nbf5 * d.nbf5;
+  DN_TRY(h2d(s, s->C, C, (size_t)d.nbf * d.nbf));
+  DN_TRY(h2d(s, s->H, H, (size_t)d.nbf * d.nbf));
+  DN_TRY(h2d(s, s->n, n, d.nbf5));
+  DN_TRY(h2d(s, s->cj12, cj12, nn));
+  DN_TRY(h2d(s, s->ck12, ck12, nn));
+  DN_TRY(lagrange_resident(s, nalpha, nbeta));
+  DN_TRY(d2h(s, elag, s->elag, (size_t)d.nbf * d.nbf));
+  DN_TRY(d2h(s, Hmat, s->Hmat, (size_t)d.nbf * d.nbf5));
+  double e = 0.0;
+  DN_TRY(d2h(s, &e, s->scal, 1));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  if (E) *E = e;
+  if (!std::isfinite(e) || (elag && !all_finite(elag, (size_t)d.nbf * d.nbf)))
+    return fail(DN_ERR_NONFINITE, "non-finite value in the Lagrangian");
+  return DN_OK;
+}
+
+int dn_sys_set_timing(dn_sys* s, int enable) {
+  DN_TRY(check_sys(s));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  s->timing = enable != 0;
+  s->ev_used = 0;
+  s->transform_ms = 0.0;
+  s->transform_launches = 0;
+  return DN_OK;
+}
+
+int dn_sys_get_timing(dn_sys* s, double* ms_total, long* launches) {
+  DN_TRY(check_sys(s));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  for (size_t i = 0; i < s->ev_used; ++i) {
+    float ms = 0.f;
+    DN_HIP(hipEventElapsedTime(&ms, s->ev_start[i], s->ev_stop[i]));
+    s->transform_ms += ms;
+    s->transform_launches += 1;
+  }
+  s->ev_used = 0;
+  if (ms_total) *ms_total = s->transform_ms;
+  if (launches) *launches = s->transform_launches;
+  return DN_OK;
+}
+
+}  // extern "C"

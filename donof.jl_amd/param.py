@@ -1,0 +1,79 @@
+"""Size and hyper-parameter record of a solve, mirroring DoNOF.jl's mutable Param
+(param.jl:3-57) with the constructor defaults of param.jl:59-165 and set_ncwo of
+param.jl:242-266.  Field names are the reference's, so code written against
+p.ipnof, p.orb_method, p.maxloop, p.alpha ... reads the same."""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+
+@dataclass
+class Param:
+    nbf: int
+    nbfaux: int
+    ne: int
+    mul: int = 1
+    no1: int = 0
+    title: str = "donof"
+    maxit: int = 1000
+    maxloop: int = 10
+    ipnof: int = 7
+    ista: int = 0
+    threshen: float = 1e-4
+    threshgorb: float = 4e-3
+    threshgocc: float = 1e-3
+    alpha: float = 0.01
+    RI: bool = True
+    HighSpin: bool = False
+    MSpin: int = 0
+    occ_method: str = "Softmax"
+    orb_method: str = "ADABelief"
+    h_cut: float = field(default_factory=lambda: 0.02 * math.sqrt(2.0))
+    # derived sizes (param.jl:71-125)
+    nalpha: int = 0
+    nbeta: int = 0
+    ndoc: int = 0
+    nsoc: int = 0
+    ndns: int = 0
+    nvir: int = 0
+    ncwo: int = -1
+    nac: int = 0
+    nbf5: int = 0
+    no0: int = 0
+    nv: int = 0
+    nvar: int = 0
+    noptorb: int = 0
+
+    def __post_init__(self):
+        self.nalpha = (self.ne + self.mul - 1) // 2
+        self.nbeta = (self.ne - self.mul + 1) // 2
+        self.ndoc = self.nbeta - self.no1
+        self.nsoc = self.nalpha - self.nbeta
+        self.ndns = self.ndoc + self.nsoc
+        self.nvir = self.nbf - self.nalpha
+        self.noptorb = self.nbf
+        self.set_ncwo(self.ncwo)
+
+    def set_ncwo(self, ncwo):
+        """param.jl:242-266."""
+        if self.ndns != 0:
+            if self.ndoc > 0:
+                if ncwo != 1 and (ncwo == -1 or ncwo > self.nvir / self.ndoc):
+                    ncwo = int(self.nvir / self.ndoc)
+            else:
+                ncwo = 0
+        self.ncwo = ncwo
+        self.nac = self.ndoc * (1 + ncwo)
+        self.nbf5 = self.no1 + self.nac + self.nsoc
+        self.no0 = self.nbf - self.nbf5
+        self.nv = self.ncwo * self.ndoc
+        self.nvar = round(self.nbf * (self.nbf - 1) / 2 - self.no0 * (self.no0 - 1) / 2)
+
+    def copy(self):
+        import copy
+        return copy.copy(self)
+
+
+def set_ncwo(p, ncwo):
+    p.set_ncwo(ncwo)

@@ -1,0 +1,108 @@
+/* libdonofhip: MI355X (gfx950) implementation of DoNOF.jl's orbital-optimization hot
+ * path behind a C ABI.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host arrays are column-major exactly as Julia
+ *    stores them: b[m,n,Q] at m + nbf*n + nbf*nbf*Q, C[m,p] at m + nbf*p.
+ *  - Host pointers are borrowed for the duration of the call; outputs go to
+ *    caller-owned buffers.  The library owns every device buffer of a dn_sys.
+ *  - Every entry point returns 0 (DN_OK) on success or a DN_ERR_* code; the message of
+ *    the last failure on the calling thread is returned by dn_last_error().
+ *  - A dn_sys is used by one host thread at a time and runs on its own HIP stream.
+ *
+ * Each entry point names the reference interface it replaces (file:line in DoNOF.jl
+ * v1.1.2).  The reference binds its GPU path by Julia dispatch on CuArray
+ * (ext/DoNOFGPUExt.jl); the ccall bindings that replace it are in INTEGRATION.md.
+ */
+#ifndef DONOF_HIP_H
+#define DONOF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DN_OK = 0,
+  DN_ERR_ARG = 1,       /* invalid argument or size */
+  DN_ERR_HIP = 2,       /* HIP runtime error */
+  DN_ERR_NOMEM = 3,     /* device allocation failed */
+  DN_ERR_BUDGET = 4,    /* allocation would exceed the context's HBM budget */
+  DN_ERR_NONFINITE = 5, /* NaN/Inf found in an output */
+  DN_ERR_STATE = 6      /* call not valid in the current state (e.g. b not uploaded) */
+};
+
+/* Arithmetic of the RI transform and J/K contractions. */
+enum { DN_FP64 = 0 };
+
+typedef struct dn_ctx dn_ctx;
+typedef struct dn_sys dn_sys;
+
+/* Sizes of one system; field meaning as DoNOF's Param (param.jl:3-57). */
+typedef struct dn_dims {
+  int32_t nbf;    /* AO basis size */
+  int32_t nbfaux; /* RI auxiliary basis size */
+  int32_t nbf5;   /* active natural orbitals */
+  int32_t no1;    /* frozen core */
+  int32_t ndoc;   /* strongly doubly occupied */
+  int32_t nsoc;   /* singly occupied */
+  int32_t nbeta;
+  int32_t nalpha;
+  int32_t ncwo;   /* weak orbitals per pair */
+  int32_t nv;     /* occupation variables = ncwo*ndoc */
+} dn_dims;
+
+const char* dn_last_error(void);
+int dn_device_count(int* count);
+
+/* Device context with an HBM budget (bytes; 0 = the device's free memory). */
+int dn_ctx_create(int device, uint64_t hbm_budget_bytes, dn_ctx** out);
+int dn_ctx_destroy(dn_ctx* ctx);
+int dn_ctx_hbm_used(const dn_ctx* ctx, uint64_t* used_bytes);
+
+/* One resident system: b, C, H, occupations, J/K/Lagrangian outputs and workspaces. */
+int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out);
+int dn_sys_destroy(dn_sys* sys);
+/* Bytes dn_sys_create would reserve for these sizes (admission control). */
+int dn_sys_footprint(const dn_dims* dims, int precision, uint64_t* bytes);
+
+/* b becomes device-resident once: replaces eris_to_gpu (ext/DoNOFGPUExt.jl:36-50,
+ * called from integrals.jl:59-62). */
+int dn_sys_upload_b(dn_sys* sys, const double* b);
+int dn_sys_download_b(dn_sys* sys, double* b);
+/* Synthetic b generated in HBM (Philox4x32-10 normals, see donof.jl_amd/synthetic.py). */
+int dn_sys_generate_b(dn_sys* sys, uint64_t seed);
+
+/* J_MO, K_MO (nbf5 x nbf5) and H_core (nbf5) from C (nbf x nbf) and H (nbf x nbf):
+ * replaces computeJKH_MO / JKH_MO (integrals.jl:131-172; ext/DoNOFGPUExt.jl:52-94). */
+int dn_jkh_mo(dn_sys* sys, const double* C, const double* H, double* J, double* K,
+              double* Hcore);
+
+/* elag (nbf x nbf, columns >= nbf5 zero) and H_mat = C^T H C5 (nbf x nbf5):
+ * replaces compute_Lagrange2 (utils.jl:1-59; ext/DoNOFGPUExt.jl:150-245).
+ * E_elec of computeE_elec(Hmat, n, elag) (utils.jl:216-227) is returned in *E if E is
+ * not NULL. */
+int dn_lagrange2(dn_sys* sys, const double* C, const double* n, const double* H,
+                 const double* cj12, const double* ck12, int nalpha, int nbeta,
+                 double* elag, double* Hmat, double* E);
+
+/* Resident variants: inputs already in HBM (set once), outputs stay in HBM.  They
+ * enqueue on the system's stream and return without synchronising. */
+int dn_sys_set_orbitals(dn_sys* sys, const double* C);
+int dn_sys_set_hcore(dn_sys* sys, const double* H);
+int dn_jkh_mo_resident(dn_sys* sys);
+int dn_sys_get_jkh(dn_sys* sys, double* J, double* K, double* Hcore);
+int dn_sys_sync(dn_sys* sys);
+
+/* Per-launch timing of the dominant kernel (the first RI transform) with HIP events on
+ * the system's stream. */
+int dn_sys_set_timing(dn_sys* sys, int enable);
+int dn_sys_get_timing(dn_sys* sys, double* transform_ms_total, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DONOF_HIP_H */
