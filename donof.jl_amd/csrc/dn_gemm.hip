@@ -23,17 +23,29 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned total) {
   return x * q + (x < r ? x : r) + i;
 }
 
+// LDS layouts (doubles):
+//   K-contiguous operand  -> [mn][BK+1]   rows of one m (or n), odd stride
+//   MN-contiguous operand -> [k][BMN+16]  rows of one k
+// Both make the 16-lane groups of an MFMA fragment read hit distinct banks, and the
+// global->LDS copy keeps consecutive threads on consecutive addresses.
+template <int BMN, int BK, bool KC>
+struct TileLds {
+  static constexpr int LD = KC ? (BK + 1) : (BMN + 16);
+  static constexpr int SIZE = KC ? BMN * (BK + 1) : BK * (BMN + 16);
+  static constexpr int PAIRS = BMN * BK / 2 / kThreads;  // 16-byte loads per thread
+  __device__ static int off(int mn, int k) { return KC ? mn * LD + k : k * LD + mn; }
+};
+
 template <int BM, int BN, int BK, bool AK, bool BKC>
 __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int tiles_m,
                                                                 int tiles_n, int nsplit) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int PA = 4, PB = 4;  // LDS row padding (doubles)
-  constexpr int A_PER = BM * BK / kThreads;
-  constexpr int B_PER = BN * BK / kThreads;
-  static_assert(BM * BK % kThreads == 0 && BN * BK % kThreads == 0, "tile/threads");
-  __shared__ double As[2][BK][BM + PA];
-  __shared__ double Bs[2][BK][BN + PB];
+  using LA = TileLds<BM, BK, AK>;
+  using LB = TileLds<BN, BK, BKC>;
+  static_assert(BM * BK % (2 * kThreads) == 0 && BN * BK % (2 * kThreads) == 0, "tile/threads");
+  __shared__ double As[2][LA::SIZE];
+  __shared__ double Bs[2][LB::SIZE];
 
   const unsigned total = (unsigned)tiles_m * tiles_n * nsplit;
   const unsigned hw = blockIdx.x;
@@ -60,48 +72,79 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  double ra[A_PER], rb[B_PER];
+  double2 ra[LA::PAIRS], rb[LB::PAIRS];
 
+  // element pair p of a tile: (mn, k) of its first element; pairs run along the
+  // contiguous dimension of the operand.
+  auto pair_coord = [](int idx, bool kc, int bmn, int* mn, int* k) {
+    if (kc) { *k = 2 * (idx % (BK / 2)); *mn = idx / (BK / 2); }
+    else { *mn = 2 * (idx % (bmn / 2)); *k = idx / (bmn / 2); }
+  };
+  auto load_pair = [](const double* base, long ld, bool kc, bool vec, int gmn, int gk, int MN,
+                      int KK) -> double2 {
+    // elements (gmn, gk) and the next one along the contiguous dimension
+    double2 v = {0.0, 0.0};
+    if (kc) {
+      if (gmn < MN) {
+        const double* p = base + (long)gmn * ld + gk;
+        if (vec && gk + 1 < KK) v = *reinterpret_cast<const double2*>(p);
+        else {
+          if (gk < KK) v.x = p[0];
+          if (gk + 1 < KK) v.y = p[1];
+        }
+      }
+    } else {
+      if (gk < KK) {
+        const double* p = base + gmn + (long)gk * ld;
+        if (vec && gmn + 1 < MN) v = *reinterpret_cast<const double2*>(p);
+        else {
+          if (gmn < MN) v.x = p[0];
+          if (gmn + 1 < MN) v.y = p[1];
+        }
+      }
+    }
+    return v;
+  };
   auto load_tile = [&](int t) {
     const int r = r0 + t / nkt;
     const int k0 = (t % nkt) * BK;
     const double* A = g.A + (long)r * g.sA;
     const double* B = g.B + (long)r * g.sB;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      const int idx = tid + i * kThreads;
-      int m, k;
-      if (AK) { k = idx % BK; m = idx / BK; } else { m = idx % BM; k = idx / BM; }
-      const int gm = m0 + m, gk = k0 + k;
-      double v = 0.0;
-      if (gm < M && gk < K) v = AK ? A[(long)gm * g.lda + gk] : A[gm + (long)gk * g.lda];
-      ra[i] = v;
+    for (int i = 0; i < LA::PAIRS; ++i) {
+      int mn, k;
+      pair_coord(tid + i * kThreads, AK, BM, &mn, &k);
+      ra[i] = load_pair(A, g.lda, AK, g.vec_a, m0 + mn, k0 + k, M, K);
     }
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int idx = tid + i * kThreads;
-      int n, k;
-      if (BKC) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
-      const int gn = n0 + n, gk = k0 + k;
-      double v = 0.0;
-      if (gn < N && gk < K) v = BKC ? B[gk + (long)gn * g.ldb] : B[(long)gk * g.ldb + gn];
-      rb[i] = v;
+    for (int i = 0; i < LB::PAIRS; ++i) {
+      int mn, k;
+      pair_coord(tid + i * kThreads, BKC, BN, &mn, &k);
+      rb[i] = load_pair(B, g.ldb, BKC, g.vec_b, n0 + mn, k0 + k, N, K);
     }
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      const int idx = tid + i * kThreads;
-      int m, k;
-      if (AK) { k = idx % BK; m = idx / BK; } else { m = idx % BM; k = idx / BM; }
-      As[buf][k][m] = ra[i];
+    for (int i = 0; i < LA::PAIRS; ++i) {
+      int mn, k;
+      pair_coord(tid + i * kThreads, AK, BM, &mn, &k);
+      if (AK) {
+        As[buf][LA::off(mn, k)] = ra[i].x;
+        As[buf][LA::off(mn, k + 1)] = ra[i].y;
+      } else {
+        *reinterpret_cast<double2*>(&As[buf][LA::off(mn, k)]) = ra[i];
+      }
     }
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int idx = tid + i * kThreads;
-      int n, k;
-      if (BKC) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
-      Bs[buf][k][n] = rb[i];
+    for (int i = 0; i < LB::PAIRS; ++i) {
+      int mn, k;
+      pair_coord(tid + i * kThreads, BKC, BN, &mn, &k);
+      if (BKC) {
+        Bs[buf][LB::off(mn, k)] = rb[i].x;
+        Bs[buf][LB::off(mn, k + 1)] = rb[i].y;
+      } else {
+        *reinterpret_cast<double2*>(&Bs[buf][LB::off(mn, k)]) = rb[i];
+      }
     }
   };
 
@@ -118,9 +161,9 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
     for (int kk = 0; kk < BK; kk += 4) {
       double a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) a[i] = As[buf][kk + fk][wm * WM + i * 16 + fr];
+      for (int i = 0; i < FM; ++i) a[i] = As[buf][LA::off(wm * WM + i * 16 + fr, kk + fk)];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kk + fk][wn * WN + j * 16 + fr];
+      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][LB::off(wn * WN + j * 16 + fr, kk + fk)];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -131,8 +174,9 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
     __syncthreads();
   }
 
-  // D(16x16) layout of v_mfma_f64_16x16x4f64: lane l holds column l%16, rows
-  // 4*(l/16) .. 4*(l/16)+3 in its four accumulator registers.
+  // D(16x16) layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l holds
+  // column l%16 and rows l/16 + 4*r in accumulator register r.  A store instruction
+  // (fixed r) therefore writes a 4-row x 16-column block.
   double* Cz = g.C + (long)z * g.sC;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -142,7 +186,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
       if (col >= N) continue;
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int row = m0 + wm * WM + i * 16 + 4 * fk + rr;
+        const int row = m0 + wm * WM + i * 16 + fk + 4 * rr;
         if (row < M) {
           double* c = Cz + row + (long)col * g.ldc;
           const double v = g.alpha * acc[i][j][rr];
@@ -190,10 +234,17 @@ int dispatch_layout(hipStream_t s, const Gemm& g) {
 
 }  // namespace
 
-int gemm_f64(hipStream_t s, const Gemm& g) {
-  if (g.M <= 0 || g.N <= 0) return DN_OK;
-  if (g.K <= 0 || g.nbatch <= 0 || g.bps <= 0)
+int gemm_f64(hipStream_t s, const Gemm& g_in) {
+  if (g_in.M <= 0 || g_in.N <= 0) return DN_OK;
+  if (g_in.K <= 0 || g_in.nbatch <= 0 || g_in.bps <= 0)
     return fail(DN_ERR_ARG, "gemm_f64: bad K/nbatch/bps");
+  Gemm g = g_in;
+  // 16-byte loads need an even leading dimension and batch stride and an aligned base
+  auto aligned = [](const double* p, long ld, long sb, int nb) {
+    return ((uintptr_t)p % 16 == 0) && (ld % 2 == 0) && (nb <= 1 || sb % 2 == 0);
+  };
+  g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch);
+  g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch);
   // Tall-M operands get the 128-row tile (the RI transforms have M = nbf5 <= 128 or
   // a multiple of it); everything else uses 64 x 64.
   if (g.M > 64 && g.N >= 64) return dispatch_layout<128, 64>(s, g);

@@ -48,6 +48,7 @@ struct Gemm {
   double alpha = 1.0, beta = 0.0;
   int nbatch = 1;
   int bps = 1;  // batches per split
+  bool vec_a = false, vec_b = false;  // set by gemm_f64: 16-byte loads allowed
 };
 
 int gemm_f64(hipStream_t s, const Gemm& g);

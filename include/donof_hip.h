@@ -37,6 +37,13 @@ enum {
 /* Arithmetic of the RI transform and J/K contractions. */
 enum { DN_FP64 = 0 };
 
+/* Orbital optimiser of the ADAM family: p.orb_method "ADAM", "ADABelief", "YOGI",
+ * "Demon" (minimization.jl:242-630). */
+enum { DN_ADAM = 0, DN_ADABELIEF = 1, DN_YOGI = 2, DN_DEMON = 3 };
+
+/* Occupation parametrisation: p.occ_method "Softmax", "Trigonometric" (pnof.jl:1024-1045). */
+enum { DN_OCC_SOFTMAX = 0, DN_OCC_TRIGONOMETRIC = 1 };
+
 typedef struct dn_ctx dn_ctx;
 typedef struct dn_sys dn_sys;
 
