@@ -30,6 +30,20 @@ class dn_dims(ct.Structure):
                 ("nbf", "nbfaux", "nbf5", "no1", "ndoc", "nsoc", "nbeta", "nalpha", "ncwo", "nv")]
 
 
+class dn_functional(ct.Structure):
+    _fields_ = [("ipnof", ct.c_int32), ("ista", ct.c_int32), ("occ_method", ct.c_int32),
+                ("highspin", ct.c_int32), ("mspin", ct.c_int32), ("h_cut", ct.c_double)]
+
+
+class dn_orbopt_state(ct.Structure):
+    _fields_ = [("method", ct.c_int32), ("maxloop", ct.c_int32), ("alpha", ct.c_double),
+                ("threshgorb", ct.c_double), ("nit", ct.c_int32), ("success", ct.c_int32),
+                ("E", ct.c_double)]
+
+
+DN_ADAM, DN_ADABELIEF, DN_YOGI, DN_DEMON = range(4)
+DN_OCC_SOFTMAX, DN_OCC_TRIGONOMETRIC = range(2)
+
 _P = ct.c_void_p
 _D = ct.POINTER(ct.c_double)
 _I = ct.c_int
@@ -54,6 +68,17 @@ _SIGS = {
     "dn_sys_sync": (_I, [_P]),
     "dn_sys_set_timing": (_I, [_P, _I]),
     "dn_sys_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),
+    "dn_ocupacion": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D]),
+    "dn_pnof_coefficients": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D]),
+    "dn_der_pnof_coefficients": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D]),
+    "dn_calce": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, _D, _D, _D]),
+    "dn_calcocce": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, _D]),
+    "dn_calcoccg": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, _D]),
+    "dn_rotate_orbital": (_I, [_P, _D, _D, _D]),
+    "dn_orbopt": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state), _D, _D, _D]),
+    "dn_sys_set_gamma": (_I, [_P, _D]),
+    "dn_sys_get_orbitals": (_I, [_P, _D]),
+    "dn_orbopt_resident": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state)]),
 }
 
 _lib = None

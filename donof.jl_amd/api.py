@@ -226,5 +226,177 @@ def compute_Lagrange2(C, n, H, b, cj12, ck12, nalpha, nbeta, return_energy=False
     return elag, Hmat
 
 
+# --------------------------------------------------------------------------------------
+# pnof.jl: occupations, coefficients, occupation energy and gradient
+# --------------------------------------------------------------------------------------
+
+_OCC = {"Softmax": _lib.DN_OCC_SOFTMAX, "Trigonometric": _lib.DN_OCC_TRIGONOMETRIC}
+_ORB = {"ADAM": _lib.DN_ADAM, "ADABelief": _lib.DN_ADABELIEF, "YOGI": _lib.DN_YOGI,
+        "Demon": _lib.DN_DEMON}
+
+
+def functional_of(p):
+    """The dn_functional record of a Param (p.ipnof, p.ista, p.occ_method, ...)."""
+    if p.occ_method not in _OCC:
+        raise NotImplementedError("occ_method %r (EBI is out of scope)" % p.occ_method)
+    f = _lib.dn_functional()
+    f.ipnof, f.ista, f.occ_method = int(p.ipnof), int(p.ista), _OCC[p.occ_method]
+    f.highspin, f.mspin, f.h_cut = int(bool(p.HighSpin)), int(p.MSpin), float(p.h_cut)
+    return f
+
+
+def ocupacion(gamma, p, I):
+    """(n, dn_dgamma) of pnof.jl:1024-1045 on the device of system I."""
+    s = _require_system(I)
+    f = functional_of(p)
+    g = _vec(gamma, p.nv, "gamma")
+    n = np.empty(p.nbf5)
+    dn = np.empty((p.nbf5, p.nv), order="F")
+    check(s.lib.dn_ocupacion(s.handle, ct.byref(f), dptr(g), dptr(n), dptr(dn)))
+    return n, dn
+
+
+def PNOFi_selector(n, p, I):
+    """(cj12, ck12) with zero diagonal (pnof.jl:1-53)."""
+    s = _require_system(I)
+    f = functional_of(p)
+    n = _vec(n, p.nbf5, "n")
+    cj = np.empty((p.nbf5, p.nbf5), order="F")
+    ck = np.empty((p.nbf5, p.nbf5), order="F")
+    check(s.lib.dn_pnof_coefficients(s.handle, ct.byref(f), dptr(n), dptr(cj), dptr(ck)))
+    return cj, ck
+
+
+def der_PNOFi_selector(n, dn_dgamma, p, I):
+    """(Dcj12r, Dck12r), nbf5 x nbf5 x nv (pnof.jl:55-117)."""
+    s = _require_system(I)
+    f = functional_of(p)
+    n = _vec(n, p.nbf5, "n")
+    dn = _colmajor(dn_dgamma, (p.nbf5, p.nv), "dn_dgamma")
+    Dcj = np.empty((p.nbf5, p.nbf5, p.nv), order="F")
+    Dck = np.empty((p.nbf5, p.nbf5, p.nv), order="F")
+    check(s.lib.dn_der_pnof_coefficients(s.handle, ct.byref(f), dptr(n), dptr(dn), dptr(Dcj),
+                                         dptr(Dck)))
+    return Dcj, Dck
+
+
+def _jkh_args(p, J, K, H_core):
+    if J is None and K is None and H_core is None:
+        return None, None, None
+    return (_colmajor(J, (p.nbf5, p.nbf5), "J_MO"), _colmajor(K, (p.nbf5, p.nbf5), "K_MO"),
+            _vec(H_core, p.nbf5, "H_core"))
+
+
+def calce(n, cj12, ck12, J_MO, K_MO, H_core, p, I):
+    """pnof.jl:1273-1300.  J_MO, K_MO, H_core = None uses the resident ones."""
+    s = _require_system(I)
+    f = functional_of(p)
+    n = _vec(n, p.nbf5, "n")
+    cj = _colmajor(cj12, (p.nbf5, p.nbf5), "cj12")
+    ck = _colmajor(ck12, (p.nbf5, p.nbf5), "ck12")
+    J, K, Hc = _jkh_args(p, J_MO, K_MO, H_core)
+    E = ct.c_double()
+    check(s.lib.dn_calce(s.handle, ct.byref(f), dptr(n), dptr(cj), dptr(ck), dptr(J), dptr(K),
+                         dptr(Hc), ct.byref(E)))
+    return E.value
+
+
+def calcocce(gamma, J_MO, K_MO, H_core, p, I):
+    """pnof.jl:1302-1322."""
+    s = _require_system(I)
+    f = functional_of(p)
+    g = _vec(gamma, p.nv, "gamma")
+    J, K, Hc = _jkh_args(p, J_MO, K_MO, H_core)
+    E = ct.c_double()
+    check(s.lib.dn_calcocce(s.handle, ct.byref(f), dptr(g), dptr(J), dptr(K), dptr(Hc),
+                            ct.byref(E)))
+    return E.value
+
+
+def calcoccg(gamma, J_MO, K_MO, H_core, p, I):
+    """pnof.jl:1324-1377."""
+    s = _require_system(I)
+    f = functional_of(p)
+    g = _vec(gamma, p.nv, "gamma")
+    J, K, Hc = _jkh_args(p, J_MO, K_MO, H_core)
+    grad = np.empty(p.nv)
+    check(s.lib.dn_calcoccg(s.handle, ct.byref(f), dptr(g), dptr(J), dptr(K), dptr(Hc),
+                            dptr(grad)))
+    return grad
+
+
+# --------------------------------------------------------------------------------------
+# utils.jl:499-521, pnof.jl:1389-1407
+# --------------------------------------------------------------------------------------
+
+
+def rotate_orbital(y, C, p, I):
+    """Cnew = C exp(Y(y)) in FP64 on the device (utils.jl:499-521)."""
+    s = _require_system(I)
+    y = _vec(y, p.nvar, "y")
+    C = _colmajor(C, (p.nbf, p.nbf), "C")
+    out = np.empty((p.nbf, p.nbf), order="F")
+    check(s.lib.dn_rotate_orbital(s.handle, dptr(y), dptr(C), dptr(out)))
+    return out
+
+
+def pack_grad(elag, nbf5):
+    """grads = 4 elag - 4 elag^T packed row-wise over i < nbf5, j > i
+    (pnof.jl:1395-1403)."""
+    nbf = elag.shape[0]
+    g = 4.0 * elag - 4.0 * elag.T
+    rows = np.repeat(np.arange(nbf5), [nbf - i - 1 for i in range(nbf5)])
+    cols = np.concatenate([np.arange(i + 1, nbf) for i in range(nbf5)]) if nbf5 else \
+        np.zeros(0, dtype=int)
+    return g[rows, cols]
+
+
+def calcorbg(y, n, cj12, ck12, C, H, I, pa):
+    """Packed orbital gradient at rotate(y, C) (pnof.jl:1389-1407)."""
+    Cnew = rotate_orbital(y, C, pa, I)
+    elag, _ = compute_Lagrange2(Cnew, n, H, I, cj12, ck12, pa.nalpha, pa.nbeta)
+    return pack_grad(elag, pa.nbf5)
+
+
+def orbopt(method, gamma, C, H, I, p):
+    """ADAM-family orbital optimiser (minimization.jl:242-630) run entirely on the
+    device: returns (E, C, nit, success) and updates p.alpha / p.maxloop as the
+    reference does."""
+    s = _require_system(I)
+    f = functional_of(p)
+    st = _lib.dn_orbopt_state()
+    st.method = _ORB[method]
+    st.maxloop, st.alpha, st.threshgorb = int(p.maxloop), float(p.alpha), float(p.threshgorb)
+    g = _vec(gamma, p.nv, "gamma")
+    Cio = np.array(_colmajor(C, (p.nbf, p.nbf), "C"), order="F", copy=True)
+    H = _colmajor(H, (p.nbf, p.nbf), "H")
+    check(s.lib.dn_orbopt(s.handle, ct.byref(f), ct.byref(st), dptr(g), dptr(Cio), dptr(H)))
+    p.maxloop, p.alpha = int(st.maxloop), float(st.alpha)
+    return st.E, Cio, int(st.nit), bool(st.success)
+
+
+def orbopt_adam(gamma, C, H, I, p):
+    """minimization.jl:344-438."""
+    return orbopt("ADAM", gamma, C, H, I, p)
+
+
+def orbopt_adabelief(gamma, C, H, I, p):
+    """minimization.jl:441-536."""
+    return orbopt("ADABelief", gamma, C, H, I, p)
+
+
+def orbopt_yogi(gamma, C, H, I, p):
+    """minimization.jl:539-630."""
+    return orbopt("YOGI", gamma, C, H, I, p)
+
+
+def orbopt_demon(gamma, C, H, I, p):
+    """minimization.jl:242-340."""
+    return orbopt("Demon", gamma, C, H, I, p)
+
+
 __all__ = ["Context", "System", "default_context", "footprint", "JKH_MO", "computeJKH_MO",
-           "compute_Lagrange2", "DnError"]
+           "compute_Lagrange2", "DnError", "functional_of", "ocupacion", "PNOFi_selector",
+           "der_PNOFi_selector", "calce", "calcocce", "calcoccg", "rotate_orbital",
+           "calcorbg", "pack_grad", "orbopt", "orbopt_adam", "orbopt_adabelief",
+           "orbopt_yogi", "orbopt_demon"]

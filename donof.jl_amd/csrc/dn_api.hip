@@ -45,9 +45,10 @@ bool dims_valid(const dn_dims& d, std::string* why) {
 // Element counts of every device buffer of a system.
 struct Layout {
   size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bmat, xw, part, scal;
+  size_t nv, dnm, nvar, part2;
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + 2 * big + bmat + 2 * xw +
-                part + scal);
+                part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2);
   }
 };
 
@@ -78,6 +79,10 @@ Layout layout_of(const dn_dims& d) {
   const size_t lag_part = lag_splits(d) * L.nbf_nbf5;
   L.part = std::max(jk_part, lag_part);
   L.scal = 64;
+  L.nv = std::max<size_t>(d.nv, 1);
+  L.dnm = std::max<size_t>(nbf5 * d.nv, 1);
+  L.nvar = nbf * nbf5 - nbf5 * (nbf5 + 1) / 2;
+  L.part2 = 16 * nbf * nbf;
   return L;
 }
 
@@ -96,10 +101,6 @@ void release(dn_sys* s) {
   s->owned.clear();
 }
 
-int check_sys(dn_sys* s) {
-  if (!s) return fail(DN_ERR_ARG, "null dn_sys");
-  return DN_OK;
-}
 
 // b_pn[p,(n,Q)] = sum_m C5[m,p] b[m,(n,Q)]: one GEMM with N = nbf*naux (integrals.jl:159).
 int transform1(dn_sys* s) {
@@ -153,15 +154,22 @@ int hc_products(dn_sys* s, bool want_hmat) {
   g.A = s->H; g.lda = d.nbf; g.a_kcontig = false;
   g.B = s->C; g.ldb = d.nbf; g.b_kcontig = true;
   g.C = s->HC; g.ldc = d.nbf;
-  DN_TRY(gemm_f64(s->stream, g));
+  DN_TRY(gemm_f64_splitk(s->stream, g, s->part2, s->part2_elems));
   if (want_hmat) {
     Gemm h;
     h.M = d.nbf; h.N = d.nbf5; h.K = d.nbf;
     h.A = s->C; h.lda = d.nbf; h.a_kcontig = true;
     h.B = s->HC; h.ldb = d.nbf; h.b_kcontig = true;
     h.C = s->Hmat; h.ldc = d.nbf;
-    DN_TRY(gemm_f64(s->stream, h));
+    DN_TRY(gemm_f64_splitk(s->stream, h, s->part2, s->part2_elems));
   }
+  return DN_OK;
+}
+
+}  // namespace
+
+int check_sys(dn_sys* s) {
+  if (!s) return fail(DN_ERR_ARG, "null dn_sys");
   return DN_OK;
 }
 
@@ -226,8 +234,6 @@ bool all_finite(const double* x, size_t n) {
     if (!std::isfinite(x[i])) return false;
   return true;
 }
-
-}  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -338,6 +344,26 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.xw, &s->w);
   A(L.part, &s->part);
   A(L.scal, &s->scal);
+  A(L.nv, &s->gamma);
+  A(L.dnm, &s->dn);
+  A(L.nv, &s->goc);
+  A(6 * L.n, &s->orb);
+  double* orbi = nullptr;
+  A(3 * L.n, &orbi);  // 3*nbf5 ints fit in 3*nbf5 doubles
+  s->orbi = reinterpret_cast<int*>(orbi);
+  s->nvar = (long)L.nvar;
+  A(L.nvar, &s->gv);
+  A(L.nvar, &s->mv);
+  A(L.nvar, &s->vv);
+  A(L.nvar, &s->vmax);
+  A(L.nvar, &s->yv);
+  A(L.nbf2, &s->Y);
+  A(L.nbf2, &s->U);
+  A(5 * L.nbf2, &s->rwork);
+  A(L.nbf2, &s->Ca);
+  A(L.nbf2, &s->bestC);
+  A(L.part2, &s->part2);
+  s->part2_elems = L.part2;
   s->part_elems = L.part;
   hipError_t e = hipSuccess;
   if (rc == DN_OK) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);

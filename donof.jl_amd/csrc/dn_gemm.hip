@@ -13,8 +13,6 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int kThreads = 256;  // 4 waves in a 2 x 2 wave grid
-
 // Logical tile id for the hardware workgroup id `hw` so that consecutive logical ids
 // run on the same XCD (the dispatcher deals workgroups round-robin over 8 XCDs).
 __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned total) {
@@ -23,27 +21,64 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned total) {
   return x * q + (x < r ? x : r) + i;
 }
 
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 // LDS layouts (doubles):
 //   K-contiguous operand  -> [mn][BK+1]   rows of one m (or n), odd stride
 //   MN-contiguous operand -> [k][BMN+16]  rows of one k
 // Both make the 16-lane groups of an MFMA fragment read hit distinct banks, and the
 // global->LDS copy keeps consecutive threads on consecutive addresses.
-template <int BMN, int BK, bool KC>
+template <int BMN, int BK, bool KC, int NT>
 struct TileLds {
   static constexpr int LD = KC ? (BK + 1) : (BMN + 16);
   static constexpr int SIZE = KC ? BMN * (BK + 1) : BK * (BMN + 16);
-  static constexpr int PAIRS = BMN * BK / 2 / kThreads;  // 16-byte loads per thread
+  static constexpr int PAIRS = BMN * BK / 2 / NT;  // 16-byte loads per thread
   __device__ static int off(int mn, int k) { return KC ? mn * LD + k : k * LD + mn; }
+  // (mn, k) of the first element of pair idx; pairs run along the contiguous dimension
+  __device__ static void coord(int idx, int* mn, int* k) {
+    if (KC) { *k = 2 * (idx % (BK / 2)); *mn = idx / (BK / 2); }
+    else { *mn = 2 * (idx % (BMN / 2)); *k = idx / (BMN / 2); }
+  }
 };
 
-template <int BM, int BN, int BK, bool AK, bool BKC>
-__global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int tiles_m,
-                                                                int tiles_n, int nsplit) {
-  constexpr int WM = BM / 2, WN = BN / 2;
+// Two consecutive elements along the contiguous dimension, zero outside [0,MN)x[0,KK).
+// Branch-free: the address is clamped into the operand and the value selected.  With
+// `vec` (uniform) the pair is one 16-byte load; it requires an even extent along the
+// contiguous dimension, an even leading dimension and a 16-byte aligned base.
+template <bool KC>
+__device__ __forceinline__ v2d load_pair(const double* __restrict__ base, long ld, bool vec,
+                                         int gmn, int gk, int MN, int KK) {
+  v2d v;
+  if (vec) {
+    const bool ok = (gmn < MN) && (gk < KK);
+    const int cmn = min(gmn, MN - (KC ? 1 : 2));
+    const int ck = min(gk, KK - (KC ? 2 : 1));
+    const double* p = KC ? base + (long)cmn * ld + ck : base + cmn + (long)ck * ld;
+    v = *reinterpret_cast<const v2d*>(p);
+    if (!ok) v = v2d{0.0, 0.0};
+  } else {
+    const int cmn0 = min(gmn, MN - 1), ck0 = min(gk, KK - 1);
+    const int gmn1 = KC ? gmn : gmn + 1, gk1 = KC ? gk + 1 : gk;
+    const int cmn1 = min(gmn1, MN - 1), ck1 = min(gk1, KK - 1);
+    const double x0 = KC ? base[(long)cmn0 * ld + ck0] : base[cmn0 + (long)ck0 * ld];
+    const double x1 = KC ? base[(long)cmn1 * ld + ck1] : base[cmn1 + (long)ck1 * ld];
+    v.x = (gmn < MN && gk < KK) ? x0 : 0.0;
+    v.y = (gmn1 < MN && gk1 < KK) ? x1 : 0.0;
+  }
+  return v;
+}
+
+// Tile BM x BN per workgroup of WGM x WGN waves; each wave owns (BM/WGM) x (BN/WGN).
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
+                                                                   int tiles_n, int nsplit) {
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
-  using LA = TileLds<BM, BK, AK>;
-  using LB = TileLds<BN, BK, BKC>;
-  static_assert(BM * BK % (2 * kThreads) == 0 && BN * BK % (2 * kThreads) == 0, "tile/threads");
+  using LA = TileLds<BM, BK, AK, NT>;
+  using LB = TileLds<BN, BK, BKC, NT>;
+  static_assert(FM >= 1 && FN >= 1 && WM % 16 == 0 && WN % 16 == 0, "wave tile");
+  static_assert(BM * BK % (2 * NT) == 0 && BN * BK % (2 * NT) == 0, "tile/threads");
   __shared__ double As[2][LA::SIZE];
   __shared__ double Bs[2][LB::SIZE];
 
@@ -57,7 +92,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
   const int M = g.M, N = g.N, K = g.K;
 
@@ -72,39 +107,8 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  double2 ra[LA::PAIRS], rb[LB::PAIRS];
+  v2d ra[LA::PAIRS], rb[LB::PAIRS];
 
-  // element pair p of a tile: (mn, k) of its first element; pairs run along the
-  // contiguous dimension of the operand.
-  auto pair_coord = [](int idx, bool kc, int bmn, int* mn, int* k) {
-    if (kc) { *k = 2 * (idx % (BK / 2)); *mn = idx / (BK / 2); }
-    else { *mn = 2 * (idx % (bmn / 2)); *k = idx / (bmn / 2); }
-  };
-  auto load_pair = [](const double* base, long ld, bool kc, bool vec, int gmn, int gk, int MN,
-                      int KK) -> double2 {
-    // elements (gmn, gk) and the next one along the contiguous dimension
-    double2 v = {0.0, 0.0};
-    if (kc) {
-      if (gmn < MN) {
-        const double* p = base + (long)gmn * ld + gk;
-        if (vec && gk + 1 < KK) v = *reinterpret_cast<const double2*>(p);
-        else {
-          if (gk < KK) v.x = p[0];
-          if (gk + 1 < KK) v.y = p[1];
-        }
-      }
-    } else {
-      if (gk < KK) {
-        const double* p = base + gmn + (long)gk * ld;
-        if (vec && gmn + 1 < MN) v = *reinterpret_cast<const double2*>(p);
-        else {
-          if (gmn < MN) v.x = p[0];
-          if (gmn + 1 < MN) v.y = p[1];
-        }
-      }
-    }
-    return v;
-  };
   auto load_tile = [&](int t) {
     const int r = r0 + t / nkt;
     const int k0 = (t % nkt) * BK;
@@ -113,37 +117,37 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
 #pragma unroll
     for (int i = 0; i < LA::PAIRS; ++i) {
       int mn, k;
-      pair_coord(tid + i * kThreads, AK, BM, &mn, &k);
-      ra[i] = load_pair(A, g.lda, AK, g.vec_a, m0 + mn, k0 + k, M, K);
+      LA::coord(tid + i * NT, &mn, &k);
+      ra[i] = load_pair<AK>(A, g.lda, g.vec_a, m0 + mn, k0 + k, M, K);
     }
 #pragma unroll
     for (int i = 0; i < LB::PAIRS; ++i) {
       int mn, k;
-      pair_coord(tid + i * kThreads, BKC, BN, &mn, &k);
-      rb[i] = load_pair(B, g.ldb, BKC, g.vec_b, n0 + mn, k0 + k, N, K);
+      LB::coord(tid + i * NT, &mn, &k);
+      rb[i] = load_pair<BKC>(B, g.ldb, g.vec_b, n0 + mn, k0 + k, N, K);
     }
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < LA::PAIRS; ++i) {
       int mn, k;
-      pair_coord(tid + i * kThreads, AK, BM, &mn, &k);
+      LA::coord(tid + i * NT, &mn, &k);
       if (AK) {
         As[buf][LA::off(mn, k)] = ra[i].x;
         As[buf][LA::off(mn, k + 1)] = ra[i].y;
       } else {
-        *reinterpret_cast<double2*>(&As[buf][LA::off(mn, k)]) = ra[i];
+        *reinterpret_cast<v2d*>(&As[buf][LA::off(mn, k)]) = ra[i];
       }
     }
 #pragma unroll
     for (int i = 0; i < LB::PAIRS; ++i) {
       int mn, k;
-      pair_coord(tid + i * kThreads, BKC, BN, &mn, &k);
+      LB::coord(tid + i * NT, &mn, &k);
       if (BKC) {
         Bs[buf][LB::off(mn, k)] = rb[i].x;
         Bs[buf][LB::off(mn, k + 1)] = rb[i].y;
       } else {
-        *reinterpret_cast<double2*>(&Bs[buf][LB::off(mn, k)]) = rb[i];
+        *reinterpret_cast<v2d*>(&Bs[buf][LB::off(mn, k)]) = rb[i];
       }
     }
   };
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(kThreads, 2) void dgemm_mfma_kernel(Gemm g, int til
 
 __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int nsplit, int M,
                                      int N, long ld, double* __restrict__ out, long ldo,
-                                     double beta) {
+                                     double beta, double scale) {
   const long total = (long)M * N;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
@@ -206,11 +210,12 @@ __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int 
     double s = 0.0;
     for (int z = 0; z < nsplit; ++z) s += P[z * sP + m + n * ld];
     double* o = out + m + n * ldo;
+    s *= scale;
     *o = (beta == 0.0) ? s : s + beta * (*o);
   }
 }
 
-template <int BM, int BN, bool AK, bool BKC>
+template <int BM, int BN, int WGM, int WGN, bool AK, bool BKC>
 int launch(hipStream_t s, const Gemm& g) {
   constexpr int BK = 16;
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
@@ -218,18 +223,20 @@ int launch(hipStream_t s, const Gemm& g) {
   const long total = (long)tm * tn * nsplit;
   if (total <= 0) return DN_OK;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "gemm grid too large");
-  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, AK, BKC>), dim3((unsigned)total),
-                     dim3(kThreads), 0, s, g, tm, tn, nsplit);
+  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC>), dim3((unsigned)total),
+                     dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int WGM, int WGN>
 int dispatch_layout(hipStream_t s, const Gemm& g) {
   if (g.a_kcontig) {
-    return g.b_kcontig ? launch<BM, BN, true, true>(s, g) : launch<BM, BN, true, false>(s, g);
+    return g.b_kcontig ? launch<BM, BN, WGM, WGN, true, true>(s, g)
+                       : launch<BM, BN, WGM, WGN, true, false>(s, g);
   }
-  return g.b_kcontig ? launch<BM, BN, false, true>(s, g) : launch<BM, BN, false, false>(s, g);
+  return g.b_kcontig ? launch<BM, BN, WGM, WGN, false, true>(s, g)
+                     : launch<BM, BN, WGM, WGN, false, false>(s, g);
 }
 
 }  // namespace
@@ -240,25 +247,54 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
     return fail(DN_ERR_ARG, "gemm_f64: bad K/nbatch/bps");
   Gemm g = g_in;
   // 16-byte loads need an even leading dimension and batch stride and an aligned base
-  auto aligned = [](const double* p, long ld, long sb, int nb) {
-    return ((uintptr_t)p % 16 == 0) && (ld % 2 == 0) && (nb <= 1 || sb % 2 == 0);
+  auto aligned = [](const double* p, long ld, long sb, int nb, int extent) {
+    return ((uintptr_t)p % 16 == 0) && (ld % 2 == 0) && (nb <= 1 || sb % 2 == 0) &&
+           (extent % 2 == 0) && extent >= 2;
   };
-  g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch);
-  g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch);
+  g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
+  g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
   // Tall-M operands get the 128-row tile (the RI transforms have M = nbf5 <= 128 or
   // a multiple of it); everything else uses 64 x 64.
-  if (g.M > 64 && g.N >= 64) return dispatch_layout<128, 64>(s, g);
-  return dispatch_layout<64, 64>(s, g);
+  if (g.M > 64 && g.N >= 64) return dispatch_layout<128, 64, 2, 2>(s, g);
+  return dispatch_layout<64, 64, 2, 2>(s, g);
+}
+
+int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {
+  if (g.nbatch != 1) return gemm_f64(s, g);
+  const long tiles = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+  // largest split count (<= 16, dividing K, chunks >= 32) that keeps ~1024 workgroups
+  int best = 1;
+  for (int d = 2; d <= 16; ++d) {
+    if (g.K % d || g.K / d < 32) continue;
+    if ((size_t)d * g.M * g.N > part_elems) break;
+    if (tiles * d > 2048) break;
+    best = d;
+  }
+  if (best == 1 || g.beta != 0.0) return gemm_f64(s, g);
+  const int kc = g.K / best;
+  Gemm h = g;
+  h.K = kc;
+  h.sA = (long)kc * (g.a_kcontig ? 1 : g.lda);
+  h.sB = (long)kc * (g.b_kcontig ? 1 : g.ldb);
+  h.nbatch = best;
+  h.bps = 1;
+  h.C = part;
+  h.ldc = g.M;
+  h.sC = (long)g.M * g.N;
+  h.alpha = 1.0;
+  h.beta = 0.0;
+  DN_TRY(gemm_f64(s, h));
+  return reduce_splits(s, part, (long)g.M * g.N, best, g.M, g.N, g.M, g.C, g.ldc, 0.0, g.alpha);
 }
 
 int reduce_splits(hipStream_t s, const double* P, long sP, int nsplit, int M, int N, long ld,
-                  double* out, long ldo, double beta) {
+                  double* out, long ldo, double beta, double scale) {
   const long total = (long)M * N;
   if (total <= 0) return DN_OK;
   const int threads = 256;
   const long blocks = (total + threads - 1) / threads;
   hipLaunchKernelGGL(reduce_splits_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)),
-                     dim3(threads), 0, s, P, sP, nsplit, M, N, ld, out, ldo, beta);
+                     dim3(threads), 0, s, P, sP, nsplit, M, N, ld, out, ldo, beta, scale);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

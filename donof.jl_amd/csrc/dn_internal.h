@@ -54,7 +54,10 @@ struct Gemm {
 int gemm_f64(hipStream_t s, const Gemm& g);
 // Sum `nsplit` partial matrices (each M x N, leading dim ld, stride sP) into out.
 int reduce_splits(hipStream_t s, const double* P, long sP, int nsplit, int M, int N,
-                  long ld, double* out, long ldo, double beta);
+                  long ld, double* out, long ldo, double beta, double scale = 1.0);
+// Single GEMM split along K into partials (part: >= 16*M*N doubles) and reduced; for
+// small outputs whose tile count alone cannot fill the GPU.
+int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems);
 
 // ---- device buffers -----------------------------------------------------------------
 struct DevBuf {
@@ -62,6 +65,16 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+}  // namespace dn
+
+namespace dn {
+// resident building blocks (dn_api.hip)
+int check_sys(dn_sys* s);
+int h2d(dn_sys* s, double* dst, const double* src, size_t elems);
+int d2h(dn_sys* s, double* dst, const double* src, size_t elems);
+int jkh_resident(dn_sys* s);
+int lagrange_resident(dn_sys* s, int nalpha, int nbeta);
+bool all_finite(const double* x, size_t n);
 }  // namespace dn
 
 struct dn_ctx {
@@ -99,6 +112,26 @@ struct dn_sys {
   double* part = nullptr;   // split partials
   size_t part_elems = 0;
   double* scal = nullptr;   // small scalars (energy, norms)
+  // occupation functional
+  double* gamma = nullptr;  // nv
+  double* dn = nullptr;     // nbf5*nv : dn/dgamma
+  double* goc = nullptr;    // nv      : occupation gradient
+  double* orb = nullptr;    // 6*nbf5  : fi, nd, nd12, dfic, ndc, ndx
+  int* orbi = nullptr;      // 3*nbf5  : strong, pair, role
+  // orbital optimiser and rotation
+  long nvar = 0;
+  double* gv = nullptr;     // nvar: packed gradient
+  double* mv = nullptr;     // nvar: first moment
+  double* vv = nullptr;     // nvar: second moment
+  double* vmax = nullptr;   // nvar: AMSGrad max
+  double* yv = nullptr;     // nvar: step
+  double* Y = nullptr;      // nbf*nbf generator
+  double* U = nullptr;      // nbf*nbf exp(Y)
+  double* rwork = nullptr;  // 5*nbf*nbf exp workspace
+  double* Ca = nullptr;     // nbf*nbf spare orbitals
+  double* bestC = nullptr;  // nbf*nbf
+  double* part2 = nullptr;  // 16*nbf*nbf split-K partials
+  size_t part2_elems = 0;
   bool b_ready = false;
   // timing of the dominant transform kernel: one event pair per launch, read back
   // (and recycled) by dn_sys_get_timing so that timing adds no synchronisation.

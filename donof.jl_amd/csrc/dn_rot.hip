@@ -168,14 +168,20 @@ int norm1(hipStream_t s, const double* Y, int nbf, double* out) {
   return DN_OK;
 }
 
-// Square GEMM out = A B (nbf x nbf, column-major)
-static int sq_gemm(hipStream_t s, const double* A, const double* B, double* out, int nbf) {
+// Square GEMM out = A B (nbf x nbf, column-major), split along K
+static int sq_gemm(hipStream_t s, const double* A, const double* B, double* out, int nbf,
+                   double* part, size_t part_elems) {
   Gemm g;
   g.M = nbf; g.N = nbf; g.K = nbf;
   g.A = A; g.lda = nbf; g.a_kcontig = false;
   g.B = B; g.ldb = nbf; g.b_kcontig = true;
   g.C = out; g.ldc = nbf;
-  return gemm_f64(s, g);
+  return gemm_f64_splitk(s, g, part, part_elems);
+}
+
+int matmul_sq(hipStream_t s, const double* A, const double* B, double* out, int nbf,
+              double* part, size_t part_elems) {
+  return sq_gemm(s, A, B, out, nbf, part, part_elems);
 }
 
 // Number of squarings for ||Y||_1 = nrm so that ||Y / 2^s||_1 <= 1/4.
@@ -186,7 +192,8 @@ int squarings_for(double nrm) {
 }
 
 // U = exp(Y) for ||Y||_1 = nrm.  work holds 5 nbf x nbf matrices.
-int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* work, double* U) {
+int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* work, double* U,
+                double* part, size_t part_elems) {
   const long nn = (long)nbf * nbf;
   double* X = work;
   double* X2 = work + nn;
@@ -197,9 +204,9 @@ int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* wor
   const double f = std::ldexp(1.0, -sq);
   hipLaunchKernelGGL(scale_kernel, dim3(grid_for(nn)), dim3(256), 0, s, Y, f, nn, X);
   DN_HIP(hipGetLastError());
-  DN_TRY(sq_gemm(s, X, X, X2, nbf));
-  DN_TRY(sq_gemm(s, X2, X, X3, nbf));
-  DN_TRY(sq_gemm(s, X2, X2, X4, nbf));
+  DN_TRY(sq_gemm(s, X, X, X2, nbf, part, part_elems));
+  DN_TRY(sq_gemm(s, X2, X, X3, nbf, part, part_elems));
+  DN_TRY(sq_gemm(s, X2, X2, X4, nbf, part, part_elems));
   // Taylor coefficients c_k = 1/k!
   double c[13];
   c[0] = 1.0;
@@ -208,13 +215,13 @@ int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* wor
   hipLaunchKernelGGL(poly_block_kernel, dim3(grid_for(nn)), dim3(256), 0, s, X, X2, X3, X4, c[8],
                      c[9], c[10], c[11], c[12], nbf, T);  // T = B2
   DN_HIP(hipGetLastError());
-  DN_TRY(sq_gemm(s, X4, T, U, nbf));                         // U = X4 B2
+  DN_TRY(sq_gemm(s, X4, T, U, nbf, part, part_elems));                         // U = X4 B2
   hipLaunchKernelGGL(poly_block_kernel, dim3(grid_for(nn)), dim3(256), 0, s, X, X2, X3,
                      (const double*)nullptr, c[4], c[5], c[6], c[7], 0.0, nbf, T);  // B1
   DN_HIP(hipGetLastError());
   hipLaunchKernelGGL(add_kernel, dim3(grid_for(nn)), dim3(256), 0, s, T, U, nn);  // B1 + X4 B2
   DN_HIP(hipGetLastError());
-  DN_TRY(sq_gemm(s, X4, T, U, nbf));                         // X4 (B1 + X4 B2)
+  DN_TRY(sq_gemm(s, X4, T, U, nbf, part, part_elems));                         // X4 (B1 + X4 B2)
   hipLaunchKernelGGL(poly_block_kernel, dim3(grid_for(nn)), dim3(256), 0, s, X, X2, X3,
                      (const double*)nullptr, c[0], c[1], c[2], c[3], 0.0, nbf, T);  // B0
   DN_HIP(hipGetLastError());
@@ -224,7 +231,7 @@ int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* wor
   double* cur = U;
   double* oth = T;
   for (int k = 0; k < sq; ++k) {
-    DN_TRY(sq_gemm(s, cur, cur, oth, nbf));
+    DN_TRY(sq_gemm(s, cur, cur, oth, nbf, part, part_elems));
     std::swap(cur, oth);
   }
   if (cur != U) DN_HIP(hipMemcpyAsync(U, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -108,6 +108,66 @@ int dn_sys_sync(dn_sys* sys);
 int dn_sys_set_timing(dn_sys* sys, int enable);
 int dn_sys_get_timing(dn_sys* sys, double* transform_ms_total, long* launches);
 
+/* ---- occupation functional --------------------------------------------------------- */
+
+/* The functional options of Param read by the occupation code (param.jl:26-27,49-51,56). */
+typedef struct dn_functional {
+  int32_t ipnof;      /* p.ipnof: 4, 5, 7, 8 (GNOF) or 9 */
+  int32_t ista;       /* p.ista */
+  int32_t occ_method; /* DN_OCC_SOFTMAX or DN_OCC_TRIGONOMETRIC */
+  int32_t highspin;   /* p.HighSpin */
+  int32_t mspin;      /* p.MSpin */
+  double h_cut;       /* p.h_cut */
+} dn_functional;
+
+/* n (nbf5) and dn/dgamma (nbf5 x nv) from gamma (nv): ocupacion (pnof.jl:1024-1163). */
+int dn_ocupacion(dn_sys* sys, const dn_functional* f, const double* gamma, double* n,
+                 double* dn_dgamma);
+/* cj12, ck12 (nbf5 x nbf5, zero diagonal): PNOFi_selector (pnof.jl:1-53). */
+int dn_pnof_coefficients(dn_sys* sys, const dn_functional* f, const double* n, double* cj12,
+                         double* ck12);
+/* Dcj12r, Dck12r (nbf5 x nbf5 x nv): der_PNOFi_selector (pnof.jl:55-117). */
+int dn_der_pnof_coefficients(dn_sys* sys, const dn_functional* f, const double* n,
+                             const double* dn_dgamma, double* Dcj12r, double* Dck12r);
+/* Occupation energy: calce (pnof.jl:1273-1300).  J, K, Hcore NULL = the resident ones
+ * of the last J/K build (all three given or all NULL). */
+int dn_calce(dn_sys* sys, const dn_functional* f, const double* n, const double* cj12,
+             const double* ck12, const double* J, const double* K, const double* Hcore,
+             double* E);
+/* calcocce (pnof.jl:1302-1322) and calcoccg (pnof.jl:1324-1377). */
+int dn_calcocce(dn_sys* sys, const dn_functional* f, const double* gamma, const double* J,
+                const double* K, const double* Hcore, double* E);
+int dn_calcoccg(dn_sys* sys, const dn_functional* f, const double* gamma, const double* J,
+                const double* K, const double* Hcore, double* grad);
+
+/* ---- orbital rotation and optimiser ------------------------------------------------ */
+
+/* Cnew = C exp(Y(y)), y packed row-wise over i < nbf5, j > i (nvar entries):
+ * rotate_orbital (utils.jl:499-521; ext/DoNOFGPUExt.jl:355-382, there in FP32). */
+int dn_rotate_orbital(dn_sys* sys, const double* y, const double* C, double* Cnew);
+
+/* State of one orbital-optimiser call; maxloop and alpha are updated as the reference
+ * mutates p.maxloop and p.alpha (minimization.jl:426-431). */
+typedef struct dn_orbopt_state {
+  int32_t method;     /* DN_ADAM, DN_ADABELIEF, DN_YOGI or DN_DEMON */
+  int32_t maxloop;    /* in/out: p.maxloop */
+  double alpha;       /* in/out: p.alpha */
+  double threshgorb;  /* p.threshgorb */
+  int32_t nit;        /* out: iterations */
+  int32_t success;    /* out: gradient criterion met */
+  double E;           /* out: returned electronic energy */
+} dn_orbopt_state;
+
+/* orbopt_adam / orbopt_adabelief / orbopt_yogi / orbopt_demon (minimization.jl:242-630):
+ * C (nbf x nbf) is replaced by the returned orbitals. */
+int dn_orbopt(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st, const double* gamma,
+              double* C, const double* H);
+/* Resident form: gamma, C and H already set on the system (dn_sys_set_gamma,
+ * dn_sys_set_orbitals, dn_sys_set_hcore); C stays in HBM. */
+int dn_sys_set_gamma(dn_sys* sys, const double* gamma);
+int dn_sys_get_orbitals(dn_sys* sys, double* C);
+int dn_orbopt_resident(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -786,3 +786,56 @@ def orbopt(method, gamma, C, H, b, p, trace=None):
     if p.maxloop < 100:
         return best_E, best_C, nit, success
     return E, C, nit, success
+
+
+# --------------------------------------------------------------------------------------
+# Outer driver (energy.jl:140-220) with the helpers it calls
+# --------------------------------------------------------------------------------------
+
+
+def order_occupations_softmax(old_C, old_gamma, p):
+    """utils.jl:586-616: each pair's columns and gammas sorted by decreasing gamma
+    (stable, as Julia's sortperm), gammas shifted so the leader has gamma 0."""
+    C = np.array(old_C, copy=True)
+    gamma = np.array(old_gamma, copy=True)
+    if np.any(gamma > 0):
+        for i in range(1, p.ndoc + 1):
+            cols = np.r_[p.no1 + i - 1, pair_weak(p, i)]
+            gi = pair_gamma(p, i)
+            gt = np.r_[0.0, gamma[gi]]
+            idx = np.argsort(gt, kind="stable")[::-1]
+            gt = gt[idx] - gt.max()
+            gamma[gi] = gt[1:]
+            C[:, cols] = old_C[:, cols][:, idx]
+    return C, gamma
+
+
+def energy_loop(b, H, C, p, n=None, freeze_occ=False):
+    """energy.jl:140-220 for given integrals (no1 = 0).  Returns (E, C, n, history)."""
+    if n is None:
+        gamma = (guess_gamma_softmax(p.ndoc, p.ncwo) if p.occ_method == "Softmax"
+                 else guess_gamma_trigonometric(p.ndoc, p.ncwo))
+    else:
+        gamma = n_to_gammas_softmax(n, p)
+    E = 9999.0
+    hist = []
+    for i_ext in range(1, p.maxit + 1):
+        E_orb, C, nit_orb, _ = orbopt(p.orb_method, gamma, C, H, b, p)
+        if p.ndoc > 0 and not freeze_occ:
+            J, K, Hc = jkh_mo(C, H, b, p.nbf5)
+            _, nit_occ, gamma = occoptr(gamma, J, K, Hc, p)
+            if nit_occ < 1:
+                p.maxloop += 10
+        n, _ = ocupacion(gamma, p)
+        if p.occ_method == "Softmax":
+            C, gamma = order_occupations_softmax(C, gamma, p)
+        E = E_orb
+        n, _ = ocupacion(gamma, p)
+        cj12, ck12 = pnof_selector(n, p)
+        g_orb = calcorbg(np.zeros(p.nvar), n, cj12, ck12, C, H, b, p)
+        J, K, Hc = jkh_mo(C, H, b, p.nbf5)
+        g_occ = calcoccg(gamma, J, K, Hc, p)
+        hist.append((E, float(np.max(np.abs(g_orb))), float(np.linalg.norm(g_occ))))
+        if np.max(np.abs(g_orb)) < p.threshgorb and np.linalg.norm(g_occ) < p.threshgocc:
+            break
+    return E, C, n, hist

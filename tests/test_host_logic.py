@@ -1,0 +1,83 @@
+"""CPU checks of the host-side logic of the package (no GPU): the conjugate-gradient
+occupation optimiser, the Softmax re-ordering and the gamma conversions, against the
+oracle's functions."""
+import numpy as np
+import pytest
+
+from donof_amd import minimization as mini, synthetic
+
+
+def system(occ="Softmax"):
+    s = synthetic.SyntheticSystem(20, 40, 15, 21, ndoc=3)
+    s.p.occ_method = occ
+    return s
+
+
+def test_gamma_guesses_and_conversions(oracle):
+    p = system().p
+    assert np.array_equal(mini.guess_gamma_softmax(p.ndoc, p.ncwo),
+                          oracle.guess_gamma_softmax(p.ndoc, p.ncwo))
+    assert np.array_equal(mini.guess_gamma_trigonometric(p.ndoc, p.ncwo),
+                          oracle.guess_gamma_trigonometric(p.ndoc, p.ncwo))
+    rng = np.random.default_rng(0)
+    g = rng.standard_normal(p.nv) - 2
+    n, _ = oracle.ocupacion_softmax(g, p)
+    assert np.abs(mini.n_to_gammas_softmax(n, p) - oracle.n_to_gammas_softmax(n, p)).max() == 0
+    pt = system("Trigonometric").p
+    gt = rng.uniform(0.2, 1.2, pt.nv)
+    nt, _ = oracle.ocupacion_trigonometric(gt, pt)
+    # round trip through the trigonometric parametrisation
+    g2 = mini.n_to_gammas_trigonometric(nt, pt)
+    assert np.abs(oracle.ocupacion_trigonometric(g2, pt)[0] - nt).max() < 1e-12
+
+
+def test_order_occupations_softmax(oracle):
+    s = system()
+    p = s.p
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal(p.nv)          # some gammas > 0: pairs get re-ordered
+    C2, g2 = mini.order_occupations_softmax(s.C, g, p)
+    Cr, gr = oracle.order_occupations_softmax(s.C, g, p)
+    assert np.array_equal(C2, Cr) and np.array_equal(g2, gr)
+    # occupations are permuted within each pair, the strongest becomes the leader
+    n1, _ = oracle.ocupacion_softmax(g, p)
+    n2, _ = oracle.ocupacion_softmax(g2, p)
+    for l in range(1, p.ndoc + 1):
+        cols = np.r_[p.no1 + l - 1, oracle.pair_weak(p, l)]
+        assert np.allclose(np.sort(n1[cols]), np.sort(n2[cols]))
+        assert n2[p.no1 + l - 1] == n2[cols].max()
+    # no gamma > 0: unchanged
+    C3, g3 = mini.order_occupations_softmax(s.C, -np.abs(g), p)
+    assert np.array_equal(C3, s.C) and np.array_equal(g3, -np.abs(g))
+
+
+@pytest.mark.parametrize("ipnof", [7, 8, 9])
+@pytest.mark.parametrize("occ", ["Softmax", "Trigonometric"])
+def test_conjugate_gradient_reaches_stationary_point(oracle, ipnof, occ):
+    """(PNOF5 on these random integrals has several stationary points and unbounded
+    directions, so only the physical-like functionals are compared.)"""
+    s = system(occ)
+    p = s.p
+    p.ipnof = ipnof
+    J, K, Hc = oracle.jkh_mo(s.C, s.H, s.host_b(), p.nbf5)
+    g0 = (mini.guess_gamma_softmax(p.ndoc, p.ncwo) if occ == "Softmax"
+          else mini.guess_gamma_trigonometric(p.ndoc, p.ncwo))
+    res = mini.conjugate_gradient(lambda g: oracle.calcocce(g, J, K, Hc, p),
+                                  lambda g: oracle.calcoccg(g, J, K, Hc, p), g0, g_abstol=1e-7)
+    assert res.g_converged
+    assert np.max(np.abs(oracle.calcoccg(res.x, J, K, Hc, p))) <= 1e-7
+    assert res.minimum <= oracle.calcocce(g0, J, K, Hc, p)
+    # scipy's CG (the oracle's occoptr, gtol = threshen = 1e-4) reaches the same minimum
+    Eo, _, _ = oracle.occoptr(g0, J, K, Hc, p)
+    assert abs(res.minimum - Eo) < 1e-7
+
+
+def test_conjugate_gradient_quadratic():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((12, 12))
+    A = A @ A.T + 12 * np.eye(12)
+    bvec = rng.standard_normal(12)
+    res = mini.conjugate_gradient(lambda x: 0.5 * x @ A @ x - bvec @ x, lambda x: A @ x - bvec,
+                                  np.zeros(12), g_abstol=1e-12)
+    assert res.g_converged
+    assert np.abs(res.x - np.linalg.solve(A, bvec)).max() < 1e-10
