@@ -1,8 +1,8 @@
 """Benchmark: RI J/K-MO builds per second on one MI355X.
 
-Workload (the FP64 J/K-MO kernel on synthetic tensors): nbf=400, nbfaux=1600, nbf5=120,
-b generated in HBM by the Philox generator, C = Q of a seeded QR, H symmetric N(0,1),
-GNOF pairing ndoc=5, ncwo=23.  One step = one J/K-MO build (the RI transform
+Headline workload (the FP64 J/K-MO kernel on synthetic tensors): nbf=400, nbfaux=1600,
+nbf5=120, b generated in HBM by the Philox generator, C = Q of a seeded QR, H symmetric
+N(0,1), GNOF pairing ndoc=5, ncwo=23.  One step = one J/K-MO build (the RI transform
 b_pqQ = C5^T b_mnQ C5, J, K and H_core) with b, C and H already resident in HBM.
 
 The dominant kernel is the first transform b_pn = C5^T b (2 nbf5 nbf^2 nbfaux FLOP per
@@ -10,7 +10,11 @@ launch), timed with HIP events on the library's stream around every launch; its
 roofline is the FP64 matrix peak.  The CPU baseline times the NumPy oracle (BLAS-backed)
 on the same tensor on the host cores.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline]
+Extra legs (--no-extra to skip): the same build at nbf=800, nbfaux=3200, nbf5=240 in FP64
+and in split precision (FP16 hi+lo on MFMA, FP32 accumulation), their ratio and the split
+path's J/K/energy error against the FP64 device path on the same tensor and orbitals.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline] [--no-extra]
 """
 from __future__ import annotations
 
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RI J/K-MO builds/s + %peak at ΔE≤1e-6 Ha; p50/p99 solve latency"
 FP64_PEAK_TFLOPS = 78.6
+F16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -34,6 +39,10 @@ def build_flops(nbf, naux, nbf5):
     """Algorithmic FLOP of one J/K-MO build in the reference's contraction order."""
     return (2 * nbf5 * nbf * nbf * naux + 2 * nbf5 * nbf5 * nbf * naux + 4 * nbf5 * nbf5 * naux
             + 2 * nbf * nbf * nbf5 + 2 * nbf * nbf5)
+
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(C, H, b, nbf5, budget_s=20.0):
@@ -58,8 +67,72 @@ def cpu_baseline(C, H, b, nbf5, budget_s=20.0):
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "builds/s", "cores": threads or os.cpu_count(),
             "kind": "port",
-            "sample": "%d full J/K-MO builds of the same nbf=400/nbfaux=1600/nbf5=120 tensor "
-                      "with the NumPy FP64 oracle (OpenBLAS threads=%s), %.1f s" % (n, threads, dt)}
+            "sample": "%d full J/K-MO builds of the same nbf=%d/nbfaux=%d/nbf5=%d tensor with the "
+                      "NumPy FP64 oracle (OpenBLAS threads=%s), %.1f s"
+                      % (n, b.shape[0], b.shape[2], nbf5, threads, dt)}
+
+
+def time_builds(dev, steps, warmup):
+    """(ms per build, dominant-kernel ms per launch, launches) with HIP-event timing."""
+    for _ in range(warmup):
+        dev.jkh_mo_resident()
+    dev.sync()
+    dev.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.jkh_mo_resident()
+    dev.sync()
+    t1 = time.perf_counter()
+    k_ms, k_n = dev.get_timing()
+    dev.set_timing(False)
+    return (t1 - t0) * 1e3 / steps, k_ms / max(k_n, 1), k_n
+
+
+def make_system(api, synthetic, nbf, naux, nbf5, seed, precision):
+    sysd = synthetic.SyntheticSystem(nbf, naux, nbf5, seed)
+    dev = api.System(sysd.p, precision=precision)
+    dev.generate_b(seed)
+    dev.set_orbitals(sysd.C)
+    dev.set_hcore(sysd.H)
+    return sysd, dev
+
+
+def extra_legs(api, synthetic, orc, steps, warmup, seed):
+    """FP64 and split-precision builds at nbf=800, nbfaux=3200, nbf5=240."""
+    nbf, naux, nbf5 = 800, 3200, 240
+    out = {"config": {"nbf": nbf, "nbfaux": naux, "nbf5": nbf5}}
+    F = build_flops(nbf, naux, nbf5)
+    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "fp64")
+    ms, kms, _ = time_builds(dev, steps, warmup)
+    J64, K64, H64 = dev.get_jkh()
+    dev.close()
+    out["fp64"] = {"builds_per_s": 1e3 / ms, "ms_per_build": ms, "tflops": F / (ms * 1e-3) / 1e12,
+                   "transform_kernel_ms": kms,
+                   "transform_tflops": 2.0 * nbf5 * nbf * nbf * naux / (kms * 1e-3) / 1e12}
+    log("fp64 800/3200/240: %.3f ms/build" % ms)
+    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "split")
+    ms16, kms16, _ = time_builds(dev, steps, warmup)
+    J16, K16, H16 = dev.get_jkh()
+    dev.close()
+    k1_useful = 2.0 * nbf5 * nbf * nbf * naux
+    p = sysd.p
+    cj, ck = orc.pnof_selector(sysd.n, p)
+    E64 = orc.calce(sysd.n, cj, ck, J64, K64, H64, p)
+    E16 = orc.calce(sysd.n, cj, ck, J16, K16, H16, p)
+    out["split_f16"] = {
+        "builds_per_s": 1e3 / ms16, "ms_per_build": ms16, "tflops": F / (ms16 * 1e-3) / 1e12,
+        "transform_kernel_ms": kms16,
+        "transform_useful_tflops": k1_useful / (kms16 * 1e-3) / 1e12,
+        "transform_mfma_tflops": 3 * k1_useful / (kms16 * 1e-3) / 1e12,
+        "transform_mfma_frac_of_f16_peak": 3 * k1_useful / (kms16 * 1e-3) / 1e12 / F16_PEAK_TFLOPS,
+        "J_max_rel_err": float(np.abs(J16 - J64).max() / np.abs(J64).max()),
+        "K_max_rel_err": float(np.abs(K16 - K64).max() / np.abs(K64).max()),
+        "gnof_energy_abs_err_Ha": float(abs(E16 - E64)),
+    }
+    out["split_speedup_vs_fp64"] = ms / ms16
+    log("split 800/3200/240: %.3f ms/build (x%.2f), |dE|=%.2e"
+        % (ms16, ms / ms16, abs(E16 - E64)))
+    return out
 
 
 def main():
@@ -72,6 +145,7 @@ def main():
     ap.add_argument("--nbf5", type=int, default=120)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -84,28 +158,10 @@ def main():
     from donof_amd import api, synthetic
 
     nbf, naux, nbf5 = args.nbf, args.nbfaux, args.nbf5
-    sysd = synthetic.SyntheticSystem(nbf, naux, nbf5, args.seed)
-    dev = api.System(sysd.p)
-    dev.generate_b(args.seed)
-    dev.set_orbitals(sysd.C)
-    dev.set_hcore(sysd.H)
-    print("bench: system resident, warmup %d" % args.warmup, file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        dev.jkh_mo_resident()
-    dev.sync()
-    dev.set_timing(True)
-    dev.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dev.jkh_mo_resident()
-    dev.sync()
-    t1 = time.perf_counter()
-    k_ms, k_n = dev.get_timing()
-    dev.set_timing(False)
-    J, K, Hc = dev.get_jkh()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, args.seed, "fp64")
+    log("system resident, warmup %d" % args.warmup)
+    ms_step, k_avg_ms, k_n = time_builds(dev, args.steps, args.warmup)
     F = build_flops(nbf, naux, nbf5)
-    k_avg_ms = k_ms / max(k_n, 1)
     k_flops = 2.0 * nbf5 * nbf * nbf * naux
     k_tflops = k_flops / (k_avg_ms * 1e-3) / 1e12
     out = {
@@ -130,16 +186,21 @@ def main():
                      "frac": k_tflops / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel_ms": k_avg_ms, "launches": k_n},
     }
-    print("bench: %.3f ms/build, transform kernel %.3f ms (%.1f TFLOP/s)"
-          % (ms_step, k_avg_ms, k_tflops), file=sys.stderr, flush=True)
+    log("%.3f ms/build, transform kernel %.3f ms (%.1f TFLOP/s)" % (ms_step, k_avg_ms, k_tflops))
+    b = None
     if not args.no_cpu_baseline:
-        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         b = dev.download_b()
+    dev.close()
+    if b is not None:
         out["cpu_baseline"] = cpu_baseline(sysd.C, sysd.H, b, nbf5, args.cpu_seconds)
         del b
     else:
         out["cpu_baseline"] = None
-    dev.close()
+    if not args.no_extra:
+        orc = __graft_entry__.load_oracle()
+        out["extra_800_3200_240"] = extra_legs(api, synthetic, orc, max(5, args.steps // 2),
+                                               args.warmup, args.seed)
     print(json.dumps(out))
     return 0
 

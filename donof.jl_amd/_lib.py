@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(PKG, "libdonofhip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "donof_hip.h")
 
 DN_OK, DN_ERR_ARG, DN_ERR_HIP, DN_ERR_NOMEM, DN_ERR_BUDGET, DN_ERR_NONFINITE, DN_ERR_STATE = range(7)
-DN_FP64 = 0
+DN_FP64, DN_SPLIT_F16 = 0, 1
 ERROR_NAMES = {1: "DN_ERR_ARG", 2: "DN_ERR_HIP", 3: "DN_ERR_NOMEM", 4: "DN_ERR_BUDGET",
                5: "DN_ERR_NONFINITE", 6: "DN_ERR_STATE"}
 

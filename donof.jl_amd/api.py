@@ -17,7 +17,7 @@ import ctypes as ct
 import numpy as np
 
 from . import _lib
-from ._lib import DN_FP64, DnError, check, dptr, dn_dims
+from ._lib import DN_FP64, DN_SPLIT_F16, DnError, check, dptr, dn_dims
 
 
 class Context:
@@ -80,6 +80,11 @@ class System:
     outputs and workspaces, on its own HIP stream."""
 
     def __init__(self, p, ctx=None, precision=DN_FP64):
+        if precision in ("fp64", "FP64"):
+            precision = DN_FP64
+        elif precision in ("split", "split_f16"):
+            precision = DN_SPLIT_F16
+        self.precision = precision
         self.ctx = ctx if ctx is not None else default_context()
         self.lib = self.ctx.lib
         self.p = p

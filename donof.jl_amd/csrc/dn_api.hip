@@ -22,6 +22,19 @@ int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nb
              double* elag);
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
                 int nbf, int nbf5, double* out);
+// dn_split.hip
+int split_prepare_b(hipStream_t s, const double* b, int nbf, int naux, int ld8, float* scale,
+                    float* inv_scale, _Float16* hi, _Float16* lo);
+int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
+                    float* cscale, _Float16* hi, _Float16* lo);
+int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+             const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+             const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
+             float* qmax);
+int split_k2_jk(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+                const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
+                const float* qmax, float* qs, float* qinv, double* part, size_t part_elems,
+                double* J, double* K);
 
 namespace {
 thread_local std::string g_last_error;
@@ -46,11 +59,21 @@ bool dims_valid(const dn_dims& d, std::string* why) {
 struct Layout {
   size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bmat, xw, part, scal;
   size_t nv, dnm, nvar, part2;
+  // split precision (0 when FP64): half counts of the b and C planes, float counts
+  size_t bplane, cplane, f_big, f_bpq, f_naux;
+  static size_t d8(size_t halves) { return (halves + 3) / 4; }   // halves -> doubles
+  static size_t f8(size_t floats) { return (floats + 1) / 2; }   // floats -> doubles
+  size_t split_doubles() const {
+    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 5 * f8(f_naux) + 1;
+  }
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + 2 * big + bmat + 2 * xw +
-                part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2);
+                part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2 +
+                split_doubles());
   }
 };
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 size_t lag_splits(const dn_dims& d) {
   // batch-reduce splits over Q for the Lagrangian GEMM: aim at ~1024 workgroups
@@ -62,7 +85,7 @@ size_t lag_splits(const dn_dims& d) {
   return (size_t)((d.nbfaux + bps - 1) / bps);
 }
 
-Layout layout_of(const dn_dims& d) {
+Layout layout_of(const dn_dims& d, int precision) {
   Layout L{};
   const size_t nbf = d.nbf, nbf5 = d.nbf5, naux = d.nbfaux;
   L.b = nbf * nbf * naux;
@@ -83,6 +106,13 @@ Layout layout_of(const dn_dims& d) {
   L.dnm = std::max<size_t>(nbf5 * d.nv, 1);
   L.nvar = nbf * nbf5 - nbf5 * (nbf5 + 1) / 2;
   L.part2 = 16 * nbf * nbf;
+  if (precision == DN_SPLIT_F16) {
+    L.bplane = (size_t)round_up(d.nbf, 8) * nbf * naux + 64;
+    L.cplane = (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32);
+    L.f_big = nbf5 * nbf * naux;
+    L.f_bpq = nbf5 * nbf5 * naux;
+    L.f_naux = naux;
+  }
   return L;
 }
 
@@ -112,22 +142,9 @@ int transform1(dn_sys* s) {
   g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
   g.B = s->b; g.ldb = d.nbf; g.b_kcontig = true;
   g.C = s->bpn; g.ldc = d.nbf5;
-  if (s->timing) {
-    if (s->ev_used == s->ev_start.size()) {
-      hipEvent_t a = nullptr, b = nullptr;
-      DN_HIP(hipEventCreate(&a));
-      DN_HIP(hipEventCreate(&b));
-      s->ev_start.push_back(a);
-      s->ev_stop.push_back(b);
-    }
-    DN_HIP(hipEventRecord(s->ev_start[s->ev_used], s->stream));
-  }
+  DN_TRY(begin_timing(s));
   DN_TRY(gemm_f64(s->stream, g));
-  if (s->timing) {
-    DN_HIP(hipEventRecord(s->ev_stop[s->ev_used], s->stream));
-    s->ev_used += 1;
-  }
-  return DN_OK;
+  return end_timing(s);
 }
 
 // out[:, :, Q] = b_pn[:, :, Q] * C[:, 0:ncols]  (integrals.jl:160 for ncols = nbf5; the
@@ -173,9 +190,52 @@ int check_sys(dn_sys* s) {
   return DN_OK;
 }
 
+int prepare_b(dn_sys* s) {
+  if (s->precision != DN_SPLIT_F16) return DN_OK;
+  return split_prepare_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, s->ld8, s->bscale, s->binv,
+                         s->bhi, s->blo);
+}
+
+int begin_timing(dn_sys* s) {
+  if (!s->timing) return DN_OK;
+  if (s->ev_used == s->ev_start.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    DN_HIP(hipEventCreate(&a));
+    DN_HIP(hipEventCreate(&b));
+    s->ev_start.push_back(a);
+    s->ev_stop.push_back(b);
+  }
+  DN_HIP(hipEventRecord(s->ev_start[s->ev_used], s->stream));
+  return DN_OK;
+}
+
+int end_timing(dn_sys* s) {
+  if (!s->timing) return DN_OK;
+  DN_HIP(hipEventRecord(s->ev_stop[s->ev_used], s->stream));
+  s->ev_used += 1;
+  return DN_OK;
+}
+
+int jkh_split(dn_sys* s) {
+  const dn_dims& d = s->d;
+  DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
+                         s->clo));
+  DN_TRY(begin_timing(s));
+  DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo, s->binv,
+                  s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
+  DN_TRY(end_timing(s));
+  DN_TRY(split_k2_jk(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale,
+                     s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv, s->part, s->part_elems, s->J,
+                     s->K));
+  DN_TRY(hc_products(s, false));
+  DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
+  return DN_OK;
+}
+
 int jkh_resident(dn_sys* s) {
   const dn_dims& d = s->d;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
+  if (s->precision == DN_SPLIT_F16) return jkh_split(s);
   DN_TRY(transform1(s));
   DN_TRY(transform2(s, d.nbf5, s->bt));
   DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->part,
@@ -295,8 +355,9 @@ int dn_sys_footprint(const dn_dims* dims, int precision, uint64_t* bytes) {
   if (!dims || !bytes) return fail(DN_ERR_ARG, "null argument");
   std::string why;
   if (!dims_valid(*dims, &why)) return fail(DN_ERR_ARG, why);
-  if (precision != DN_FP64) return fail(DN_ERR_ARG, "unknown precision");
-  *bytes = layout_of(*dims).total_bytes();
+  if (precision != DN_FP64 && precision != DN_SPLIT_F16)
+    return fail(DN_ERR_ARG, "unknown precision");
+  *bytes = layout_of(*dims, precision).total_bytes();
   return DN_OK;
 }
 
@@ -304,8 +365,9 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   if (!ctx || !dims || !out) return fail(DN_ERR_ARG, "null argument");
   std::string why;
   if (!dims_valid(*dims, &why)) return fail(DN_ERR_ARG, why);
-  if (precision != DN_FP64) return fail(DN_ERR_ARG, "unknown precision");
-  const Layout L = layout_of(*dims);
+  if (precision != DN_FP64 && precision != DN_SPLIT_F16)
+    return fail(DN_ERR_ARG, "unknown precision");
+  const Layout L = layout_of(*dims, precision);
   const uint64_t need = L.total_bytes();
   {
     std::lock_guard<std::mutex> lock(g_ctx_mutex);
@@ -364,6 +426,30 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf2, &s->bestC);
   A(L.part2, &s->part2);
   s->part2_elems = L.part2;
+  if (precision == DN_SPLIT_F16) {
+    s->ld8 = round_up(dims->nbf, 8);
+    s->Kpad = round_up(dims->nbf, 32);
+    s->Mpad = round_up(dims->nbf5, 128);
+    double* q = nullptr;
+    A(Layout::d8(L.bplane), &q); s->bhi = reinterpret_cast<_Float16*>(q);
+    A(Layout::d8(L.bplane), &q); s->blo = reinterpret_cast<_Float16*>(q);
+    A(Layout::d8(L.cplane), &q); s->chi = reinterpret_cast<_Float16*>(q);
+    A(Layout::d8(L.cplane), &q); s->clo = reinterpret_cast<_Float16*>(q);
+    A(Layout::f8(L.f_big), &q); s->bpn32 = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_bpq), &q); s->bpq32 = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->bscale = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->binv = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->qmax = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->qs = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->qinv = reinterpret_cast<float*>(q);
+    A(1, &q); s->cscale = reinterpret_cast<float*>(q);
+    if (rc == DN_OK) {
+      // plane padding must read as zero
+      hipError_t e1 = hipMemset(s->bhi, 0, Layout::d8(L.bplane) * 8);
+      hipError_t e2 = hipMemset(s->blo, 0, Layout::d8(L.bplane) * 8);
+      if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(DN_ERR_HIP, "hipMemset of b planes");
+    }
+  }
   s->part_elems = L.part;
   hipError_t e = hipSuccess;
   if (rc == DN_OK) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -405,6 +491,7 @@ int dn_sys_upload_b(dn_sys* s, const double* b) {
   DN_TRY(check_sys(s));
   const size_t n = (size_t)s->d.nbf * s->d.nbf * s->d.nbfaux;
   DN_TRY(h2d(s, s->b, b, n));
+  DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;
   return DN_OK;
@@ -422,6 +509,7 @@ int dn_sys_download_b(dn_sys* s, double* b) {
 int dn_sys_generate_b(dn_sys* s, uint64_t seed) {
   DN_TRY(check_sys(s));
   DN_TRY(synth_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, seed));
+  DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;
   return DN_OK;

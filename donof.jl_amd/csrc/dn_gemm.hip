@@ -41,35 +41,42 @@ struct TileLds {
   }
 };
 
-// Two consecutive elements along the contiguous dimension, zero outside [0,MN)x[0,KK).
-// Branch-free: the address is clamped into the operand and the value selected.  With
-// `vec` (uniform) the pair is one 16-byte load; it requires an even extent along the
-// contiguous dimension, an even leading dimension and a 16-byte aligned base.
-template <bool KC>
-__device__ __forceinline__ v2d load_pair(const double* __restrict__ base, long ld, bool vec,
-                                         int gmn, int gk, int MN, int KK) {
+// Two consecutive elements along the contiguous dimension from an address clamped into
+// the operand (always in bounds); the caller zeroes the out-of-range ones with
+// pair_mask() when it writes the tile to LDS, i.e. after the MFMAs of the current tile,
+// so the loads stay in flight meanwhile.  With VEC the pair is one 16-byte load; it
+// requires an even extent along the contiguous dimension, an even leading dimension and
+// a 16-byte aligned base.  (A run-time VEC switch would make the compiler wait for every
+// load at the control-flow merge.)
+template <bool KC, bool VEC>
+__device__ __forceinline__ v2d load_pair(const double* __restrict__ base, long ld, int gmn,
+                                         int gk, int MN, int KK) {
   v2d v;
-  if (vec) {
-    const bool ok = (gmn < MN) && (gk < KK);
+  if constexpr (VEC) {
     const int cmn = min(gmn, MN - (KC ? 1 : 2));
     const int ck = min(gk, KK - (KC ? 2 : 1));
     const double* p = KC ? base + (long)cmn * ld + ck : base + cmn + (long)ck * ld;
     v = *reinterpret_cast<const v2d*>(p);
-    if (!ok) v = v2d{0.0, 0.0};
   } else {
     const int cmn0 = min(gmn, MN - 1), ck0 = min(gk, KK - 1);
     const int gmn1 = KC ? gmn : gmn + 1, gk1 = KC ? gk + 1 : gk;
     const int cmn1 = min(gmn1, MN - 1), ck1 = min(gk1, KK - 1);
-    const double x0 = KC ? base[(long)cmn0 * ld + ck0] : base[cmn0 + (long)ck0 * ld];
-    const double x1 = KC ? base[(long)cmn1 * ld + ck1] : base[cmn1 + (long)ck1 * ld];
-    v.x = (gmn < MN && gk < KK) ? x0 : 0.0;
-    v.y = (gmn1 < MN && gk1 < KK) ? x1 : 0.0;
+    v.x = KC ? base[(long)cmn0 * ld + ck0] : base[cmn0 + (long)ck0 * ld];
+    v.y = KC ? base[(long)cmn1 * ld + ck1] : base[cmn1 + (long)ck1 * ld];
   }
   return v;
 }
 
+template <bool KC>
+__device__ __forceinline__ v2d pair_mask(v2d v, int gmn, int gk, int MN, int KK) {
+  const int gmn1 = KC ? gmn : gmn + 1, gk1 = KC ? gk + 1 : gk;
+  v.x = (gmn < MN && gk < KK) ? v.x : 0.0;
+  v.y = (gmn1 < MN && gk1 < KK) ? v.y : 0.0;
+  return v;
+}
+
 // Tile BM x BN per workgroup of WGM x WGN waves; each wave owns (BM/WGM) x (BN/WGN).
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
 __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
                                                                    int tiles_n, int nsplit) {
   constexpr int NT = 64 * WGM * WGN;
@@ -118,20 +125,22 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
     for (int i = 0; i < LA::PAIRS; ++i) {
       int mn, k;
       LA::coord(tid + i * NT, &mn, &k);
-      ra[i] = load_pair<AK>(A, g.lda, g.vec_a, m0 + mn, k0 + k, M, K);
+      ra[i] = load_pair<AK, VA>(A, g.lda, m0 + mn, k0 + k, M, K);
     }
 #pragma unroll
     for (int i = 0; i < LB::PAIRS; ++i) {
       int mn, k;
       LB::coord(tid + i * NT, &mn, &k);
-      rb[i] = load_pair<BKC>(B, g.ldb, g.vec_b, n0 + mn, k0 + k, N, K);
+      rb[i] = load_pair<BKC, VB>(B, g.ldb, n0 + mn, k0 + k, N, K);
     }
   };
-  auto store_tile = [&](int buf) {
+  auto store_tile = [&](int buf, int t) {
+    const int k0 = (t % nkt) * BK;
 #pragma unroll
     for (int i = 0; i < LA::PAIRS; ++i) {
       int mn, k;
       LA::coord(tid + i * NT, &mn, &k);
+      ra[i] = pair_mask<AK>(ra[i], m0 + mn, k0 + k, M, K);
       if (AK) {
         As[buf][LA::off(mn, k)] = ra[i].x;
         As[buf][LA::off(mn, k + 1)] = ra[i].y;
@@ -143,6 +152,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
     for (int i = 0; i < LB::PAIRS; ++i) {
       int mn, k;
       LB::coord(tid + i * NT, &mn, &k);
+      rb[i] = pair_mask<BKC>(rb[i], n0 + mn, k0 + k, N, K);
       if (BKC) {
         Bs[buf][LB::off(mn, k)] = rb[i].x;
         Bs[buf][LB::off(mn, k + 1)] = rb[i].y;
@@ -154,7 +164,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
 
   if (T > 0) {
     load_tile(0);
-    store_tile(0);
+    store_tile(0, 0);
   }
   __syncthreads();
   const int fr = lane & 15, fk = lane >> 4;
@@ -174,7 +184,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
         for (int j = 0; j < FN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (t + 1 < T) store_tile(buf ^ 1);
+    if (t + 1 < T) store_tile(buf ^ 1, t + 1);
     __syncthreads();
   }
 
@@ -215,28 +225,37 @@ __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int 
   }
 }
 
-template <int BM, int BN, int WGM, int WGN, bool AK, bool BKC>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
 int launch(hipStream_t s, const Gemm& g) {
-  constexpr int BK = 16;
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int nsplit = (g.nbatch + g.bps - 1) / g.bps;
   const long total = (long)tm * tn * nsplit;
   if (total <= 0) return DN_OK;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "gemm grid too large");
-  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC>), dim3((unsigned)total),
-                     dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
+  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>),
+                     dim3((unsigned)total), dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
-template <int BM, int BN, int WGM, int WGN>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>
+int dispatch_vec(hipStream_t s, const Gemm& g) {
+  if (g.vec_a) {
+    return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, true, true>(s, g)
+                   : launch<BM, BN, BK, WGM, WGN, AK, BKC, true, false>(s, g);
+  }
+  return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, false, true>(s, g)
+                 : launch<BM, BN, BK, WGM, WGN, AK, BKC, false, false>(s, g);
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN>
 int dispatch_layout(hipStream_t s, const Gemm& g) {
   if (g.a_kcontig) {
-    return g.b_kcontig ? launch<BM, BN, WGM, WGN, true, true>(s, g)
-                       : launch<BM, BN, WGM, WGN, true, false>(s, g);
+    return g.b_kcontig ? dispatch_vec<BM, BN, BK, WGM, WGN, true, true>(s, g)
+                       : dispatch_vec<BM, BN, BK, WGM, WGN, true, false>(s, g);
   }
-  return g.b_kcontig ? launch<BM, BN, WGM, WGN, false, true>(s, g)
-                     : launch<BM, BN, WGM, WGN, false, false>(s, g);
+  return g.b_kcontig ? dispatch_vec<BM, BN, BK, WGM, WGN, false, true>(s, g)
+                     : dispatch_vec<BM, BN, BK, WGM, WGN, false, false>(s, g);
 }
 
 }  // namespace
@@ -255,8 +274,8 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
   // Tall-M operands get the 128-row tile (the RI transforms have M = nbf5 <= 128 or
   // a multiple of it); everything else uses 64 x 64.
-  if (g.M > 64 && g.N >= 64) return dispatch_layout<128, 64, 2, 2>(s, g);
-  return dispatch_layout<64, 64, 2, 2>(s, g);
+  if (g.M > 64 && g.N >= 128) return dispatch_layout<128, 128, 16, 2, 4>(s, g);
+  return dispatch_layout<64, 64, 16, 2, 2>(s, g);
 }
 
 int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {

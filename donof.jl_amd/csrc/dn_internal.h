@@ -75,6 +75,9 @@ int d2h(dn_sys* s, double* dst, const double* src, size_t elems);
 int jkh_resident(dn_sys* s);
 int lagrange_resident(dn_sys* s, int nalpha, int nbeta);
 bool all_finite(const double* x, size_t n);
+int prepare_b(dn_sys* s);
+int begin_timing(dn_sys* s);  // HIP events around the dominant kernel
+int end_timing(dn_sys* s);
 }  // namespace dn
 
 struct dn_ctx {
@@ -132,6 +135,20 @@ struct dn_sys {
   double* bestC = nullptr;  // nbf*nbf
   double* part2 = nullptr;  // 16*nbf*nbf split-K partials
   size_t part2_elems = 0;
+  // split-precision J/K build (precision == DN_SPLIT_F16)
+  int ld8 = 0, Kpad = 0, Mpad = 0;
+  _Float16* bhi = nullptr;  // [(n,Q)][m] rows of ld8 halves: hi of b*scale[Q]
+  _Float16* blo = nullptr;
+  _Float16* chi = nullptr;  // [Mpad][Kpad]: hi of C5^T*cscale
+  _Float16* clo = nullptr;
+  float* bscale = nullptr;  // naux scales and inverses of b
+  float* binv = nullptr;
+  float* cscale = nullptr;  // (scale, 1/scale) of C5
+  float* bpn32 = nullptr;   // nbf5*nbf*naux
+  float* bpq32 = nullptr;   // nbf5*nbf5*naux
+  float* qmax = nullptr;    // naux
+  float* qs = nullptr;      // naux
+  float* qinv = nullptr;    // naux
   bool b_ready = false;
   // timing of the dominant transform kernel: one event pair per launch, read back
   // (and recycled) by dn_sys_get_timing so that timing adds no synchronisation.
