@@ -53,6 +53,7 @@ _SIGS = {
     "dn_ctx_create": (_I, [_I, ct.c_uint64, ct.POINTER(_P)]),
     "dn_ctx_destroy": (_I, [_P]),
     "dn_ctx_hbm_used": (_I, [_P, ct.POINTER(ct.c_uint64)]),
+    "dn_ctx_hbm_budget": (_I, [_P, ct.POINTER(ct.c_uint64)]),
     "dn_sys_create": (_I, [_P, ct.POINTER(dn_dims), _I, ct.POINTER(_P)]),
     "dn_sys_destroy": (_I, [_P]),
     "dn_sys_footprint": (_I, [ct.POINTER(dn_dims), _I, ct.POINTER(ct.c_uint64)]),

@@ -31,6 +31,12 @@ class Context:
         self.device = device
 
     @property
+    def hbm_budget(self):
+        u = ct.c_uint64()
+        check(self.lib.dn_ctx_hbm_budget(self.handle, ct.byref(u)))
+        return u.value
+
+    @property
     def hbm_used(self):
         u = ct.c_uint64()
         check(self.lib.dn_ctx_hbm_used(self.handle, ct.byref(u)))

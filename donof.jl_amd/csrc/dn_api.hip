@@ -9,8 +9,9 @@
 namespace dn {
 
 // kernels of dn_jk.hip / dn_lag.hip
-int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux, double* part,
-                size_t part_elems, double* J, double* K);
+int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux, double* X,
+                double* part, size_t part_elems, double* part2, size_t part2_elems, double* J,
+                double* K);
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
                double* Hcore);
 int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
@@ -31,10 +32,12 @@ int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
              const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
              const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
              float* qmax);
-int split_k2_jk(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
-                const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
-                const float* qmax, float* qs, float* qinv, double* part, size_t part_elems,
-                double* J, double* K);
+int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+             const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
+             const float* qmax, float* qs, float* qinv);
+int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
+                    double* part, size_t part_elems, double* part2, size_t part2_elems,
+                    double* J, double* K);
 
 namespace {
 thread_local std::string g_last_error;
@@ -187,6 +190,8 @@ int hc_products(dn_sys* s, bool want_hmat) {
 
 int check_sys(dn_sys* s) {
   if (!s) return fail(DN_ERR_ARG, "null dn_sys");
+  // the calling thread may not have selected the system's device yet
+  DN_HIP(hipSetDevice(s->ctx->device));
   return DN_OK;
 }
 
@@ -224,9 +229,10 @@ int jkh_split(dn_sys* s) {
   DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo, s->binv,
                   s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   DN_TRY(end_timing(s));
-  DN_TRY(split_k2_jk(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale,
-                     s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv, s->part, s->part_elems, s->J,
-                     s->K));
+  DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale,
+                  s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
+  DN_TRY(jk_from_bpq_f32(s->stream, s->bpq32, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X,
+                         s->part, s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
   DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
   return DN_OK;
@@ -238,8 +244,8 @@ int jkh_resident(dn_sys* s) {
   if (s->precision == DN_SPLIT_F16) return jkh_split(s);
   DN_TRY(transform1(s));
   DN_TRY(transform2(s, d.nbf5, s->bt));
-  DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->part,
-                     s->part_elems, s->J, s->K));
+  DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X, s->part,
+                     s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
   DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
   return DN_OK;
@@ -348,6 +354,12 @@ int dn_ctx_hbm_used(const dn_ctx* ctx, uint64_t* used) {
   if (!ctx || !used) return fail(DN_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lock(g_ctx_mutex);
   *used = ctx->used;
+  return DN_OK;
+}
+
+int dn_ctx_hbm_budget(const dn_ctx* ctx, uint64_t* budget) {
+  if (!ctx || !budget) return fail(DN_ERR_ARG, "null argument");
+  *budget = ctx->budget;
   return DN_OK;
 }
 

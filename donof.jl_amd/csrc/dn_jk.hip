@@ -4,30 +4,81 @@
 
 namespace dn {
 
-// J[p,q] = sum_Q b_ppQ b_qqQ and K[p,q] = sum_Q b_pqQ^2 over the Q range of split z.
-// bpq is [p,q,Q] with leading dimension nbf5 and slice stride `slice` (nbf5^2 for
-// the J/K build, nbf5*nbf when it is read out of the Lagrangian's b_MO^T).  Each thread owns
-// one (p,q); consecutive threads read consecutive p of a slice (coalesced), the
-// diagonal reads hit the cache lines shared by the whole block.
-__global__ void jk_partial_kernel(const double* __restrict__ bpq, long slice, int nbf5,
-                                  int naux, int q_per_split, double* __restrict__ Jp,
-                                  double* __restrict__ Kp) {
+// K[p,q] = sum_Q b_pqQ^2 over the Q range of split z (integrals.jl:166).  bpq is
+// [p,q,Q] with leading dimension nbf5 and slice stride `slice` (nbf5^2 for the J/K
+// build, nbf5*nbf when read out of the Lagrangian's b_MO^T); each thread owns one (p,q)
+// and consecutive threads read consecutive addresses of a slice.
+template <typename T>
+__global__ void k_partial_kernel(const T* __restrict__ bpq, long slice, int nbf5, int naux,
+                                 int q_per_split, double* __restrict__ Kp) {
   const long nn = (long)nbf5 * nbf5;
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const int z = blockIdx.y;
   if (e >= nn) return;
-  const int p = e % nbf5, q = e / nbf5;
   const int Q0 = z * q_per_split;
   const int Q1 = min(Q0 + q_per_split, naux);
-  double j = 0.0, k = 0.0;
+  double k = 0.0;
   for (int Q = Q0; Q < Q1; ++Q) {
-    const double* s = bpq + (long)Q * slice;
-    const double v = s[e];
+    const double v = bpq[(long)Q * slice + e];
     k = fma(v, v, k);
-    j = fma(s[(long)p * (nbf5 + 1)], s[(long)q * (nbf5 + 1)], j);
   }
-  Jp[z * nn + e] = j;
   Kp[z * nn + e] = k;
+}
+
+// X[p,Q] = b_ppQ (the diagonal that J = X X^T contracts, integrals.jl:163)
+template <typename T>
+__global__ void diag_kernel(const T* __restrict__ bpq, long slice, int nbf5, int naux,
+                            double* __restrict__ X) {
+  const long total = (long)nbf5 * naux;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long Q = e / nbf5;
+    const int p = e - Q * nbf5;
+    X[e] = bpq[Q * slice + (long)p * (nbf5 + 1)];
+  }
+}
+
+template <typename T>
+int jk_from_bpq_t(hipStream_t s, const T* bpq, long slice, int nbf5, int naux, double* X,
+                  double* part, size_t part_elems, double* part2, size_t part2_elems, double* J,
+                  double* K) {
+  const long nn = (long)nbf5 * nbf5;
+  const int threads = 256;
+  const int bx = (int)((nn + threads - 1) / threads);
+  int splits = (2048 + bx - 1) / bx;
+  if (splits > naux) splits = naux;
+  while (splits > 1 && (size_t)splits * nn > part_elems) --splits;
+  const int qps = (naux + splits - 1) / splits;
+  splits = (naux + qps - 1) / qps;
+  hipLaunchKernelGGL(k_partial_kernel<T>, dim3(bx, splits), dim3(threads), 0, s, bpq, slice, nbf5,
+                     naux, qps, part);
+  DN_HIP(hipGetLastError());
+  DN_TRY(reduce_splits(s, part, nn, splits, nbf5, nbf5, nbf5, K, nbf5, 0.0));
+  const long nx = (long)nbf5 * naux;
+  hipLaunchKernelGGL(diag_kernel<T>, dim3((unsigned)std::min<long>((nx + 255) / 256, 4096)),
+                     dim3(256), 0, s, bpq, slice, nbf5, naux, X);
+  DN_HIP(hipGetLastError());
+  // J = X X^T: A(p,Q) = X[p + nbf5 Q], B(Q,q) = X[q + nbf5 Q]
+  Gemm g;
+  g.M = nbf5; g.N = nbf5; g.K = naux;
+  g.A = X; g.lda = nbf5; g.a_kcontig = false;
+  g.B = X; g.ldb = nbf5; g.b_kcontig = false;
+  g.C = J; g.ldc = nbf5;
+  return gemm_f64_splitk(s, g, part2, part2_elems);
+}
+
+int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux, double* X,
+                double* part, size_t part_elems, double* part2, size_t part2_elems, double* J,
+                double* K) {
+  return jk_from_bpq_t<double>(s, bpq, slice, nbf5, naux, X, part, part_elems, part2,
+                               part2_elems, J, K);
+}
+
+int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
+                    double* part, size_t part_elems, double* part2, size_t part2_elems,
+                    double* J, double* K) {
+  return jk_from_bpq_t<float>(s, bpq, slice, nbf5, naux, X, part, part_elems, part2,
+                              part2_elems, J, K);
 }
 
 // H_core[i] = sum_m C5[m,i] (H C5)[m,i]   (integrals.jl:156,169)
@@ -44,26 +95,6 @@ __global__ void hcore_kernel(const double* __restrict__ C, const double* __restr
     __syncthreads();
   }
   if (threadIdx.x == 0) Hcore[i] = red[0];
-}
-
-int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux, double* part,
-                size_t part_elems, double* J, double* K) {
-  const long nn = (long)nbf5 * nbf5;
-  const int threads = 256;
-  const int bx = (int)((nn + threads - 1) / threads);
-  int splits = (2048 + bx - 1) / bx;
-  if (splits > naux) splits = naux;
-  while (splits > 1 && (size_t)(2 * splits) * nn > part_elems) --splits;
-  const int qps = (naux + splits - 1) / splits;
-  splits = (naux + qps - 1) / qps;
-  double* Jp = part;
-  double* Kp = part + (size_t)splits * nn;
-  hipLaunchKernelGGL(jk_partial_kernel, dim3(bx, splits), dim3(threads), 0, s, bpq, slice, nbf5,
-                     naux, qps, Jp, Kp);
-  DN_HIP(hipGetLastError());
-  DN_TRY(reduce_splits(s, Jp, nn, splits, nbf5, nbf5, nbf5, J, nbf5, 0.0));
-  DN_TRY(reduce_splits(s, Kp, nn, splits, nbf5, nbf5, nbf5, K, nbf5, 0.0));
-  return DN_OK;
 }
 
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,

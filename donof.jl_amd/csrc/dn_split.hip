@@ -154,9 +154,11 @@ struct SplitGemm {
 constexpr int SB_K = 32;           // k per MFMA / per LDS tile
 constexpr int SB_LD = SB_K + 8;    // halves per LDS row (80 bytes: conflict-free b128 reads)
 
-template <int BM, int BN, bool AF32>
-__global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_m, int tiles_n) {
-  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+template <int BM, int BN, int WGM, int WGN, bool AF32>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g, int tiles_m,
+                                                                   int tiles_n) {
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
   __shared__ _Float16 Ah[2][BM * SB_LD], Al[2][BM * SB_LD];
   __shared__ _Float16 Bh[2][BN * SB_LD], Bl[2][BN * SB_LD];
 
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
   const int nt = (lid / tiles_m) % tiles_n;
   const int z = lid / (tiles_m * tiles_n);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
   const int M = g.M, N = g.N, K = g.K;
   const int nkt = (K + SB_K - 1) / SB_K;
@@ -178,10 +180,12 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // staging registers
-  constexpr int AV = BM * 4 / 256;        // 16-byte loads per plane (f16 source)
-  constexpr int AP = BM * SB_K / 2 / 256; // k-pairs per thread (FP32 source)
-  constexpr int BV = BN * 4 / 256;
+  // staging registers: the loads of tile t+1 are issued before tile t is computed and
+  // written to LDS after it (LDS is double-buffered, one barrier per tile)
+  constexpr int AV = BM * 4 / NT;         // 16-byte loads per plane (f16 source)
+  constexpr int AP = BM * SB_K / 2 / NT;  // k-pairs per thread (FP32 source)
+  constexpr int BV = BN * 4 / NT;
+  static_assert((AF32 ? AP : AV) >= 1 && BV >= 1, "tile too small for the thread count");
   v8h rah[AF32 ? 1 : AV], ral[AF32 ? 1 : AV];
   float rf0[AF32 ? AP : 1], rf1[AF32 ? AP : 1];
   v8h rbh[BV], rbl[BV];
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     if constexpr (AF32) {
 #pragma unroll
       for (int i = 0; i < AP; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int m = idx % BM, kp = idx / BM;
         const int gm = min(m0 + m, M - 1);
         const int gk0 = min(k0 + 2 * kp, K - 1), gk1 = min(k0 + 2 * kp + 1, K - 1);
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     } else {
 #pragma unroll
       for (int i = 0; i < AV; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int r = idx / 4, kq = idx % 4;
         const long off = (long)(m0 + r) * g.lda + k0 + 8 * kq;  // planes are row/K padded
         rah[i] = *reinterpret_cast<const v8h*>(g.ahi + off);
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
-      const int idx = tid + i * 256;
+      const int idx = tid + i * NT;
       const int r = idx / 4, kq = idx % 4;
       const int col = min(n0 + r, N - 1);
       const long off = (long)col * g.ldb + k0 + 8 * kq;
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     if constexpr (AF32) {
 #pragma unroll
       for (int i = 0; i < AP; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int m = idx % BM, kp = idx / BM;
         _Float16 h0, l0, h1, l1;
         split_f16(rf0[i] * ascale, &h0, &l0);
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     } else {
 #pragma unroll
       for (int i = 0; i < AV; ++i) {
-        const int idx = tid + i * 256;
+        const int idx = tid + i * NT;
         const int r = idx / 4, kq = idx % 4;
         *reinterpret_cast<v8h*>(&Ah[buf][r * SB_LD + 8 * kq]) = rah[i];
         *reinterpret_cast<v8h*>(&Al[buf][r * SB_LD + 8 * kq]) = ral[i];
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
-      const int idx = tid + i * 256;
+      const int idx = tid + i * NT;
       const int r = idx / 4, kq = idx % 4;
       *reinterpret_cast<v8h*>(&Bh[buf][r * SB_LD + 8 * kq]) = rbh[i];
       *reinterpret_cast<v8h*>(&Bl[buf][r * SB_LD + 8 * kq]) = rbl[i];
@@ -257,13 +261,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
   for (int t = 0; t < nkt; ++t) {
     const int buf = t & 1;
     if (t + 1 < nkt) load(t + 1);
-    v8h ah[FM], al[FM], bh[FN], bl[FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-      const int r = wm * WM + i * 16 + fr;
-      ah[i] = *reinterpret_cast<const v8h*>(&Ah[buf][r * SB_LD + fk]);
-      al[i] = *reinterpret_cast<const v8h*>(&Al[buf][r * SB_LD + fk]);
-    }
+    v8h bh[FN], bl[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int r = wn * WN + j * 16 + fr;
@@ -271,13 +269,17 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitGemm g, int tiles_
       bl[j] = *reinterpret_cast<const v8h*>(&Bl[buf][r * SB_LD + fk]);
     }
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+      const int r = wm * WM + i * 16 + fr;
+      const v8h ah = *reinterpret_cast<const v8h*>(&Ah[buf][r * SB_LD + fk]);
+      const v8h al = *reinterpret_cast<const v8h*>(&Al[buf][r * SB_LD + fk]);
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
       }
+    }
     if (t + 1 < nkt) store(buf ^ 1);
     __syncthreads();
   }
@@ -335,27 +337,6 @@ __global__ void bpn_scale_kernel(const float* __restrict__ qmax, int naux,
   inv[Q] = 1.0f / s;
 }
 
-// J, K partials from FP32 bpq in FP64 (as jk_partial_kernel)
-__global__ void jk_partial_f32_kernel(const float* __restrict__ bpq, int nbf5, int naux,
-                                      int q_per_split, double* __restrict__ Jp,
-                                      double* __restrict__ Kp) {
-  const long nn = (long)nbf5 * nbf5;
-  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int z = blockIdx.y;
-  if (e >= nn) return;
-  const int p = e % nbf5, q = e / nbf5;
-  const int Q0 = z * q_per_split, Q1 = min(Q0 + q_per_split, naux);
-  double j = 0.0, k = 0.0;
-  for (int Q = Q0; Q < Q1; ++Q) {
-    const float* s = bpq + (long)Q * nn;
-    const double v = s[e];
-    k = fma(v, v, k);
-    j = fma((double)s[(long)p * (nbf5 + 1)], (double)s[(long)q * (nbf5 + 1)], j);
-  }
-  Jp[z * nn + e] = j;
-  Kp[z * nn + e] = k;
-}
-
 // ---- host side ---------------------------------------------------------------------------
 
 static unsigned grid_of(long n) {
@@ -383,13 +364,13 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
   return DN_OK;
 }
 
-template <int BM, int BN, bool AF32>
+template <int BM, int BN, int WGM, int WGN, bool AF32>
 static int launch_split(hipStream_t s, const SplitGemm& g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const long total = (long)tm * tn * g.nbatch;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split gemm grid too large");
-  hipLaunchKernelGGL((split_gemm_kernel<BM, BN, AF32>), dim3((unsigned)total), dim3(256), 0, s, g,
-                     tm, tn);
+  hipLaunchKernelGGL((split_gemm_kernel<BM, BN, WGM, WGN, AF32>), dim3((unsigned)total),
+                     dim3(64 * WGM * WGN), 0, s, g, tm, tn);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -411,13 +392,12 @@ int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
   g1.C = bpn; g1.ldc = nbf5; g1.sC = 0;
   g1.cs = cscale_dev; g1.col_inv_scale = b_inv_scale; g1.col_div = nbf;
   g1.qmax = qmax; g1.nbatch = 1;
-  return launch_split<128, 128, false>(s, g1);
+  return launch_split<128, 128, 2, 4, false>(s, g1);
 }
 
-int split_k2_jk(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
-                const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
-                const float* qmax, float* qs, float* qinv, double* part, size_t part_elems,
-                double* J, double* K) {
+int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+             const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
+             const float* qmax, float* qs, float* qinv) {
   hipLaunchKernelGGL(bpn_scale_kernel, dim3((naux + 255) / 256), dim3(256), 0, s, qmax, naux, qs,
                      qinv);
   DN_HIP(hipGetLastError());
@@ -428,20 +408,7 @@ int split_k2_jk(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Flo
   g2.C = bpq; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
   g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.col_div = 0;
   g2.nbatch = naux;
-  DN_TRY((launch_split<128, 128, true>(s, g2)));
-  // J, K in FP64
-  const long nn = (long)nbf5 * nbf5;
-  const int bx = (int)((nn + 255) / 256);
-  int splits = std::min<int>(naux, std::max(1, (2048 + bx - 1) / bx));
-  while (splits > 1 && (size_t)(2 * splits) * nn > part_elems) --splits;
-  const int qps = (naux + splits - 1) / splits;
-  splits = (naux + qps - 1) / qps;
-  hipLaunchKernelGGL(jk_partial_f32_kernel, dim3(bx, splits), dim3(256), 0, s, bpq, nbf5, naux,
-                     qps, part, part + (size_t)splits * nn);
-  DN_HIP(hipGetLastError());
-  DN_TRY(reduce_splits(s, part, nn, splits, nbf5, nbf5, nbf5, J, nbf5, 0.0));
-  DN_TRY(reduce_splits(s, part + (size_t)splits * nn, nn, splits, nbf5, nbf5, nbf5, K, nbf5, 0.0));
-  return DN_OK;
+  return launch_split<128, 128, 2, 4, true>(s, g2);
 }
 
 }  // namespace dn

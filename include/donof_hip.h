@@ -71,6 +71,7 @@ int dn_device_count(int* count);
 int dn_ctx_create(int device, uint64_t hbm_budget_bytes, dn_ctx** out);
 int dn_ctx_destroy(dn_ctx* ctx);
 int dn_ctx_hbm_used(const dn_ctx* ctx, uint64_t* used_bytes);
+int dn_ctx_hbm_budget(const dn_ctx* ctx, uint64_t* budget_bytes);
 
 /* One resident system: b, C, H, occupations, J/K/Lagrangian outputs and workspaces. */
 int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out);
