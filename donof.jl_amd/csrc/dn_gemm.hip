@@ -75,10 +75,11 @@ __device__ __forceinline__ v2d pair_mask(v2d v, int gmn, int gk, int MN, int KK)
   return v;
 }
 
-// Tile BM x BN per workgroup of WGM x WGN waves; each wave owns (BM/WGM) x (BN/WGN).
+// One output tile (mt, nt) of split z of a GEMM, by a workgroup of WGM x WGN waves; each
+// wave owns (BM/WGM) x (BN/WGN).  As/Bs are the workgroup's LDS double buffers.
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
-__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
-                                                                   int tiles_n, int nsplit) {
+__device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, double* As_,
+                                          double* Bs_) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
@@ -86,16 +87,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
   using LB = TileLds<BN, BK, BKC, NT>;
   static_assert(FM >= 1 && FN >= 1 && WM % 16 == 0 && WN % 16 == 0, "wave tile");
   static_assert(BM * BK % (2 * NT) == 0 && BN * BK % (2 * NT) == 0, "tile/threads");
-  __shared__ double As[2][LA::SIZE];
-  __shared__ double Bs[2][LB::SIZE];
-
-  const unsigned total = (unsigned)tiles_m * tiles_n * nsplit;
-  const unsigned hw = blockIdx.x;
-  if (hw >= total) return;
-  const unsigned lid = xcd_remap(hw, total);
-  const int mt = lid % tiles_m;
-  const int nt = (lid / tiles_m) % tiles_n;
-  const int z = lid / (tiles_m * tiles_n);
+  double (*As)[LA::SIZE] = reinterpret_cast<double (*)[LA::SIZE]>(As_);
+  double (*Bs)[LB::SIZE] = reinterpret_cast<double (*)[LB::SIZE]>(Bs_);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -210,6 +203,56 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
     }
 }
 
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>
+struct GemmLds {
+  static constexpr int A = TileLds<BM, BK, AK, 64 * WGM * WGN>::SIZE;
+  static constexpr int B = TileLds<BN, BK, BKC, 64 * WGM * WGN>::SIZE;
+};
+
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
+                                                                   int tiles_n, int nsplit) {
+  using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
+  __shared__ double As[2 * L::A];
+  __shared__ double Bs[2 * L::B];
+  const unsigned total = (unsigned)tiles_m * tiles_n * nsplit;
+  const unsigned hw = blockIdx.x;
+  if (hw >= total) return;
+  const unsigned lid = xcd_remap(hw, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>(g, mt, nt, z, As, Bs);
+}
+
+// Persistent grouped GEMM: the tiles of all problems form one list (problem i owns tiles
+// [desc[i].tile_begin, desc[i].tile_begin + tiles_m*tiles_n*nsplit)); workgroup b takes
+// tiles b, b + gridDim.x, ... until the list is exhausted (a static schedule, so every
+// workgroup's loop ends).
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const GroupedGemm* desc,
+                                                                      int nprob,
+                                                                      long total_tiles) {
+  using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
+  __shared__ double As[2 * L::A];
+  __shared__ double Bs[2 * L::B];
+  for (long t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (desc[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const GroupedGemm& d = desc[lo];
+    const long local = t - d.tile_begin;
+    const int mt = local % d.tiles_m;
+    const int nt = (local / d.tiles_m) % d.tiles_n;
+    const int z = local / ((long)d.tiles_m * d.tiles_n);
+    gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>(d.g, mt, nt, z, As, Bs);
+    __syncthreads();  // the next tile reuses the LDS buffers
+  }
+}
+
 __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int nsplit, int M,
                                      int N, long ld, double* __restrict__ out, long ldo,
                                      double beta, double scale) {
@@ -276,6 +319,79 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   // a multiple of it); everything else uses 64 x 64.
   if (g.M > 64 && g.N >= 128) return dispatch_layout<128, 128, 16, 2, 4>(s, g);
   return dispatch_layout<64, 64, 16, 2, 2>(s, g);
+}
+
+namespace {
+
+bool vec_ok(const double* p, long ld, long sb, int nb, int extent) {
+  return ((uintptr_t)p % 16 == 0) && (ld % 2 == 0) && (nb <= 1 || sb % 2 == 0) &&
+         (extent % 2 == 0) && extent >= 2;
+}
+
+void config_tile(int config, int* bm, int* bn) {
+  switch (config) {
+    case kTile64x64: *bm = 64; *bn = 64; break;
+    case kTile64x128: *bm = 64; *bn = 128; break;
+    default: *bm = 128; *bn = 128; break;
+  }
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, int grid) {
+  hipLaunchKernelGGL((dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>), dim3(grid),
+                     dim3(64 * WGM * WGN), 0, s, d, n, total);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN>
+int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, const GroupedGemm* d, int n,
+                   long total, int grid) {
+#define DN_G(AKv, BKv, VAv, VBv) \
+  launch_grouped<BM, BN, BK, WGM, WGN, AKv, BKv, VAv, VBv>(s, d, n, total, grid)
+  if (ak && bk) return va ? (vb ? DN_G(true, true, true, true) : DN_G(true, true, true, false))
+                          : (vb ? DN_G(true, true, false, true) : DN_G(true, true, false, false));
+  if (!ak && bk) return va ? (vb ? DN_G(false, true, true, true) : DN_G(false, true, true, false))
+                           : (vb ? DN_G(false, true, false, true) : DN_G(false, true, false, false));
+  return fail(DN_ERR_ARG, "grouped GEMM supports a_kcontig in {0,1} with b_kcontig = 1");
+#undef DN_G
+}
+
+}  // namespace
+
+long gemm_grouped_plan(GroupedGemm* desc, int n, int config) {
+  int bm, bn;
+  config_tile(config, &bm, &bn);
+  long begin = 0;
+  for (int i = 0; i < n; ++i) {
+    Gemm& g = desc[i].g;
+    g.vec_a = vec_ok(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
+    g.vec_b = vec_ok(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
+    desc[i].tiles_m = (g.M + bm - 1) / bm;
+    desc[i].tiles_n = (g.N + bn - 1) / bn;
+    desc[i].nsplit = (g.nbatch + g.bps - 1) / g.bps;
+    desc[i].tile_begin = begin;
+    begin += (long)desc[i].tiles_m * desc[i].tiles_n * desc[i].nsplit;
+  }
+  return begin;
+}
+
+int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd, int n,
+                     long total, int config, int grid) {
+  if (n <= 0 || total <= 0) return DN_OK;
+  const bool ak = dh[0].g.a_kcontig, bk = dh[0].g.b_kcontig;
+  bool va = true, vb = true;
+  for (int i = 0; i < n; ++i) {
+    if (dh[i].g.a_kcontig != ak || dh[i].g.b_kcontig != bk)
+      return fail(DN_ERR_ARG, "grouped GEMM problems must share operand layouts");
+    va = va && dh[i].g.vec_a;
+    vb = vb && dh[i].g.vec_b;
+  }
+  switch (config) {
+    case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, dd, n, total, grid);
+    case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, dd, n, total, grid);
+    default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, dd, n, total, grid);
+  }
 }
 
 int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {

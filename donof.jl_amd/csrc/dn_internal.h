@@ -52,6 +52,21 @@ struct Gemm {
 };
 
 int gemm_f64(hipStream_t s, const Gemm& g);
+
+// One problem of a grouped GEMM launch (tile list offsets filled by gemm_grouped_plan).
+struct GroupedGemm {
+  Gemm g;
+  int tiles_m = 0, tiles_n = 0, nsplit = 0;
+  long tile_begin = 0;
+};
+// Tile configurations of the grouped launches (chosen by the autotuner).
+enum GemmConfig { kTile128x128 = 0, kTile64x64 = 1, kTile64x128 = 2, kNumGemmConfigs = 3 };
+// Fill tile counts/offsets of desc[0..n) for a config; returns the total tile count.
+long gemm_grouped_plan(GroupedGemm* desc, int n, int config);
+// One persistent launch over all problems (descriptors already on the device).  All
+// problems share the operand layouts (a_kcontig, b_kcontig) of desc_host[0].
+int gemm_f64_grouped(hipStream_t s, const GroupedGemm* desc_host, const GroupedGemm* desc_dev,
+                     int n, long total_tiles, int config, int grid);
 // Sum `nsplit` partial matrices (each M x N, leading dim ld, stride sP) into out.
 int reduce_splits(hipStream_t s, const double* P, long sP, int nsplit, int M, int N,
                   long ld, double* out, long ldo, double beta, double scale = 1.0);
