@@ -60,6 +60,7 @@ _SIGS = {
     "dn_sys_upload_b": (_I, [_P, _D]),
     "dn_sys_download_b": (_I, [_P, _D]),
     "dn_sys_generate_b": (_I, [_P, ct.c_uint64]),
+    "dn_sys_ri_metric": (_I, [_P, _D, _D]),
     "dn_jkh_mo": (_I, [_P, _D, _D, _D, _D, _D]),
     "dn_lagrange2": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _D, _D, _D]),
     "dn_sys_set_orbitals": (_I, [_P, _D]),

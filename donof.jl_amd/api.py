@@ -114,6 +114,13 @@ class System:
         check(self.lib.dn_sys_generate_b(self.handle, ct.c_uint64(int(seed))))
         return self
 
+    def ri_metric(self, G, I3c):
+        """b = I3c G^{-1/2} on the device (FP64 Cholesky route; integrals.jl:68-91)."""
+        G = _colmajor(G, (self.nbfaux, self.nbfaux), "G")
+        I3c = _colmajor(I3c, (self.nbf, self.nbf, self.nbfaux), "I3c")
+        check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))
+        return self
+
     def download_b(self):
         out = np.empty((self.nbf, self.nbf, self.nbfaux), order="F")
         check(self.lib.dn_sys_download_b(self.handle, dptr(out)))
@@ -187,8 +194,17 @@ def _require_system(b):
 
 
 # --------------------------------------------------------------------------------------
-# integrals.jl:131-172
+# integrals.jl:68-91, 131-172
 # --------------------------------------------------------------------------------------
+
+
+def Iaux(G, I, p, ctx=None, precision=DN_FP64):
+    """integrals.jl:68-91: the RI tensor from the metric G and the 3-centre integrals I.
+    Returns the device-resident System holding b (the reference returns b itself; every
+    later call takes that System where the reference takes b).  b is I L^{-T} with
+    G = L L^T, i.e. the reference's I G^{-1/2} rotated by an orthogonal matrix in the
+    auxiliary index, which no J, K, Lagrangian or energy can see."""
+    return System(p, ctx, precision).ri_metric(G, I)
 
 
 def JKH_MO(C, H, b, nbf, nbf5, nbfaux):

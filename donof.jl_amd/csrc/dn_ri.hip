@@ -1,0 +1,195 @@
+// RI metric on the device: b[m,n,:] = I3c[m,n,:] * G^{-1/2} up to an orthogonal
+// rotation of the auxiliary index (integrals.jl:68-91, Iaux; ext/DoNOFGPUExt.jl:9-33).
+//
+// The reference diagonalises G and forms G^{-1/2}.  Here G = L L^T (blocked Cholesky,
+// FP64) and b = I3c L^{-T}.  The two results differ by b_ref = b Q with the orthogonal
+// Q = L^T G^{-1/2}; every consumer of b (J, K, the Lagrangian, the energy) contracts the
+// auxiliary index in pairs, sum_Q b[.,Q] b[.,Q], so all of them are identical.  G must be
+// positive definite (a Coulomb metric is); the eigen route's clamp of negative
+// eigenvalues has no Cholesky counterpart, so such a G is rejected.
+//
+// Layout: G column-major naux x naux (lower triangle read), I3c column-major
+// (nbf, nbf, naux), i.e. an (nbf^2 x naux) matrix with the auxiliary index as columns.
+#include "dn_internal.h"
+
+namespace dn {
+namespace {
+
+constexpr int NB = 64;  // Cholesky block size
+
+// Factor the kb x kb diagonal block at (k, k) in LDS, write L_kk back into G and
+// L_kk^{-1} (full kb x kb, zero above the diagonal) into rows k..k+kb of Dinv.
+__global__ __launch_bounds__(256) void potrf_block_kernel(double* G, long ld, int k, int kb,
+                                                          double* Dinv, int* flag) {
+  __shared__ double a[NB][NB + 1];
+  __shared__ double inv[NB][NB + 1];
+  __shared__ double piv;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kb * kb; i += blockDim.x) {
+    const int r = i % kb, c = i / kb;
+    a[r][c] = G[(k + r) + (long)(k + c) * ld];
+  }
+  for (int j = 0; j < kb; ++j) {
+    __syncthreads();
+    if (tid == 0) {
+      double d = a[j][j];
+      if (!(d > 0.0) || !isfinite(d)) {
+        atomicOr(flag, 1);
+        d = 1.0;
+      }
+      piv = sqrt(d);
+      a[j][j] = piv;
+    }
+    __syncthreads();
+    for (int r = j + 1 + tid; r < kb; r += blockDim.x) a[r][j] /= piv;
+    __syncthreads();
+    const int m = kb - j - 1;
+    for (int i = tid; i < m * m; i += blockDim.x) {
+      const int r = j + 1 + i % m, c = j + 1 + i / m;
+      if (c <= r) a[r][c] -= a[r][j] * a[c][j];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < kb * kb; i += blockDim.x) {
+    const int r = i % kb, c = i / kb;
+    if (r >= c) G[(k + r) + (long)(k + c) * ld] = a[r][c];
+  }
+  // column c of L^{-1} by forward substitution: L x = e_c
+  if (tid < kb) {
+    const int c = tid;
+    for (int r = 0; r < c; ++r) inv[r][c] = 0.0;
+    inv[c][c] = 1.0 / a[c][c];
+    for (int r = c + 1; r < kb; ++r) {
+      double acc = 0.0;
+      for (int i = c; i < r; ++i) acc += a[r][i] * inv[i][c];
+      inv[r][c] = -acc / a[r][r];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < kb * kb; i += blockDim.x) {
+    const int r = i % kb, c = i / kb;
+    Dinv[(k + r) + (long)c * ld] = inv[r][c];
+  }
+}
+
+// dst[r + c*ldd] = src[r + c*lds] for r < rows, c < cols
+__global__ void copy_block_kernel(double* dst, long ldd, const double* src, long lds, int rows,
+                                  int cols) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  const int r = i % rows, c = i / rows;
+  dst[r + c * ldd] = src[r + c * lds];
+}
+
+int copy_block(hipStream_t s, double* dst, long ldd, const double* src, long lds, int rows,
+               int cols) {
+  const long n = (long)rows * cols;
+  if (n <= 0) return DN_OK;
+  hipLaunchKernelGGL(copy_block_kernel, dim3((n + 255) / 256), dim3(256), 0, s, dst, ldd, src,
+                     lds, rows, cols);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+}  // namespace
+
+// In place: the lower triangle of G (naux x naux, ld = naux) becomes L with G = L L^T;
+// Dinv (naux x NB) receives the inverses of the diagonal blocks; T (naux x NB) is the
+// panel workspace.  *flag (device) is set when a pivot is not positive.
+int cholesky_f64(hipStream_t s, double* G, int naux, double* Dinv, double* T, int* flag) {
+  const long ld = naux;
+  for (int k = 0; k < naux; k += NB) {
+    const int kb = std::min(NB, naux - k);
+    hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(256), 0, s, G, ld, k, kb, Dinv, flag);
+    DN_HIP(hipGetLastError());
+    const int rem = naux - k - kb;
+    if (rem <= 0) break;
+    // panel: T = G[k+kb:, k:k+kb] L_kk^{-T}
+    Gemm p;
+    p.M = rem; p.N = kb; p.K = kb;
+    p.A = G + (k + kb) + (long)k * ld; p.lda = ld; p.a_kcontig = false;
+    p.B = Dinv + k; p.ldb = ld; p.b_kcontig = false;  // B(q, c) = Dinv[(k + c) + q ld]
+    p.C = T; p.ldc = ld;
+    DN_TRY(gemm_f64(s, p));
+    // trailing update: G[k+kb:, k+kb:] -= T T^T
+    Gemm u;
+    u.M = rem; u.N = rem; u.K = kb;
+    u.A = T; u.lda = ld; u.a_kcontig = false;
+    u.B = T; u.ldb = ld; u.b_kcontig = false;
+    u.C = G + (k + kb) * (ld + 1); u.ldc = ld;
+    u.alpha = -1.0; u.beta = 1.0;
+    DN_TRY(gemm_f64(s, u));
+    DN_TRY(copy_block(s, G + (k + kb) + (long)k * ld, ld, T, ld, rem, kb));
+  }
+  return DN_OK;
+}
+
+// X (rows x naux, ld = rows) is overwritten with X - (solved columns) and the solution
+// Y = X L^{-T} is written to Y (rows x naux, ld = rows): column block k is
+// Y_k = (X_k - Y_{<k} L_{k,<k}^T) L_kk^{-T}.
+int trsm_right_lt(hipStream_t s, double* X, double* Y, long rows, int naux, const double* L,
+                  const double* Dinv) {
+  if (rows > INT32_MAX) return fail(DN_ERR_ARG, "RI metric: nbf^2 exceeds 2^31");
+  const long ld = naux;
+  for (int k = 0; k < naux; k += NB) {
+    const int kb = std::min(NB, naux - k);
+    if (k > 0) {
+      Gemm g;
+      g.M = (int)rows; g.N = kb; g.K = k;
+      g.A = Y; g.lda = rows; g.a_kcontig = false;
+      g.B = L + k; g.ldb = ld; g.b_kcontig = false;  // B(q, c) = L[(k + c) + q ld]
+      g.C = X + k * rows; g.ldc = rows;
+      g.alpha = -1.0; g.beta = 1.0;
+      DN_TRY(gemm_f64(s, g));
+    }
+    Gemm d;
+    d.M = (int)rows; d.N = kb; d.K = kb;
+    d.A = X + k * rows; d.lda = rows; d.a_kcontig = false;
+    d.B = Dinv + k; d.ldb = ld; d.b_kcontig = false;
+    d.C = Y + k * rows; d.ldc = rows;
+    DN_TRY(gemm_f64(s, d));
+  }
+  return DN_OK;
+}
+
+}  // namespace dn
+
+using namespace dn;
+
+extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
+  DN_TRY(check_sys(s));
+  if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
+  const int naux = s->d.nbfaux;
+  const long rows = (long)s->d.nbf * s->d.nbf;
+  const size_t nI = (size_t)rows * naux, nG = (size_t)naux * naux, nP = (size_t)naux * NB;
+  const size_t bytes = (nI + nG + 2 * nP) * sizeof(double) + 256;
+  if (s->ctx->budget && s->ctx->used + bytes > s->ctx->budget)
+    return fail(DN_ERR_BUDGET, "RI metric workspace exceeds the HBM budget");
+  char* w = nullptr;
+  DN_HIP(hipMalloc(&w, bytes));
+  double* dI = reinterpret_cast<double*>(w);
+  double* dG = dI + nI;
+  double* Dinv = dG + nG;
+  double* T = Dinv + nP;
+  int* flag = reinterpret_cast<int*>(T + nP);
+  int rc = DN_OK, host_flag = 0;
+  auto run = [&]() -> int {
+    DN_TRY(h2d(s, dI, I3c, nI));
+    DN_TRY(h2d(s, dG, G, nG));
+    DN_HIP(hipMemsetAsync(flag, 0, sizeof(int), s->stream));
+    DN_TRY(cholesky_f64(s->stream, dG, naux, Dinv, T, flag));
+    DN_HIP(hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    DN_HIP(hipStreamSynchronize(s->stream));
+    if (host_flag)
+      return fail(DN_ERR_ARG, "RI metric is not positive definite (Cholesky pivot <= 0)");
+    DN_TRY(trsm_right_lt(s->stream, dI, s->b, rows, naux, dG, Dinv));
+    DN_TRY(prepare_b(s));
+    DN_HIP(hipStreamSynchronize(s->stream));
+    return DN_OK;
+  };
+  rc = run();
+  hipStreamSynchronize(s->stream);
+  hipFree(w);
+  if (rc == DN_OK) s->b_ready = true;
+  return rc;
+}

@@ -23,7 +23,7 @@ import math
 
 import numpy as np
 
-STREAM_B, STREAM_C, STREAM_H, STREAM_OCC = 0, 1, 2, 3
+STREAM_B, STREAM_C, STREAM_H, STREAM_OCC, STREAM_METRIC = 0, 1, 2, 3, 4
 B_DECAY = 400.0
 
 _M0 = np.uint64(0xD2511F53)
@@ -93,6 +93,22 @@ def make_b(nbf, naux, seed, q0=0, q1=None):
         A = normal(e, STREAM_B, seed)
         out[:, :, t] = (A + A.T) * (math.exp(-(Q + 1) / B_DECAY) * inv)
     return out
+
+
+def make_metric(nbf, naux, seed, cond=1e6):
+    """RI setup inputs with a known answer: a positive-definite 2-centre metric G
+    (naux x naux, eigenvalues log-spaced over [1/cond, 1], seeded orthogonal
+    eigenvectors) and 3-centre integrals I3c = b G^{1/2} with b = make_b(nbf, naux, seed),
+    so that I3c G^{-1/2} = b.  Returns (G, I3c, b)."""
+    e = np.arange(naux * naux, dtype=np.uint64)
+    A = normal(e, STREAM_METRIC, seed).reshape(naux, naux, order="F")
+    V, _ = np.linalg.qr(A)
+    lam = np.logspace(0.0, -math.log10(cond), naux)
+    G = np.asfortranarray(0.5 * ((V * lam) @ V.T + ((V * lam) @ V.T).T))
+    Gh = (V * np.sqrt(lam)) @ V.T
+    b = make_b(nbf, naux, seed)
+    I3c = (b.reshape(nbf * nbf, naux, order="F") @ Gh).reshape(nbf, nbf, naux, order="F")
+    return G, np.asfortranarray(I3c), b
 
 
 def make_C(nbf, seed):

@@ -85,6 +85,15 @@ int dn_sys_upload_b(dn_sys* sys, const double* b);
 int dn_sys_download_b(dn_sys* sys, double* b);
 /* Synthetic b generated in HBM (Philox4x32-10 normals, see donof.jl_amd/synthetic.py). */
 int dn_sys_generate_b(dn_sys* sys, uint64_t seed);
+/* RI setup on the device: b = I3c G^{-1/2} from the 2-centre metric G (naux x naux,
+ * column-major, lower triangle read) and the 3-centre integrals I3c (column-major
+ * nbf x nbf x naux); replaces Iaux (integrals.jl:68-91) and Iaux_gpu
+ * (ext/DoNOFGPUExt.jl:9-33).  FP64 blocked Cholesky G = L L^T and b = I3c L^{-T}: b
+ * equals the reference's eigen-route b up to an orthogonal rotation of the auxiliary
+ * index, which leaves J, K, the Lagrangian and every energy unchanged.  A G that is
+ * not positive definite returns DN_ERR_ARG.  Host arrays are not modified; a transient
+ * device workspace of the size of b is taken for the call. */
+int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
 
 /* J_MO, K_MO (nbf5 x nbf5) and H_core (nbf5) from C (nbf x nbf) and H (nbf x nbf):
  * replaces computeJKH_MO / JKH_MO (integrals.jl:131-172; ext/DoNOFGPUExt.jl:52-94). */

@@ -8,6 +8,8 @@ reference are 1-based and column-major; here arrays keep the same axis order
 (b[m, n, Q], C[m, p], Dcj12r[i, j, k]) and indices are 0-based.
 
 How each part is pinned (tests/test_oracle_*.py):
+  iaux              G^{-1/2} G G^{-1/2} = I; the Cholesky route gives the same
+                    b b^T (auxiliary index contracted) and the same J/K.
   jkh_mo            brute-force loops at tiny sizes, symmetry/PSD invariants,
                     invariance under b -> b.Q (orthogonal Q), diag J == diag K.
   compute_lagrange2 brute-force loops; energy identity computeE_elec == calce;
@@ -90,6 +92,31 @@ def pair_gamma(p, l):
     """0-based gamma indices of pair l for Softmax: llg = (ndoc-l)+1, stride ndoc
     (pnof.jl:1142-1143)."""
     return (p.ndoc - l) + p.ndoc * np.arange(p.ncwo)
+
+
+# --------------------------------------------------------------------------------------
+# RI metric (integrals.jl:68-91)
+# --------------------------------------------------------------------------------------
+
+
+def iaux(G, I3c):
+    """b[m,n,:] = I3c[m,n,:] G^{-1/2} by the eigen route of integrals.jl:68-91: eigen of
+    the symmetric G, 1/sqrt(lambda) for lambda >= 0 and 0 for lambda < 0 (74-83)."""
+    lam, V = np.linalg.eigh(0.5 * (G + G.T))
+    s = np.where(lam < 0.0, 0.0, 1.0 / np.sqrt(np.where(lam < 0.0, 1.0, lam)))
+    Gms = (V * s) @ V.T
+    nbf, _, naux = I3c.shape
+    return (I3c.reshape(nbf * nbf, naux, order="F") @ Gms).reshape(nbf, nbf, naux, order="F")
+
+
+def iaux_cholesky(G, I3c):
+    """b = I3c L^{-T} with G = L L^T: the reference's b rotated by the orthogonal
+    L^T G^{-1/2} in the auxiliary index (same sum_Q b b)."""
+    L = np.linalg.cholesky(0.5 * (G + G.T))
+    nbf, _, naux = I3c.shape
+    X = I3c.reshape(nbf * nbf, naux, order="F")
+    Y = scipy.linalg.solve_triangular(L, X.T, lower=True).T
+    return Y.reshape(nbf, nbf, naux, order="F")
 
 
 # --------------------------------------------------------------------------------------
