@@ -35,12 +35,6 @@ F16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
-def build_flops(nbf, naux, nbf5):
-    """Algorithmic FLOP of one J/K-MO build in the reference's contraction order."""
-    return (2 * nbf5 * nbf * nbf * naux + 2 * nbf5 * nbf5 * nbf * naux + 4 * nbf5 * nbf5 * naux
-            + 2 * nbf * nbf * nbf5 + 2 * nbf * nbf5)
-
-
 def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -97,11 +91,11 @@ def make_system(api, synthetic, nbf, naux, nbf5, seed, precision):
     return sysd, dev
 
 
-def extra_legs(api, synthetic, orc, steps, warmup, seed):
+def extra_legs(api, synthetic, steps, warmup, seed):
     """FP64 and split-precision builds at nbf=800, nbfaux=3200, nbf5=240."""
     nbf, naux, nbf5 = 800, 3200, 240
     out = {"config": {"nbf": nbf, "nbfaux": naux, "nbf5": nbf5}}
-    F = build_flops(nbf, naux, nbf5)
+    F = synthetic.build_flops(nbf, naux, nbf5)
     sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "fp64")
     ms, kms, _ = time_builds(dev, steps, warmup)
     J64, K64, H64 = dev.get_jkh()
@@ -115,10 +109,13 @@ def extra_legs(api, synthetic, orc, steps, warmup, seed):
     J16, K16, H16 = dev.get_jkh()
     dev.close()
     k1_useful = 2.0 * nbf5 * nbf * nbf * naux
+    # GNOF energy (calce, pnof.jl:1273-1300) of both builds, on the device
     p = sysd.p
-    cj, ck = orc.pnof_selector(sysd.n, p)
-    E64 = orc.calce(sysd.n, cj, ck, J64, K64, H64, p)
-    E16 = orc.calce(sysd.n, cj, ck, J16, K16, H16, p)
+    aux = api.System(synthetic.synthetic_param(nbf, 4, nbf5))
+    cj, ck = api.PNOFi_selector(sysd.n, p, aux)
+    E64 = api.calce(sysd.n, cj, ck, J64, K64, H64, p, aux)
+    E16 = api.calce(sysd.n, cj, ck, J16, K16, H16, p, aux)
+    aux.close()
     out["split_f16"] = {
         "builds_per_s": 1e3 / ms16, "ms_per_build": ms16, "tflops": F / (ms16 * 1e-3) / 1e12,
         "transform_kernel_ms": kms16,
@@ -133,6 +130,70 @@ def extra_legs(api, synthetic, orc, steps, warmup, seed):
     log("split 800/3200/240: %.3f ms/build (x%.2f), |dE|=%.2e"
         % (ms16, ms / ms16, abs(E16 - E64)))
     return out
+
+
+def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
+    """64 systems, nbf ~ U[48, 400], naux = 4 nbf, nbf5 = ceil(nbf/3): one grouped build
+    of all of them per step (graph replay), against per-system builds on their own
+    streams."""
+    shapes = synthetic.grouped_shapes(nsys, seed)
+    F = sum(synthetic.build_flops(*sh) for sh in shapes)
+    systems = []
+    for i, (nbf, naux, nbf5) in enumerate(shapes):
+        s = api.System(synthetic.synthetic_param(nbf, naux, nbf5))
+        s.generate_b(seed + i)
+        s.set_orbitals(synthetic.make_C(nbf, seed + 1000 + i))
+        s.set_hcore(synthetic.make_H(nbf, seed + 2000 + i))
+        systems.append(s)
+    log("grouped: %d systems resident, %.2f TFLOP per build of all" % (nsys, F / 1e12))
+    for _ in range(warmup):
+        for s in systems:
+            s.jkh_mo_resident()
+    for s in systems:
+        s.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in systems:
+            s.jkh_mo_resident()
+    for s in systems:
+        s.sync()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ref = [s.get_jkh() for s in systems[:4]]
+    g = api.Group(systems)
+    t0 = time.perf_counter()
+    cfg = g.tune()
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        g.jkh()
+    g.sync()
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.jkh()
+    g.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ev_ms, n = g.get_timing()
+    err = 0.0
+    for s, (J, K, Hc) in zip(systems[:4], ref):
+        Jg, Kg, Hg = s.get_jkh()
+        err = max(err, float(np.abs(Jg - J).max() / np.abs(J).max()),
+                  float(np.abs(Kg - K).max() / np.abs(K).max()))
+    g.close()
+    for s in systems:
+        s.close()
+    tf = F / (ms * 1e-3) / 1e12
+    log("grouped fp64: %.2f ms per build of all %d (%.1f TFLOP/s), per-system streams %.2f ms"
+        % (ms, nsys, tf, seq_ms))
+    return {"config": {"nsys": nsys, "nbf_range": [48, 400], "seed": seed,
+                       "nbf": [sh[0] for sh in shapes],
+                       "tflop_per_build_of_all": F / 1e12,
+                       "b_gb": sum(8.0 * n * n * a for n, a, _ in shapes) / 1e9},
+            "fp64": {"builds_per_s": nsys * 1e3 / ms, "ms_per_build_of_all": ms,
+                     "graph_event_ms": ev_ms / max(n, 1), "tflops": tf,
+                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "tiles": cfg,
+                     "autotune_s": tune_s,
+                     "per_system_streams_ms": seq_ms, "speedup_vs_per_system": seq_ms / ms,
+                     "max_rel_diff_vs_per_system": err}}
 
 
 def main():
@@ -161,7 +222,7 @@ def main():
     sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, args.seed, "fp64")
     log("system resident, warmup %d" % args.warmup)
     ms_step, k_avg_ms, k_n = time_builds(dev, args.steps, args.warmup)
-    F = build_flops(nbf, naux, nbf5)
+    F = synthetic.build_flops(nbf, naux, nbf5)
     k_flops = 2.0 * nbf5 * nbf * nbf * naux
     k_tflops = k_flops / (k_avg_ms * 1e-3) / 1e12
     out = {
@@ -198,9 +259,10 @@ def main():
     else:
         out["cpu_baseline"] = None
     if not args.no_extra:
-        orc = __graft_entry__.load_oracle()
-        out["extra_800_3200_240"] = extra_legs(api, synthetic, orc, max(5, args.steps // 2),
+        out["extra_800_3200_240"] = extra_legs(api, synthetic, max(5, args.steps // 2),
                                                args.warmup, args.seed)
+        out["extra_grouped_64"] = grouped_leg(api, synthetic, max(3, args.steps // 4),
+                                              args.warmup, args.seed)
     print(json.dumps(out))
     return 0
 

@@ -61,6 +61,8 @@ _SIGS = {
     "dn_sys_download_b": (_I, [_P, _D]),
     "dn_sys_generate_b": (_I, [_P, ct.c_uint64]),
     "dn_sys_ri_metric": (_I, [_P, _D, _D]),
+    "dn_sys_set_symmetry": (_I, [_P, _I]),
+    "dn_sys_b_symmetric": (_I, [_P, ct.POINTER(ct.c_int)]),
     "dn_jkh_mo": (_I, [_P, _D, _D, _D, _D, _D]),
     "dn_lagrange2": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _D, _D, _D]),
     "dn_sys_set_orbitals": (_I, [_P, _D]),
@@ -81,6 +83,13 @@ _SIGS = {
     "dn_sys_set_gamma": (_I, [_P, _D]),
     "dn_sys_get_orbitals": (_I, [_P, _D]),
     "dn_orbopt_resident": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state)]),
+    "dn_group_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
+    "dn_group_destroy": (_I, [_P]),
+    "dn_group_tune": (_I, [_P, ct.POINTER(ct.c_int)]),
+    "dn_group_jkh": (_I, [_P]),
+    "dn_group_sync": (_I, [_P]),
+    "dn_group_set_timing": (_I, [_P, _I]),
+    "dn_group_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),
 }
 
 _lib = None

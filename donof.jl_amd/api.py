@@ -121,6 +121,17 @@ class System:
         check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))
         return self
 
+    def set_symmetry(self, use):
+        """Use (default) or ignore the (m, n) symmetry of b in the J/K build."""
+        check(self.lib.dn_sys_set_symmetry(self.handle, 1 if use else 0))
+        return self
+
+    @property
+    def b_symmetric(self):
+        v = ct.c_int()
+        check(self.lib.dn_sys_b_symmetric(self.handle, ct.byref(v)))
+        return bool(v.value)
+
     def download_b(self):
         out = np.empty((self.nbf, self.nbf, self.nbfaux), order="F")
         check(self.lib.dn_sys_download_b(self.handle, dptr(out)))
@@ -163,6 +174,62 @@ class System:
     def close(self):
         if getattr(self, "handle", None):
             self.lib.dn_sys_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Group:
+    """Resident systems of uneven size built together: grouped persistent launches
+    captured into one HIP graph (dn_group_*).  Each system's result is its own
+    computeJKH_MO (integrals.jl:131-172), read with System.get_jkh."""
+
+    STAGES = ("K1", "K2", "J", "HC")
+    CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128"}
+
+    def __init__(self, systems, ctx=None):
+        self.systems = list(systems)
+        if not self.systems:
+            raise ValueError("a group needs at least one system")
+        self.ctx = ctx if ctx is not None else self.systems[0].ctx
+        self.lib = self.ctx.lib
+        arr = (ct.c_void_p * len(self.systems))(*[s.handle for s in self.systems])
+        h = ct.c_void_p()
+        check(self.lib.dn_group_create(self.ctx.handle, arr, len(self.systems), ct.byref(h)))
+        self.handle = h
+        self.configs = None
+
+    def tune(self):
+        """Autotune the tile configuration of every grouped GEMM stage and capture the
+        build graph; returns {stage: tile shape}."""
+        cfg = (ct.c_int * 4)()
+        check(self.lib.dn_group_tune(self.handle, cfg))
+        self.configs = {st: self.CONFIGS[c] for st, c in zip(self.STAGES, cfg)}
+        return self.configs
+
+    def jkh(self):
+        check(self.lib.dn_group_jkh(self.handle))
+        return self
+
+    def sync(self):
+        check(self.lib.dn_group_sync(self.handle))
+
+    def set_timing(self, enable):
+        check(self.lib.dn_group_set_timing(self.handle, 1 if enable else 0))
+
+    def get_timing(self):
+        ms = ct.c_double()
+        n = ct.c_long()
+        check(self.lib.dn_group_get_timing(self.handle, ct.byref(ms), ct.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dn_group_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -15,6 +15,9 @@ int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
                double* Hcore);
 int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
+int sym_inplace(hipStream_t s, double* bpq, long slice, int nbf5, int naux);
+int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
+               double* dmax, double* bmax);
 int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
           const double* n, int nalpha, int nbeta, double* X, double* w);
 int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf5, int naux,
@@ -150,6 +153,35 @@ int transform1(dn_sys* s) {
   return end_timing(s);
 }
 
+// Symmetric b (b_Q = Lh_Q + Lh_Q^T, Lh the strict lower triangle plus half the
+// diagonal): C5^T b_Q C5 = W_Q + W_Q^T with W_Q = C5^T Lh_Q C5.
+//   T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p]: one batched GEMM over Q whose row block
+//              n0 only reads rows m >= n0 of b, half of b;
+//   W^T      = C5^T T, one GEMM with N = nbf5 * naux (p, Q flattened).
+int transform1_sym(dn_sys* s) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
+  g.A = s->b; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf; g.a_kcontig = true;
+  g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
+  g.C = s->bpn; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
+  g.nbatch = d.nbfaux; g.bps = 1;
+  g.tri = kTriA;
+  DN_TRY(begin_timing(s));
+  DN_TRY(gemm_f64(s->stream, g));
+  return end_timing(s);
+}
+
+int transform2_sym(dn_sys* s) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf5; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
+  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
+  g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
+  g.C = s->bt; g.ldc = d.nbf5;
+  return gemm_f64(s->stream, g);
+}
+
 // out[:, :, Q] = b_pn[:, :, Q] * C[:, 0:ncols]  (integrals.jl:160 for ncols = nbf5; the
 // transpose of b_MO of utils.jl:19-20 for ncols = nbf).  Slice stride nbf5*ncols.
 int transform2(dn_sys* s, int ncols, double* out) {
@@ -196,6 +228,11 @@ int check_sys(dn_sys* s) {
 }
 
 int prepare_b(dn_sys* s) {
+  // b[m,n,Q] = b[n,m,Q] (to 2^-50 of max|b|) lets the J/K build read half of b
+  double dmax = 0.0, bmax = 0.0;
+  DN_TRY(b_symmetry(s->stream, s->b, s->d.nbf, s->d.nbfaux,
+                    reinterpret_cast<unsigned long long*>(s->scal + 62), &dmax, &bmax));
+  s->b_sym = dmax <= std::ldexp(bmax, -50);
   if (s->precision != DN_SPLIT_F16) return DN_OK;
   return split_prepare_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, s->ld8, s->bscale, s->binv,
                          s->bhi, s->blo);
@@ -242,8 +279,14 @@ int jkh_resident(dn_sys* s) {
   const dn_dims& d = s->d;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
   if (s->precision == DN_SPLIT_F16) return jkh_split(s);
-  DN_TRY(transform1(s));
-  DN_TRY(transform2(s, d.nbf5, s->bt));
+  if (s->b_sym && s->use_sym) {
+    DN_TRY(transform1_sym(s));
+    DN_TRY(transform2_sym(s));
+    DN_TRY(sym_inplace(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux));
+  } else {
+    DN_TRY(transform1(s));
+    DN_TRY(transform2(s, d.nbf5, s->bt));
+  }
   DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X, s->part,
                      s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
@@ -552,6 +595,19 @@ int dn_sys_get_jkh(dn_sys* s, double* J, double* K, double* Hcore) {
   if ((J && !all_finite(J, nn)) || (K && !all_finite(K, nn)) ||
       (Hcore && !all_finite(Hcore, s->d.nbf5)))
     return fail(DN_ERR_NONFINITE, "non-finite value in J/K/H_core");
+  return DN_OK;
+}
+
+int dn_sys_set_symmetry(dn_sys* s, int use) {
+  DN_TRY(check_sys(s));
+  s->use_sym = use != 0;
+  return DN_OK;
+}
+
+int dn_sys_b_symmetric(dn_sys* s, int* is_symmetric) {
+  DN_TRY(check_sys(s));
+  if (!is_symmetric) return fail(DN_ERR_ARG, "null output pointer");
+  *is_symmetric = s->b_sym ? 1 : 0;
   return DN_OK;
 }
 

@@ -77,7 +77,13 @@ __device__ __forceinline__ v2d pair_mask(v2d v, int gmn, int gk, int MN, int KK)
 
 // One output tile (mt, nt) of split z of a GEMM, by a workgroup of WGM x WGN waves; each
 // wave owns (BM/WGM) x (BN/WGN).  As/Bs are the workgroup's LDS double buffers.
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+// TRI = kTriB (K-contiguous B): B is symmetric, B(k, n) = B(n, k), and the tile
+// multiplies by its lower half Lh with B = Lh + Lh^T, i.e. Lh(k, n) = B(k, n) for k > n,
+// B(n, n)/2 for k = n and 0 for k < n; the K loop of column block n0 starts at k = n0.
+// TRI = kTriA (K-contiguous A): the same for A, A(m, k) masked by k > m / k == m, and
+// the K loop of row block m0 starts at k = m0.
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
 __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, double* As_,
                                           double* Bs_) {
   constexpr int NT = 64 * WGM * WGN;
@@ -86,6 +92,8 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
   using LA = TileLds<BM, BK, AK, NT>;
   using LB = TileLds<BN, BK, BKC, NT>;
   static_assert(FM >= 1 && FN >= 1 && WM % 16 == 0 && WN % 16 == 0, "wave tile");
+  static_assert(TRI != kTriB || BKC, "the symmetric-B mode needs a K-contiguous B");
+  static_assert(TRI != kTriA || AK, "the symmetric-A mode needs a K-contiguous A");
   static_assert(BM * BK % (2 * NT) == 0 && BN * BK % (2 * NT) == 0, "tile/threads");
   double (*As)[LA::SIZE] = reinterpret_cast<double (*)[LA::SIZE]>(As_);
   double (*Bs)[LB::SIZE] = reinterpret_cast<double (*)[LB::SIZE]>(Bs_);
@@ -98,8 +106,9 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
 
   const int r0 = z * g.bps;
   const int r1 = min(r0 + g.bps, g.nbatch);
-  const int nkt = (K + BK - 1) / BK;
-  const int T = (r1 - r0) * nkt;
+  const int kt0 = TRI == kTriB ? n0 / BK : (TRI == kTriA ? m0 / BK : 0);
+  const int nkt = (K + BK - 1) / BK - kt0;
+  const int T = nkt > 0 ? (r1 - r0) * nkt : 0;
 
   v4d acc[FM][FN];
 #pragma unroll
@@ -111,7 +120,7 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
 
   auto load_tile = [&](int t) {
     const int r = r0 + t / nkt;
-    const int k0 = (t % nkt) * BK;
+    const int k0 = (kt0 + t % nkt) * BK;
     const double* A = g.A + (long)r * g.sA;
     const double* B = g.B + (long)r * g.sB;
 #pragma unroll
@@ -128,12 +137,17 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
     }
   };
   auto store_tile = [&](int buf, int t) {
-    const int k0 = (t % nkt) * BK;
+    const int k0 = (kt0 + t % nkt) * BK;
 #pragma unroll
     for (int i = 0; i < LA::PAIRS; ++i) {
       int mn, k;
       LA::coord(tid + i * NT, &mn, &k);
       ra[i] = pair_mask<AK>(ra[i], m0 + mn, k0 + k, M, K);
+      if constexpr (TRI == kTriA) {
+        const int gm = m0 + mn, gk = k0 + k;  // pairs run along k
+        ra[i].x *= gk > gm ? 1.0 : (gk == gm ? 0.5 : 0.0);
+        ra[i].y *= gk + 1 > gm ? 1.0 : (gk + 1 == gm ? 0.5 : 0.0);
+      }
       if (AK) {
         As[buf][LA::off(mn, k)] = ra[i].x;
         As[buf][LA::off(mn, k + 1)] = ra[i].y;
@@ -146,6 +160,11 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
       int mn, k;
       LB::coord(tid + i * NT, &mn, &k);
       rb[i] = pair_mask<BKC>(rb[i], n0 + mn, k0 + k, N, K);
+      if constexpr (TRI == kTriB) {
+        const int gn = n0 + mn, gk = k0 + k;  // pairs run along k
+        rb[i].x *= gk > gn ? 1.0 : (gk == gn ? 0.5 : 0.0);
+        rb[i].y *= gk + 1 > gn ? 1.0 : (gk + 1 == gn ? 0.5 : 0.0);
+      }
       if (BKC) {
         Bs[buf][LB::off(mn, k)] = rb[i].x;
         Bs[buf][LB::off(mn, k + 1)] = rb[i].y;
@@ -209,7 +228,8 @@ struct GemmLds {
   static constexpr int B = TileLds<BN, BK, BKC, 64 * WGM * WGN>::SIZE;
 };
 
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
 __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
                                                                    int tiles_n, int nsplit) {
   using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
@@ -222,21 +242,30 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
   const int mt = lid % tiles_m;
   const int nt = (lid / tiles_m) % tiles_n;
   const int z = lid / (tiles_m * tiles_n);
-  gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>(g, mt, nt, z, As, Bs);
+  gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(g, mt, nt, z, As, Bs);
 }
 
 // Persistent grouped GEMM: the tiles of all problems form one list (problem i owns tiles
 // [desc[i].tile_begin, desc[i].tile_begin + tiles_m*tiles_n*nsplit)); workgroup b takes
-// tiles b, b + gridDim.x, ... until the list is exhausted (a static schedule, so every
-// workgroup's loop ends).
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+// list positions b, b + gridDim.x, ... until the list is exhausted (a static schedule, so
+// every workgroup's loop ends).  With a grid that is a multiple of the 8 XCDs, workgroup
+// b runs on XCD b % 8, and position t maps to tile (t % 8) * per + t / 8: each XCD works
+// through one contiguous eighth of the list, so the M-tiles that share a B block (and
+// consecutive B blocks) meet in the same L2.
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
 __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const GroupedGemm* desc,
                                                                       int nprob,
                                                                       long total_tiles) {
   using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
   __shared__ double As[2 * L::A];
   __shared__ double Bs[2 * L::B];
-  for (long t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+  const bool chunk = (gridDim.x % 8u) == 0;
+  const long per = (total_tiles + 7) / 8;
+  const long span = chunk ? 8 * per : total_tiles;
+  for (long pos = blockIdx.x; pos < span; pos += gridDim.x) {
+    const long t = chunk ? (pos % 8) * per + pos / 8 : pos;
+    if (t >= total_tiles) continue;
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -248,7 +277,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const Gro
     const int mt = local % d.tiles_m;
     const int nt = (local / d.tiles_m) % d.tiles_n;
     const int z = local / ((long)d.tiles_m * d.tiles_n);
-    gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>(d.g, mt, nt, z, As, Bs);
+    gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(d.g, mt, nt, z, As, Bs);
     __syncthreads();  // the next tile reuses the LDS buffers
   }
 }
@@ -268,27 +297,28 @@ __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int 
   }
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
 int launch(hipStream_t s, const Gemm& g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int nsplit = (g.nbatch + g.bps - 1) / g.bps;
   const long total = (long)tm * tn * nsplit;
   if (total <= 0) return DN_OK;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "gemm grid too large");
-  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>),
+  hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>),
                      dim3((unsigned)total), dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, int TRI = 0>
 int dispatch_vec(hipStream_t s, const Gemm& g) {
   if (g.vec_a) {
-    return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, true, true>(s, g)
-                   : launch<BM, BN, BK, WGM, WGN, AK, BKC, true, false>(s, g);
+    return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, true, true, TRI>(s, g)
+                   : launch<BM, BN, BK, WGM, WGN, AK, BKC, true, false, TRI>(s, g);
   }
-  return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, false, true>(s, g)
-                 : launch<BM, BN, BK, WGM, WGN, AK, BKC, false, false>(s, g);
+  return g.vec_b ? launch<BM, BN, BK, WGM, WGN, AK, BKC, false, true, TRI>(s, g)
+                 : launch<BM, BN, BK, WGM, WGN, AK, BKC, false, false, TRI>(s, g);
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN>
@@ -315,9 +345,24 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   };
   g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
   g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
-  // Tall-M operands get the 128-row tile (the RI transforms have M = nbf5 <= 128 or
-  // a multiple of it); everything else uses 64 x 64.
-  if (g.M > 64 && g.N >= 128) return dispatch_layout<128, 128, 16, 2, 4>(s, g);
+  if (g.tri == kTriA) {
+    if (!g.a_kcontig || !g.b_kcontig || g.M != g.K)
+      return fail(DN_ERR_ARG, "gemm_f64: symmetric-A mode needs K-contiguous A and B, M == K");
+    // 64-row blocks (the K loop of row block m0 covers k >= m0 only): the short, uneven
+    // K loops balance best over small workgroups (measured on MI355X at nbf = 400, 800)
+    return dispatch_vec<64, 128, 16, 2, 4, true, true, kTriA>(s, g);
+  }
+  if (g.tri) {
+    if (!g.a_kcontig || !g.b_kcontig || g.N != g.K)
+      return fail(DN_ERR_ARG, "gemm_f64: symmetric-B mode needs K-contiguous A and B, N == K");
+    // 64 x 64 tiles: the K loop of column block n0 covers k >= n0 only, and the short,
+    // uneven K loops balance best over many small workgroups (measured on MI355X at
+    // nbf = 400 and 800 against 128 x 64 and 128 x 128 tiles)
+    return dispatch_vec<64, 64, 16, 2, 2, true, true, kTriB>(s, g);
+  }
+  // Operands larger than 64 in both M and N get the 128 x 128 tile (the RI transforms
+  // have M = nbf5, N = nbf5 or nbf); everything else uses 64 x 64.
+  if (g.M > 64 && g.N > 64) return dispatch_layout<128, 128, 16, 2, 4>(s, g);
   return dispatch_layout<64, 64, 16, 2, 2>(s, g);
 }
 
@@ -332,28 +377,50 @@ void config_tile(int config, int* bm, int* bn) {
   switch (config) {
     case kTile64x64: *bm = 64; *bn = 64; break;
     case kTile64x128: *bm = 64; *bn = 128; break;
+    case kTile128x64: *bm = 128; *bn = 64; break;
     default: *bm = 128; *bn = 128; break;
   }
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB>
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
 int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, int grid) {
-  hipLaunchKernelGGL((dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB>), dim3(grid),
-                     dim3(64 * WGM * WGN), 0, s, d, n, total);
+  auto kern = dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>;
+  if (grid <= 0) {  // as many workgroups as stay resident, a multiple of the 8 XCDs
+    int dev = 0, cus = 0, occ = 0;
+    DN_HIP(hipGetDevice(&dev));
+    DN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    DN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WGM * WGN, 0));
+    const long full = (long)std::max(occ, 1) * cus;
+    grid = (int)std::min(full, (total + 7) / 8 * 8);
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WGM * WGN), 0, s, d, n, total);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN>
-int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, const GroupedGemm* d, int n,
-                   long total, int grid) {
+int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, int tri,
+                   const GroupedGemm* d, int n, long total, int grid) {
 #define DN_G(AKv, BKv, VAv, VBv) \
   launch_grouped<BM, BN, BK, WGM, WGN, AKv, BKv, VAv, VBv>(s, d, n, total, grid)
-  if (ak && bk) return va ? (vb ? DN_G(true, true, true, true) : DN_G(true, true, true, false))
-                          : (vb ? DN_G(true, true, false, true) : DN_G(true, true, false, false));
-  if (!ak && bk) return va ? (vb ? DN_G(false, true, true, true) : DN_G(false, true, true, false))
-                           : (vb ? DN_G(false, true, false, true) : DN_G(false, true, false, false));
-  return fail(DN_ERR_ARG, "grouped GEMM supports a_kcontig in {0,1} with b_kcontig = 1");
+#define DN_GT(VAv, VBv, T) \
+  launch_grouped<BM, BN, BK, WGM, WGN, true, true, VAv, VBv, T>(s, d, n, total, grid)
+  if (tri) {
+    if (!ak || !bk) return fail(DN_ERR_ARG, "symmetric-operand grouped GEMM needs K-contiguous A, B");
+    if (tri == kTriA)
+      return va ? (vb ? DN_GT(true, true, kTriA) : DN_GT(true, false, kTriA))
+                : (vb ? DN_GT(false, true, kTriA) : DN_GT(false, false, kTriA));
+    return va ? (vb ? DN_GT(true, true, kTriB) : DN_GT(true, false, kTriB))
+              : (vb ? DN_GT(false, true, kTriB) : DN_GT(false, false, kTriB));
+  }
+#undef DN_GT
+#define DN_GV(AKv, BKv) \
+  (va ? (vb ? DN_G(AKv, BKv, true, true) : DN_G(AKv, BKv, true, false)) \
+      : (vb ? DN_G(AKv, BKv, false, true) : DN_G(AKv, BKv, false, false)))
+  if (ak) return bk ? DN_GV(true, true) : DN_GV(true, false);
+  return bk ? DN_GV(false, true) : DN_GV(false, false);
+#undef DN_GV
 #undef DN_G
 }
 
@@ -380,17 +447,19 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
                      long total, int config, int grid) {
   if (n <= 0 || total <= 0) return DN_OK;
   const bool ak = dh[0].g.a_kcontig, bk = dh[0].g.b_kcontig;
+  const int tri = dh[0].g.tri;
   bool va = true, vb = true;
   for (int i = 0; i < n; ++i) {
-    if (dh[i].g.a_kcontig != ak || dh[i].g.b_kcontig != bk)
+    if (dh[i].g.a_kcontig != ak || dh[i].g.b_kcontig != bk || dh[i].g.tri != tri)
       return fail(DN_ERR_ARG, "grouped GEMM problems must share operand layouts");
     va = va && dh[i].g.vec_a;
     vb = vb && dh[i].g.vec_b;
   }
   switch (config) {
-    case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, dd, n, total, grid);
-    case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, dd, n, total, grid);
-    default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, dd, n, total, grid);
+    case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+    case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+    case kTile128x64: return grouped_layout<128, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+    default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
   }
 }
 

@@ -49,7 +49,12 @@ struct Gemm {
   int nbatch = 1;
   int bps = 1;  // batches per split
   bool vec_a = false, vec_b = false;  // set by gemm_f64: 16-byte loads allowed
+  // kTriB: symmetric B (B(k, n) = B(n, k), N == K), multiply by its lower half Lh with
+  // B = Lh + Lh^T (k > n as is, k == n halved), skipping the K range above each column
+  // block.  kTriA: the same for a symmetric A (M == K), masked by k > m.
+  int tri = 0;
 };
+enum { kTriNone = 0, kTriB = 1, kTriA = 2 };
 
 int gemm_f64(hipStream_t s, const Gemm& g);
 
@@ -60,7 +65,13 @@ struct GroupedGemm {
   long tile_begin = 0;
 };
 // Tile configurations of the grouped launches (chosen by the autotuner).
-enum GemmConfig { kTile128x128 = 0, kTile64x64 = 1, kTile64x128 = 2, kNumGemmConfigs = 3 };
+enum GemmConfig {
+  kTile128x128 = 0,
+  kTile64x64 = 1,
+  kTile64x128 = 2,
+  kTile128x64 = 3,
+  kNumGemmConfigs = 4
+};
 // Fill tile counts/offsets of desc[0..n) for a config; returns the total tile count.
 long gemm_grouped_plan(GroupedGemm* desc, int n, int config);
 // One persistent launch over all problems (descriptors already on the device).  All
@@ -73,6 +84,16 @@ int reduce_splits(hipStream_t s, const double* P, long sP, int nsplit, int M, in
 // Single GEMM split along K into partials (part: >= 16*M*N doubles) and reduced; for
 // small outputs whose tile count alone cannot fill the GPU.
 int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems);
+
+// In-place b_pq = W + W^T over the Q slices of several systems in one launch.
+struct SymDesc {
+  double* bpq;  // nbf5 x nbf5 x naux
+  int nbf5 = 0, naux = 0;
+  int q_per_block = 1;
+  long blk_begin = 0;
+};
+long sym_grouped_plan(SymDesc* desc, int n);  // fills q_per_block/blk_begin; returns blocks
+int sym_grouped(hipStream_t s, const SymDesc* dev, int n, long blocks);
 
 // ---- device buffers -----------------------------------------------------------------
 struct DevBuf {
@@ -165,6 +186,8 @@ struct dn_sys {
   float* qs = nullptr;      // naux
   float* qinv = nullptr;    // naux
   bool b_ready = false;
+  bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
+  bool use_sym = true;   // let the J/K build use that symmetry
   // timing of the dominant transform kernel: one event pair per launch, read back
   // (and recycled) by dn_sys_get_timing so that timing adds no synchronisation.
   bool timing = false;

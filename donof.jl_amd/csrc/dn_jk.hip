@@ -1,5 +1,7 @@
 // J/K/H_core epilogues of the RI J/K-MO build (integrals.jl:144-172) and the device
 // generator of the synthetic three-index tensor.
+#include <cstring>
+
 #include "dn_internal.h"
 
 namespace dn {
@@ -79,6 +81,149 @@ int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int n
                     double* J, double* K) {
   return jk_from_bpq_t<float>(s, bpq, slice, nbf5, naux, X, part, part_elems, part2,
                               part2_elems, J, K);
+}
+
+// b_pq = W + W^T in place, slice by slice (W = C5^T Lh C5 of the symmetric-b transform):
+// workgroup (pair of 32 x 32 tiles (I, J), I <= J; range of Q) stages both tiles in LDS
+// and writes both back, so every element is read and written once, coalesced.
+__device__ __forceinline__ void sym_pair(double* __restrict__ bpq, long slice, int n5, int I,
+                                         int J, int Q0, int Q1, double (*a)[33],
+                                         double (*b)[33]) {
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int pI = I * 32 + tx, pJ = J * 32 + tx;
+  for (int Q = Q0; Q < Q1; ++Q) {
+    double* W = bpq + (long)Q * slice;
+    for (int r = ty; r < 32; r += 8) {
+      const int qJ = J * 32 + r, qI = I * 32 + r;
+      a[tx][r] = (pI < n5 && qJ < n5) ? W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
+      b[tx][r] = (pJ < n5 && qI < n5) ? W[pJ + (long)qI * n5] : 0.0;  // tile (J, I)
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      const int qJ = J * 32 + r, qI = I * 32 + r;
+      if (pI < n5 && qJ < n5) W[pI + (long)qJ * n5] = a[tx][r] + b[r][tx];
+      if (I != J && pJ < n5 && qI < n5) W[pJ + (long)qI * n5] = b[tx][r] + a[r][tx];
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void pair_of(int pidx, int ntile, int* I, int* J) {
+  int i = 0;
+  while (pidx >= ntile - i) {
+    pidx -= ntile - i;
+    ++i;
+  }
+  *I = i;
+  *J = i + pidx;
+}
+
+__global__ __launch_bounds__(256) void sym_inplace_kernel(double* __restrict__ bpq, long slice,
+                                                          int n5, int naux, int ntile,
+                                                          int q_per_block) {
+  __shared__ double a[32][33], b[32][33];
+  int I, J;
+  pair_of(blockIdx.x, ntile, &I, &J);
+  const int Q0 = blockIdx.y * q_per_block;
+  sym_pair(bpq, slice, n5, I, J, Q0, min(Q0 + q_per_block, naux), a, b);
+}
+
+// Grouped form: workgroup g belongs to the system whose block range holds it.
+__global__ __launch_bounds__(256) void sym_grouped_kernel(const SymDesc* desc, int n) {
+  __shared__ double a[32][33], b[32][33];
+  const long g = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (desc[mid].blk_begin <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const SymDesc& d = desc[lo];
+  const long local = g - d.blk_begin;
+  const int ntile = (d.nbf5 + 31) / 32;
+  const int pairs = ntile * (ntile + 1) / 2;
+  int I, J;
+  pair_of((int)(local % pairs), ntile, &I, &J);
+  const int Q0 = (int)(local / pairs) * d.q_per_block;
+  sym_pair(d.bpq, (long)d.nbf5 * d.nbf5, d.nbf5, I, J, Q0, min(Q0 + d.q_per_block, d.naux), a,
+           b);
+}
+
+long sym_grouped_plan(SymDesc* desc, int n) {
+  long begin = 0;
+  for (int i = 0; i < n; ++i) {
+    const int ntile = (desc[i].nbf5 + 31) / 32;
+    const int pairs = ntile * (ntile + 1) / 2;
+    desc[i].q_per_block = (int)std::max<long>(1, (long)desc[i].naux * pairs / 256);
+    desc[i].blk_begin = begin;
+    begin += (long)pairs * ((desc[i].naux + desc[i].q_per_block - 1) / desc[i].q_per_block);
+  }
+  return begin;
+}
+
+int sym_grouped(hipStream_t s, const SymDesc* dev, int n, long blocks) {
+  if (n <= 0 || blocks <= 0) return DN_OK;
+  hipLaunchKernelGGL(sym_grouped_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int sym_inplace(hipStream_t s, double* bpq, long slice, int nbf5, int naux) {
+  const int ntile = (nbf5 + 31) / 32;
+  const int pairs = ntile * (ntile + 1) / 2;
+  int qpb = (int)std::max<long>(1, (long)naux * pairs / 4096);
+  const int nq = (naux + qpb - 1) / qpb;
+  hipLaunchKernelGGL(sym_inplace_kernel, dim3(pairs, nq), dim3(256), 0, s, bpq, slice, nbf5, naux,
+                     ntile, qpb);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+// out[0] = max |b[m,n,Q] - b[n,m,Q]|, out[1] = max |b| (as the bit patterns of
+// non-negative doubles, which order like the values).
+__global__ __launch_bounds__(256) void sym_check_kernel(const double* __restrict__ b, int nbf,
+                                                        long total,
+                                                        unsigned long long* __restrict__ out) {
+  const long nn = (long)nbf * nbf;
+  double dmax = 0.0, bmax = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const long Q = e / nn;
+    const long r = e - Q * nn;
+    const int m = (int)(r % nbf), n = (int)(r / nbf);
+    const double v = b[e];
+    dmax = fmax(dmax, fabs(v - b[Q * nn + n + (long)m * nbf]));
+    bmax = fmax(bmax, fabs(v));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_down(dmax, off, 64));
+    bmax = fmax(bmax, __shfl_down(bmax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(dmax));
+    atomicMax(out + 1, (unsigned long long)__double_as_longlong(bmax));
+  }
+}
+
+namespace {
+__global__ void clear_u64_kernel(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+}  // namespace
+
+int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
+               double* dmax, double* bmax) {
+  hipLaunchKernelGGL(clear_u64_kernel, dim3(1), dim3(64), 0, s, dev2, 2);
+  const long total = (long)nbf * nbf * naux;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
+  hipLaunchKernelGGL(sym_check_kernel, dim3(blocks), dim3(256), 0, s, b, nbf, total, dev2);
+  DN_HIP(hipGetLastError());
+  unsigned long long h[2] = {0, 0};
+  DN_HIP(hipMemcpyAsync(h, dev2, sizeof(h), hipMemcpyDeviceToHost, s));
+  DN_HIP(hipStreamSynchronize(s));
+  std::memcpy(dmax, &h[0], sizeof(double));
+  std::memcpy(bmax, &h[1], sizeof(double));
+  return DN_OK;
 }
 
 // H_core[i] = sum_m C5[m,i] (H C5)[m,i]   (integrals.jl:156,169)

@@ -23,7 +23,7 @@ import math
 
 import numpy as np
 
-STREAM_B, STREAM_C, STREAM_H, STREAM_OCC, STREAM_METRIC = 0, 1, 2, 3, 4
+STREAM_B, STREAM_C, STREAM_H, STREAM_OCC, STREAM_METRIC, STREAM_SHAPE = 0, 1, 2, 3, 4, 5
 B_DECAY = 400.0
 
 _M0 = np.uint64(0xD2511F53)
@@ -176,6 +176,22 @@ def synthetic_param(nbf, naux, nbf5, ndoc=None, ipnof=8, orb_method="ADAM",
     p.orb_method = orb_method
     p.occ_method = occ_method
     return p
+
+
+def grouped_shapes(nsys=64, seed=2024, nbf_min=48, nbf_max=400):
+    """Sizes of a grouped batch: nbf uniform on the integers [nbf_min, nbf_max] (seeded
+    Philox), naux = 4 nbf, nbf5 = ceil(nbf / 3).  Returns a list of (nbf, naux, nbf5)."""
+    u = uniform(np.arange(nsys, dtype=np.uint64), STREAM_SHAPE, seed)
+    nbf = nbf_min + np.minimum((u * (nbf_max - nbf_min + 1)).astype(np.int64),
+                               nbf_max - nbf_min)
+    return [(int(n), 4 * int(n), -(-int(n) // 3)) for n in nbf]
+
+
+def build_flops(nbf, naux, nbf5):
+    """Algorithmic FLOP of one J/K-MO build in the reference's contraction order
+    (integrals.jl:156-169): both transform halves, J, K and H_core."""
+    return (2.0 * nbf5 * nbf * nbf * naux + 2.0 * nbf5 * nbf5 * nbf * naux
+            + 4.0 * nbf5 * nbf5 * naux + 2.0 * nbf * nbf * nbf5 + 2.0 * nbf * nbf5)
 
 
 class SyntheticSystem:

@@ -94,6 +94,12 @@ int dn_sys_generate_b(dn_sys* sys, uint64_t seed);
  * not positive definite returns DN_ERR_ARG.  Host arrays are not modified; a transient
  * device workspace of the size of b is taken for the call. */
 int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
+/* Whenever b is set, the library checks b[m,n,Q] == b[n,m,Q] (to 2^-50 of max|b|; the
+ * RI tensor is symmetric in m, n, integrals.jl:86-88).  A symmetric b lets the FP64
+ * J/K build contract only its lower half: C5^T b_Q C5 = W + W^T with W = C5^T Lh_Q C5,
+ * b_Q = Lh_Q + Lh_Q^T.  use = 0 forces the full contraction of integrals.jl:159-160. */
+int dn_sys_set_symmetry(dn_sys* sys, int use);
+int dn_sys_b_symmetric(dn_sys* sys, int* is_symmetric);
 
 /* J_MO, K_MO (nbf5 x nbf5) and H_core (nbf5) from C (nbf x nbf) and H (nbf x nbf):
  * replaces computeJKH_MO / JKH_MO (integrals.jl:131-172; ext/DoNOFGPUExt.jl:52-94). */
@@ -180,6 +186,27 @@ int dn_orbopt(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st, const do
 int dn_sys_set_gamma(dn_sys* sys, const double* gamma);
 int dn_sys_get_orbitals(dn_sys* sys, double* C);
 int dn_orbopt_resident(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st);
+
+/* ---- grouped builds --------------------------------------------------------------
+ * Many resident systems of uneven size built in one sequence of grouped, persistent
+ * launches, captured once into a HIP graph.  Each system's build is computeJKH_MO /
+ * JKH_MO (integrals.jl:131-172) on its resident b, C (dn_sys_set_orbitals) and H
+ * (dn_sys_set_hcore); the reference has no grouped form, it builds one system per call.
+ * Results stay in each system (read them with dn_sys_get_jkh).  The systems must outlive
+ * the group and share its context and precision. */
+typedef struct dn_group dn_group;
+int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out);
+int dn_group_destroy(dn_group* group);
+/* Times every tile configuration of each grouped GEMM stage (K1, K2, J, H C5) on this
+ * group, keeps the fastest, and captures the build graph.  configs (4 ints, may be NULL)
+ * receives the choices. */
+int dn_group_tune(dn_group* group, int* configs);
+/* One J/K/H_core build of every system (a graph replay; asynchronous). */
+int dn_group_jkh(dn_group* group);
+int dn_group_sync(dn_group* group);
+int dn_group_set_timing(dn_group* group, int on);
+/* Summed HIP-event time of the builds since the last call, and their count. */
+int dn_group_get_timing(dn_group* group, double* total_ms, long* builds);
 
 #ifdef __cplusplus
 }

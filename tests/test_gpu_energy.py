@@ -1,7 +1,11 @@
 """The outer energy() loop (energy.jl:140-220) on the device against the oracle's loop on
-the same inputs.  The occupation optimisers differ (the reference's Optim.jl CG is not
-available; the oracle uses scipy's CG, the package its own Hager-Zhang CG), so both are
-driven to a tight gradient tolerance and the per-iteration energies must agree."""
+the same inputs.  With frozen occupations every step is a deterministic recurrence, so
+the per-iteration energies must agree to roundoff.  With the occupation optimiser on,
+the two CG implementations differ (the reference's Optim.jl CG is not available; the
+oracle uses scipy's CG, the package its own Hager-Zhang CG): both are driven to a tight
+gradient tolerance and must reach the same stationary occupations after the first
+iteration; later iterations are not compared, since the optimisers' discrete decisions
+(best energy, step-size cuts) can flip on roundoff (parity unpinned there)."""
 import numpy as np
 import pytest
 
@@ -10,24 +14,42 @@ from donof_amd import api, energy, synthetic
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("orb_method", ["ADAM", "ADABelief"])
-def test_energy_outer_iterations_match_oracle(oracle, orb_method):
+def _h2o_sized(orb_method, seed=11):
     nbf, naux, nbf5 = 24, 60, 20       # the H2O / cc-pVDZ sizes: ndoc = 5, ncwo = 3
-    s = synthetic.SyntheticSystem(nbf, naux, nbf5, 11, ndoc=5, orb_method=orb_method)
+    s = synthetic.SyntheticSystem(nbf, naux, nbf5, seed, ndoc=5, orb_method=orb_method)
+    b = synthetic.make_b(nbf, naux, seed)
+    return s, b
+
+
+@pytest.mark.parametrize("orb_method", ["ADAM", "ADABelief", "YOGI", "Demon"])
+def test_energy_frozen_occupations_match_oracle(oracle, orb_method):
+    s, b = _h2o_sized(orb_method)
     p = s.p
-    p.maxit, p.threshen = 3, 1e-9
-    p.threshgorb, p.threshgocc = 1e-12, 1e-12      # run all maxit iterations
-    b = synthetic.make_b(nbf, naux, 11)
+    p.maxit, p.threshgorb, p.threshgocc = 3, 1e-12, 1e-12      # run all maxit iterations
     dev = api.System(p).upload_b(b)
-    pr = oracle.make_param(nbf, naux, 10, ncwo=3, ipnof=8, orb_method=orb_method)
-    pr.maxit, pr.threshen, pr.threshgorb, pr.threshgocc = 3, 1e-9, 1e-12, 1e-12
-    Et, C, n, elag, hist = energy.energy(dev, s.H, s.C, p, printmode=False)
-    Er, Cr, nr, histr = oracle.energy_loop(b, s.H, s.C, pr)
+    pr = oracle.make_param(24, 60, 10, ncwo=3, ipnof=8, orb_method=orb_method)
+    pr.maxit, pr.threshgorb, pr.threshgocc = 3, 1e-12, 1e-12
+    Et, C, n, elag, hist = energy.energy(dev, s.H, s.C, p, n=s.n, freeze_occ=True,
+                                         printmode=False)
+    Er, Cr, nr, histr = oracle.energy_loop(b, s.H, s.C, pr, n=s.n, freeze_occ=True)
     assert len(hist) == len(histr) == 3
     for h, hr in zip(hist, histr):
-        assert abs(h["E"] - hr[0]) < 1e-7
+        assert abs(h["E"] - hr[0]) < 1e-8
+    assert np.abs(C - Cr).max() < 1e-7
+    dev.close()
+
+
+def test_energy_first_iteration_matches_oracle(oracle):
+    s, b = _h2o_sized("ADAM")
+    p = s.p
+    p.maxit, p.threshen, p.threshgorb, p.threshgocc = 1, 1e-10, 1e-12, 1e-12
+    dev = api.System(p).upload_b(b)
+    pr = oracle.make_param(24, 60, 10, ncwo=3, ipnof=8, orb_method="ADAM")
+    pr.maxit, pr.threshen, pr.threshgorb, pr.threshgocc = 1, 1e-10, 1e-12, 1e-12
+    Et, C, n, elag, hist = energy.energy(dev, s.H, s.C, p, printmode=False)
+    Er, Cr, nr, histr = oracle.energy_loop(b, s.H, s.C, pr)
+    assert abs(hist[0]["E"] - histr[0][0]) < 1e-9
     assert np.abs(n - nr).max() < 1e-6
-    assert abs(Et - Er) < 1e-7
     dev.close()
 
 

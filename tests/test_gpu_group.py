@@ -1,0 +1,83 @@
+"""Grouped J/K/H_core builds (dn_group_*): every system of a group of uneven sizes gets
+its own computeJKH_MO (integrals.jl:131-172) to the FP64 bar, on every graph replay,
+with new orbitals set between replays."""
+import numpy as np
+import pytest
+
+from donof_amd import api, synthetic
+from donof_amd._lib import DnError
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _group(oracle_needed_shapes, seed=7):
+    systems, inputs = [], []
+    for i, (nbf, naux, nbf5) in enumerate(oracle_needed_shapes):
+        p = synthetic.synthetic_param(nbf, naux, nbf5)
+        b = synthetic.make_b(nbf, naux, seed + i)
+        s = api.System(p).upload_b(b)
+        C = synthetic.make_C(nbf, 100 + i)
+        H = synthetic.make_H(nbf, 200 + i)
+        s.set_orbitals(C)
+        s.set_hcore(H)
+        systems.append(s)
+        inputs.append((b, C, H, nbf5))
+    return systems, inputs
+
+
+SHAPES = [(20, 80, 7), (48, 192, 16), (33, 132, 11), (70, 280, 24), (9, 36, 3), (130, 520, 44)]
+
+
+def test_group_matches_oracle_on_every_replay(oracle):
+    systems, inputs = _group(SHAPES)
+    g = api.Group(systems)
+    cfg = g.tune()
+    print("configs", cfg, flush=True)
+    assert set(cfg) == {"K1", "K2", "J", "HC"}
+    for rep in range(3):
+        if rep == 2:   # new orbitals: the replayed graph must read them
+            for i, s in enumerate(systems):
+                C = synthetic.make_C(s.nbf, 300 + i)
+                s.set_orbitals(C)
+                inputs[i] = (inputs[i][0], C, inputs[i][2], inputs[i][3])
+        g.jkh()
+        g.sync()
+        for s, (b, C, H, nbf5) in zip(systems, inputs):
+            J, K, Hc = s.get_jkh()
+            Jr, Kr, Hr = oracle.jkh_mo(C, H, b, nbf5)
+            assert np.abs(J - Jr).max() <= TOL, (rep, s.nbf)
+            assert np.abs(K - Kr).max() <= TOL, (rep, s.nbf)
+            assert np.abs(Hc - Hr).max() <= TOL, (rep, s.nbf)
+        # eager single-system work between replays
+        systems[0].jkh_mo_resident()
+        systems[0].sync()
+    g.close()
+    for s in systems:
+        s.close()
+
+
+def test_group_untuned_replay_and_timing(oracle):
+    systems, inputs = _group(SHAPES[:3], seed=30)
+    g = api.Group(systems)
+    g.set_timing(True)
+    g.jkh()
+    g.jkh()
+    ms, n = g.get_timing()
+    assert n == 2 and ms > 0
+    for s, (b, C, H, nbf5) in zip(systems, inputs):
+        J, K, Hc = s.get_jkh()
+        Jr, Kr, Hr = oracle.jkh_mo(C, H, b, nbf5)
+        assert np.abs(J - Jr).max() <= TOL and np.abs(K - Kr).max() <= TOL
+    g.close()
+    for s in systems:
+        s.close()
+
+
+def test_group_requires_ready_systems():
+    p = synthetic.synthetic_param(20, 80, 7)
+    s = api.System(p)                       # b never uploaded
+    with pytest.raises(DnError, match="not been uploaded"):
+        api.Group([s])
+    s.close()

@@ -38,6 +38,40 @@ def test_jkh_mo_parity(oracle, nbf, naux, nbf5, ndoc):
 
 
 @pytest.mark.parametrize("nbf,naux,nbf5,ndoc", SHAPES)
+def test_jkh_mo_full_contraction_parity(oracle, nbf, naux, nbf5, ndoc):
+    """The same build with the (m, n) symmetry of b switched off (integrals.jl:159-160
+    as written)."""
+    print("case jkh full", nbf, naux, nbf5, flush=True)
+    s, b = _system(nbf, naux, nbf5, 100 + nbf, ndoc)
+    dev = api.System(s.p)
+    dev.upload_b(b)
+    assert dev.b_symmetric
+    dev.set_symmetry(False)
+    J, K, Hc = api.computeJKH_MO(s.C, s.H, dev, s.p)
+    Jr, Kr, Hr = oracle.jkh_mo(s.C, s.H, b, nbf5)
+    assert np.abs(J - Jr).max() <= TOL
+    assert np.abs(K - Kr).max() <= TOL
+    dev.set_symmetry(True)
+    J2, K2, _ = api.computeJKH_MO(s.C, s.H, dev, s.p)
+    assert np.abs(J2 - J).max() <= TOL and np.abs(K2 - K).max() <= TOL
+    dev.close()
+
+
+def test_nonsymmetric_b_uses_full_contraction(oracle):
+    nbf, naux, nbf5 = 37, 45, 9
+    s, b = _system(nbf, naux, nbf5, 5, 3)
+    b = b + 1e-3 * np.random.default_rng(3).standard_normal(b.shape)   # b[m,n] != b[n,m]
+    dev = api.System(s.p)
+    dev.upload_b(b)
+    assert not dev.b_symmetric
+    J, K, Hc = api.computeJKH_MO(s.C, s.H, dev, s.p)
+    Jr, Kr, Hr = oracle.jkh_mo(s.C, s.H, b, nbf5)
+    assert np.abs(J - Jr).max() <= TOL
+    assert np.abs(K - Kr).max() <= TOL
+    dev.close()
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5,ndoc", SHAPES)
 def test_lagrange2_parity(oracle, nbf, naux, nbf5, ndoc):
     print("case lag", nbf, naux, nbf5, flush=True)
     s, b = _system(nbf, naux, nbf5, 200 + nbf, ndoc)

@@ -81,3 +81,13 @@ def test_conjugate_gradient_quadratic():
                                   np.zeros(12), g_abstol=1e-12)
     assert res.g_converged
     assert np.abs(res.x - np.linalg.solve(A, bvec)).max() < 1e-10
+
+
+def test_grouped_shapes():
+    from donof_amd import synthetic
+    sh = synthetic.grouped_shapes(64, seed=2024)
+    assert len(sh) == 64
+    assert all(48 <= n <= 400 and a == 4 * n and n5 == -(-n // 3) for n, a, n5 in sh)
+    assert len({n for n, _, _ in sh}) > 40                     # spread over the range
+    assert sh == synthetic.grouped_shapes(64, seed=2024)       # seeded
+    assert synthetic.build_flops(400, 1600, 120) == pytest.approx(80.0e9, rel=1e-3)
