@@ -163,6 +163,7 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
     t0 = time.perf_counter()
     cfg = g.tune()
     tune_s = time.perf_counter() - t0
+    plan = g.plan()
     for _ in range(warmup):
         g.jkh()
     g.sync()
@@ -190,7 +191,7 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
                        "b_gb": sum(8.0 * n * n * a for n, a, _ in shapes) / 1e9},
             "fp64": {"builds_per_s": nsys * 1e3 / ms, "ms_per_build_of_all": ms,
                      "graph_event_ms": ev_ms / max(n, 1), "tflops": tf,
-                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "tiles": cfg,
+                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "tiles": cfg, "plan": plan,
                      "autotune_s": tune_s,
                      "per_system_streams_ms": seq_ms, "speedup_vs_per_system": seq_ms / ms,
                      "max_rel_diff_vs_per_system": err}}

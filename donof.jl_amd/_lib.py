@@ -87,6 +87,7 @@ _SIGS = {
     "dn_group_destroy": (_I, [_P]),
     "dn_group_tune": (_I, [_P, ct.POINTER(ct.c_int)]),
     "dn_group_jkh": (_I, [_P]),
+    "dn_group_plan_text": (_I, [_P, ct.c_char_p, _I]),
     "dn_group_sync": (_I, [_P]),
     "dn_group_set_timing": (_I, [_P, _I]),
     "dn_group_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),

@@ -189,7 +189,7 @@ class Group:
     computeJKH_MO (integrals.jl:131-172), read with System.get_jkh."""
 
     STAGES = ("K1", "K2", "J", "HC")
-    CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128"}
+    CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128", 3: "128x64", 4: "64x32", 5: "32x64"}
 
     def __init__(self, systems, ctx=None):
         self.systems = list(systems)
@@ -210,6 +210,12 @@ class Group:
         check(self.lib.dn_group_tune(self.handle, cfg))
         self.configs = {st: self.CONFIGS[c] for st, c in zip(self.STAGES, cfg)}
         return self.configs
+
+    def plan(self):
+        """Per stage: each class's system count, tile shape and GFLOP."""
+        buf = ct.create_string_buffer(4096)
+        check(self.lib.dn_group_plan_text(self.handle, buf, len(buf)))
+        return buf.value.decode()
 
     def jkh(self):
         check(self.lib.dn_group_jkh(self.handle))

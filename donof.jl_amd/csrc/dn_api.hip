@@ -15,7 +15,6 @@ int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
                double* Hcore);
 int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
-int sym_inplace(hipStream_t s, double* bpq, long slice, int nbf5, int naux);
 int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
                double* dmax, double* bmax);
 int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
@@ -282,11 +281,14 @@ int jkh_resident(dn_sys* s) {
   if (s->b_sym && s->use_sym) {
     DN_TRY(transform1_sym(s));
     DN_TRY(transform2_sym(s));
-    DN_TRY(sym_inplace(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux));
-  } else {
-    DN_TRY(transform1(s));
-    DN_TRY(transform2(s, d.nbf5, s->bt));
+    DN_TRY(symk(s->stream, s->symk_dev, 1, s->symk_blocks, s->symk_red));
+    DN_TRY(j_from_x(s->stream, s->X, d.nbf5, d.nbfaux, s->part2, s->part2_elems, s->J));
+    DN_TRY(hc_products(s, false));
+    DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
+    return DN_OK;
   }
+  DN_TRY(transform1(s));
+  DN_TRY(transform2(s, d.nbf5, s->bt));
   DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X, s->part,
                      s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
@@ -506,6 +508,21 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     }
   }
   s->part_elems = L.part;
+  if (rc == DN_OK) {  // descriptor of the symmetric build's K/X pass, kept in scal[16..31]
+    static_assert(sizeof(SymKDesc) <= 16 * sizeof(double), "SymKDesc fits scal[16..31]");
+    SymKDesc k;
+    k.W = s->bt;
+    k.K = s->K;
+    k.X = s->X;
+    k.part = s->part;
+    k.part_elems = s->part_elems;
+    k.nbf5 = dims->nbf5;
+    k.naux = dims->nbfaux;
+    symk_plan(&k, 1, &s->symk_blocks, &s->symk_red);
+    s->symk_dev = reinterpret_cast<SymKDesc*>(s->scal + 16);
+    if (hipMemcpy(s->symk_dev, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(DN_ERR_HIP, "descriptor upload");
+  }
   hipError_t e = hipSuccess;
   if (rc == DN_OK) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (rc == DN_OK && e != hipSuccess) rc = fail(DN_ERR_HIP, hipGetErrorString(e));

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
 // consecutive B blocks) meet in the same L2.
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
           int TRI = 0>
-__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const GroupedGemm* desc,
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const GroupedGemm* __restrict__ desc,
                                                                       int nprob,
                                                                       long total_tiles) {
   using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
@@ -272,7 +272,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const Gro
       if (desc[mid].tile_begin <= t) lo = mid;
       else hi = mid - 1;
     }
-    const GroupedGemm& d = desc[lo];
+    // lo is uniform over the workgroup; saying so makes the descriptor loads scalar (SGPRs)
+    // instead of per-lane vector loads.  A private copy: the tile's stores to C cannot
+    // alias it, so it is not reloaded after every store.
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const GroupedGemm d = desc[lo];
     const long local = t - d.tile_begin;
     const int mt = local % d.tiles_m;
     const int nt = (local / d.tiles_m) % d.tiles_n;
@@ -378,6 +382,8 @@ void config_tile(int config, int* bm, int* bn) {
     case kTile64x64: *bm = 64; *bn = 64; break;
     case kTile64x128: *bm = 64; *bn = 128; break;
     case kTile128x64: *bm = 128; *bn = 64; break;
+    case kTile64x32: *bm = 64; *bn = 32; break;
+    case kTile32x64: *bm = 32; *bn = 64; break;
     default: *bm = 128; *bn = 128; break;
   }
 }
@@ -386,13 +392,15 @@ template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, 
           int TRI = 0>
 int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, int grid) {
   auto kern = dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>;
-  if (grid <= 0) {  // as many workgroups as stay resident, a multiple of the 8 XCDs
+  // grid <= 0: -grid times as many workgroups as stay resident (0 = once), a multiple
+  // of the 8 XCDs, at most one per tile (rounded up to the XCD multiple)
+  if (grid <= 0) {
     int dev = 0, cus = 0, occ = 0;
     DN_HIP(hipGetDevice(&dev));
     DN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     DN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WGM * WGN, 0));
-    const long full = (long)std::max(occ, 1) * cus;
-    grid = (int)std::min(full, (total + 7) / 8 * 8);
+    const long full = (long)std::max(occ, 1) * cus * std::max(-grid, 1);
+    grid = (int)std::min(std::min(full, (total + 7) / 8 * 8), 0x7ffffff8L);
   }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WGM * WGN), 0, s, d, n, total);
   DN_HIP(hipGetLastError());
@@ -407,18 +415,21 @@ int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, int tri,
 #define DN_GT(VAv, VBv, T) \
   launch_grouped<BM, BN, BK, WGM, WGN, true, true, VAv, VBv, T>(s, d, n, total, grid)
   if (tri) {
-    if (!ak || !bk) return fail(DN_ERR_ARG, "symmetric-operand grouped GEMM needs K-contiguous A, B");
-    if (tri == kTriA)
-      return va ? (vb ? DN_GT(true, true, kTriA) : DN_GT(true, false, kTriA))
-                : (vb ? DN_GT(false, true, kTriA) : DN_GT(false, false, kTriA));
-    return va ? (vb ? DN_GT(true, true, kTriB) : DN_GT(true, false, kTriB))
-              : (vb ? DN_GT(false, true, kTriB) : DN_GT(false, false, kTriB));
+    if (!ak || !bk || tri != kTriA)
+      return fail(DN_ERR_ARG, "grouped GEMM: symmetric mode is kTriA with K-contiguous A, B");
+    return va ? (vb ? DN_GT(true, true, kTriA) : DN_GT(true, false, kTriA))
+              : (vb ? DN_GT(false, true, kTriA) : DN_GT(false, false, kTriA));
   }
 #undef DN_GT
 #define DN_GV(AKv, BKv) \
   (va ? (vb ? DN_G(AKv, BKv, true, true) : DN_G(AKv, BKv, true, false)) \
       : (vb ? DN_G(AKv, BKv, false, true) : DN_G(AKv, BKv, false, false)))
-  if (ak) return bk ? DN_GV(true, true) : DN_GV(true, false);
+  // the layouts of the J/K build's stages: (A, B) K-contiguous (K1, flattened K2),
+  // (MN, K) (batched K2, H C5) and (MN, MN) (J = X X^T)
+  if (ak) {
+    if (!bk) return fail(DN_ERR_ARG, "grouped GEMM: layout (K-contiguous A, MN B) unsupported");
+    return DN_GV(true, true);
+  }
   return bk ? DN_GV(false, true) : DN_GV(false, false);
 #undef DN_GV
 #undef DN_G
@@ -459,6 +470,8 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
     case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
     case kTile128x64: return grouped_layout<128, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+    case kTile64x32: return grouped_layout<64, 32, 16, 2, 1>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+    case kTile32x64: return grouped_layout<32, 64, 16, 1, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
     default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
   }
 }

@@ -5,8 +5,9 @@
 //   K1  b_pn = C5^T b                 persistent grouped GEMM (the dominant stage); for a
 //                                     symmetric b only its lower half: T = C5^T Lh
 //   K2  b_pq[:,:,Q] = b_pn[:,:,Q] C5  persistent grouped GEMM, one tile list over (sys, Q)
-//   SY  b_pq = W + W^T                 systems whose K1 used the lower half
-//   KX  K = sum_Q b_pq^2, X = diag     one workgroup per (system, column q)
+//   KX  K = sum_Q b_pq^2, X = diag     partial sums over Q ranges, then their reduction;
+//                                     for the lower-half systems K and X come straight
+//                                     from W with b_pq = W + W^T (never stored)
 //   J   J = X X^T                     persistent grouped GEMM
 //   HC  H C5                          persistent grouped GEMM
 //   HD  H_core_i = C5[:,i] . (H C5)[:,i]
@@ -18,14 +19,20 @@
 namespace dn {
 namespace {
 
+// K = sum_Q b_pq^2 and X[p,Q] = b_ppQ for one system of the group, in three launches
+// over all systems: partial sums of K over Q ranges (thread per (p,q), coalesced over the
+// slice), their reduction, and the diagonal gather.
 struct KxDesc {
   const double* bpq;  // nbf5 x nbf5 x naux, slice stride nbf5^2
   double* K;
   double* X;          // nbf5 x naux
+  double* part;       // splits x nbf5^2 partial sums
   int nbf5, naux;
-  long col_begin;     // first column of this system in the grouped column list
+  int splits, qps;    // Q ranges of qps slices
+  long part_begin;    // first workgroup of this system in each launch
+  long red_begin;
+  long x_begin;
 };
-
 struct DiagDesc {
   const double* C;
   const double* HC;
@@ -35,47 +42,67 @@ struct DiagDesc {
 };
 
 template <typename D>
-__device__ __forceinline__ int find_desc(const D* d, int n, long col) {
+__device__ __forceinline__ int find_desc(const D* __restrict__ d, int n, long col) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (d[mid].col_begin <= col) lo = mid;
     else hi = mid - 1;
   }
-  return lo;
+  return __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar descriptor loads
 }
 
-// One workgroup of 256 threads per (system, q): K[:, q] and X[q, :].
-__global__ __launch_bounds__(256) void group_kx_kernel(const KxDesc* desc, int n) {
-  __shared__ double red[4][64];
-  const long col = blockIdx.x;
-  const KxDesc& d = desc[find_desc(desc, n, col)];
-  const int q = (int)(col - d.col_begin);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const long slice = (long)d.nbf5 * d.nbf5;
-  const double* base = d.bpq + (long)q * d.nbf5;
-  for (int p0 = 0; p0 < d.nbf5; p0 += 64) {
-    const int p = p0 + lane;
-    double acc = 0.0;
-    if (p < d.nbf5)
-      for (int Q = w; Q < d.naux; Q += 4) {
-        const double v = base[p + Q * slice];
-        acc = fma(v, v, acc);
-      }
-    red[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && p < d.nbf5)
-      d.K[p + (long)q * d.nbf5] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    __syncthreads();
+template <long KxDesc::*BEGIN>
+__device__ __forceinline__ int find_kx(const KxDesc* __restrict__ d, int n, long blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].*BEGIN <= blk) lo = mid;
+    else hi = mid - 1;
   }
-  for (int Q = threadIdx.x; Q < d.naux; Q += blockDim.x)
-    d.X[q + (long)Q * d.nbf5] = base[q + Q * slice];
+  return __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar descriptor loads
+}
+
+__global__ __launch_bounds__(256) void group_kpart_kernel(const KxDesc* __restrict__ desc, int n) {
+  const KxDesc d = desc[find_kx<&KxDesc::part_begin>(desc, n, blockIdx.x)];
+  const long nn = (long)d.nbf5 * d.nbf5;
+  const long chunks = (nn + 255) / 256;
+  const long local = blockIdx.x - d.part_begin;
+  const long e = (local % chunks) * 256 + threadIdx.x;
+  const int z = (int)(local / chunks);
+  if (e >= nn) return;
+  const int Q0 = z * d.qps, Q1 = min(Q0 + d.qps, d.naux);
+  double k = 0.0;
+  for (int Q = Q0; Q < Q1; ++Q) {
+    const double v = d.bpq[(long)Q * nn + e];
+    k = fma(v, v, k);
+  }
+  d.part[z * nn + e] = k;
+}
+
+__global__ __launch_bounds__(256) void group_kred_kernel(const KxDesc* __restrict__ desc, int n) {
+  const KxDesc d = desc[find_kx<&KxDesc::red_begin>(desc, n, blockIdx.x)];
+  const long nn = (long)d.nbf5 * d.nbf5;
+  const long e = (blockIdx.x - d.red_begin) * 256 + threadIdx.x;
+  if (e >= nn) return;
+  double k = 0.0;
+  for (int z = 0; z < d.splits; ++z) k += d.part[z * nn + e];
+  d.K[e] = k;
+}
+
+__global__ __launch_bounds__(256) void group_diag_kernel(const KxDesc* __restrict__ desc, int n) {
+  const KxDesc d = desc[find_kx<&KxDesc::x_begin>(desc, n, blockIdx.x)];
+  const long e = (blockIdx.x - d.x_begin) * 256 + threadIdx.x;
+  if (e >= (long)d.nbf5 * d.naux) return;
+  const long Q = e / d.nbf5;
+  const int p = (int)(e - Q * d.nbf5);
+  d.X[e] = d.bpq[Q * d.nbf5 * d.nbf5 + (long)p * (d.nbf5 + 1)];
 }
 
 // One wavefront per (system, i): H_core_i = sum_m C[m, i] HC[m, i].
-__global__ __launch_bounds__(64) void group_hdiag_kernel(const DiagDesc* desc, int n) {
+__global__ __launch_bounds__(64) void group_hdiag_kernel(const DiagDesc* __restrict__ desc, int n) {
   const long col = blockIdx.x;
-  const DiagDesc& d = desc[find_desc(desc, n, col)];
+  const DiagDesc d = desc[find_desc(desc, n, col)];
   const int i = (int)(col - d.col_begin);
   double acc = 0.0;
   for (int m = threadIdx.x; m < d.nbf; m += 64)
@@ -97,22 +124,28 @@ struct dn_group {
   int precision = DN_FP64;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  // Problems of a stage sorted into classes by operand alignment (16-byte loads allowed
-  // for A, for B): one persistent launch per class, so one odd leading dimension does
-  // not take the vector loads away from every other system.
+  // Problems of a stage sorted into classes by operand layout and alignment (16-byte
+  // loads allowed for A, for B) and by nbf5 (<= 32, <= 64, <= 128, larger): one
+  // persistent launch per class, each with the tile shape the autotuner found fastest
+  // for it, so neither an odd leading dimension nor a small system sets the tiles of
+  // the others.
   struct Launch {
     int first, count;
     long tiles;
+    int config;
+    double flops;
+    int grid = 0;  // gemm_f64_grouped's grid argument
   };
   std::vector<GroupedGemm> host[kNumStages];
   std::vector<Launch> launches[kNumStages];
   GroupedGemm* dev[kNumStages] = {};
-  int config[kNumStages] = {kTile128x128, kTile128x128, kTile64x64, kTile64x64};
   KxDesc* kx = nullptr;
+  long kx_blocks[3] = {0, 0, 0};
   DiagDesc* diag = nullptr;
-  SymDesc* sym = nullptr;
+  SymKDesc* symk = nullptr;  // systems with the symmetric build
   int nsym = 0;
-  long sym_blocks = 0;
+  long symk_blocks = 0, symk_red = 0;
+  int nkx = 0;               // systems with the full build
   long ncols = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -177,14 +210,27 @@ Gemm stage_gemm(dn_sys* s, int stage) {
   return g;
 }
 
-int plan_stage(dn_group* G, int stage, int config) {
+int size_bucket(const Gemm& g, int stage) {
+  const int n5 = stage == kHC ? g.N : (g.tri ? g.N : g.M);
+  return n5 <= 32 ? 0 : n5 <= 64 ? 1 : n5 <= 128 ? 2 : 3;
+}
+
+int default_config(int stage, int bucket) {
+  if (bucket == 0) return kTile64x32;
+  if (stage == kK2 && bucket >= 2) return kTile128x128;
+  return stage == kK1 ? kTile64x128 : kTile64x64;
+}
+
+// Classes of one stage with their default tile configurations; descriptors uploaded.
+int plan_stage(dn_group* G, int stage) {
   DN_HIP(hipStreamSynchronize(G->stream));  // a replay in flight may read the descriptors
   auto& h = G->host[stage];
   h.resize(G->sys.size());
   for (size_t i = 0; i < G->sys.size(); ++i) h[i].g = stage_gemm(G->sys[i], stage);
-  gemm_grouped_plan(h.data(), (int)h.size(), config);  // sets the alignment flags
-  auto key = [](const GroupedGemm& d) {
-    return 16 * d.g.tri + 8 * d.g.a_kcontig + 4 * d.g.b_kcontig + 2 * !d.g.vec_a + !d.g.vec_b;
+  gemm_grouped_plan(h.data(), (int)h.size(), kTile64x64);  // sets the alignment flags
+  auto key = [stage](const GroupedGemm& d) {
+    return 64 * d.g.tri + 32 * d.g.a_kcontig + 16 * d.g.b_kcontig + 8 * !d.g.vec_a +
+           4 * !d.g.vec_b + size_bucket(d.g, stage);
   };
   std::stable_sort(h.begin(), h.end(),
                    [&](const GroupedGemm& a, const GroupedGemm& b) { return key(a) < key(b); });
@@ -192,20 +238,40 @@ int plan_stage(dn_group* G, int stage, int config) {
   L.clear();
   for (int i = 0; i < (int)h.size();) {
     int j = i;
-    while (j < (int)h.size() && key(h[j]) == key(h[i])) ++j;
-    L.push_back({i, j - i, gemm_grouped_plan(h.data() + i, j - i, config)});
+    double f = 0.0;
+    while (j < (int)h.size() && key(h[j]) == key(h[i])) {
+      f += 2.0 * h[j].g.M * h[j].g.N * (double)h[j].g.K * h[j].g.nbatch;
+      ++j;
+    }
+    const int cfg = default_config(stage, size_bucket(h[i].g, stage));
+    L.push_back({i, j - i, gemm_grouped_plan(h.data() + i, j - i, cfg), cfg, f, 0});
     i = j;
   }
-  G->config[stage] = config;
   DN_HIP(hipMemcpy(G->dev[stage], h.data(), h.size() * sizeof(GroupedGemm),
                    hipMemcpyHostToDevice));
   return DN_OK;
 }
 
+// Re-plan one class for another tile configuration.
+int set_config(dn_group* G, int stage, size_t li, int config) {
+  DN_HIP(hipStreamSynchronize(G->stream));
+  auto& l = G->launches[stage][li];
+  GroupedGemm* h = G->host[stage].data() + l.first;
+  l.tiles = gemm_grouped_plan(h, l.count, config);
+  l.config = config;
+  DN_HIP(hipMemcpy(G->dev[stage] + l.first, h, l.count * sizeof(GroupedGemm),
+                   hipMemcpyHostToDevice));
+  return DN_OK;
+}
+
+int launch_one(dn_group* G, int stage, size_t li) {
+  const auto& l = G->launches[stage][li];
+  return gemm_f64_grouped(G->stream, G->host[stage].data() + l.first, G->dev[stage] + l.first,
+                          l.count, l.tiles, l.config, l.grid);
+}
+
 int launch_stage(dn_group* G, int stage) {
-  for (const auto& l : G->launches[stage])
-    DN_TRY(gemm_f64_grouped(G->stream, G->host[stage].data() + l.first,
-                            G->dev[stage] + l.first, l.count, l.tiles, G->config[stage], 0));
+  for (size_t li = 0; li < G->launches[stage].size(); ++li) DN_TRY(launch_one(G, stage, li));
   return DN_OK;
 }
 
@@ -213,9 +279,14 @@ int launch_build(dn_group* G) {
   const int n = (int)G->sys.size();
   DN_TRY(launch_stage(G, kK1));
   DN_TRY(launch_stage(G, kK2));
-  DN_TRY(sym_grouped(G->stream, G->sym, G->nsym, G->sym_blocks));
-  hipLaunchKernelGGL(group_kx_kernel, dim3(G->ncols), dim3(256), 0, G->stream, G->kx, n);
-  DN_HIP(hipGetLastError());
+  DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red));
+  if (G->nkx) {
+    const int m = G->nkx;
+    hipLaunchKernelGGL(group_kpart_kernel, dim3(G->kx_blocks[0]), dim3(256), 0, G->stream, G->kx, m);
+    hipLaunchKernelGGL(group_kred_kernel, dim3(G->kx_blocks[1]), dim3(256), 0, G->stream, G->kx, m);
+    hipLaunchKernelGGL(group_diag_kernel, dim3(G->kx_blocks[2]), dim3(256), 0, G->stream, G->kx, m);
+    DN_HIP(hipGetLastError());
+  }
   DN_TRY(launch_stage(G, kJ));
   DN_TRY(launch_stage(G, kHC));
   hipLaunchKernelGGL(group_hdiag_kernel, dim3(G->ncols), dim3(64), 0, G->stream, G->diag, n);
@@ -263,14 +334,14 @@ int capture(dn_group* G) {
   return DN_OK;
 }
 
-int time_stage(dn_group* G, int stage, float* ms) {
+int time_launch(dn_group* G, int stage, size_t li, float* ms) {
   hipEvent_t a, b;
   DN_HIP(hipEventCreate(&a));
   DN_HIP(hipEventCreate(&b));
-  int rc = launch_stage(G, stage);  // warm-up (code object load, caches)
+  int rc = launch_one(G, stage, li);  // warm-up (code object load, caches)
   if (rc == DN_OK) {
     (void)hipEventRecord(a, G->stream);
-    for (int r = 0; r < 2 && rc == DN_OK; ++r) rc = launch_stage(G, stage);
+    for (int r = 0; r < 2 && rc == DN_OK; ++r) rc = launch_one(G, stage, li);
     (void)hipEventRecord(b, G->stream);
     (void)hipEventSynchronize(b);
     (void)hipEventElapsedTime(ms, a, b);
@@ -312,38 +383,63 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
     e = hipMalloc(&G->dev[st], nsys * sizeof(GroupedGemm));
   if (e == hipSuccess) e = hipMalloc(&G->kx, nsys * sizeof(KxDesc));
   if (e == hipSuccess) e = hipMalloc(&G->diag, nsys * sizeof(DiagDesc));
-  if (e == hipSuccess) e = hipMalloc(&G->sym, nsys * sizeof(SymDesc));
+  if (e == hipSuccess) e = hipMalloc(&G->symk, nsys * sizeof(SymKDesc));
   if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, std::string("dn_group_create: ") +
                                                           hipGetErrorString(e)));
-  std::vector<KxDesc> kx(nsys);
+  std::vector<KxDesc> kx;
+  std::vector<SymKDesc> sk;
   std::vector<DiagDesc> dg(nsys);
   long col = 0;
   for (int i = 0; i < nsys; ++i) {
     dn_sys* s = G->sys[i];
-    kx[i] = KxDesc{s->bt, s->K, s->X, s->d.nbf5, s->d.nbfaux, col};
     dg[i] = DiagDesc{s->C, s->HC, s->Hcore, s->d.nbf, s->d.nbf5, col};
     col += s->d.nbf5;
-  }
-  G->ncols = col;
-  std::vector<SymDesc> sy;
-  for (dn_sys* s : G->sys)
-    if (s->b_sym && s->use_sym) {
-      SymDesc q;
-      q.bpq = s->bt;
+    if (s->b_sym && s->use_sym) {  // K, X straight from W (b_pq = W + W^T)
+      SymKDesc q;
+      q.W = s->bt;
+      q.K = s->K;
+      q.X = s->X;
+      q.part = s->part;
+      q.part_elems = s->part_elems;
       q.nbf5 = s->d.nbf5;
       q.naux = s->d.nbfaux;
-      sy.push_back(q);
+      sk.push_back(q);
+      continue;
     }
-  G->nsym = (int)sy.size();
-  G->sym_blocks = sym_grouped_plan(sy.data(), G->nsym);
+    KxDesc k;
+    const long nn = (long)s->d.nbf5 * s->d.nbf5;
+    k.bpq = s->bt;
+    k.K = s->K;
+    k.X = s->X;
+    k.part = s->part;
+    k.nbf5 = s->d.nbf5;
+    k.naux = s->d.nbfaux;
+    k.qps = 64;  // Q slices per partial sum
+    k.splits = (k.naux + k.qps - 1) / k.qps;
+    while ((size_t)k.splits * nn > s->part_elems) {
+      k.qps *= 2;
+      k.splits = (k.naux + k.qps - 1) / k.qps;
+    }
+    k.part_begin = G->kx_blocks[0];
+    k.red_begin = G->kx_blocks[1];
+    k.x_begin = G->kx_blocks[2];
+    G->kx_blocks[0] += (long)k.splits * ((nn + 255) / 256);
+    G->kx_blocks[1] += (nn + 255) / 256;
+    G->kx_blocks[2] += ((long)k.nbf5 * k.naux + 255) / 256;
+    kx.push_back(k);
+  }
+  G->ncols = col;
+  G->nsym = (int)sk.size();
+  G->nkx = (int)kx.size();
+  symk_plan(sk.data(), G->nsym, &G->symk_blocks, &G->symk_red);
   if (G->nsym)
-    e = hipMemcpy(G->sym, sy.data(), sy.size() * sizeof(SymDesc), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
-  e = hipMemcpy(G->kx, kx.data(), nsys * sizeof(KxDesc), hipMemcpyHostToDevice);
+    e = hipMemcpy(G->symk, sk.data(), sk.size() * sizeof(SymKDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess && G->nkx)
+    e = hipMemcpy(G->kx, kx.data(), kx.size() * sizeof(KxDesc), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(G->diag, dg.data(), nsys * sizeof(DiagDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
   for (int st = 0; st < kNumStages; ++st) {
-    const int rc = plan_stage(G, st, G->config[st]);
+    const int rc = plan_stage(G, st);
     if (rc != DN_OK) return cleanup(rc);
   }
   *out = G;
@@ -359,7 +455,7 @@ int dn_group_destroy(dn_group* G) {
     if (G->dev[st]) (void)hipFree(G->dev[st]);
   if (G->kx) (void)hipFree(G->kx);
   if (G->diag) (void)hipFree(G->diag);
-  if (G->sym) (void)hipFree(G->sym);
+  if (G->symk) (void)hipFree(G->symk);
   for (auto e : G->ev_start) (void)hipEventDestroy(e);
   for (auto e : G->ev_stop) (void)hipEventDestroy(e);
   if (G->done) (void)hipEventDestroy(G->done);
@@ -373,24 +469,63 @@ int dn_group_tune(dn_group* G, int* configs) {
   DN_HIP(hipSetDevice(G->ctx->device));
   DN_TRY(join_systems(G));
   for (int st = 0; st < kNumStages; ++st) {
-    int best = G->config[st];
-    float best_ms = 1e30f;
-    for (int c = 0; c < kNumGemmConfigs; ++c) {
-      DN_TRY(plan_stage(G, st, c));
-      float ms = 0.f;
-      DN_TRY(time_stage(G, st, &ms));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = c;
+    for (size_t li = 0; li < G->launches[st].size(); ++li) {
+      // tile shape x workgroup count (1, 2 or 4 times the resident count, or one per tile)
+      static const int grids[] = {0, -2, -4, -(1 << 20)};
+      int best = G->launches[st][li].config, best_grid = 0;
+      float best_ms = 1e30f;
+      for (int c = 0; c < kNumGemmConfigs; ++c) {
+        DN_TRY(set_config(G, st, li, c));
+        for (int gr : grids) {
+          G->launches[st][li].grid = gr;
+          float ms = 0.f;
+          DN_TRY(time_launch(G, st, li, &ms));
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+            best_grid = gr;
+          }
+        }
       }
+      DN_TRY(set_config(G, st, li, best));
+      G->launches[st][li].grid = best_grid;
     }
-    DN_TRY(plan_stage(G, st, best));
   }
   DN_TRY(capture(G));
   DN_HIP(hipStreamSynchronize(G->stream));
   G->tuned = true;
-  if (configs)
-    for (int st = 0; st < kNumStages; ++st) configs[st] = G->config[st];
+  if (configs)  // the choice for each stage's class with the most work
+    for (int st = 0; st < kNumStages; ++st) {
+      double fmax = -1.0;
+      for (const auto& l : G->launches[st])
+        if (l.flops > fmax) {
+          fmax = l.flops;
+          configs[st] = l.config;
+        }
+    }
+  return DN_OK;
+}
+
+int dn_group_plan_text(dn_group* G, char* buf, int len) {
+  if (!G || !buf || len <= 0) return fail(DN_ERR_ARG, "bad dn_group_plan_text arguments");
+  static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64"};
+  static const char* stages[] = {"K1", "K2", "J", "HC"};
+  std::string t;
+  for (int st = 0; st < kNumStages; ++st) {
+    t += stages[st];
+    t += ":";
+    for (const auto& l : G->launches[st]) {
+      char one[96];
+      char grid[16];
+      if (l.grid <= -(1 << 20)) std::snprintf(grid, sizeof(grid), "all");
+      else std::snprintf(grid, sizeof(grid), "r%d", l.grid < 0 ? -l.grid : 1);
+      std::snprintf(one, sizeof(one), " %dx%s/%s(%.1fGF)", l.count, names[l.config], grid,
+                    l.flops * 1e-9);
+      t += one;
+    }
+    t += st + 1 < kNumStages ? "; " : "";
+  }
+  std::snprintf(buf, (size_t)len, "%s", t.c_str());
   return DN_OK;
 }
 

@@ -70,12 +70,16 @@ enum GemmConfig {
   kTile64x64 = 1,
   kTile64x128 = 2,
   kTile128x64 = 3,
-  kNumGemmConfigs = 4
+  kTile64x32 = 4,
+  kTile32x64 = 5,
+  kNumGemmConfigs = 6
 };
 // Fill tile counts/offsets of desc[0..n) for a config; returns the total tile count.
 long gemm_grouped_plan(GroupedGemm* desc, int n, int config);
 // One persistent launch over all problems (descriptors already on the device).  All
-// problems share the operand layouts (a_kcontig, b_kcontig) of desc_host[0].
+// problems share the operand layouts (a_kcontig, b_kcontig) of desc_host[0].  grid > 0
+// is the workgroup count; grid <= 0 asks for -grid (at least 1) times the resident
+// count, capped at one workgroup per tile.
 int gemm_f64_grouped(hipStream_t s, const GroupedGemm* desc_host, const GroupedGemm* desc_dev,
                      int n, long total_tiles, int config, int grid);
 // Sum `nsplit` partial matrices (each M x N, leading dim ld, stride sP) into out.
@@ -85,15 +89,22 @@ int reduce_splits(hipStream_t s, const double* P, long sP, int nsplit, int M, in
 // small outputs whose tile count alone cannot fill the GPU.
 int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems);
 
-// In-place b_pq = W + W^T over the Q slices of several systems in one launch.
-struct SymDesc {
-  double* bpq;  // nbf5 x nbf5 x naux
+// K = sum_Q (W + W^T)^2 and X = 2 diag(W) per Q (b_pq = W + W^T of the symmetric-b
+// build) for one or several systems in two launches.
+struct SymKDesc {
+  const double* W;     // nbf5 x nbf5 x naux
+  double* K;           // nbf5 x nbf5
+  double* X;           // nbf5 x naux
+  double* part;        // partial sums
+  size_t part_elems = 0;
   int nbf5 = 0, naux = 0;
-  int q_per_block = 1;
-  long blk_begin = 0;
+  int qps = 1, splits = 1;  // set by symk_plan
+  long blk_begin = 0, red_begin = 0;
 };
-long sym_grouped_plan(SymDesc* desc, int n);  // fills q_per_block/blk_begin; returns blocks
-int sym_grouped(hipStream_t s, const SymDesc* dev, int n, long blocks);
+void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks);
+int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
+             size_t part2_elems, double* J);
+int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks);
 
 // ---- device buffers -----------------------------------------------------------------
 struct DevBuf {
@@ -187,6 +198,8 @@ struct dn_sys {
   float* qinv = nullptr;    // naux
   bool b_ready = false;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
+  dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16)
+  long symk_blocks = 0, symk_red = 0;
   bool use_sym = true;   // let the J/K build use that symmetry
   // timing of the dominant transform kernel: one event pair per launch, read back
   // (and recycled) by dn_sys_get_timing so that timing adds no synchronisation.

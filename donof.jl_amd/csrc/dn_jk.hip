@@ -40,6 +40,9 @@ __global__ void diag_kernel(const T* __restrict__ bpq, long slice, int nbf5, int
   }
 }
 
+int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
+             size_t part2_elems, double* J);
+
 template <typename T>
 int jk_from_bpq_t(hipStream_t s, const T* bpq, long slice, int nbf5, int naux, double* X,
                   double* part, size_t part_elems, double* part2, size_t part2_elems, double* J,
@@ -60,7 +63,12 @@ int jk_from_bpq_t(hipStream_t s, const T* bpq, long slice, int nbf5, int naux, d
   hipLaunchKernelGGL(diag_kernel<T>, dim3((unsigned)std::min<long>((nx + 255) / 256, 4096)),
                      dim3(256), 0, s, bpq, slice, nbf5, naux, X);
   DN_HIP(hipGetLastError());
-  // J = X X^T: A(p,Q) = X[p + nbf5 Q], B(Q,q) = X[q + nbf5 Q]
+  return j_from_x(s, X, nbf5, naux, part2, part2_elems, J);
+}
+
+// J = X X^T (integrals.jl:163): A(p,Q) = X[p + nbf5 Q], B(Q,q) = X[q + nbf5 Q]
+int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
+             size_t part2_elems, double* J) {
   Gemm g;
   g.M = nbf5; g.N = nbf5; g.K = naux;
   g.A = X; g.lda = nbf5; g.a_kcontig = false;
@@ -83,98 +91,116 @@ int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int n
                               part2_elems, J, K);
 }
 
-// b_pq = W + W^T in place, slice by slice (W = C5^T Lh C5 of the symmetric-b transform):
-// workgroup (pair of 32 x 32 tiles (I, J), I <= J; range of Q) stages both tiles in LDS
-// and writes both back, so every element is read and written once, coalesced.
-__device__ __forceinline__ void sym_pair(double* __restrict__ bpq, long slice, int n5, int I,
-                                         int J, int Q0, int Q1, double (*a)[33],
-                                         double (*b)[33]) {
+// J/K of the symmetric-b build straight from W (b_pq = W + W^T is never stored):
+//   K[p,q] = sum_Q (W[p,q,Q] + W[q,p,Q])^2,   X[p,Q] = b_ppQ = 2 W[p,p,Q].
+// Pass 1: workgroup (system, 32 x 32 tile pair (I, J) with I <= J, range z of Q) stages
+// the tiles (I, J) and (J, I) of each slice in LDS and accumulates the squares of tile
+// (I, J) in registers; every element of W is read once, coalesced.  Diagonal tiles also
+// write X.  Pass 2 sums the ranges and mirrors the lower tiles (K is symmetric).
+__device__ __forceinline__ int find_symk(const SymKDesc* __restrict__ d, int n, long blk,
+                                         bool red) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((red ? d[mid].red_begin : d[mid].blk_begin) <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar descriptor loads
+}
+
+__global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __restrict__ desc,
+                                                           int n) {
+  __shared__ double a[32][33], b[32][33];
+  const SymKDesc d = desc[find_symk(desc, n, blockIdx.x, false)];
+  const int n5 = d.nbf5;
+  const long nn = (long)n5 * n5;
+  const int ntile = (n5 + 31) / 32;
+  const int pairs = ntile * (ntile + 1) / 2;
+  const long local = blockIdx.x - d.blk_begin;
+  int pidx = (int)(local % pairs), I = 0;
+  while (pidx >= ntile - I) {
+    pidx -= ntile - I;
+    ++I;
+  }
+  const int J = I + pidx;
+  const int z = (int)(local / pairs);
+  const int Q0 = z * d.qps, Q1 = min(Q0 + d.qps, d.naux);
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int pI = I * 32 + tx, pJ = J * 32 + tx;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int Q = Q0; Q < Q1; ++Q) {
-    double* W = bpq + (long)Q * slice;
-    for (int r = ty; r < 32; r += 8) {
-      const int qJ = J * 32 + r, qI = I * 32 + r;
+    const double* W = d.W + (long)Q * nn;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = ty + 8 * k, qJ = J * 32 + r, qI = I * 32 + r;
       a[tx][r] = (pI < n5 && qJ < n5) ? W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
       b[tx][r] = (pJ < n5 && qI < n5) ? W[pJ + (long)qI * n5] : 0.0;  // tile (J, I)
     }
     __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-      const int qJ = J * 32 + r, qI = I * 32 + r;
-      if (pI < n5 && qJ < n5) W[pI + (long)qJ * n5] = a[tx][r] + b[r][tx];
-      if (I != J && pJ < n5 && qI < n5) W[pJ + (long)qI * n5] = b[tx][r] + a[r][tx];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = ty + 8 * k;
+      const double v = a[tx][r] + b[r][tx];  // b_pq at p = I*32 + tx, q = J*32 + r
+      acc[k] = fma(v, v, acc[k]);
+      if (I == J && r == tx && pI < n5) d.X[pI + (long)Q * n5] = v;
     }
     __syncthreads();
   }
-}
-
-__device__ __forceinline__ void pair_of(int pidx, int ntile, int* I, int* J) {
-  int i = 0;
-  while (pidx >= ntile - i) {
-    pidx -= ntile - i;
-    ++i;
+  double* part = d.part + (long)z * nn;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int qJ = J * 32 + ty + 8 * k;
+    if (pI < n5 && qJ < n5) part[pI + (long)qJ * n5] = acc[k];
   }
-  *I = i;
-  *J = i + pidx;
 }
 
-__global__ __launch_bounds__(256) void sym_inplace_kernel(double* __restrict__ bpq, long slice,
-                                                          int n5, int naux, int ntile,
-                                                          int q_per_block) {
-  __shared__ double a[32][33], b[32][33];
-  int I, J;
-  pair_of(blockIdx.x, ntile, &I, &J);
-  const int Q0 = blockIdx.y * q_per_block;
-  sym_pair(bpq, slice, n5, I, J, Q0, min(Q0 + q_per_block, naux), a, b);
-}
-
-// Grouped form: workgroup g belongs to the system whose block range holds it.
-__global__ __launch_bounds__(256) void sym_grouped_kernel(const SymDesc* desc, int n) {
-  __shared__ double a[32][33], b[32][33];
-  const long g = blockIdx.x;
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (desc[mid].blk_begin <= g) lo = mid;
-    else hi = mid - 1;
+// 64 elements per workgroup, the four wavefronts summing interleaved Q ranges.
+__global__ __launch_bounds__(256) void symk_reduce_kernel(const SymKDesc* __restrict__ desc,
+                                                          int n) {
+  __shared__ double red[4][64];
+  const SymKDesc d = desc[find_symk(desc, n, blockIdx.x, true)];
+  const int n5 = d.nbf5;
+  const long nn = (long)n5 * n5;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long e = (blockIdx.x - d.red_begin) * 64 + lane;
+  double k = 0.0;
+  if (e < nn) {
+    const int p = (int)(e % n5), q = (int)(e / n5);
+    // partials exist for tile(p) <= tile(q); K is symmetric
+    const long src = (p / 32 <= q / 32) ? e : q + (long)p * n5;
+    for (int z = w; z < d.splits; z += 4) k += d.part[z * nn + src];
   }
-  const SymDesc& d = desc[lo];
-  const long local = g - d.blk_begin;
-  const int ntile = (d.nbf5 + 31) / 32;
-  const int pairs = ntile * (ntile + 1) / 2;
-  int I, J;
-  pair_of((int)(local % pairs), ntile, &I, &J);
-  const int Q0 = (int)(local / pairs) * d.q_per_block;
-  sym_pair(d.bpq, (long)d.nbf5 * d.nbf5, d.nbf5, I, J, Q0, min(Q0 + d.q_per_block, d.naux), a,
-           b);
+  red[w][lane] = k;
+  __syncthreads();
+  if (w == 0 && e < nn) d.K[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
-long sym_grouped_plan(SymDesc* desc, int n) {
-  long begin = 0;
+void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks) {
+  long b = 0, r = 0;
   for (int i = 0; i < n; ++i) {
-    const int ntile = (desc[i].nbf5 + 31) / 32;
+    SymKDesc& d = desc[i];
+    const long nn = (long)d.nbf5 * d.nbf5;
+    const int ntile = (d.nbf5 + 31) / 32;
     const int pairs = ntile * (ntile + 1) / 2;
-    desc[i].q_per_block = (int)std::max<long>(1, (long)desc[i].naux * pairs / 256);
-    desc[i].blk_begin = begin;
-    begin += (long)pairs * ((desc[i].naux + desc[i].q_per_block - 1) / desc[i].q_per_block);
+    d.qps = 8;
+    d.splits = (d.naux + d.qps - 1) / d.qps;
+    while ((size_t)d.splits * nn > d.part_elems) {
+      d.qps *= 2;
+      d.splits = (d.naux + d.qps - 1) / d.qps;
+    }
+    d.blk_begin = b;
+    d.red_begin = r;
+    b += (long)pairs * d.splits;
+    r += (nn + 63) / 64;
   }
-  return begin;
+  *blocks = b;
+  *red_blocks = r;
 }
 
-int sym_grouped(hipStream_t s, const SymDesc* dev, int n, long blocks) {
-  if (n <= 0 || blocks <= 0) return DN_OK;
-  hipLaunchKernelGGL(sym_grouped_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
-  DN_HIP(hipGetLastError());
-  return DN_OK;
-}
-
-int sym_inplace(hipStream_t s, double* bpq, long slice, int nbf5, int naux) {
-  const int ntile = (nbf5 + 31) / 32;
-  const int pairs = ntile * (ntile + 1) / 2;
-  int qpb = (int)std::max<long>(1, (long)naux * pairs / 4096);
-  const int nq = (naux + qpb - 1) / qpb;
-  hipLaunchKernelGGL(sym_inplace_kernel, dim3(pairs, nq), dim3(256), 0, s, bpq, slice, nbf5, naux,
-                     ntile, qpb);
+int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks) {
+  if (n <= 0) return DN_OK;
+  hipLaunchKernelGGL(symk_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
+  hipLaunchKernelGGL(symk_reduce_kernel, dim3((unsigned)red_blocks), dim3(256), 0, s, dev, n);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

@@ -197,10 +197,13 @@ int dn_orbopt_resident(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st)
 typedef struct dn_group dn_group;
 int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out);
 int dn_group_destroy(dn_group* group);
-/* Times every tile configuration of each grouped GEMM stage (K1, K2, J, H C5) on this
- * group, keeps the fastest, and captures the build graph.  configs (4 ints, may be NULL)
- * receives the choices. */
+/* The systems of each grouped GEMM stage (K1, K2, J, H C5) are split into classes by
+ * operand layout, alignment and size; this times every tile configuration of every
+ * class on this group, keeps the fastest, and captures the build graph.  configs
+ * (4 ints, may be NULL) receives each stage's choice for its class with the most work. */
 int dn_group_tune(dn_group* group, int* configs);
+/* The launch plan as text: per stage, each class's system count, tile shape and GFLOP. */
+int dn_group_plan_text(dn_group* group, char* buf, int len);
 /* One J/K/H_core build of every system (a graph replay; asynchronous). */
 int dn_group_jkh(dn_group* group);
 int dn_group_sync(dn_group* group);
