@@ -34,6 +34,10 @@ int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
              const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
              const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
              float* qmax);
+int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+                 const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                 const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
+                 float* qmax);
 int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
              const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
              const float* qmax, float* qs, float* qinv);
@@ -261,12 +265,24 @@ int jkh_split(dn_sys* s) {
   const dn_dims& d = s->d;
   DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
                          s->clo));
+  const bool sym = s->b_sym && s->use_sym;
   DN_TRY(begin_timing(s));
-  DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo, s->binv,
-                  s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
+  if (sym)
+    DN_TRY(split_k1_sym(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo,
+                        s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
+  else
+    DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo,
+                    s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   DN_TRY(end_timing(s));
   DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale,
                   s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
+  if (sym) {  // bpq32 holds W with b_pq = W + W^T
+    DN_TRY(symk(s->stream, reinterpret_cast<const SymKDesc*>(s->scal + 32), 1, s->symk_blocks,
+                s->symk_red, true));
+    DN_TRY(j_from_x(s->stream, s->X, d.nbf5, d.nbfaux, s->part2, s->part2_elems, s->J));
+    DN_TRY(hc_products(s, false));
+    return hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore);
+  }
   DN_TRY(jk_from_bpq_f32(s->stream, s->bpq32, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X,
                          s->part, s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
@@ -499,7 +515,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     A(Layout::f8(L.f_naux), &q); s->qmax = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->qs = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->qinv = reinterpret_cast<float*>(q);
-    A(1, &q); s->cscale = reinterpret_cast<float*>(q);
+    A(2, &q); s->cscale = reinterpret_cast<float*>(q);  // scale, 1/scale, max bits
     if (rc == DN_OK) {
       // plane padding must read as zero
       hipError_t e1 = hipMemset(s->bhi, 0, Layout::d8(L.bplane) * 8);
@@ -511,6 +527,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   if (rc == DN_OK) {  // descriptor of the symmetric build's K/X pass, kept in scal[16..31]
     static_assert(sizeof(SymKDesc) <= 16 * sizeof(double), "SymKDesc fits scal[16..31]");
     SymKDesc k;
+    k.Wf = nullptr;
     k.W = s->bt;
     k.K = s->K;
     k.X = s->X;
@@ -520,7 +537,12 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     k.naux = dims->nbfaux;
     symk_plan(&k, 1, &s->symk_blocks, &s->symk_red);
     s->symk_dev = reinterpret_cast<SymKDesc*>(s->scal + 16);
-    if (hipMemcpy(s->symk_dev, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess)
+    SymKDesc k32 = k;  // the split-precision build's W (FP32), at scal[32..47]
+    k32.W = nullptr;
+    k32.Wf = s->bpq32;
+    static_assert(sizeof(SymKDesc) % sizeof(double) == 0 && sizeof(SymKDesc) <= 128, "layout");
+    if (hipMemcpy(s->symk_dev, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->scal + 32, &k32, sizeof(k32), hipMemcpyHostToDevice) != hipSuccess)
       rc = fail(DN_ERR_HIP, "descriptor upload");
   }
   hipError_t e = hipSuccess;

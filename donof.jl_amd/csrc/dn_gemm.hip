@@ -194,32 +194,34 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
     }
     if (t + 1 < T) store_tile(buf ^ 1, t + 1);
     __syncthreads();
   }
 
   // D(16x16) layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l holds
-  // column l%16 and rows l/16 + 4*r in accumulator register r.  A store instruction
-  // (fixed r) therefore writes a 4-row x 16-column block.
+  // column l%16 and rows l/16 + 4*r in accumulator register r.  The product is formed
+  // transposed (B fragment first), so a lane holds row l%16 of the C tile and columns
+  // l/16 + 4*r: one store instruction (fixed r) writes 16 consecutive rows (128 bytes)
+  // of each of 4 columns.
   double* Cz = g.C + (long)z * g.sC;
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+  for (int i = 0; i < FM; ++i) {
+    const int row = m0 + wm * WM + i * 16 + fr;
+    if (row >= M) continue;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WN + j * 16 + fr;
-      if (col >= N) continue;
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int row = m0 + wm * WM + i * 16 + fk + 4 * rr;
-        if (row < M) {
+        const int col = n0 + wn * WN + j * 16 + fk + 4 * rr;
+        if (col < N) {
           double* c = Cz + row + (long)col * g.ldc;
           const double v = g.alpha * acc[i][j][rr];
           *c = (g.beta == 0.0) ? v : v + g.beta * (*c);
         }
       }
-    }
+  }
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC>

@@ -92,7 +92,8 @@ int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elem
 // K = sum_Q (W + W^T)^2 and X = 2 diag(W) per Q (b_pq = W + W^T of the symmetric-b
 // build) for one or several systems in two launches.
 struct SymKDesc {
-  const double* W;     // nbf5 x nbf5 x naux
+  const double* W;     // nbf5 x nbf5 x naux (FP64 build)
+  const float* Wf;     // the same in FP32 (split-precision build)
   double* K;           // nbf5 x nbf5
   double* X;           // nbf5 x naux
   double* part;        // partial sums
@@ -104,7 +105,8 @@ struct SymKDesc {
 void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks);
 int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
              size_t part2_elems, double* J);
-int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks);
+int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks,
+         bool f32 = false);
 
 // ---- device buffers -----------------------------------------------------------------
 struct DevBuf {
@@ -198,7 +200,8 @@ struct dn_sys {
   float* qinv = nullptr;    // naux
   bool b_ready = false;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
-  dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16)
+  dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
+                                     // the split-precision one at scal + 32)
   long symk_blocks = 0, symk_red = 0;
   bool use_sym = true;   // let the J/K build use that symmetry
   // timing of the dominant transform kernel: one event pair per launch, read back

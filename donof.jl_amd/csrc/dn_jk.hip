@@ -1,6 +1,7 @@
 // J/K/H_core epilogues of the RI J/K-MO build (integrals.jl:144-172) and the device
 // generator of the synthetic three-index tensor.
 #include <cstring>
+#include <type_traits>
 
 #include "dn_internal.h"
 
@@ -108,10 +109,13 @@ __device__ __forceinline__ int find_symk(const SymKDesc* __restrict__ d, int n, 
   return __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar descriptor loads
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __restrict__ desc,
                                                            int n) {
   __shared__ double a[32][33], b[32][33];
   const SymKDesc d = desc[find_symk(desc, n, blockIdx.x, false)];
+  const T* Wall = std::is_same<T, float>::value ? reinterpret_cast<const T*>(d.Wf)
+                                                : reinterpret_cast<const T*>(d.W);
   const int n5 = d.nbf5;
   const long nn = (long)n5 * n5;
   const int ntile = (n5 + 31) / 32;
@@ -129,12 +133,12 @@ __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __res
   const int pI = I * 32 + tx, pJ = J * 32 + tx;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int Q = Q0; Q < Q1; ++Q) {
-    const double* W = d.W + (long)Q * nn;
+    const T* W = Wall + (long)Q * nn;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int r = ty + 8 * k, qJ = J * 32 + r, qI = I * 32 + r;
-      a[tx][r] = (pI < n5 && qJ < n5) ? W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
-      b[tx][r] = (pJ < n5 && qI < n5) ? W[pJ + (long)qI * n5] : 0.0;  // tile (J, I)
+      a[tx][r] = (pI < n5 && qJ < n5) ? (double)W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
+      b[tx][r] = (pJ < n5 && qI < n5) ? (double)W[pJ + (long)qI * n5] : 0.0;  // tile (J, I)
     }
     __syncthreads();
 #pragma unroll
@@ -197,9 +201,12 @@ void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks) {
   *red_blocks = r;
 }
 
-int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks) {
+int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks, bool f32) {
   if (n <= 0) return DN_OK;
-  hipLaunchKernelGGL(symk_partial_kernel, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
+  if (f32)
+    hipLaunchKernelGGL(symk_partial_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
+  else
+    hipLaunchKernelGGL(symk_partial_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
   hipLaunchKernelGGL(symk_reduce_kernel, dim3((unsigned)red_blocks), dim3(256), 0, s, dev, n);
   DN_HIP(hipGetLastError());
   return DN_OK;
@@ -208,18 +215,35 @@ int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks
 // out[0] = max |b[m,n,Q] - b[n,m,Q]|, out[1] = max |b| (as the bit patterns of
 // non-negative doubles, which order like the values).
 __global__ __launch_bounds__(256) void sym_check_kernel(const double* __restrict__ b, int nbf,
-                                                        long total,
+                                                        int naux, int ntile, int q_per_block,
                                                         unsigned long long* __restrict__ out) {
+  // workgroup: 32 x 32 tile pair (I, J), I <= J, of a range of slabs; both tiles staged in
+  // LDS so that b[m,n] and b[n,m] are each read once, coalesced
+  __shared__ double a[32][33], t[32][33];
+  int pidx = blockIdx.x, I = 0;
+  while (pidx >= ntile - I) {
+    pidx -= ntile - I;
+    ++I;
+  }
+  const int J = I + pidx;
   const long nn = (long)nbf * nbf;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int mI = I * 32 + tx, mJ = J * 32 + tx;
   double dmax = 0.0, bmax = 0.0;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const long Q = e / nn;
-    const long r = e - Q * nn;
-    const int m = (int)(r % nbf), n = (int)(r / nbf);
-    const double v = b[e];
-    dmax = fmax(dmax, fabs(v - b[Q * nn + n + (long)m * nbf]));
-    bmax = fmax(bmax, fabs(v));
+  const int Q0 = blockIdx.y * q_per_block, Q1 = min(Q0 + q_per_block, naux);
+  for (int Q = Q0; Q < Q1; ++Q) {
+    const double* s = b + (long)Q * nn;
+    for (int r = ty; r < 32; r += 8) {
+      const int nJ = J * 32 + r, nI = I * 32 + r;
+      a[tx][r] = (mI < nbf && nJ < nbf) ? s[mI + (long)nJ * nbf] : 0.0;  // tile (I, J)
+      t[tx][r] = (mJ < nbf && nI < nbf) ? s[mJ + (long)nI * nbf] : 0.0;  // tile (J, I)
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      dmax = fmax(dmax, fabs(a[tx][r] - t[r][tx]));
+      bmax = fmax(bmax, fmax(fabs(a[tx][r]), fabs(t[tx][r])));
+    }
+    __syncthreads();
   }
   for (int off = 32; off > 0; off >>= 1) {
     dmax = fmax(dmax, __shfl_down(dmax, off, 64));
@@ -240,9 +264,12 @@ __global__ void clear_u64_kernel(unsigned long long* p, int n) {
 int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
                double* dmax, double* bmax) {
   hipLaunchKernelGGL(clear_u64_kernel, dim3(1), dim3(64), 0, s, dev2, 2);
-  const long total = (long)nbf * nbf * naux;
-  const int blocks = (int)std::min<long>((total + 255) / 256, 8192);
-  hipLaunchKernelGGL(sym_check_kernel, dim3(blocks), dim3(256), 0, s, b, nbf, total, dev2);
+  const int ntile = (nbf + 31) / 32;
+  const int pairs = ntile * (ntile + 1) / 2;
+  const int qpb = (int)std::max<long>(1, (long)naux * pairs / 8192);
+  const int nq = (naux + qpb - 1) / qpb;
+  hipLaunchKernelGGL(sym_check_kernel, dim3(pairs, nq), dim3(256), 0, s, b, nbf, naux, ntile, qpb,
+                     dev2);
   DN_HIP(hipGetLastError());
   unsigned long long h[2] = {0, 0};
   DN_HIP(hipMemcpyAsync(h, dev2, sizeof(h), hipMemcpyDeviceToHost, s));

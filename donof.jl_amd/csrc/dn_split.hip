@@ -91,28 +91,32 @@ __global__ void b_split_kernel(const double* __restrict__ b, int nbf, long slab,
   }
 }
 
-// C5^T planes [Mpad][Kpad] (row = orbital p, k = AO index m), zero padded; one block
-// computes the scale first.
-__global__ void c_split_kernel(const double* __restrict__ C, int nbf, int nbf5, int Mpad,
-                               int Kpad, float* __restrict__ cscale, _Float16* __restrict__ hi,
-                               _Float16* __restrict__ lo) {
-  __shared__ float red[16];
-  __shared__ float sc_sh;
+// C5^T planes [Mpad][Kpad] (row = orbital p, k = AO index m), zero padded, in two
+// launches over the whole GPU: max |C5| into cscale[2] (bits of a non-negative float, which
+// order like the value; cleared first), then the planes with the power-of-two scale.
+__global__ void c_max_kernel(const double* __restrict__ C, long n, float* __restrict__ cscale) {
   float m = 0.f;
-  for (long e = threadIdx.x; e < (long)nbf * nbf5; e += blockDim.x)
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x)
     m = fmaxf(m, (float)fabs(C[e]));
-  m = block_max(m, red);
-  if (threadIdx.x == 0) {
-    sc_sh = pow2_scale(m * 1.0001f);
-    cscale[0] = sc_sh;
-    cscale[1] = 1.0f / sc_sh;
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned*>(cscale + 2), __float_as_uint(m));
+}
+
+__global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5, int Mpad,
+                                int Kpad, float* __restrict__ cscale, _Float16* __restrict__ hi,
+                                _Float16* __restrict__ lo) {
+  const float sc = pow2_scale(cscale[2] * 1.0001f);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cscale[0] = sc;
+    cscale[1] = 1.0f / sc;
   }
-  __syncthreads();
-  const double sc = sc_sh;
-  for (long e = threadIdx.x; e < (long)Mpad * Kpad; e += blockDim.x) {
+  const long total = (long)Mpad * Kpad;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
     const int p = e / Kpad, k = e % Kpad;
     _Float16 h = 0, l = 0;
-    if (p < nbf5 && k < nbf) split_f16_d(C[k + (long)p * nbf] * sc, &h, &l);
+    if (p < nbf5 && k < nbf) split_f16_d(C[k + (long)p * nbf] * (double)sc, &h, &l);
     hi[e] = h;
     lo[e] = l;
   }
@@ -134,10 +138,10 @@ struct SplitGemm {
   const float* af32;
   long lda, sA;             // batch stride (FP32 source)
   const float* a_scale;     // per batch: scale applied to the FP32 A before splitting
-  // B: f16 planes [col][k] (ld = ldb)
+  // B: f16 planes [col][k] (ld = ldb), batch stride sB (halves)
   const _Float16* bhi;
   const _Float16* blo;
-  long ldb;
+  long ldb, sB;
   // C: FP32 column-major, batch stride sC
   float* C;
   long ldc, sC;
@@ -146,21 +150,99 @@ struct SplitGemm {
   const float* cs;
   const float* col_inv_scale;  // per Q = col / col_div (K1), nullable
   int col_div;
-  const float* batch_inv_scale;  // per batch (K2), nullable
-  float* qmax;                   // per Q max |out| (K1), nullable
+  const float* batch_inv_scale;  // per batch (K2, symmetric K1), nullable
+  float* qmax;                   // per Q max |out| (K1: by column group, or by batch when
+                                 // col_div == 0), nullable
   int nbatch;
 };
 
-constexpr int SB_K = 32;           // k per MFMA / per LDS tile
-constexpr int SB_LD = SB_K + 8;    // halves per LDS row (80 bytes: conflict-free b128 reads)
+constexpr int SB_K = 32;  // k per MFMA / per LDS tile: four 16-byte chunks of 8 halves
+// LDS tiles are K-major in 16-byte chunks: chunk c (k = 8c..8c+7) of row r at
+// (c * ROWS + r) * 8 halves.  An MFMA fragment read (lane l: row l % 16, chunk l / 16) and
+// a staging store (lane l: row l % 16 + 16 (l / 64), chunk (l / 16) % 4) are then runs of
+// 16 consecutive 16-byte chunks, free of bank conflicts (a row-major tile with padded
+// rows conflicts on one of the two).
+__device__ __forceinline__ int chunk_off(int rows, int r, int c) { return (c * rows + r) * 8; }
+__device__ __forceinline__ void stage_rc(int idx, int* r, int* c) {
+  *r = (idx & 15) + 16 * (idx >> 6);
+  *c = (idx >> 4) & 3;
+}
 
-template <int BM, int BN, int WGM, int WGN, bool AF32>
+// Output of one wave's FM x FN fragments at rows r0.., columns c0..; n0 is the tile's first
+// column (for the per-Q maxima).  D(16x16) of v_mfma_f32_16x16x32_f16: lane l holds
+// column l % 16, rows 4 (l / 16) + r.
+template <int FM, int FN>
+__device__ __forceinline__ void split_epilogue(const SplitGemm& g, v4f (&acc)[FM][FN], int r0,
+                                               int c0, int n0, int z, int lane) {
+  const int fr = lane & 15;
+  const int M = g.M, N = g.N;
+  float* Cz = g.C + (long)z * g.sC;
+  const float bscale = g.batch_inv_scale ? g.batch_inv_scale[z] : 1.f;
+  const int qa = g.col_div > 0 ? n0 / g.col_div : 0;
+  float mx_a = 0.f, mx_b = 0.f;
+  // each lane holds 4 consecutive rows of one column: one 16-byte store when the column
+  // stride and the base keep it aligned
+  const bool vec = (g.ldc % 4 == 0) && ((uintptr_t)Cz % 16 == 0);
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = c0 + j * 16 + fr;
+    if (col >= N) continue;
+    float f = g.cs[1] * bscale;
+    int q = 0;
+    if (g.col_inv_scale) {
+      q = col / g.col_div;
+      f *= g.col_inv_scale[q];
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int row0 = r0 + i * 16 + 4 * (lane >> 4);
+      const v4f v = acc[i][j] * f;
+      float* c = Cz + row0 + (long)col * g.ldc;
+      float m = 0.f;
+      if (vec && row0 + 3 < M) {
+        *reinterpret_cast<v4f*>(c) = v;
+        m = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+      } else {
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+          if (row0 + rr < M) {
+            c[rr] = v[rr];
+            m = fmaxf(m, fabsf(v[rr]));
+          }
+      }
+      if (g.col_div == 0 || q == qa) mx_a = fmaxf(mx_a, m);
+      else if (q == qa + 1) mx_b = fmaxf(mx_b, m);
+      else if (g.qmax) atomicMax(reinterpret_cast<unsigned*>(&g.qmax[q]), __float_as_uint(m));
+    }
+  }
+  if (g.qmax) {
+    for (int off = 32; off > 0; off >>= 1) {
+      mx_a = fmaxf(mx_a, __shfl_down(mx_a, off, 64));
+      mx_b = fmaxf(mx_b, __shfl_down(mx_b, off, 64));
+    }
+    if (lane == 0) {
+      if (g.col_div == 0) {
+        atomicMax(reinterpret_cast<unsigned*>(&g.qmax[z]), __float_as_uint(mx_a));
+      } else {
+        const int nq = N / g.col_div;
+        atomicMax(reinterpret_cast<unsigned*>(&g.qmax[qa]), __float_as_uint(mx_a));
+        if (qa + 1 < nq)
+          atomicMax(reinterpret_cast<unsigned*>(&g.qmax[qa + 1]), __float_as_uint(mx_b));
+      }
+    }
+  }
+}
+
+// TRIB: B is the symmetric slab b_Q (col n, k = m) and the tile multiplies by its lower
+// half Lh (m > n as is, m == n halved, m < n zero; b_Q = Lh + Lh^T), so the K loop of
+// column block n0 starts at m = n0.
+template <int BM, int BN, int WGM, int WGN, bool AF32, bool TRIB = false>
 __global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g, int tiles_m,
                                                                    int tiles_n) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
-  __shared__ _Float16 Ah[2][BM * SB_LD], Al[2][BM * SB_LD];
-  __shared__ _Float16 Bh[2][BN * SB_LD], Bl[2][BN * SB_LD];
+  __shared__ _Float16 Ah[2][BM * SB_K], Al[2][BM * SB_K];
+  __shared__ _Float16 Bh[2][BN * SB_K], Bl[2][BN * SB_K];
 
   const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
   if (blockIdx.x >= total) return;
@@ -172,7 +254,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g,
   const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
   const int M = g.M, N = g.N, K = g.K;
-  const int nkt = (K + SB_K - 1) / SB_K;
+  const int kt0 = TRIB ? n0 / SB_K : 0;
+  const int nkt = (K + SB_K - 1) / SB_K - kt0;
 
   v4f acc[FM][FN];
 #pragma unroll
@@ -182,149 +265,453 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g,
 
   // staging registers: the loads of tile t+1 are issued before tile t is computed and
   // written to LDS after it (LDS is double-buffered, one barrier per tile)
-  constexpr int AV = BM * 4 / NT;         // 16-byte loads per plane (f16 source)
-  constexpr int AP = BM * SB_K / 2 / NT;  // k-pairs per thread (FP32 source)
+  constexpr int AV = BM * 4 / NT;  // 16-byte chunks per thread and plane (A, B)
   constexpr int BV = BN * 4 / NT;
-  static_assert((AF32 ? AP : AV) >= 1 && BV >= 1, "tile too small for the thread count");
+  static_assert(AV >= 1 && BV >= 1 && BM * 4 % NT == 0 && BN * 4 % NT == 0,
+                "tile too small for the thread count");
+  static_assert(!AF32 || BM % 64 == 0, "FP32 A staging covers 64-row groups");
   v8h rah[AF32 ? 1 : AV], ral[AF32 ? 1 : AV];
-  float rf0[AF32 ? AP : 1], rf1[AF32 ? AP : 1];
+  float rf[AF32 ? AV : 1][8];
   v8h rbh[BV], rbl[BV];
   const float* Af = AF32 ? g.af32 + (long)z * g.sA : nullptr;
   const float ascale = AF32 ? g.a_scale[z] : 1.f;
+  const _Float16* Bhi = g.bhi + (long)z * g.sB;
+  const _Float16* Blo = g.blo + (long)z * g.sB;
 
+  // FP32 A: task idx = (row m = idx % BM, chunk c = idx / BM), eight column-major loads
+  // (consecutive lanes read consecutive rows)
   auto load = [&](int t) {
-    const int k0 = t * SB_K;
+    const int k0 = (kt0 + t) * SB_K;
     if constexpr (AF32) {
 #pragma unroll
-      for (int i = 0; i < AP; ++i) {
+      for (int i = 0; i < AV; ++i) {
         const int idx = tid + i * NT;
-        const int m = idx % BM, kp = idx / BM;
+        const int m = idx % BM, c = idx / BM;
         const int gm = min(m0 + m, M - 1);
-        const int gk0 = min(k0 + 2 * kp, K - 1), gk1 = min(k0 + 2 * kp + 1, K - 1);
-        rf0[i] = Af[gm + (long)gk0 * g.lda];
-        rf1[i] = Af[gm + (long)gk1 * g.lda];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          rf[i][e] = Af[gm + (long)min(k0 + 8 * c + e, K - 1) * g.lda];
       }
     } else {
 #pragma unroll
       for (int i = 0; i < AV; ++i) {
-        const int idx = tid + i * NT;
-        const int r = idx / 4, kq = idx % 4;
-        const long off = (long)(m0 + r) * g.lda + k0 + 8 * kq;  // planes are row/K padded
+        int r, c;
+        stage_rc(tid + i * NT, &r, &c);
+        const long off = (long)(m0 + r) * g.lda + k0 + 8 * c;  // planes are row/K padded
         rah[i] = *reinterpret_cast<const v8h*>(g.ahi + off);
         ral[i] = *reinterpret_cast<const v8h*>(g.alo + off);
       }
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
-      const int idx = tid + i * NT;
-      const int r = idx / 4, kq = idx % 4;
+      int r, c;
+      stage_rc(tid + i * NT, &r, &c);
       const int col = min(n0 + r, N - 1);
-      const long off = (long)col * g.ldb + k0 + 8 * kq;
-      rbh[i] = *reinterpret_cast<const v8h*>(g.bhi + off);
-      rbl[i] = *reinterpret_cast<const v8h*>(g.blo + off);
+      const long off = (long)col * g.ldb + k0 + 8 * c;
+      rbh[i] = *reinterpret_cast<const v8h*>(Bhi + off);
+      rbl[i] = *reinterpret_cast<const v8h*>(Blo + off);
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, int t) {
     if constexpr (AF32) {
+      const int k0 = (kt0 + t) * SB_K;
 #pragma unroll
-      for (int i = 0; i < AP; ++i) {
+      for (int i = 0; i < AV; ++i) {
         const int idx = tid + i * NT;
-        const int m = idx % BM, kp = idx / BM;
-        _Float16 h0, l0, h1, l1;
-        split_f16(rf0[i] * ascale, &h0, &l0);
-        split_f16(rf1[i] * ascale, &h1, &l1);
-        *reinterpret_cast<v2h*>(&Ah[buf][m * SB_LD + 2 * kp]) = v2h{h0, h1};
-        *reinterpret_cast<v2h*>(&Al[buf][m * SB_LD + 2 * kp]) = v2h{l0, l1};
+        const int m = idx % BM, c = idx / BM;
+        v8h h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          _Float16 he, le;
+          // k beyond K reads a clamped element: zero it (the B planes are zero there too)
+          split_f16(k0 + 8 * c + e < K ? rf[i][e] * ascale : 0.f, &he, &le);
+          h[e] = he;
+          l[e] = le;
+        }
+        *reinterpret_cast<v8h*>(&Ah[buf][chunk_off(BM, m, c)]) = h;
+        *reinterpret_cast<v8h*>(&Al[buf][chunk_off(BM, m, c)]) = l;
       }
     } else {
 #pragma unroll
       for (int i = 0; i < AV; ++i) {
-        const int idx = tid + i * NT;
-        const int r = idx / 4, kq = idx % 4;
-        *reinterpret_cast<v8h*>(&Ah[buf][r * SB_LD + 8 * kq]) = rah[i];
-        *reinterpret_cast<v8h*>(&Al[buf][r * SB_LD + 8 * kq]) = ral[i];
+        int r, c;
+        stage_rc(tid + i * NT, &r, &c);
+        *reinterpret_cast<v8h*>(&Ah[buf][chunk_off(BM, r, c)]) = rah[i];
+        *reinterpret_cast<v8h*>(&Al[buf][chunk_off(BM, r, c)]) = ral[i];
       }
     }
 #pragma unroll
     for (int i = 0; i < BV; ++i) {
-      const int idx = tid + i * NT;
-      const int r = idx / 4, kq = idx % 4;
-      *reinterpret_cast<v8h*>(&Bh[buf][r * SB_LD + 8 * kq]) = rbh[i];
-      *reinterpret_cast<v8h*>(&Bl[buf][r * SB_LD + 8 * kq]) = rbl[i];
+      int r, c;
+      stage_rc(tid + i * NT, &r, &c);
+      if constexpr (TRIB) {
+        // m = k0 + 8 c + e against n = n0 + r; only the k tiles that reach into this
+        // column block's diagonal need the mask (a uniform branch)
+        const int kk = (kt0 + t) * SB_K;
+        if (kk < n0 + BN) {
+          const int d = kk + 8 * c - (n0 + r);
+          v8h f;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {  // bit patterns of 1.0, 0.5, 0.0 in FP16
+            const unsigned short bits = d + e > 0 ? 0x3C00 : (d + e == 0 ? 0x3800 : 0);
+            f[e] = __builtin_bit_cast(_Float16, bits);
+          }
+          rbh[i] *= f;
+          rbl[i] *= f;
+        }
+      }
+      *reinterpret_cast<v8h*>(&Bh[buf][chunk_off(BN, r, c)]) = rbh[i];
+      *reinterpret_cast<v8h*>(&Bl[buf][chunk_off(BN, r, c)]) = rbl[i];
     }
   };
 
-  load(0);
-  store(0);
+  if (nkt > 0) {
+    load(0);
+    store(0, 0);
+  }
   __syncthreads();
-  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  const int fr = lane & 15, fc = lane >> 4;
   for (int t = 0; t < nkt; ++t) {
     const int buf = t & 1;
     if (t + 1 < nkt) load(t + 1);
-    v8h bh[FN], bl[FN];
+    v8h bh[FN], bl[FN], ah[FM], al[FM];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int r = wn * WN + j * 16 + fr;
-      bh[j] = *reinterpret_cast<const v8h*>(&Bh[buf][r * SB_LD + fk]);
-      bl[j] = *reinterpret_cast<const v8h*>(&Bl[buf][r * SB_LD + fk]);
+      bh[j] = *reinterpret_cast<const v8h*>(&Bh[buf][chunk_off(BN, r, fc)]);
+      bl[j] = *reinterpret_cast<const v8h*>(&Bl[buf][chunk_off(BN, r, fc)]);
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const int r = wm * WM + i * 16 + fr;
-      const v8h ah = *reinterpret_cast<const v8h*>(&Ah[buf][r * SB_LD + fk]);
-      const v8h al = *reinterpret_cast<const v8h*>(&Al[buf][r * SB_LD + fk]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
-      }
+      ah[i] = *reinterpret_cast<const v8h*>(&Ah[buf][chunk_off(BM, r, fc)]);
+      al[i] = *reinterpret_cast<const v8h*>(&Al[buf][chunk_off(BM, r, fc)]);
     }
-    if (t + 1 < nkt) store(buf ^ 1);
+    // the three products, each over all FM x FN accumulators in turn: consecutive MFMAs
+    // write different accumulators, so none waits for the previous one's result
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    if (t + 1 < nkt) store(buf ^ 1, t + 1);
     __syncthreads();
   }
 
-  // D(16x16) of v_mfma_f32_16x16x32_f16: lane l holds column l%16, rows 4*(l/16) + r.
-  float* Cz = g.C + (long)z * g.sC;
-  const float bscale = g.batch_inv_scale ? g.batch_inv_scale[z] : 1.f;
-  const int qa = g.col_div > 0 ? n0 / g.col_div : 0;
-  float mx_a = 0.f, mx_b = 0.f;
+  split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
+}
+
+// ---- f16-plane GEMM with LDS-direct loads ---------------------------------------------
+//
+// Both operands are FP16 planes (A [row][k], B [col][k]).  Every k tile of A and B goes
+// from HBM straight into LDS (global_load_lds_dwordx4: lane l writes 16 bytes at the
+// instruction's LDS base + 16 l), NST - 1 tiles ahead of the one being multiplied, so no
+// staging registers are spent on it and the loads of several tiles are in flight.
+// LDS tiles come in blocks of 16 rows, K-major inside a block: chunk c (k = 8c..8c+7) of
+// row r at (r / 16) * 1024 + c * 256 + (r % 16) * 16 bytes.  One load instruction fills
+// one block (lane l: row l % 16, chunk l / 16 -> the 64 contiguous global bytes of each
+// of 16 rows), and the 16 lanes of an MFMA fragment read that share a chunk read 256
+// contiguous bytes, free of bank conflicts.
+
+// s_waitcnt vmcnt(N), no wait on the other counters (gfx9 encoding: vmcnt in [3:0] and
+// [15:14], expcnt [6:4], lgkmcnt [11:8])
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, int tiles_m,
+                                                                  int tiles_n) {
+  constexpr int NW = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
+  constexpr int A_BYTES = BM * 128, B_BYTES = BN * 128;  // 2 planes x rows x 64 B
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int NA = BM / 8, NB = BN / 8;  // load instructions per k tile (16 rows x 64 B)
+  constexpr int LW = (NA + NB) / NW;       // per wavefront
+  static_assert(BM % 16 == 0 && BN % 16 == 0 && (NA + NB) % NW == 0, "load split");
+  static_assert(NST >= 2 && NST <= 4 && (NST - 2) * LW < 64, "pipeline depth");
+  static_assert(FM >= 1 && FN >= 1, "wave tile");
+  __shared__ __attribute__((aligned(16))) unsigned char smem[NST * STAGE];
+
+  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
+  if (blockIdx.x >= total) return;
+  const unsigned lid = xcd_remap16(blockIdx.x, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int kt0 = TRIB ? n0 / SB_K : 0;
+  const int nkt = (g.K + SB_K - 1) / SB_K - kt0;
+
+  // this wavefront's load instructions: q = wave + NW i; A for q < NA, else B; each moves
+  // the 16-row block grp of one plane
+  const _Float16* src[LW];
+  int dst[LW];
 #pragma unroll
-  for (int j = 0; j < FN; ++j) {
-    const int col = n0 + wn * WN + j * 16 + fr;
-    if (col >= N) continue;
-    float f = g.cs[1] * bscale;
-    int q = 0;
-    if (g.col_inv_scale) {
-      q = col / g.col_div;
-      f *= g.col_inv_scale[q];
+  for (int i = 0; i < LW; ++i) {
+    const int q = wave + NW * i;
+    const bool isA = q < NA;
+    const int qq = isA ? q : q - NA;
+    const int rows = isA ? BM : BN;
+    const int p = qq / (rows / 16), grp = qq % (rows / 16);
+    const int r = 16 * grp + (lane & 15), c = lane >> 4;
+    if (isA) {
+      src[i] = (p ? g.alo : g.ahi) + (long)(m0 + r) * g.lda + 8 * c;  // planes row-padded
+    } else {
+      const int col = min(n0 + r, g.N - 1);
+      src[i] = (p ? g.blo : g.bhi) + (long)z * g.sB + (long)col * g.ldb + 8 * c;
+    }
+    dst[i] = (isA ? 0 : A_BYTES) + p * rows * 64 + grp * 1024;
+  }
+  auto issue = [&](int t) {
+    const int kofs = (kt0 + t) * SB_K;
+    unsigned char* base = smem + (t % NST) * STAGE;
+#if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: the host pass only needs the stub
+#pragma unroll
+    for (int i = 0; i < LW; ++i)
+      __builtin_amdgcn_global_load_lds(src[i] + kofs, base + dst[i], 16, 0, 0);
+#else
+    (void)kofs;
+    (void)base;
+#endif
+  };
+
+  v4f acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int s = 0; s < NST - 1; ++s)
+    if (s < nkt) issue(s);
+  const int fr = lane & 15, fc = lane >> 4;
+  for (int t = 0; t < nkt; ++t) {
+    // this wavefront's loads of tile t have landed once at most `ahead` tiles' remain
+    const int ahead = min(NST - 2, nkt - 1 - t);
+    if (ahead >= 2) wait_vmcnt<2 * LW>();
+    else if (ahead == 1) wait_vmcnt<LW>();
+    else wait_vmcnt<0>();
+    // Everyone's tile t is in LDS and everyone is done reading tile t - 1.  A bare
+    // s_barrier: __syncthreads() would also wait for every outstanding load (vmcnt(0)),
+    // draining the tiles in flight; the LDS reads of tile t - 1 are complete already
+    // (their values went into the MFMAs), and lgkmcnt(0) makes sure of it.
+    __builtin_amdgcn_s_waitcnt((15 & 15) | (3 << 14) | (7 << 4) | (0 << 8));  // lgkmcnt(0)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (t + NST - 1 < nkt) issue(t + NST - 1);
+    const unsigned char* A = smem + (t % NST) * STAGE;
+    const unsigned char* B = A + A_BYTES;
+    v8h bh[FN], bl[FN], ah[FM], al[FM];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int o = (wn * WN + j * 16) * 64 + fc * 256 + fr * 16;  // 16-row block, chunk fc
+      bh[j] = *reinterpret_cast<const v8h*>(B + o);
+      bl[j] = *reinterpret_cast<const v8h*>(B + BN * 64 + o);
+    }
+    if constexpr (TRIB) {
+      // Lh: m = kk + 8 fc + e against column n; only k tiles reaching into this column
+      // block's diagonal need it (a uniform branch)
+      const int kk = (kt0 + t) * SB_K;
+      if (kk < n0 + BN) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int d = kk + 8 * fc - (n0 + wn * WN + j * 16 + fr);
+          v8h f;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {  // bit patterns of 1.0, 0.5, 0.0 in FP16
+            const unsigned short bits = d + e > 0 ? 0x3C00 : (d + e == 0 ? 0x3800 : 0);
+            f[e] = __builtin_bit_cast(_Float16, bits);
+          }
+          bh[j] *= f;
+          bl[j] *= f;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int o = (wm * WM + i * 16) * 64 + fc * 256 + fr * 16;
+      ah[i] = *reinterpret_cast<const v8h*>(A + o);
+      al[i] = *reinterpret_cast<const v8h*>(A + BM * 64 + o);
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + rr;
-        if (row < M) {
-          const float v = acc[i][j][rr] * f;
-          Cz[row + (long)col * g.ldc] = v;
-          if (q == qa) mx_a = fmaxf(mx_a, fabsf(v));
-          else if (q == qa + 1) mx_b = fmaxf(mx_b, fabsf(v));
-          else if (g.qmax) atomicMax(reinterpret_cast<unsigned*>(&g.qmax[q]),
-                                     __float_as_uint(fabsf(v)));
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+  }
+  split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
+}
+
+// ---- f16-plane GEMM, B streamed straight into registers ---------------------------
+//
+// For the first transform, where B (the b planes) is the large stream and A (C5^T
+// planes) is small and shared: every wavefront owns all BM = 128 rows of the tile and FN
+// 16-column fragments, loads its B fragments from global memory directly in MFMA layout
+// (lane l: column l % 16, k = 8 (l / 16)..+7, 16 contiguous bytes), one k tile ahead, and
+// reads only A from LDS.  LDS traffic per MFMA is then set by the A fragments alone
+// (FN MFMA columns share each), a quarter of what a tile staging both operands needs.
+template <int FN, int NWAVE, bool TRIB>
+__global__ __launch_bounds__(64 * NWAVE) void split_bdirect_kernel(SplitGemm g, int tiles_m,
+                                                                  int tiles_n) {
+  constexpr int BM = 128, FM = BM / 16, WN = 16 * FN, BN = WN * NWAVE, NT = 64 * NWAVE;
+  constexpr int AV = BM * 4 / NT;  // A chunks per thread and plane
+  static_assert(BM * 4 % NT == 0 && AV >= 1, "A staging");
+  __shared__ _Float16 Ah[2][BM * SB_K], Al[2][BM * SB_K];
+
+  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
+  if (blockIdx.x >= total) return;
+  const unsigned lid = xcd_remap16(blockIdx.x, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int kt0 = TRIB ? n0 / SB_K : 0;
+  const int nkt = (g.K + SB_K - 1) / SB_K - kt0;
+  const int fr = lane & 15, fc = lane >> 4;
+
+  // B: this lane's column of each fragment j (clamped; columns past N are not stored)
+  const _Float16* bh_p[FN];
+  const _Float16* bl_p[FN];
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = min(n0 + wave * WN + j * 16 + fr, g.N - 1);
+    const long off = (long)z * g.sB + (long)col * g.ldb + 8 * fc;
+    bh_p[j] = g.bhi + off;
+    bl_p[j] = g.blo + off;
+  }
+  // A staging: chunk (r, c) of the K-major LDS tile per thread and plane
+  const _Float16* ah_p[AV];
+  const _Float16* al_p[AV];
+  int a_lds[AV];
+#pragma unroll
+  for (int i = 0; i < AV; ++i) {
+    int r, c;
+    stage_rc(tid + i * NT, &r, &c);
+    const long off = (long)(m0 + r) * g.lda + 8 * c;  // planes are row/K padded
+    ah_p[i] = g.ahi + off;
+    al_p[i] = g.alo + off;
+    a_lds[i] = chunk_off(BM, r, c);
+  }
+
+  v4f acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  v8h rah[AV], ral[AV];
+  auto load_a = [&](int t) {
+    const int k0 = (kt0 + t) * SB_K;
+#pragma unroll
+    for (int i = 0; i < AV; ++i) {
+      rah[i] = *reinterpret_cast<const v8h*>(ah_p[i] + k0);
+      ral[i] = *reinterpret_cast<const v8h*>(al_p[i] + k0);
+    }
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < AV; ++i) {
+      *reinterpret_cast<v8h*>(&Ah[buf][a_lds[i]]) = rah[i];
+      *reinterpret_cast<v8h*>(&Al[buf][a_lds[i]]) = ral[i];
+    }
+  };
+  auto load_b = [&](int t, v8h (&bh)[FN], v8h (&bl)[FN]) {
+    const int k0 = (kt0 + t) * SB_K;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      bh[j] = *reinterpret_cast<const v8h*>(bh_p[j] + k0);
+      bl[j] = *reinterpret_cast<const v8h*>(bl_p[j] + k0);
+    }
+  };
+  auto step = [&](int t, int buf, v8h (&bh)[FN], v8h (&bl)[FN]) {
+    if constexpr (TRIB) {
+      // Lh: m = kk + 8 fc + e against column n; only k tiles reaching into the block's
+      // diagonal need it (a uniform branch)
+      const int kk = (kt0 + t) * SB_K;
+      if (kk < n0 + BN) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int d = kk + 8 * fc - (n0 + wave * WN + j * 16 + fr);
+          v8h f;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {  // bit patterns of 1.0, 0.5, 0.0 in FP16
+            const unsigned short bits = d + e > 0 ? 0x3C00 : (d + e == 0 ? 0x3800 : 0);
+            f[e] = __builtin_bit_cast(_Float16, bits);
+          }
+          bh[j] *= f;
+          bl[j] *= f;
         }
       }
-  }
-  if (g.qmax) {
-    for (int off = 32; off > 0; off >>= 1) {
-      mx_a = fmaxf(mx_a, __shfl_down(mx_a, off, 64));
-      mx_b = fmaxf(mx_b, __shfl_down(mx_b, off, 64));
     }
-    if (lane == 0) {
-      const int nq = N / g.col_div;
-      atomicMax(reinterpret_cast<unsigned*>(&g.qmax[qa]), __float_as_uint(mx_a));
-      if (qa + 1 < nq) atomicMax(reinterpret_cast<unsigned*>(&g.qmax[qa + 1]), __float_as_uint(mx_b));
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const v8h ah = *reinterpret_cast<const v8h*>(&Ah[buf][chunk_off(BM, i * 16 + fr, fc)]);
+      const v8h al = *reinterpret_cast<const v8h*>(&Al[buf][chunk_off(BM, i * 16 + fr, fc)]);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
     }
+  };
+
+  // two B register sets, ping-pong: tile t + 1's fragments load while tile t multiplies
+  v8h bh0[FN], bl0[FN], bh1[FN], bl1[FN];
+  if (nkt > 0) {
+    load_a(0);
+    load_b(0, bh0, bl0);
+    store_a(0);
   }
+  __syncthreads();
+  for (int t = 0; t < nkt; t += 2) {
+    if (t + 1 < nkt) {
+      load_a(t + 1);
+      load_b(t + 1, bh1, bl1);
+    }
+    step(t, 0, bh0, bl0);
+    if (t + 1 < nkt) store_a(1);
+    __syncthreads();
+    if (t + 1 >= nkt) break;
+    if (t + 2 < nkt) {
+      load_a(t + 2);
+      load_b(t + 2, bh0, bl0);
+    }
+    step(t + 1, 1, bh1, bl1);
+    if (t + 2 < nkt) store_a(0);
+    __syncthreads();
+  }
+  split_epilogue<FM, FN>(g, acc, m0, n0 + wave * WN, n0, z, lane);
 }
 
 // per-batch scale for the FP32 A of the second transform
@@ -358,18 +745,21 @@ int split_prepare_b(hipStream_t s, const double* b, int nbf, int naux, int ld8, 
 
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
                     float* cscale, _Float16* hi, _Float16* lo) {
-  hipLaunchKernelGGL(c_split_kernel, dim3(1), dim3(1024), 0, s, C, nbf, nbf5, Mpad, Kpad,
-                     cscale, hi, lo);
+  const long n = (long)nbf * nbf5;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, cscale + 2, 1L, 0.f);
+  hipLaunchKernelGGL(c_max_kernel, dim3(grid_of(n)), dim3(256), 0, s, C, n, cscale);
+  hipLaunchKernelGGL(c_planes_kernel, dim3(grid_of((long)Mpad * Kpad)), dim3(256), 0, s, C, nbf,
+                     nbf5, Mpad, Kpad, cscale, hi, lo);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
-template <int BM, int BN, int WGM, int WGN, bool AF32>
+template <int BM, int BN, int WGM, int WGN, bool AF32, bool TRIB = false>
 static int launch_split(hipStream_t s, const SplitGemm& g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const long total = (long)tm * tn * g.nbatch;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split gemm grid too large");
-  hipLaunchKernelGGL((split_gemm_kernel<BM, BN, WGM, WGN, AF32>), dim3((unsigned)total),
+  hipLaunchKernelGGL((split_gemm_kernel<BM, BN, WGM, WGN, AF32, TRIB>), dim3((unsigned)total),
                      dim3(64 * WGM * WGN), 0, s, g, tm, tn);
   DN_HIP(hipGetLastError());
   return DN_OK;
@@ -392,7 +782,32 @@ int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
   g1.C = bpn; g1.ldc = nbf5; g1.sC = 0;
   g1.cs = cscale_dev; g1.col_inv_scale = b_inv_scale; g1.col_div = nbf;
   g1.qmax = qmax; g1.nbatch = 1;
-  return launch_split<128, 128, 2, 4, false>(s, g1);
+  return launch_split<128, 128, 2, 2, false>(s, g1);
+}
+
+// Symmetric b: bpn[p,n,Q] = sum_{m >= n} C5[m,p] Lh[m,n,Q] (b_Q = Lh + Lh^T), batched
+// over Q so that column block n0 reads only rows m >= n0 of the slab; the second
+// transform then yields W with b_pq = W + W^T.
+int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+                 const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                 const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
+                 float* qmax) {
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_of(naux)), dim3(256), 0, s, qmax, (long)naux, 0.f);
+  DN_HIP(hipGetLastError());
+  SplitGemm g1{};
+  g1.M = nbf5; g1.N = nbf; g1.K = nbf;
+  g1.ahi = chi; g1.alo = clo; g1.lda = Kpad;
+  g1.bhi = bhi; g1.blo = blo; g1.ldb = ld8; g1.sB = (long)nbf * ld8;
+  g1.C = bpn; g1.ldc = nbf5; g1.sC = (long)nbf5 * nbf;
+  g1.cs = cscale_dev; g1.batch_inv_scale = b_inv_scale; g1.col_div = 0;
+  g1.qmax = qmax; g1.nbatch = naux;
+  const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
+  const long total = (long)tm * tn * g1.nbatch;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split gemm grid too large");
+  hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 2, 2, true>), dim3((unsigned)total), dim3(256),
+                     0, s, g1, tm, tn);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
 }
 
 int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
@@ -408,7 +823,7 @@ int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float1
   g2.C = bpq; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
   g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.col_div = 0;
   g2.nbatch = naux;
-  return launch_split<128, 128, 2, 4, true>(s, g2);
+  return launch_split<128, 128, 2, 2, true>(s, g2);
 }
 
 }  // namespace dn

@@ -10,14 +10,18 @@ from donof_amd._lib import DN_SPLIT_F16
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("use_sym", [True, False])
 @pytest.mark.parametrize("nbf,naux,nbf5,ndoc", [(64, 128, 20, 5), (130, 300, 45, 5),
-                                                (200, 800, 60, 5), (256, 512, 120, 5)])
-def test_split_jk_against_oracle(oracle, nbf, naux, nbf5, ndoc):
-    print("case split", nbf, naux, nbf5, flush=True)
+                                                (200, 800, 60, 5), (256, 512, 120, 5),
+                                                (37, 70, 10, 5)])
+def test_split_jk_against_oracle(oracle, nbf, naux, nbf5, ndoc, use_sym):
+    print("case split", nbf, naux, nbf5, use_sym, flush=True)
     s = synthetic.SyntheticSystem(nbf, naux, nbf5, 31 + nbf, ndoc=ndoc)
     p = s.p
     b = synthetic.make_b(nbf, naux, s.seed)
     dev = api.System(p, precision=DN_SPLIT_F16).upload_b(b)
+    assert dev.b_symmetric
+    dev.set_symmetry(use_sym)
     J, K, Hc = api.computeJKH_MO(s.C, s.H, dev, p)
     Jr, Kr, Hr = oracle.jkh_mo(s.C, s.H, b, nbf5)
     ej = np.abs(J - Jr).max() / np.abs(Jr).max()
