@@ -5,16 +5,25 @@ nbf5=120, b generated in HBM by the Philox generator, C = Q of a seeded QR, H sy
 N(0,1), GNOF pairing ndoc=5, ncwo=23.  One step = one J/K-MO build (the RI transform
 b_pqQ = C5^T b_mnQ C5, J, K and H_core) with b, C and H already resident in HBM.
 
-The dominant kernel is the first transform b_pn = C5^T b (2 nbf5 nbf^2 nbfaux FLOP per
-launch), timed with HIP events on the library's stream around every launch; its
-roofline is the FP64 matrix peak.  The CPU baseline times the NumPy oracle (BLAS-backed)
-on the same tensor on the host cores.
+The dominant kernel is the first transform.  b is symmetric in (m, n), so it runs on the
+lower half of b: T = C5^T Lh with b_Q = Lh_Q + Lh_Q^T (and C5^T b_Q C5 = W + W^T,
+W = T C5).  It is timed with HIP events on the library's stream around every launch.
+roofline.achieved counts the reference's 2 nbf5 nbf^2 nbfaux FLOP per launch; the
+executed_* fields count the MFMA work the kernel really does.  roofline.traffic is the
+HBM bytes per launch that rocprofv3 counted (profiles/jk_build_<precision>_nbf<nbf>_
+traffic.md), or null.  The CPU baseline times the NumPy oracle (BLAS-backed) on the same
+tensor on the host cores for about --cpu-seconds.
 
-Extra legs (--no-extra to skip): the same build at nbf=800, nbfaux=3200, nbf5=240 in FP64
-and in split precision (FP16 hi+lo on MFMA, FP32 accumulation), their ratio and the split
-path's J/K/energy error against the FP64 device path on the same tensor and orbitals.
+Extra legs (--legs, default all; --no-extra for none):
+  800     the same build at nbf=800, nbfaux=3200, nbf5=240 in FP64 and in split precision
+          (FP16 hi+lo on MFMA, FP32 accumulation): their ratio and the split path's J/K and
+          GNOF-energy error against the FP64 device path on the same tensor and orbitals;
+  grouped 64 systems, nbf ~ U[48, 400], built together by grouped persistent kernels in
+          one captured HIP graph, against per-system builds on their own streams;
+  served  Poisson arrivals of orbital-optimisation solves, p50/p99 latency and solves/s.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu-baseline] [--no-extra]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
+                    [--legs 800,grouped,served] [--no-cpu-baseline] [--no-extra]
 """
 from __future__ import annotations
 
@@ -56,7 +65,7 @@ def cpu_baseline(C, H, b, nbf5, budget_s=20.0):
     while True:
         orc.jkh_mo(C, H, b, nbf5)
         n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 10:
+        if time.perf_counter() - t0 > budget_s or n >= 10000:
             break
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "builds/s", "cores": threads or os.cpu_count(),
@@ -66,13 +75,20 @@ def cpu_baseline(C, H, b, nbf5, budget_s=20.0):
                       % (n, b.shape[0], b.shape[2], nbf5, threads, dt)}
 
 
-def executed_flops_sym_k1(nbf, naux, nbf5, bm=64, bn=128, bk=16):
-    """MFMA flops the symmetric-b first transform really performs: 64-row blocks of
-    T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p] start their K loop at the 16-aligned k below
-    the block's first row, and the 128-wide column tiles are padded."""
-    tiles_n = -(-nbf5 // bn)
-    rows = sum(nbf - (m0 // bk) * bk for m0 in range(0, nbf, bm))
-    return 2.0 * bm * bn * tiles_n * rows * naux
+def executed_flops_sym_k1(nbf, naux, nbf5, precision="fp64"):
+    """MFMA flops the symmetric-b first transform really performs.
+    fp64: 64-row blocks of T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p] start their K loop at
+    the 16-aligned k below the block's first row; the 128-wide p tiles are padded.
+    split: 128-column blocks of T[p,n,Q] start at the 32-aligned k below the block's first
+    column; 128-row p tiles are padded; every tile issues three products (hi*hi, hi*lo,
+    lo*hi)."""
+    if precision == "fp64":
+        tiles = -(-nbf5 // 128)
+        rows = sum(nbf - (m0 // 16) * 16 for m0 in range(0, nbf, 64))
+        return 2.0 * 64 * 128 * tiles * rows * naux
+    mpad = -(-nbf5 // 128) * 128
+    cols = sum(nbf - (n0 // 32) * 32 for n0 in range(0, nbf, 128))
+    return 3 * 2.0 * mpad * 128 * cols * naux
 
 
 def profile_value(name, key):
@@ -250,6 +266,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--precision", choices=("fp64", "split"), default="fp64",
+                    help="arithmetic of the headline build (fp64 is the metric's)")
+    ap.add_argument("--legs", default="800,grouped,served",
+                    help="extra legs to run: any of 800, grouped, served (or none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X and this "
@@ -261,17 +281,17 @@ def main():
     from donof_amd import api, synthetic
 
     nbf, naux, nbf5 = args.nbf, args.nbfaux, args.nbf5
-    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, args.seed, "fp64")
+    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, args.seed, args.precision)
     log("system resident, warmup %d" % args.warmup)
     ms_step, k_avg_ms, k_n = time_builds(dev, args.steps, args.warmup)
     F = synthetic.build_flops(nbf, naux, nbf5)
     k_flops = 2.0 * nbf5 * nbf * nbf * naux
     k_tflops = k_flops / (k_avg_ms * 1e-3) / 1e12
-    x_flops = executed_flops_sym_k1(nbf, naux, nbf5)
+    x_flops = executed_flops_sym_k1(nbf, naux, nbf5, args.precision)
     x_tflops = x_flops / (k_avg_ms * 1e-3) / 1e12
-    traffic = None
-    if (nbf, naux, nbf5) == (400, 1600, 120):
-        traffic = profile_value("jk_build_fp64_nbf400_traffic.md", "traffic_bytes_per_launch")
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else F16_PEAK_TFLOPS
+    traffic = profile_value("jk_build_%s_nbf%d_traffic.md" % (args.precision, nbf),
+                            "traffic_bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": 1e3 / ms_step,
@@ -283,27 +303,36 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f16x2 (hi+lo) products, f32 accumulate",
         "data": "synthetic (Philox-generated b in HBM, seeded QR orbitals)",
-        "config": {"workload": "fp64 J/K-MO build nbf=%d nbfaux=%d nbf5=%d" % (nbf, naux, nbf5),
-                   "nbf": nbf, "nbfaux": naux, "nbf5": nbf5, "precision": "fp64"},
+        "config": {"workload": "%s J/K-MO build nbf=%d nbfaux=%d nbf5=%d"
+                               % (args.precision, nbf, naux, nbf5),
+                   "nbf": nbf, "nbfaux": naux, "nbf5": nbf5, "precision": args.precision},
         "build_tflops": F / (ms_step * 1e-3) / 1e12,
         "pct_fp64_peak": 100.0 * F / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "roofline": {"bound": "mfma",
-                     "kernel": "dgemm_mfma symmetric-b first transform (T = C5^T Lh, half of b)",
-                     "achieved": k_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": k_tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": ("dgemm_mfma" if args.precision == "fp64" else "split_lds_kernel")
+                               + " symmetric-b first transform (lower half of b)",
+                     "achieved": k_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": k_tflops / peak, "traffic": traffic,
                      "kernel_ms": k_avg_ms, "launches": k_n,
                      "algorithmic_flops_per_launch": k_flops,
                      "executed_flops_per_launch": x_flops,
                      "executed_tflops": x_tflops,
-                     "executed_frac": x_tflops / FP64_PEAK_TFLOPS,
+                     "executed_frac": x_tflops / peak,
                      "note": "achieved counts 2*nbf5*nbf^2*nbfaux (the reference's contraction); "
                              "the kernel uses b[m,n,Q] = b[n,m,Q] and performs only the executed "
                              "flops, so achieved can exceed the FP64 peak; executed_frac is the "
                              "matrix-core utilisation"},
     }
     log("%.3f ms/build, transform kernel %.3f ms (%.1f TFLOP/s)" % (ms_step, k_avg_ms, k_tflops))
+    # the same build through the host-array boundary (dn_jkh_mo: C and H uploaded, J, K,
+    # H_core downloaded on every call), for the PCIe-inclusive rate
+    api.computeJKH_MO(sysd.C, sysd.H, dev, sysd.p)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        api.computeJKH_MO(sysd.C, sysd.H, dev, sysd.p)
+    out["host_boundary_ms_per_build"] = (time.perf_counter() - t0) * 1e3 / 5
     b = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -314,11 +343,14 @@ def main():
         del b
     else:
         out["cpu_baseline"] = None
-    if not args.no_extra:
+    legs = set() if args.no_extra else set(x for x in args.legs.split(",") if x)
+    if "800" in legs:
         out["extra_800_3200_240"] = extra_legs(api, synthetic, max(5, args.steps // 2),
                                                args.warmup, args.seed)
+    if "grouped" in legs:
         out["extra_grouped_64"] = grouped_leg(api, synthetic, max(3, args.steps // 4),
                                               args.warmup, args.seed)
+    if "served" in legs:
         from donof_amd import serve
         out["extra_served"] = serving_leg(api, serve)
     print(json.dumps(out))
