@@ -237,21 +237,24 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
 def serving_leg(api, serve, n=24, rate=4.0, seed=7, max_live=8):
     """Served solves: Poisson arrivals at `rate`/s of GNOF orbital optimisations (ADAM, at
     most 200 iterations, stop when 2 max|g| < 1e-4), nbf log-uniform in [50, 600],
-    naux = 4 nbf, nbf5 = ceil(nbf/3); admission by the device's HBM budget."""
-    reqs = serve.synthetic_requests(n, rate, seed=seed, nbf_min=50, nbf_max=600,
-                                    max_iter=200, gtol=1e-4)
-    srv = serve.Server(hbm_budget=0, max_live=max_live)
-    budget = srv.ctx.hbm_budget
-    t0 = time.perf_counter()
-    res = srv.run(reqs)
-    wall = time.perf_counter() - t0
-    srv.close()
-    st = serve.latency_stats(res, wall)
-    log("serving: %d solves in %.1f s, p50 %.3f s, p99 %.3f s"
-        % (st["solves"], wall, st["p50_s"] or 0.0, st["p99_s"] or 0.0))
-    return {"requests": n, "arrival_rate_per_s": rate, "nbf_range": [50, 600],
-            "max_live": max_live, "hbm_budget_bytes": budget, "wall_s": wall,
-            "errors": sum(1 for r in res if r.error), **st}
+    naux = 4 nbf, nbf5 = ceil(nbf/3); admission by the device's HBM budget.  The lock-step
+    grouped engine is the served path; one stream per solve is timed beside it."""
+    out = {"requests": n, "arrival_rate_per_s": rate, "nbf_range": [50, 600],
+           "max_live": max_live, "max_iter": 200, "gtol": 1e-4}
+    for mode in ("grouped", "streams"):
+        reqs = serve.synthetic_requests(n, rate, seed=seed, nbf_min=50, nbf_max=600,
+                                        max_iter=200, gtol=1e-4)
+        srv = serve.Server(hbm_budget=0, max_live=max_live, mode=mode)
+        out["hbm_budget_bytes"] = srv.ctx.hbm_budget
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        srv.close()
+        st = serve.latency_stats(res, wall)
+        log("serving (%s): %d solves in %.1f s, p50 %.3f s, p99 %.3f s"
+            % (mode, st["solves"], wall, st["p50_s"] or 0.0, st["p99_s"] or 0.0))
+        out[mode] = {"wall_s": wall, "errors": sum(1 for r in res if r.error), **st}
+    return out
 
 
 def main():

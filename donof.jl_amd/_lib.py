@@ -83,6 +83,12 @@ _SIGS = {
     "dn_sys_set_gamma": (_I, [_P, _D]),
     "dn_sys_get_orbitals": (_I, [_P, _D]),
     "dn_orbopt_resident": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state)]),
+    "dn_orbopt_start": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state)]),
+    "dn_orbopt_done": (_I, [_P, ct.POINTER(ct.c_int)]),
+    "dn_orbopt_finish": (_I, [_P, ct.POINTER(dn_orbopt_state)]),
+    "dn_ogroup_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
+    "dn_ogroup_destroy": (_I, [_P]),
+    "dn_ogroup_step": (_I, [_P, ct.POINTER(ct.c_int)]),
     "dn_group_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
     "dn_group_destroy": (_I, [_P]),
     "dn_group_tune": (_I, [_P, ct.POINTER(ct.c_int)]),

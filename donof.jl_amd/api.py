@@ -121,6 +121,40 @@ class System:
         check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))
         return self
 
+    def set_gamma(self, gamma):
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        check(self.lib.dn_sys_set_gamma(self.handle, dptr(g)))
+        self.sync()
+
+    def get_orbitals(self):
+        out = np.empty((self.nbf, self.nbf), order="F")
+        check(self.lib.dn_sys_get_orbitals(self.handle, dptr(out)))
+        return out
+
+    # -- step-wise ADAM family (dn_orbopt_start / dn_ogroup_step / dn_orbopt_finish) ----
+    def orbopt_start(self, method, p):
+        """Start an ADAM-family run at the resident gamma, C and H (minimization.jl:344-360)."""
+        f = functional_of(p)
+        st = _lib.dn_orbopt_state()
+        st.method = _ORB[method]
+        st.maxloop, st.alpha, st.threshgorb = int(p.maxloop), float(p.alpha), float(p.threshgorb)
+        check(self.lib.dn_orbopt_start(self.handle, ct.byref(f), ct.byref(st)))
+        return self
+
+    def orbopt_done(self):
+        v = ct.c_int()
+        check(self.lib.dn_orbopt_done(self.handle, ct.byref(v)))
+        return bool(v.value)
+
+    def orbopt_finish(self, p=None):
+        """(E, nit, success) of the finished run; p.alpha / p.maxloop updated as the
+        reference does when p is given; the orbitals stay resident (get_orbitals)."""
+        st = _lib.dn_orbopt_state()
+        check(self.lib.dn_orbopt_finish(self.handle, ct.byref(st)))
+        if p is not None:
+            p.maxloop, p.alpha = int(st.maxloop), float(st.alpha)
+        return st.E, int(st.nit), bool(st.success)
+
     def set_symmetry(self, use):
         """Use (default) or ignore the (m, n) symmetry of b in the J/K build."""
         check(self.lib.dn_sys_set_symmetry(self.handle, 1 if use else 0))
@@ -236,6 +270,36 @@ class Group:
     def close(self):
         if getattr(self, "handle", None):
             self.lib.dn_group_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OGroup:
+    """Started systems advanced together, one iteration per step (dn_ogroup_*): the
+    Lagrangian of all of them runs as grouped persistent GEMMs in one HIP graph."""
+
+    def __init__(self, systems, ctx=None):
+        self.systems = list(systems)
+        self.ctx = ctx if ctx is not None else self.systems[0].ctx
+        self.lib = self.ctx.lib
+        arr = (ct.c_void_p * len(self.systems))(*[s.handle for s in self.systems])
+        h = ct.c_void_p()
+        check(self.lib.dn_ogroup_create(self.ctx.handle, arr, len(self.systems), ct.byref(h)))
+        self.handle = h
+
+    def step(self):
+        n = ct.c_int()
+        check(self.lib.dn_ogroup_step(self.handle, ct.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.dn_ogroup_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

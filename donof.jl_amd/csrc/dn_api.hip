@@ -201,27 +201,30 @@ int transform2(dn_sys* s, int ncols, double* out) {
   return gemm_f64(s->stream, g);
 }
 
-// HC = H C5 ; optionally Hmat = C^T HC
-int hc_products(dn_sys* s, bool want_hmat) {
+// HC = H C5 ; optionally Hmat = C^T HC.  The J/K build splits K for parallelism; the
+// Lagrangian uses plain GEMMs, the arithmetic the grouped lock-step optimiser repeats.
+int hc_products(dn_sys* s, bool want_hmat, bool splitk = true) {
   const dn_dims& d = s->d;
   Gemm g;
   g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
   g.A = s->H; g.lda = d.nbf; g.a_kcontig = false;
   g.B = s->C; g.ldb = d.nbf; g.b_kcontig = true;
   g.C = s->HC; g.ldc = d.nbf;
-  DN_TRY(gemm_f64_splitk(s->stream, g, s->part2, s->part2_elems));
+  DN_TRY(splitk ? gemm_f64_splitk(s->stream, g, s->part2, s->part2_elems) : gemm_f64(s->stream, g));
   if (want_hmat) {
     Gemm h;
     h.M = d.nbf; h.N = d.nbf5; h.K = d.nbf;
     h.A = s->C; h.lda = d.nbf; h.a_kcontig = true;
     h.B = s->HC; h.ldb = d.nbf; h.b_kcontig = true;
     h.C = s->Hmat; h.ldc = d.nbf;
-    DN_TRY(gemm_f64_splitk(s->stream, h, s->part2, s->part2_elems));
+    DN_TRY(splitk ? gemm_f64_splitk(s->stream, h, s->part2, s->part2_elems) : gemm_f64(s->stream, h));
   }
   return DN_OK;
 }
 
 }  // namespace
+
+size_t lag_split_count(const dn_dims& d) { return lag_splits(d); }
 
 int check_sys(dn_sys* s) {
   if (!s) return fail(DN_ERR_ARG, "null dn_sys");
@@ -320,7 +323,7 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   const long slice = (long)d.nbf5 * d.nbf;
   DN_TRY(transform1(s));
   DN_TRY(transform2(s, d.nbf, s->bt));
-  DN_TRY(hc_products(s, true));
+  DN_TRY(hc_products(s, true, false));
   DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta, s->X,
                s->w));
   DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));

@@ -199,6 +199,13 @@ struct dn_sys {
   float* qs = nullptr;      // naux
   float* qinv = nullptr;    // naux
   bool b_ready = false;
+  // an ADAM-family run advanced step by step (dn_orbopt_start / dn_ogroup_step)
+  struct OptRun {
+    bool started = false, done = false, improved = false, success = false;
+    int i = 0, nit = 0, method = 0, maxloop = 0, nalpha = 0, nbeta = 0;
+    double alpha = 0, beta1 = 0, binit = 0, beta2 = 0, threshgorb = 0;
+    double E = 0, gmax = 0, best_E = 0, nrm = 0;
+  } opt;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
   dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
                                      // the split-precision one at scal + 32)

@@ -138,8 +138,9 @@ int lagrangian_and_grad(dn_sys* s, int nalpha, int nbeta, double* E, double* gma
 
 }  // namespace
 
-// The ADAM-family loop of minimization.jl:242-630 with C resident on the device.
-int orbopt_resident(dn_sys* s, const dn_functional* f, dn_orbopt_state* st) {
+// Start of an ADAM-family run (minimization.jl:344-360): occupations and coefficients from
+// the resident gamma, the Lagrangian and gradient at the resident C, zero moments.
+int orbopt_begin(dn_sys* s, const dn_functional* f, const dn_orbopt_state* st) {
   const dn_dims& d = s->d;
   OccDims P;
   DN_TRY(functional_of(s, f, &P));
@@ -150,15 +151,41 @@ int orbopt_resident(dn_sys* s, const dn_functional* f, dn_orbopt_state* st) {
   DN_TRY(occupations_resident(s, P));
   double E = 0.0, gmax = 0.0;
   DN_TRY(lagrangian_and_grad(s, d.nalpha, d.nbeta, &E, &gmax));
-  const double alpha = st->alpha;
-  double beta1 = (st->method == DN_DEMON) ? 0.6 : 0.5;
-  const double beta2 = (st->method == DN_DEMON) ? 0.999 : 0.9;
-  const double binit = beta1;
   DN_TRY(fill(s->stream, s->mv, s->nvar, 0.0));
   DN_TRY(fill(s->stream, s->vv, s->nvar, 0.0));
   DN_TRY(fill(s->stream, s->vmax, s->nvar, 0.0));
   DN_TRY(fill(s->stream, s->Y, nn, 0.0));
   DN_TRY(copy(s->stream, s->bestC, s->C, nn));
+  auto& o = s->opt;
+  o = dn_sys::OptRun{};
+  o.started = true;
+  o.method = st->method;
+  o.maxloop = st->maxloop;
+  o.alpha = st->alpha;
+  o.threshgorb = st->threshgorb;
+  o.beta1 = o.binit = (st->method == DN_DEMON) ? 0.6 : 0.5;
+  o.beta2 = (st->method == DN_DEMON) ? 0.999 : 0.9;
+  o.E = o.best_E = E;
+  o.gmax = gmax;
+  o.nalpha = d.nalpha;
+  o.nbeta = d.nbeta;
+  DN_HIP(hipStreamSynchronize(s->stream));
+  return DN_OK;
+}
+
+int fill_dev(hipStream_t s, double* p, long n, double v) { return fill(s, p, n, v); }
+int copy_dev(hipStream_t s, double* dst, const double* src, long n) { return copy(s, dst, src, n); }
+
+// The ADAM-family loop of minimization.jl:242-630 with C resident on the device.
+int orbopt_resident(dn_sys* s, const dn_functional* f, dn_orbopt_state* st) {
+  const dn_dims& d = s->d;
+  DN_TRY(orbopt_begin(s, f, st));
+  const long nn = (long)d.nbf * d.nbf;
+  double E = s->opt.E, gmax = s->opt.gmax;
+  const double alpha = st->alpha;
+  double beta1 = s->opt.beta1;
+  const double beta2 = s->opt.beta2;
+  const double binit = beta1;
   bool improved = false, success = false;
   double best_E = E;
   int nit = 0;
@@ -177,7 +204,7 @@ int orbopt_resident(dn_sys* s, const dn_functional* f, dn_orbopt_state* st) {
     DN_TRY(adam_step(s->stream, s->gv, s->mv, s->vv, s->vmax, s->yv, st->method, beta1, beta2,
                      bc1, bc2, alpha, d.nbf, d.nbf5, s->Y));
     DN_TRY(rotate_resident(s, s->C, s->Ca, false));  // C <- C exp(Y)   (utils.jl:499-521)
-    std::swap(s->C, s->Ca);
+    DN_TRY(copy(s->stream, s->C, s->Ca, nn));
     DN_TRY(lagrangian_and_grad(s, d.nalpha, d.nbeta, &E, &gmax));
     if (E < best_E) {
       best_E = E;
@@ -185,6 +212,7 @@ int orbopt_resident(dn_sys* s, const dn_functional* f, dn_orbopt_state* st) {
       improved = true;
     }
   }
+  s->opt.started = false;
   if (!improved) {  // minimization.jl:426-431
     st->alpha = 0.3 * st->alpha;
     st->maxloop = st->maxloop + 10;

@@ -1,17 +1,23 @@
 """Served orbital-optimisation solves with continuous batching on one GPU.
 
 Requests (synthetic RI systems of uneven size) arrive over time.  The scheduler admits
-them first-come-first-served whenever a worker slot is free and the system's HBM
-footprint (dn_sys_footprint) fits in what the budget has left; otherwise they wait.
-An admitted request gets its own resident system (b generated in HBM, C, H and the
-occupations uploaded once) and its own HIP stream, and runs the device-resident ADAM
-loop (minimization.jl:344-438 with maxloop = max_iter, stop when 2 max|g| < gtol).
-Solves admitted together run concurrently on the GPU; when one finishes its memory is
-released and the next waiting request is admitted while the others keep running.
+them first-come-first-served whenever a slot is free and the system's HBM footprint
+(dn_sys_footprint) fits in what the budget has left; otherwise they wait.  An admitted
+request gets its own resident system (b generated in HBM, C, H and the occupations set
+once) and runs the ADAM loop of minimization.jl:344-438 (maxloop = max_iter, stop when
+2 max|g| < gtol).
 
-Every kernel of a solve is deterministic and reads only its own system's buffers, so a
-result does not depend on what else is running: tests check that each result is
-bitwise the same alone and in the mix.
+mode "grouped" (default): one engine advances all live solves in lock step
+(dn_ogroup_step): per step every solve takes its ADAM step and rotation, and the orbital
+Lagrangians of all of them run as grouped persistent GEMMs in one captured HIP graph.
+When a solve finishes it leaves, and waiting requests join at the next step (the group is
+rebuilt).  mode "streams": one host thread per live solve, each running the whole loop
+on its own HIP stream (dn_orbopt_resident).
+
+Every kernel of a solve is deterministic, reads only its own system's buffers, and the
+tile shapes of the grouped GEMMs depend only on that system's sizes, so a result does
+not depend on what else is running: tests check that each result is bitwise the same
+alone and in the mix.
 """
 from __future__ import annotations
 
@@ -87,12 +93,34 @@ def solve_one(req, ctx):
     return E, nit, ok
 
 
-class Server:
-    """Admission by HBM budget and worker slots; one thread per live solve."""
+def start_solve(req, ctx):
+    """A resident system for the request with its ADAM run started (gamma, C, H set)."""
+    sysd = synthetic.SyntheticSystem(req.nbf, req.naux, req.nbf5, req.seed)
+    p = sysd.p
+    p.maxloop, p.alpha, p.threshgorb = req.max_iter, req.alpha, req.gtol
+    dev = api.System(p, ctx=ctx)
+    try:
+        dev.generate_b(req.seed)
+        dev.set_orbitals(sysd.C)
+        dev.set_hcore(sysd.H)
+        dev.set_gamma(sysd.gamma)
+        dev.orbopt_start("ADAM", p)
+    except Exception:
+        dev.close()
+        raise
+    return dev, p
 
-    def __init__(self, hbm_budget=0, max_live=8, device=0):
+
+class Server:
+    """Admission by HBM budget and slots; live solves advanced in lock step by grouped
+    kernels (mode "grouped") or by one thread and stream each (mode "streams")."""
+
+    def __init__(self, hbm_budget=0, max_live=8, device=0, mode="grouped"):
+        if mode not in ("grouped", "streams"):
+            raise ValueError("mode must be 'grouped' or 'streams'")
         self.ctx = api.Context(device, hbm_budget)
         self.max_live = max_live
+        self.mode = mode
         self.results = []
         self._lock = threading.Lock()
         self._live = 0
@@ -105,6 +133,80 @@ class Server:
     def run(self, requests, log=None):
         """Serve `requests` (arrival times relative to now).  Blocks until all are done;
         returns the results ordered by request id."""
+        if self.mode == "grouped":
+            return self._run_grouped(requests, log)
+        return self._run_streams(requests, log)
+
+    def _run_grouped(self, requests, log=None):
+        t0 = time.perf_counter()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting, results = [], []
+        live = {}            # rid -> (req, dev, p, need, admitted)
+        reserved = 0
+        group = None
+        changed = False
+        while pending or waiting or live:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.pop(0))
+            while waiting:   # FIFO admission while slots and memory allow
+                req = waiting[0]
+                need = api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5))
+                if need > self.ctx.hbm_budget:
+                    waiting.pop(0)
+                    results.append(SolveResult(req.rid, req.nbf, float("nan"), 0, False,
+                                               req.arrival, now, now,
+                                               "larger than the HBM budget"))
+                    continue
+                if len(live) >= self.max_live or reserved + need > self.ctx.hbm_budget:
+                    break
+                waiting.pop(0)
+                if group is not None:       # the systems leave the group's stream first
+                    group.close()
+                    group = None
+                try:
+                    dev, p = start_solve(req, self.ctx)
+                except DnError as e:
+                    results.append(SolveResult(req.rid, req.nbf, float("nan"), 0, False,
+                                               req.arrival, now, time.perf_counter() - t0,
+                                               str(e)))
+                    continue
+                live[req.rid] = (req, dev, p, need, now)
+                reserved += need
+                changed = True
+                if log:
+                    log("admit %d nbf=%d at %.3fs" % (req.rid, req.nbf, now))
+            if not live:
+                if pending and not waiting:
+                    time.sleep(min(0.002, max(0.0, pending[0].arrival - now)))
+                continue
+            if changed or group is None:
+                if group is not None:
+                    group.close()
+                group = api.OGroup([v[1] for v in live.values()], self.ctx)
+                changed = False
+            group.step()
+            finished = [rid for rid, v in live.items() if v[1].orbopt_done()]
+            if finished:
+                group.close()
+                group = None
+                t = time.perf_counter() - t0
+                for rid in finished:
+                    req, dev, p, need, admitted = live.pop(rid)
+                    E, nit, ok = dev.orbopt_finish(p)
+                    dev.close()
+                    reserved -= need
+                    results.append(SolveResult(rid, req.nbf, E, nit, ok, req.arrival, admitted,
+                                               t))
+                    if log:
+                        log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"
+                            % (rid, req.nbf, E, nit, t - req.arrival))
+        if group is not None:
+            group.close()
+        self.results = sorted(results, key=lambda r: r.rid)
+        return self.results
+
+    def _run_streams(self, requests, log=None):
         t0 = time.perf_counter()
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting = []

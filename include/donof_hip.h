@@ -187,6 +187,27 @@ int dn_sys_set_gamma(dn_sys* sys, const double* gamma);
 int dn_sys_get_orbitals(dn_sys* sys, double* C);
 int dn_orbopt_resident(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st);
 
+/* ---- step-wise optimisation and lock-step groups ---------------------------------
+ * The loop of dn_orbopt_resident (orbopt_adam & co., minimization.jl:242-630) split into
+ * start / steps / finish, so that several systems can advance together: a dn_ogroup
+ * steps every started system by one iteration per dn_ogroup_step, the orbital Lagrangian
+ * of all of them (utils.jl:1-59) running as grouped persistent GEMMs in one HIP graph.
+ * The reference has no grouped form (it optimises one system per call); per system the
+ * recurrence and its decisions are the single-system loop's, with the same arithmetic.
+ * dn_orbopt_start takes gamma, C and H already set on the system (dn_sys_set_gamma,
+ * dn_sys_set_orbitals, dn_sys_set_hcore); dn_orbopt_finish fills st as dn_orbopt_resident
+ * does (E, nit, success, updated alpha and maxloop) and leaves the orbitals in the system
+ * (dn_sys_get_orbitals).  While a system belongs to a dn_ogroup it runs on the group's
+ * stream; destroy the group before using the system on its own again. */
+int dn_orbopt_start(dn_sys* sys, const dn_functional* f, const dn_orbopt_state* st);
+int dn_orbopt_done(dn_sys* sys, int* done);
+int dn_orbopt_finish(dn_sys* sys, dn_orbopt_state* st);
+typedef struct dn_ogroup dn_ogroup;
+int dn_ogroup_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_ogroup** out);
+int dn_ogroup_destroy(dn_ogroup* group);
+/* One iteration of every unfinished system; *active = systems that stepped. */
+int dn_ogroup_step(dn_ogroup* group, int* active);
+
 /* ---- grouped builds --------------------------------------------------------------
  * Many resident systems of uneven size built in one sequence of grouped, persistent
  * launches, captured once into a HIP graph.  Each system's build is computeJKH_MO /

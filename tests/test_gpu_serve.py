@@ -1,5 +1,7 @@
 """Served solves: continuous batching on one GPU returns, for every request, exactly the
-result the same request gets when it runs alone; admission honours the HBM budget."""
+result the same request gets when it runs alone; admission honours the HBM budget.
+The lock-step grouped engine (dn_ogroup_step) must also reproduce the single-system
+device loop (dn_orbopt_resident) bit for bit."""
 import numpy as np
 import pytest
 
@@ -13,31 +15,46 @@ def requests(n=6, rate=200.0):
                                     gtol=1e-4)
 
 
-def test_alone_equals_in_the_mix():
-    reqs = requests()
-    ctx = api.Context(0)
-    alone = {r.rid: serve.solve_one(r, ctx) for r in reqs}
-    ctx.close()
-    srv = serve.Server(max_live=4)
+def _serve(reqs, mode, **kw):
+    srv = serve.Server(mode=mode, **kw)
     res = srv.run(reqs)
     srv.close()
+    return res
+
+
+def test_grouped_engine_matches_single_system_loop():
+    reqs = requests(4)
+    ctx = api.Context(0)
+    resident = {r.rid: serve.solve_one(r, ctx) for r in reqs}
+    ctx.close()
+    for r in reqs:
+        (g,) = _serve([r], "grouped", max_live=1)
+        assert g.error == ""
+        assert (g.E, g.nit, g.success) == resident[r.rid]     # bitwise the same
+
+
+@pytest.mark.parametrize("mode", ["grouped", "streams"])
+def test_alone_equals_in_the_mix(mode):
+    reqs = requests()
+    alone = {}
+    for r in reqs:
+        (a,) = _serve([r], mode, max_live=1)
+        alone[r.rid] = (a.E, a.nit, a.success)
+    res = _serve(reqs, mode, max_live=4)
     assert len(res) == len(reqs)
     for r in res:
         assert r.error == ""
-        E, nit, ok = alone[r.rid]
-        assert (r.E, r.nit, r.success) == (E, nit, ok)   # bitwise the same
-    assert max(r.admitted for r in res) > 0 or len(res) <= 4
+        assert (r.E, r.nit, r.success) == alone[r.rid]       # bitwise the same
     st = serve.latency_stats(res, max(r.done for r in res))
     assert st["solves"] == len(reqs) and st["p99_s"] >= st["p50_s"] > 0
 
 
-def test_admission_by_hbm_budget():
+@pytest.mark.parametrize("mode", ["grouped", "streams"])
+def test_admission_by_hbm_budget(mode):
     reqs = requests(5, rate=0.0)             # all arrive at t = 0
     fp = {r.rid: api.footprint(synthetic.synthetic_param(r.nbf, r.naux, r.nbf5)) for r in reqs}
     budget = int(1.2 * max(fp.values()))
-    srv = serve.Server(hbm_budget=budget, max_live=8)
-    res = srv.run(reqs)
-    srv.close()
+    res = _serve(reqs, mode, hbm_budget=budget, max_live=8)
     assert all(r.error == "" for r in res)
     # at every admission, the footprints of the solves then resident fit in the budget
     for r in res:
@@ -46,8 +63,5 @@ def test_admission_by_hbm_budget():
 
 
 def test_request_larger_than_budget_is_rejected():
-    reqs = requests(2, rate=0.0)
-    srv = serve.Server(hbm_budget=1 << 20, max_live=2)
-    res = srv.run(reqs)
-    srv.close()
+    res = _serve(requests(2, rate=0.0), "grouped", hbm_budget=1 << 20, max_live=2)
     assert all("budget" in r.error for r in res)
