@@ -223,7 +223,8 @@ class Group:
     computeJKH_MO (integrals.jl:131-172), read with System.get_jkh."""
 
     STAGES = ("K1", "K2", "J", "HC")
-    CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128", 3: "128x64", 4: "64x32", 5: "32x64"}
+    CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128", 3: "128x64", 4: "64x32", 5: "32x64",
+               6: "lds64x128", 7: "lds128x128", 8: "lds128x256"}
 
     def __init__(self, systems, ctx=None):
         self.systems = list(systems)

@@ -170,6 +170,7 @@ int transform1_sym(dn_sys* s) {
   g.C = s->bpn; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
   g.nbatch = d.nbfaux; g.bps = 1;
   g.tri = kTriA;
+  g.lds = true;
   DN_TRY(begin_timing(s));
   DN_TRY(gemm_f64(s->stream, g));
   return end_timing(s);
@@ -182,6 +183,7 @@ int transform2_sym(dn_sys* s) {
   g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
   g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
   g.C = s->bt; g.ldc = d.nbf5;
+  g.lds = true;
   return gemm_f64(s->stream, g);
 }
 

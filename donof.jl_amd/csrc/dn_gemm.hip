@@ -230,6 +230,212 @@ struct GemmLds {
   static constexpr int B = TileLds<BN, BK, BKC, 64 * WGM * WGN>::SIZE;
 };
 
+// ---- LDS-direct tile (K-contiguous A and B) -------------------------------------------
+//
+// Every 16-wide k tile of A and B goes from HBM straight into LDS
+// (global_load_lds_dwordx4: lane l writes 16 bytes at the instruction's LDS base + 16 l),
+// NST - 1 tiles ahead of the one being multiplied: no staging registers, no VALU copy, and
+// the loads of several tiles in flight.  One barrier per k tile.
+//
+// LDS layout of one operand tile: blocks of 16 rows (2048 bytes); in a block, half h
+// (doubles 8h..8h+7 of the k tile) at h * 1024, 16-byte chunk c (doubles 2c, 2c+1) of
+// that half at (c % 4) * 256, row r at (r % 16) * 16.  One load instruction fills one half
+// of one block (lane l: row l % 16, chunk l / 16, 64 contiguous global bytes per row).
+//
+// MFMA k order: in step s (0..3) of a k tile, the lanes of fragment group g = lane / 16
+// contract k = 4 g + s (A and B alike, so the sum is the same; only its order differs
+// from the register-staged tile).  A lane's four k values are two consecutive chunks, read
+// with two 16-byte LDS loads per fragment; the 16 lanes of a group read 256 contiguous
+// bytes, free of bank conflicts.
+//
+// Rows past M (columns past N) load a clamped row and are not stored; chunks past K load a
+// clamped chunk and their A values are zeroed (K is even, so a chunk is wholly in or out).
+
+// s_waitcnt vmcnt(N) with no wait on the other counters (gfx9 encoding: vmcnt in bits
+// [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8])
+template <int N>
+__device__ __forceinline__ void wait_vmcnt_only() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int BM, int BN>
+constexpr int lds_tile_bytes(int nst) {
+  return nst * (BM + BN) * 128;
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, int TRI>
+__device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int z,
+                                              unsigned char* smem) {
+  constexpr int NW = WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
+  constexpr int A_BYTES = BM * 128;
+  constexpr int STAGE = (BM + BN) * 128;
+  constexpr int NA = BM / 8, NB = BN / 8;  // load instructions per k tile
+  constexpr int LW = (NA + NB) / NW;       // per wavefront
+  static_assert(BM % 16 == 0 && BN % 16 == 0 && (NA + NB) % NW == 0, "load split");
+  static_assert(FM >= 1 && FN >= 1 && WM % 16 == 0 && WN % 16 == 0, "wave tile");
+  static_assert(NST >= 2 && NST <= 4 && (NST - 2) * LW < 64, "pipeline depth");
+  static_assert(TRI == kTriNone || TRI == kTriA, "LDS-direct tile: symmetric-A mode only");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int M = g.M, N = g.N, K = g.K;
+  const int r0 = z * g.bps, r1 = min(r0 + g.bps, g.nbatch);
+  const int kt0 = TRI == kTriA ? m0 / 16 : 0;
+  const int nkt = (K + 15) / 16 - kt0;
+  const int T = nkt > 0 ? (r1 - r0) * nkt : 0;
+
+  // this wavefront's load instructions: q = wave + NW i, A for q < NA, else B
+  const double* src[LW];
+  long sb[LW];
+  int ck[LW], dst[LW];
+#pragma unroll
+  for (int i = 0; i < LW; ++i) {
+    const int q = wave + NW * i;
+    const bool isA = q < NA;
+    const int qq = isA ? q : q - NA;
+    const int blk = qq >> 1, h = qq & 1;
+    const int r = 16 * blk + (lane & 15);
+    src[i] = isA ? g.A + (long)min(m0 + r, M - 1) * g.lda
+                 : g.B + (long)min(n0 + r, N - 1) * g.ldb;
+    sb[i] = isA ? g.sA : g.sB;
+    ck[i] = 2 * (4 * h + (lane >> 4));
+    dst[i] = (isA ? 0 : A_BYTES) + blk * 2048 + h * 1024;
+  }
+  int ir = r0, ikt = 0;  // batch and k tile of the next load
+  auto issue = [&](int t) {
+    const int k0 = (kt0 + ikt) * 16;
+    unsigned char* base = smem + (t % NST) * STAGE;
+#if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: the host pass only needs the stub
+#pragma unroll
+    for (int i = 0; i < LW; ++i) {
+      const int k = min(k0 + ck[i], K - 2);
+      __builtin_amdgcn_global_load_lds(src[i] + (long)ir * sb[i] + k, base + dst[i], 16, 0, 0);
+    }
+#else
+    (void)k0;
+    (void)base;
+#endif
+    if (++ikt == nkt) {
+      ikt = 0;
+      ++ir;
+    }
+  };
+
+  v4d acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+#pragma unroll
+  for (int s = 0; s < NST - 1; ++s)
+    if (s < T) issue(s);
+  const int fr = lane & 15, fk = lane >> 4;
+  int ckt = 0;  // k tile being multiplied
+  for (int t = 0; t < T; ++t) {
+    // this wavefront's loads of tile t have landed once at most `ahead` tiles' remain
+    const int ahead = min(NST - 2, T - 1 - t);
+    if (NST >= 4 && ahead >= 2) wait_vmcnt_only<(NST >= 4 ? 2 * LW : 0)>();
+    else if (NST >= 3 && ahead == 1) wait_vmcnt_only<(NST >= 3 ? LW : 0)>();
+    else wait_vmcnt_only<0>();
+    // Everyone's tile t is in LDS and everyone is done reading tile t - 1 (whose stage the
+    // next load overwrites).  A bare s_barrier: __syncthreads() would also wait for every
+    // outstanding load, draining the tiles in flight; lgkmcnt(0) completes this
+    // wavefront's LDS reads of tile t - 1.
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4) | (0 << 8));
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (t + NST - 1 < T) issue(t + NST - 1);
+    const unsigned char* As = smem + (t % NST) * STAGE;
+    const unsigned char* Bs = As + A_BYTES;
+    v2d af[FM][2], bf[FN][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int c = 2 * fk + p;
+      const int o = (c >> 2) * 1024 + (c & 3) * 256 + fr * 16;
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i][p] = *reinterpret_cast<const v2d*>(As + (wm * WM + 16 * i) * 128 + o);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j][p] = *reinterpret_cast<const v2d*>(Bs + (wn * WN + 16 * j) * 128 + o);
+    }
+    const int k0 = (kt0 + ckt) * 16;
+    if (++ckt == nkt) ckt = 0;
+    const int kl = k0 + 4 * fk;  // k of step s for this lane: kl + s
+    if constexpr (TRI == kTriA) {
+      // Lh(m, k): 1 for k > m, 1/2 for k == m, 0 below; only k tiles reaching into this
+      // row block's diagonal need it (a uniform branch)
+      if (k0 < m0 + BM) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int d = kl - (m0 + wm * WM + 16 * i + fr);
+#pragma unroll
+          for (int p = 0; p < 2; ++p) {
+            af[i][p].x *= d + 2 * p > 0 ? 1.0 : (d + 2 * p == 0 ? 0.5 : 0.0);
+            af[i][p].y *= d + 2 * p + 1 > 0 ? 1.0 : (d + 2 * p + 1 == 0 ? 0.5 : 0.0);
+          }
+        }
+      }
+    }
+    if (k0 + 16 > K) {  // chunks past K hold clamped copies
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          af[i][p].x = kl + 2 * p < K ? af[i][p].x : 0.0;
+          af[i][p].y = kl + 2 * p + 1 < K ? af[i][p].y : 0.0;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const double a = (s & 1) ? af[i][s >> 1].y : af[i][s >> 1].x;
+          const double b = (s & 1) ? bf[j][s >> 1].y : bf[j][s >> 1].x;
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc[i][j], 0, 0, 0);
+        }
+  }
+
+  // same accumulator layout and stores as gemm_tile
+  double* Cz = g.C + (long)z * g.sC;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = m0 + wm * WM + i * 16 + fr;
+    if (row >= M) continue;
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int col = n0 + wn * WN + j * 16 + fk + 4 * rr;
+        if (col < N) {
+          double* c = Cz + row + (long)col * g.ldc;
+          const double v = g.alpha * acc[i][j][rr];
+          *c = (g.beta == 0.0) ? v : v + g.beta * (*c);
+        }
+      }
+  }
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, int TRI>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_lds_kernel(Gemm g, int tiles_m,
+                                                                  int tiles_n, int nsplit) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[lds_tile_bytes<BM, BN>(NST)];
+  const unsigned total = (unsigned)tiles_m * tiles_n * nsplit;
+  const unsigned hw = blockIdx.x;
+  if (hw >= total) return;
+  const unsigned lid = xcd_remap(hw, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  gemm_tile_lds<BM, BN, WGM, WGN, NST, TRI>(g, mt, nt, z, smem);
+}
+
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
           int TRI = 0>
 __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int tiles_m,
@@ -247,31 +453,30 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
   gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(g, mt, nt, z, As, Bs);
 }
 
-// Persistent grouped GEMM: the tiles of all problems form one list (problem i owns tiles
-// [desc[i].tile_begin, desc[i].tile_begin + tiles_m*tiles_n*nsplit)); workgroup b takes
-// list positions b, b + gridDim.x, ... until the list is exhausted (a static schedule, so
-// every workgroup's loop ends).  With a grid that is a multiple of the 8 XCDs, workgroup
-// b runs on XCD b % 8, and position t maps to tile (t % 8) * per + t / 8: each XCD works
-// through one contiguous eighth of the list, so the M-tiles that share a B block (and
-// consecutive B blocks) meet in the same L2.
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
-          int TRI = 0>
-__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const GroupedGemm* __restrict__ desc,
-                                                                      int nprob,
-                                                                      long total_tiles) {
-  using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
-  __shared__ double As[2 * L::A];
-  __shared__ double Bs[2 * L::B];
+// Persistent grouped GEMM schedule.  Problem i owns tiles [0, n_i), n_i =
+// tiles_m * tiles_n * nsplit.  With a grid that is a multiple of the 8 XCDs, workgroup b
+// runs on XCD x = b % 8 and takes slots b / 8, b / 8 + grid / 8, ... of that XCD's list;
+// slot s belongs to the problem i with slot_begin_i <= s < slot_begin_{i+1} and is its
+// tile x * ceil(n_i / 8) + (s - slot_begin_i).  Every XCD thus works through one
+// contiguous eighth of every problem: the M-tiles that share a B block (and consecutive B
+// blocks) meet in the same L2, and a large problem is spread over all XCDs instead of
+// landing on the one or two that a contiguous split of the whole list would give it.
+// Without the XCD multiple, workgroup b takes list positions b, b + grid, ... in
+// tile_begin order.  The schedule is static, so every workgroup's loop ends.
+template <typename Tile>
+__device__ __forceinline__ void grouped_schedule(const GroupedGemm* __restrict__ desc, int nprob,
+                                                 long total_tiles, long total_slots,
+                                                 Tile&& tile) {
   const bool chunk = (gridDim.x % 8u) == 0;
-  const long per = (total_tiles + 7) / 8;
-  const long span = chunk ? 8 * per : total_tiles;
-  for (long pos = blockIdx.x; pos < span; pos += gridDim.x) {
-    const long t = chunk ? (pos % 8) * per + pos / 8 : pos;
-    if (t >= total_tiles) continue;
-    int lo = 0, hi = nprob - 1;
+  const long x = blockIdx.x % 8u;
+  const long first = chunk ? blockIdx.x / 8u : blockIdx.x;
+  const long step = chunk ? gridDim.x / 8u : gridDim.x;
+  const long span = chunk ? total_slots : total_tiles;
+  for (long pos = first; pos < span; pos += step) {
+    int lo = 0, hi = nprob - 1;  // the last problem starting at or before pos
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (desc[mid].tile_begin <= t) lo = mid;
+      if ((chunk ? desc[mid].slot_begin : desc[mid].tile_begin) <= pos) lo = mid;
       else hi = mid - 1;
     }
     // lo is uniform over the workgroup; saying so makes the descriptor loads scalar (SGPRs)
@@ -279,13 +484,38 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(const Gro
     // alias it, so it is not reloaded after every store.
     lo = __builtin_amdgcn_readfirstlane(lo);
     const GroupedGemm d = desc[lo];
-    const long local = t - d.tile_begin;
+    const long n = (long)d.tiles_m * d.tiles_n * d.nsplit;
+    const long local = chunk ? x * ((n + 7) / 8) + (pos - d.slot_begin) : pos - d.tile_begin;
+    if (local >= n) continue;  // the last XCD's share of a problem can be short
     const int mt = local % d.tiles_m;
     const int nt = (local / d.tiles_m) % d.tiles_n;
     const int z = local / ((long)d.tiles_m * d.tiles_n);
-    gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(d.g, mt, nt, z, As, Bs);
+    tile(d.g, mt, nt, z);
     __syncthreads();  // the next tile reuses the LDS buffers
   }
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(
+    const GroupedGemm* __restrict__ desc, int nprob, long total_tiles, long total_slots) {
+  using L = GemmLds<BM, BN, BK, WGM, WGN, AK, BKC>;
+  __shared__ double As[2 * L::A];
+  __shared__ double Bs[2 * L::B];
+  grouped_schedule(desc, nprob, total_tiles, total_slots,
+                   [&](const Gemm& g, int mt, int nt, int z) {
+                     gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(g, mt, nt, z, As, Bs);
+                   });
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, int TRI>
+__global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_lds_kernel(
+    const GroupedGemm* __restrict__ desc, int nprob, long total_tiles, long total_slots) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[lds_tile_bytes<BM, BN>(NST)];
+  grouped_schedule(desc, nprob, total_tiles, total_slots,
+                   [&](const Gemm& g, int mt, int nt, int z) {
+                     gemm_tile_lds<BM, BN, WGM, WGN, NST, TRI>(g, mt, nt, z, smem);
+                   });
 }
 
 __global__ void reduce_splits_kernel(const double* __restrict__ P, long sP, int nsplit, int M,
@@ -313,6 +543,19 @@ int launch(hipStream_t s, const Gemm& g) {
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "gemm grid too large");
   hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>),
                      dim3((unsigned)total), dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, int TRI>
+int launch_lds(hipStream_t s, const Gemm& g) {
+  const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
+  const int nsplit = (g.nbatch + g.bps - 1) / g.bps;
+  const long total = (long)tm * tn * nsplit;
+  if (total <= 0) return DN_OK;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "gemm grid too large");
+  hipLaunchKernelGGL((dgemm_lds_kernel<BM, BN, WGM, WGN, NST, TRI>), dim3((unsigned)total),
+                     dim3(64 * WGM * WGN), 0, s, g, tm, tn, nsplit);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -351,9 +594,20 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   };
   g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
   g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
+  if (g.tri == kTriA && (!g.a_kcontig || !g.b_kcontig || g.M != g.K))
+    return fail(DN_ERR_ARG, "gemm_f64: symmetric-A mode needs K-contiguous A and B, M == K");
+  if (g.lds && g.a_kcontig && g.b_kcontig && g.vec_a && g.vec_b && g.tri != kTriB) {
+    // 8 wavefronts per workgroup (2 per SIMD) beat 4 with twice the wave tile on every
+    // transform shape measured on MI355X; 64-row blocks for the symmetric-A mode (short,
+    // uneven K loops), 128 x 256 tiles for long N (the flattened second transform)
+    if (g.tri == kTriA || g.M <= 64) {
+      return g.tri == kTriA ? launch_lds<64, 128, 2, 4, 3, kTriA>(s, g)
+                            : launch_lds<64, 128, 2, 4, 3, kTriNone>(s, g);
+    }
+    return g.N > 128 ? launch_lds<128, 256, 2, 4, 2, kTriNone>(s, g)
+                     : launch_lds<128, 128, 2, 4, 3, kTriNone>(s, g);
+  }
   if (g.tri == kTriA) {
-    if (!g.a_kcontig || !g.b_kcontig || g.M != g.K)
-      return fail(DN_ERR_ARG, "gemm_f64: symmetric-A mode needs K-contiguous A and B, M == K");
     // 64-row blocks (the K loop of row block m0 covers k >= m0 only): the short, uneven
     // K loops balance best over small workgroups (measured on MI355X at nbf = 400, 800)
     return dispatch_vec<64, 128, 16, 2, 4, true, true, kTriA>(s, g);
@@ -386,36 +640,56 @@ void config_tile(int config, int* bm, int* bn) {
     case kTile128x64: *bm = 128; *bn = 64; break;
     case kTile64x32: *bm = 64; *bn = 32; break;
     case kTile32x64: *bm = 32; *bn = 64; break;
-    default: *bm = 128; *bn = 128; break;
+    case kLds64x128: *bm = 64; *bn = 128; break;
+    case kLds128x256: *bm = 128; *bn = 256; break;
+    default: *bm = 128; *bn = 128; break;  // kTile128x128, kLds128x128
   }
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
-          int TRI = 0>
-int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, int grid) {
-  auto kern = dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>;
-  // grid <= 0: -grid times as many workgroups as stay resident (0 = once), a multiple
-  // of the 8 XCDs, at most one per tile (rounded up to the XCD multiple)
+// Persistent launch of a kernel over `total` tiles: grid <= 0 means -grid times as many
+// workgroups as stay resident (0 = once), a multiple of the 8 XCDs, at most one per tile
+// (rounded up to the XCD multiple).
+template <typename Kern>
+int launch_persistent(hipStream_t s, Kern kern, int threads, const GroupedGemm* d, int n,
+                      long total, long slots, int grid) {
   if (grid <= 0) {
     int dev = 0, cus = 0, occ = 0;
     DN_HIP(hipGetDevice(&dev));
     DN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    DN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WGM * WGN, 0));
+    DN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, 0));
     const long full = (long)std::max(occ, 1) * cus * std::max(-grid, 1);
     grid = (int)std::min(std::min(full, (total + 7) / 8 * 8), 0x7ffffff8L);
   }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WGM * WGN), 0, s, d, n, total);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, d, n, total, slots);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
+template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
+          int TRI = 0>
+int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, long slots,
+                   int grid) {
+  return launch_persistent(s, dgemm_grouped_kernel<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>,
+                           64 * WGM * WGN, d, n, total, slots, grid);
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST>
+int launch_grouped_lds(hipStream_t s, int tri, const GroupedGemm* d, int n, long total,
+                       long slots, int grid) {
+  return tri == kTriA
+             ? launch_persistent(s, dgemm_grouped_lds_kernel<BM, BN, WGM, WGN, NST, kTriA>,
+                                 64 * WGM * WGN, d, n, total, slots, grid)
+             : launch_persistent(s, dgemm_grouped_lds_kernel<BM, BN, WGM, WGN, NST, kTriNone>,
+                                 64 * WGM * WGN, d, n, total, slots, grid);
+}
+
 template <int BM, int BN, int BK, int WGM, int WGN>
 int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, int tri,
-                   const GroupedGemm* d, int n, long total, int grid) {
+                   const GroupedGemm* d, int n, long total, long slots, int grid) {
 #define DN_G(AKv, BKv, VAv, VBv) \
-  launch_grouped<BM, BN, BK, WGM, WGN, AKv, BKv, VAv, VBv>(s, d, n, total, grid)
+  launch_grouped<BM, BN, BK, WGM, WGN, AKv, BKv, VAv, VBv>(s, d, n, total, slots, grid)
 #define DN_GT(VAv, VBv, T) \
-  launch_grouped<BM, BN, BK, WGM, WGN, true, true, VAv, VBv, T>(s, d, n, total, grid)
+  launch_grouped<BM, BN, BK, WGM, WGN, true, true, VAv, VBv, T>(s, d, n, total, slots, grid)
   if (tri) {
     if (!ak || !bk || tri != kTriA)
       return fail(DN_ERR_ARG, "grouped GEMM: symmetric mode is kTriA with K-contiguous A, B");
@@ -442,7 +716,7 @@ int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, int tri,
 long gemm_grouped_plan(GroupedGemm* desc, int n, int config) {
   int bm, bn;
   config_tile(config, &bm, &bn);
-  long begin = 0;
+  long begin = 0, slot = 0;
   for (int i = 0; i < n; ++i) {
     Gemm& g = desc[i].g;
     g.vec_a = vec_ok(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
@@ -451,7 +725,10 @@ long gemm_grouped_plan(GroupedGemm* desc, int n, int config) {
     desc[i].tiles_n = (g.N + bn - 1) / bn;
     desc[i].nsplit = (g.nbatch + g.bps - 1) / g.bps;
     desc[i].tile_begin = begin;
-    begin += (long)desc[i].tiles_m * desc[i].tiles_n * desc[i].nsplit;
+    desc[i].slot_begin = slot;
+    const long tiles = (long)desc[i].tiles_m * desc[i].tiles_n * desc[i].nsplit;
+    begin += tiles;
+    slot += (tiles + 7) / 8;
   }
   return begin;
 }
@@ -468,14 +745,34 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     va = va && dh[i].g.vec_a;
     vb = vb && dh[i].g.vec_b;
   }
-  switch (config) {
-    case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
-    case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
-    case kTile128x64: return grouped_layout<128, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
-    case kTile64x32: return grouped_layout<64, 32, 16, 2, 1>(s, ak, bk, va, vb, tri, dd, n, total, grid);
-    case kTile32x64: return grouped_layout<32, 64, 16, 1, 2>(s, ak, bk, va, vb, tri, dd, n, total, grid);
-    default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, grid);
+  const GroupedGemm& last = dh[n - 1];
+  const long slots =
+      last.slot_begin + ((long)last.tiles_m * last.tiles_n * last.nsplit + 7) / 8;
+  if (config >= kLds64x128) {
+    if (!ak || !bk || !va || !vb || tri == kTriB)
+      return fail(DN_ERR_ARG, "grouped GEMM: LDS-direct configurations need K-contiguous, "
+                              "16-byte aligned A and B");
+    switch (config) {
+      case kLds64x128: return launch_grouped_lds<64, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
+      case kLds128x128: return launch_grouped_lds<128, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
+      default: return launch_grouped_lds<128, 256, 2, 4, 2>(s, tri, dd, n, total, slots, grid);
+    }
   }
+  switch (config) {
+    case kTile64x64: return grouped_layout<64, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    case kTile64x128: return grouped_layout<64, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    case kTile128x64: return grouped_layout<128, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    case kTile64x32: return grouped_layout<64, 32, 16, 2, 1>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    case kTile32x64: return grouped_layout<32, 64, 16, 1, 2>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+  }
+}
+
+bool gemm_config_ok(int config, const Gemm& g) {
+  if (config < 0 || config >= kNumGemmConfigs) return false;
+  if (config < kLds64x128) return g.tri == kTriNone || g.tri == kTriA;
+  return g.a_kcontig && g.b_kcontig && g.tri != kTriB &&
+         vec_ok(g.A, g.lda, g.sA, g.nbatch, g.K) && vec_ok(g.B, g.ldb, g.sB, g.nbatch, g.K);
 }
 
 int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {

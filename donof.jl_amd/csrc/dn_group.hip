@@ -215,8 +215,15 @@ int size_bucket(const Gemm& g, int stage) {
   return n5 <= 32 ? 0 : n5 <= 64 ? 1 : n5 <= 128 ? 2 : 3;
 }
 
-int default_config(int stage, int bucket) {
+int default_config(int stage, int bucket, const Gemm& g) {
   if (bucket == 0) return kTile64x32;
+  // the symmetric-b transforms (K1 with the Lh mask, K2 flattened) take the LDS-direct
+  // tiles the single-system build uses
+  const bool sym = (stage == kK1 && g.tri == kTriA) || (stage == kK2 && g.a_kcontig && g.b_kcontig);
+  if (sym && gemm_config_ok(kLds64x128, g)) {
+    if (stage == kK1 || bucket == 1) return kLds64x128;
+    return kLds128x256;
+  }
   if (stage == kK2 && bucket >= 2) return kTile128x128;
   return stage == kK1 ? kTile64x128 : kTile64x64;
 }
@@ -243,7 +250,7 @@ int plan_stage(dn_group* G, int stage) {
       f += 2.0 * h[j].g.M * h[j].g.N * (double)h[j].g.K * h[j].g.nbatch;
       ++j;
     }
-    const int cfg = default_config(stage, size_bucket(h[i].g, stage));
+    const int cfg = default_config(stage, size_bucket(h[i].g, stage), h[i].g);
     L.push_back({i, j - i, gemm_grouped_plan(h.data() + i, j - i, cfg), cfg, f, 0});
     i = j;
   }
@@ -475,6 +482,7 @@ int dn_group_tune(dn_group* G, int* configs) {
       int best = G->launches[st][li].config, best_grid = 0;
       float best_ms = 1e30f;
       for (int c = 0; c < kNumGemmConfigs; ++c) {
+        if (!gemm_config_ok(c, G->host[st][G->launches[st][li].first].g)) continue;
         DN_TRY(set_config(G, st, li, c));
         for (int gr : grids) {
           G->launches[st][li].grid = gr;
@@ -508,7 +516,8 @@ int dn_group_tune(dn_group* G, int* configs) {
 
 int dn_group_plan_text(dn_group* G, char* buf, int len) {
   if (!G || !buf || len <= 0) return fail(DN_ERR_ARG, "bad dn_group_plan_text arguments");
-  static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64"};
+  static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64",
+                                "lds64x128", "lds128x128", "lds128x256"};
   static const char* stages[] = {"K1", "K2", "J", "HC"};
   std::string t;
   for (int st = 0; st < kNumStages; ++st) {

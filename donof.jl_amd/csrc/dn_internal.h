@@ -53,6 +53,11 @@ struct Gemm {
   // B = Lh + Lh^T (k > n as is, k == n halved), skipping the K range above each column
   // block.  kTriA: the same for a symmetric A (M == K), masked by k > m.
   int tri = 0;
+  // LDS-direct pipeline (gemm_f64 and the kLds* grouped configurations; needs K-contiguous
+  // A and B with 16-byte loads, tri none or kTriA).  Its k order inside each 16-wide k
+  // tile differs from the register-staged kernel's, so results differ in the last bits:
+  // products compared bitwise across code paths must choose the same kernel.
+  bool lds = false;
 };
 enum { kTriNone = 0, kTriB = 1, kTriA = 2 };
 
@@ -62,7 +67,8 @@ int gemm_f64(hipStream_t s, const Gemm& g);
 struct GroupedGemm {
   Gemm g;
   int tiles_m = 0, tiles_n = 0, nsplit = 0;
-  long tile_begin = 0;
+  long tile_begin = 0;  // prefix sum of the tile counts n_i
+  long slot_begin = 0;  // prefix sum of ceil(n_i / 8): one XCD's share of each problem
 };
 // Tile configurations of the grouped launches (chosen by the autotuner).
 enum GemmConfig {
@@ -72,8 +78,13 @@ enum GemmConfig {
   kTile128x64 = 3,
   kTile64x32 = 4,
   kTile32x64 = 5,
-  kNumGemmConfigs = 6
+  kLds64x128 = 6,   // LDS-direct pipeline, K-contiguous A and B only
+  kLds128x128 = 7,
+  kLds128x256 = 8,
+  kNumGemmConfigs = 9
 };
+// Whether a tile configuration can run a problem of the given layout and alignment.
+bool gemm_config_ok(int config, const Gemm& g);
 // Fill tile counts/offsets of desc[0..n) for a config; returns the total tile count.
 long gemm_grouped_plan(GroupedGemm* desc, int n, int config);
 // One persistent launch over all problems (descriptors already on the device).  All

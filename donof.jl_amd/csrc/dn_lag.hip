@@ -38,19 +38,37 @@ __global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf5
 }
 
 // bmat[q,b,Q] = -ck12[b,q] bt[q,b,Q] + [q==b] w[b,Q]
-__global__ void lag_bmat_kernel(const double* __restrict__ bt, long slice, int nbf5, int naux,
-                                const double* __restrict__ ck12, const double* __restrict__ w,
-                                double* __restrict__ bmat) {
+// Block (tq, tb, zq): the 32 x 32 (q, b) tile at (32 tq, 32 tb) for Q in
+// [BMAT_Q zq, BMAT_Q (zq + 1)).  The ck12 tile is read once, along b, and kept transposed
+// in LDS, so that bt and bmat are read and written along q (coalesced).
+constexpr int BMAT_Q = 16;
+__global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict__ bt, long slice,
+                                                       int nbf5, int naux,
+                                                       const double* __restrict__ ck12,
+                                                       const double* __restrict__ w,
+                                                       double* __restrict__ bmat) {
+  __shared__ double ct[32][33];  // ct[q][b] = ck12[b, q]
+  const int q0 = 32 * blockIdx.x, b0 = 32 * blockIdx.y;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int i = ty; i < 32; i += 8) {  // row q0 + i of the tile: ck12[b0 + tx, q0 + i]
+    const int q = q0 + i, b = b0 + tx;
+    ct[i][tx] = (q < nbf5 && b < nbf5) ? ck12[b + (long)q * nbf5] : 0.0;
+  }
+  __syncthreads();
   const long nn = (long)nbf5 * nbf5;
-  const long total = nn * naux;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const long Q = e / nn;
-    const long r = e - Q * nn;
-    const int q = r % nbf5, b = r / nbf5;
-    double v = -ck12[b + (long)q * nbf5] * bt[Q * slice + q + (long)b * nbf5];
-    if (q == b) v += w[b + Q * nbf5];
-    bmat[e] = v;
+  const int q = q0 + tx;
+  if (q >= nbf5) return;
+  const int Q1 = min(naux, BMAT_Q * ((int)blockIdx.z + 1));
+  for (int Q = BMAT_Q * blockIdx.z; Q < Q1; ++Q) {
+    const double* s = bt + (long)Q * slice;
+    double* o = bmat + (long)Q * nn;
+    for (int i = ty; i < 32; i += 8) {
+      const int b = b0 + i;
+      if (b >= nbf5) break;
+      double v = -ct[tx][i] * s[q + (long)b * nbf5];
+      if (q == b) v += w[b + (long)Q * nbf5];
+      o[q + (long)b * nbf5] = v;
+    }
   }
 }
 
@@ -92,8 +110,9 @@ int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const
 
 int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat) {
-  hipLaunchKernelGGL(lag_bmat_kernel, dim3(8192), dim3(256), 0, s, bt, slice, nbf5, naux, ck12,
-                     w, bmat);
+  const unsigned t = (nbf5 + 31) / 32;
+  hipLaunchKernelGGL(lag_bmat_kernel, dim3(t, t, (naux + BMAT_Q - 1) / BMAT_Q), dim3(256), 0, s,
+                     bt, slice, nbf5, naux, ck12, w, bmat);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

@@ -32,22 +32,19 @@ __global__ void build_y_kernel(const double* __restrict__ y, int nbf, int nbf5,
   }
 }
 
-// max_j sum_i |Y[i,j]| (one block)
-__global__ void norm1_kernel(const double* __restrict__ Y, int nbf, double* __restrict__ out) {
-  __shared__ double red[256];
-  double mx = 0.0;
-  for (int j = threadIdx.x; j < nbf; j += blockDim.x) {
-    double s = 0.0;
-    for (int i = 0; i < nbf; ++i) s += fabs(Y[i + (long)j * nbf]);
-    mx = fmax(mx, s);
-  }
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = red[0];
+// max_j sum_i |Y[i,j]|: one wavefront per column (coalesced reads); the maximum is taken
+// on the bit patterns (non-negative doubles order as their bits, a NaN above all), so the
+// result does not depend on the order the columns finish in.  *out must start at zero.
+__global__ void norm1_kernel(const double* __restrict__ Y, int nbf,
+                             unsigned long long* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= nbf) return;
+  const double* col = Y + (long)j * nbf;
+  double s = 0.0;
+  for (int i = lane; i < nbf; i += 64) s += fabs(col[i]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) atomicMax(out, (unsigned long long)__double_as_longlong(s));
 }
 
 // out = c0 I + c1 X + c2 X2 + c3 X3 + c4 X4 (X4 may be null when c4 == 0), all scaled
@@ -163,7 +160,10 @@ int build_y(hipStream_t s, const double* y, int nbf, int nbf5, double* Y) {
 }
 
 int norm1(hipStream_t s, const double* Y, int nbf, double* out) {
-  hipLaunchKernelGGL(norm1_kernel, dim3(1), dim3(256), 0, s, Y, nbf, out);
+  auto* bits = reinterpret_cast<unsigned long long*>(out);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, bits, 1);
+  DN_HIP(hipGetLastError());
+  hipLaunchKernelGGL(norm1_kernel, dim3((nbf + 3) / 4), dim3(256), 0, s, Y, nbf, bits);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

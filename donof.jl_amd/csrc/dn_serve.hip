@@ -134,26 +134,24 @@ int plan_lstage(dn_ogroup* G, int stage) {
   std::vector<int> cfg(n);
   for (int i = 0; i < n; ++i) h[i].g = lag_gemm(G->sys[i], stage, &cfg[i]);
   gemm_grouped_plan(h.data(), n, kTile64x64);  // alignment flags
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  auto key = [&](int i) {
-    const Gemm& g = h[i].g;
-    return cfg[i] * 16 + 8 * g.a_kcontig + 4 * g.b_kcontig + 2 * !g.vec_a + !g.vec_b;
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
-  std::vector<GroupedGemm> sorted(n);
-  std::vector<int> scfg(n);
+  // one launch per class: tile configuration, operand layout and alignment
+  std::vector<int> key(n), order(n);
   for (int i = 0; i < n; ++i) {
-    sorted[i] = h[order[i]];
-    scfg[i] = cfg[order[i]];
+    const Gemm& g = h[i].g;
+    key[i] = cfg[i] * 16 + 8 * g.a_kcontig + 4 * g.b_kcontig + 2 * !g.vec_a + !g.vec_b;
+    order[i] = i;
   }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<GroupedGemm> sorted(n);
+  for (int i = 0; i < n; ++i) sorted[i] = h[order[i]];
   h = sorted;
   auto& L = G->launches[stage];
   L.clear();
   for (int i = 0; i < n;) {
     int j = i;
-    while (j < n && key(order[j]) == key(order[i])) ++j;
-    L.push_back({i, j - i, scfg[i], gemm_grouped_plan(h.data() + i, j - i, scfg[i])});
+    while (j < n && key[order[j]] == key[order[i]]) ++j;
+    const int c = cfg[order[i]];
+    L.push_back({i, j - i, c, gemm_grouped_plan(h.data() + i, j - i, c)});
     i = j;
   }
   DN_HIP(hipMemcpy(G->dev[stage], h.data(), n * sizeof(GroupedGemm), hipMemcpyHostToDevice));
