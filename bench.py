@@ -172,66 +172,77 @@ def extra_legs(api, synthetic, steps, warmup, seed):
 def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
     """64 systems, nbf ~ U[48, 400], naux = 4 nbf, nbf5 = ceil(nbf/3): one grouped build
     of all of them per step (graph replay), against per-system builds on their own
-    streams."""
+    streams; in FP64 and in split precision (the same tensors and orbitals, so the split
+    results are compared with the FP64 ones)."""
     shapes = synthetic.grouped_shapes(nsys, seed)
     F = sum(synthetic.build_flops(*sh) for sh in shapes)
-    systems = []
-    for i, (nbf, naux, nbf5) in enumerate(shapes):
-        s = api.System(synthetic.synthetic_param(nbf, naux, nbf5))
-        s.generate_b(seed + i)
-        s.set_orbitals(synthetic.make_C(nbf, seed + 1000 + i))
-        s.set_hcore(synthetic.make_H(nbf, seed + 2000 + i))
-        systems.append(s)
-    log("grouped: %d systems resident, %.2f TFLOP per build of all" % (nsys, F / 1e12))
-    for _ in range(warmup):
+    out = {"config": {"nsys": nsys, "nbf_range": [48, 400], "seed": seed,
+                      "nbf": [sh[0] for sh in shapes],
+                      "tflop_per_build_of_all": F / 1e12,
+                      "b_gb": sum(8.0 * n * n * a for n, a, _ in shapes) / 1e9}}
+    ref64 = None
+    for prec in ("fp64", "split"):
+        systems = []
+        for i, (nbf, naux, nbf5) in enumerate(shapes):
+            s = api.System(synthetic.synthetic_param(nbf, naux, nbf5), precision=prec)
+            s.generate_b(seed + i)
+            s.set_orbitals(synthetic.make_C(nbf, seed + 1000 + i))
+            s.set_hcore(synthetic.make_H(nbf, seed + 2000 + i))
+            systems.append(s)
+        log("grouped %s: %d systems resident, %.2f TFLOP per build of all"
+            % (prec, nsys, F / 1e12))
+        for _ in range(warmup):
+            for s in systems:
+                s.jkh_mo_resident()
         for s in systems:
-            s.jkh_mo_resident()
-    for s in systems:
-        s.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+            s.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for s in systems:
+                s.jkh_mo_resident()
         for s in systems:
-            s.jkh_mo_resident()
-    for s in systems:
-        s.sync()
-    seq_ms = (time.perf_counter() - t0) * 1e3 / steps
-    ref = [s.get_jkh() for s in systems[:4]]
-    g = api.Group(systems)
-    t0 = time.perf_counter()
-    cfg = g.tune()
-    tune_s = time.perf_counter() - t0
-    plan = g.plan()
-    for _ in range(warmup):
-        g.jkh()
-    g.sync()
-    g.set_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.jkh()
-    g.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    ev_ms, n = g.get_timing()
-    err = 0.0
-    for s, (J, K, Hc) in zip(systems[:4], ref):
-        Jg, Kg, Hg = s.get_jkh()
-        err = max(err, float(np.abs(Jg - J).max() / np.abs(J).max()),
-                  float(np.abs(Kg - K).max() / np.abs(K).max()))
-    g.close()
-    for s in systems:
-        s.close()
-    tf = F / (ms * 1e-3) / 1e12
-    log("grouped fp64: %.2f ms per build of all %d (%.1f TFLOP/s), per-system streams %.2f ms"
-        % (ms, nsys, tf, seq_ms))
-    return {"config": {"nsys": nsys, "nbf_range": [48, 400], "seed": seed,
-                       "nbf": [sh[0] for sh in shapes],
-                       "tflop_per_build_of_all": F / 1e12,
-                       "b_gb": sum(8.0 * n * n * a for n, a, _ in shapes) / 1e9},
-            "fp64": {"builds_per_s": nsys * 1e3 / ms, "ms_per_build_of_all": ms,
-                     "graph_event_ms": ev_ms / max(n, 1), "tflops": tf,
-                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "tiles": cfg, "plan": plan,
-                     "autotune_s": tune_s,
-                     "per_system_streams_ms": seq_ms, "speedup_vs_per_system": seq_ms / ms,
-                     "max_rel_diff_vs_per_system": err}}
+            s.sync()
+        seq_ms = (time.perf_counter() - t0) * 1e3 / steps
+        ref = [s.get_jkh() for s in systems[:4]]
+        g = api.Group(systems)
+        t0 = time.perf_counter()
+        cfg = g.tune()
+        tune_s = time.perf_counter() - t0
+        plan = g.plan()
+        for _ in range(warmup):
+            g.jkh()
+        g.sync()
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.jkh()
+        g.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ev_ms, n = g.get_timing()
+        got = [s.get_jkh() for s in systems[:4]]
+
+        def rel(a, b):
+            return max(float(np.abs(x - y).max() / np.abs(y).max())
+                       for (xj, xk, _), (yj, yk, _) in zip(a, b)
+                       for x, y in ((xj, yj), (xk, yk)))
+        g.close()
+        for s in systems:
+            s.close()
+        tf = F / (ms * 1e-3) / 1e12
+        log("grouped %s: %.2f ms per build of all %d (%.1f TFLOP/s), per-system streams %.2f ms"
+            % (prec, ms, nsys, tf, seq_ms))
+        leg = {"builds_per_s": nsys * 1e3 / ms, "ms_per_build_of_all": ms,
+               "graph_event_ms": ev_ms / max(n, 1), "tflops": tf, "tiles": cfg, "plan": plan,
+               "autotune_s": tune_s, "per_system_streams_ms": seq_ms,
+               "speedup_vs_per_system": seq_ms / ms,
+               "max_rel_diff_vs_per_system": rel(got, ref)}
+        if prec == "fp64":
+            ref64 = got
+            leg["frac_fp64_peak"] = tf / FP64_PEAK_TFLOPS
+        else:
+            leg["JK_max_rel_err_vs_fp64_group"] = rel(got, ref64)
+        out[prec] = leg
+    return out
 
 
 def serving_leg(api, serve, n=24, rate=4.0, seed=7, max_live=8):

@@ -6,6 +6,7 @@
 // the Lagrangian's C^K/C^J contraction (batch-reduce over Q), H C5, C^T H C5 and the
 // matrix products of the orbital-rotation exponential.
 #include "dn_internal.h"
+#include "dn_sched.h"
 
 namespace dn {
 
@@ -453,48 +454,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_mfma_kernel(Gemm g, int 
   gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(g, mt, nt, z, As, Bs);
 }
 
-// Persistent grouped GEMM schedule.  Problem i owns tiles [0, n_i), n_i =
-// tiles_m * tiles_n * nsplit.  With a grid that is a multiple of the 8 XCDs, workgroup b
-// runs on XCD x = b % 8 and takes slots b / 8, b / 8 + grid / 8, ... of that XCD's list;
-// slot s belongs to the problem i with slot_begin_i <= s < slot_begin_{i+1} and is its
-// tile x * ceil(n_i / 8) + (s - slot_begin_i).  Every XCD thus works through one
-// contiguous eighth of every problem: the M-tiles that share a B block (and consecutive B
-// blocks) meet in the same L2, and a large problem is spread over all XCDs instead of
-// landing on the one or two that a contiguous split of the whole list would give it.
-// Without the XCD multiple, workgroup b takes list positions b, b + grid, ... in
-// tile_begin order.  The schedule is static, so every workgroup's loop ends.
-template <typename Tile>
-__device__ __forceinline__ void grouped_schedule(const GroupedGemm* __restrict__ desc, int nprob,
-                                                 long total_tiles, long total_slots,
-                                                 Tile&& tile) {
-  const bool chunk = (gridDim.x % 8u) == 0;
-  const long x = blockIdx.x % 8u;
-  const long first = chunk ? blockIdx.x / 8u : blockIdx.x;
-  const long step = chunk ? gridDim.x / 8u : gridDim.x;
-  const long span = chunk ? total_slots : total_tiles;
-  for (long pos = first; pos < span; pos += step) {
-    int lo = 0, hi = nprob - 1;  // the last problem starting at or before pos
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((chunk ? desc[mid].slot_begin : desc[mid].tile_begin) <= pos) lo = mid;
-      else hi = mid - 1;
-    }
-    // lo is uniform over the workgroup; saying so makes the descriptor loads scalar (SGPRs)
-    // instead of per-lane vector loads.  A private copy: the tile's stores to C cannot
-    // alias it, so it is not reloaded after every store.
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    const GroupedGemm d = desc[lo];
-    const long n = (long)d.tiles_m * d.tiles_n * d.nsplit;
-    const long local = chunk ? x * ((n + 7) / 8) + (pos - d.slot_begin) : pos - d.tile_begin;
-    if (local >= n) continue;  // the last XCD's share of a problem can be short
-    const int mt = local % d.tiles_m;
-    const int nt = (local / d.tiles_m) % d.tiles_n;
-    const int z = local / ((long)d.tiles_m * d.tiles_n);
-    tile(d.g, mt, nt, z);
-    __syncthreads();  // the next tile reuses the LDS buffers
-  }
-}
-
+// Persistent grouped GEMMs (the schedule of dn_sched.h).
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
           int TRI = 0>
 __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(
@@ -503,8 +463,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_kernel(
   __shared__ double As[2 * L::A];
   __shared__ double Bs[2 * L::B];
   grouped_schedule(desc, nprob, total_tiles, total_slots,
-                   [&](const Gemm& g, int mt, int nt, int z) {
-                     gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(g, mt, nt, z, As, Bs);
+                   [&](const GroupedGemm& d, int mt, int nt, int z) {
+                     gemm_tile<BM, BN, BK, WGM, WGN, AK, BKC, VA, VB, TRI>(d.g, mt, nt, z, As,
+                                                                          Bs);
                    });
 }
 
@@ -513,8 +474,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_grouped_lds_kernel(
     const GroupedGemm* __restrict__ desc, int nprob, long total_tiles, long total_slots) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[lds_tile_bytes<BM, BN>(NST)];
   grouped_schedule(desc, nprob, total_tiles, total_slots,
-                   [&](const Gemm& g, int mt, int nt, int z) {
-                     gemm_tile_lds<BM, BN, WGM, WGN, NST, TRI>(g, mt, nt, z, smem);
+                   [&](const GroupedGemm& d, int mt, int nt, int z) {
+                     gemm_tile_lds<BM, BN, WGM, WGN, NST, TRI>(d.g, mt, nt, z, smem);
                    });
 }
 
@@ -646,25 +607,6 @@ void config_tile(int config, int* bm, int* bn) {
   }
 }
 
-// Persistent launch of a kernel over `total` tiles: grid <= 0 means -grid times as many
-// workgroups as stay resident (0 = once), a multiple of the 8 XCDs, at most one per tile
-// (rounded up to the XCD multiple).
-template <typename Kern>
-int launch_persistent(hipStream_t s, Kern kern, int threads, const GroupedGemm* d, int n,
-                      long total, long slots, int grid) {
-  if (grid <= 0) {
-    int dev = 0, cus = 0, occ = 0;
-    DN_HIP(hipGetDevice(&dev));
-    DN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    DN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, 0));
-    const long full = (long)std::max(occ, 1) * cus * std::max(-grid, 1);
-    grid = (int)std::min(std::min(full, (total + 7) / 8 * 8), 0x7ffffff8L);
-  }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, d, n, total, slots);
-  DN_HIP(hipGetLastError());
-  return DN_OK;
-}
-
 template <int BM, int BN, int BK, int WGM, int WGN, bool AK, bool BKC, bool VA, bool VB,
           int TRI = 0>
 int launch_grouped(hipStream_t s, const GroupedGemm* d, int n, long total, long slots,
@@ -716,21 +658,13 @@ int grouped_layout(hipStream_t s, bool ak, bool bk, bool va, bool vb, int tri,
 long gemm_grouped_plan(GroupedGemm* desc, int n, int config) {
   int bm, bn;
   config_tile(config, &bm, &bn);
-  long begin = 0, slot = 0;
   for (int i = 0; i < n; ++i) {
     Gemm& g = desc[i].g;
     g.vec_a = vec_ok(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
     g.vec_b = vec_ok(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
-    desc[i].tiles_m = (g.M + bm - 1) / bm;
-    desc[i].tiles_n = (g.N + bn - 1) / bn;
     desc[i].nsplit = (g.nbatch + g.bps - 1) / g.bps;
-    desc[i].tile_begin = begin;
-    desc[i].slot_begin = slot;
-    const long tiles = (long)desc[i].tiles_m * desc[i].tiles_n * desc[i].nsplit;
-    begin += tiles;
-    slot += (tiles + 7) / 8;
   }
-  return begin;
+  return grouped_offsets(desc, n, bm, bn);
 }
 
 int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd, int n,
@@ -745,9 +679,7 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     va = va && dh[i].g.vec_a;
     vb = vb && dh[i].g.vec_b;
   }
-  const GroupedGemm& last = dh[n - 1];
-  const long slots =
-      last.slot_begin + ((long)last.tiles_m * last.tiles_n * last.nsplit + 7) / 8;
+  const long slots = grouped_slots(dh, n);
   if (config >= kLds64x128) {
     if (!ak || !bk || !va || !vb || tri == kTriB)
       return fail(DN_ERR_ARG, "grouped GEMM: LDS-direct configurations need K-contiguous, "

@@ -139,6 +139,10 @@ struct dn_group {
   std::vector<GroupedGemm> host[kNumStages];
   std::vector<Launch> launches[kNumStages];
   GroupedGemm* dev[kNumStages] = {};
+  // split-precision groups: the two transforms run as grouped split launches (kind 0 =
+  // K1, 1 = K2) over these descriptors instead of host[kK1], host[kK2]
+  std::vector<GroupedSplit> shost[2];
+  GroupedSplit* sdev[2] = {};
   KxDesc* kx = nullptr;
   long kx_blocks[3] = {0, 0, 0};
   DiagDesc* diag = nullptr;
@@ -229,8 +233,56 @@ int default_config(int stage, int bucket, const Gemm& g) {
 }
 
 // Classes of one stage with their default tile configurations; descriptors uploaded.
+bool split_stage(const dn_group* G, int stage) {
+  return G->precision == DN_SPLIT_F16 && (stage == kK1 || stage == kK2);
+}
+
+// The grouped split transforms of one system (the operands of jkh_split's symmetric build).
+SplitGemm split_stage_gemm(dn_sys* s, int stage) {
+  const dn_dims& d = s->d;
+  if (stage == kK1)
+    return split_k1_sym_gemm(d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo, s->binv,
+                             s->chi, s->clo, s->cscale, s->bpn32, s->qmax);
+  return split_k2_gemm(d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale, s->bpn32,
+                       s->bpq32, s->qs, s->qinv);
+}
+
+int split_default_config(int stage, int bucket) {
+  if (stage == kK1) return bucket == 0 ? kSplit32x64 : bucket == 1 ? kSplit64x64 : kSplit128x128;
+  return bucket <= 1 ? kSplit64x64 : kSplit128x128;
+}
+
+int plan_split_stage(dn_group* G, int stage) {
+  auto& h = G->shost[stage];
+  h.resize(G->sys.size());
+  for (size_t i = 0; i < G->sys.size(); ++i) h[i].g = split_stage_gemm(G->sys[i], stage);
+  auto key = [](const GroupedSplit& d) {  // nbf5 class
+    const int n5 = d.g.M;
+    return n5 <= 32 ? 0 : n5 <= 64 ? 1 : n5 <= 128 ? 2 : 3;
+  };
+  std::stable_sort(h.begin(), h.end(),
+                   [&](const GroupedSplit& a, const GroupedSplit& b) { return key(a) < key(b); });
+  auto& L = G->launches[stage];
+  L.clear();
+  for (int i = 0; i < (int)h.size();) {
+    int j = i;
+    double f = 0.0;
+    while (j < (int)h.size() && key(h[j]) == key(h[i])) {
+      f += 2.0 * h[j].g.M * h[j].g.N * (double)h[j].g.K * h[j].g.nbatch;
+      ++j;
+    }
+    const int cfg = split_default_config(stage, key(h[i]));
+    L.push_back({i, j - i, split_grouped_plan(h.data() + i, j - i, cfg), cfg, f, 0});
+    i = j;
+  }
+  DN_HIP(hipMemcpy(G->sdev[stage], h.data(), h.size() * sizeof(GroupedSplit),
+                   hipMemcpyHostToDevice));
+  return DN_OK;
+}
+
 int plan_stage(dn_group* G, int stage) {
   DN_HIP(hipStreamSynchronize(G->stream));  // a replay in flight may read the descriptors
+  if (split_stage(G, stage)) return plan_split_stage(G, stage);
   auto& h = G->host[stage];
   h.resize(G->sys.size());
   for (size_t i = 0; i < G->sys.size(); ++i) h[i].g = stage_gemm(G->sys[i], stage);
@@ -263,6 +315,14 @@ int plan_stage(dn_group* G, int stage) {
 int set_config(dn_group* G, int stage, size_t li, int config) {
   DN_HIP(hipStreamSynchronize(G->stream));
   auto& l = G->launches[stage][li];
+  if (split_stage(G, stage)) {
+    GroupedSplit* h = G->shost[stage].data() + l.first;
+    l.tiles = split_grouped_plan(h, l.count, config);
+    l.config = config;
+    DN_HIP(hipMemcpy(G->sdev[stage] + l.first, h, l.count * sizeof(GroupedSplit),
+                     hipMemcpyHostToDevice));
+    return DN_OK;
+  }
   GroupedGemm* h = G->host[stage].data() + l.first;
   l.tiles = gemm_grouped_plan(h, l.count, config);
   l.config = config;
@@ -273,6 +333,9 @@ int set_config(dn_group* G, int stage, size_t li, int config) {
 
 int launch_one(dn_group* G, int stage, size_t li) {
   const auto& l = G->launches[stage][li];
+  if (split_stage(G, stage))
+    return split_grouped(G->stream, G->shost[stage].data() + l.first, G->sdev[stage] + l.first,
+                         l.count, l.tiles, l.config, stage == kK1 ? 0 : 1, l.grid);
   return gemm_f64_grouped(G->stream, G->host[stage].data() + l.first, G->dev[stage] + l.first,
                           l.count, l.tiles, l.config, l.grid);
 }
@@ -284,9 +347,19 @@ int launch_stage(dn_group* G, int stage) {
 
 int launch_build(dn_group* G) {
   const int n = (int)G->sys.size();
+  const bool split = G->precision == DN_SPLIT_F16;
+  if (split)  // C5 planes for the current orbitals, cleared per-Q maxima
+    for (dn_sys* s : G->sys) {
+      DN_TRY(split_prepare_c(G->stream, s->C, s->d.nbf, s->d.nbf5, s->Mpad, s->Kpad, s->cscale,
+                             s->chi, s->clo));
+      DN_TRY(split_k1_begin(G->stream, s->qmax, s->d.nbfaux));
+    }
   DN_TRY(launch_stage(G, kK1));
+  if (split)  // per-Q scales of the first transform's output
+    for (dn_sys* s : G->sys)
+      DN_TRY(split_k2_begin(G->stream, s->qmax, s->d.nbfaux, s->qs, s->qinv));
   DN_TRY(launch_stage(G, kK2));
-  DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red));
+  DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red, split));
   if (G->nkx) {
     const int m = G->nkx;
     hipLaunchKernelGGL(group_kpart_kernel, dim3(G->kx_blocks[0]), dim3(256), 0, G->stream, G->kx, m);
@@ -382,8 +455,13 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
     if (!s->b_ready) return cleanup(fail(DN_ERR_STATE, "b has not been uploaded or generated"));
     G->sys.push_back(s);
   }
-  if (G->precision != DN_FP64)
-    return cleanup(fail(DN_ERR_ARG, "grouped builds support DN_FP64 systems"));
+  if (G->precision != DN_FP64 && G->precision != DN_SPLIT_F16)
+    return cleanup(fail(DN_ERR_ARG, "grouped builds support DN_FP64 and DN_SPLIT_F16 systems"));
+  if (G->precision == DN_SPLIT_F16)
+    for (dn_sys* s : G->sys)
+      if (!(s->b_sym && s->use_sym))
+        return cleanup(fail(DN_ERR_ARG, "grouped split-precision builds need the symmetric-b "
+                                        "build (a symmetric b with symmetry enabled)"));
   hipError_t e = hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&G->done, hipEventDisableTiming);
   for (int st = 0; st < kNumStages && e == hipSuccess; ++st)
@@ -391,6 +469,8 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
   if (e == hipSuccess) e = hipMalloc(&G->kx, nsys * sizeof(KxDesc));
   if (e == hipSuccess) e = hipMalloc(&G->diag, nsys * sizeof(DiagDesc));
   if (e == hipSuccess) e = hipMalloc(&G->symk, nsys * sizeof(SymKDesc));
+  for (int k = 0; k < 2 && e == hipSuccess && G->precision == DN_SPLIT_F16; ++k)
+    e = hipMalloc(&G->sdev[k], nsys * sizeof(GroupedSplit));
   if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, std::string("dn_group_create: ") +
                                                           hipGetErrorString(e)));
   std::vector<KxDesc> kx;
@@ -403,7 +483,8 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
     col += s->d.nbf5;
     if (s->b_sym && s->use_sym) {  // K, X straight from W (b_pq = W + W^T)
       SymKDesc q;
-      q.W = s->bt;
+      q.W = G->precision == DN_SPLIT_F16 ? nullptr : s->bt;
+      q.Wf = G->precision == DN_SPLIT_F16 ? s->bpq32 : nullptr;
       q.K = s->K;
       q.X = s->X;
       q.part = s->part;
@@ -463,6 +544,8 @@ int dn_group_destroy(dn_group* G) {
   if (G->kx) (void)hipFree(G->kx);
   if (G->diag) (void)hipFree(G->diag);
   if (G->symk) (void)hipFree(G->symk);
+  for (int k = 0; k < 2; ++k)
+    if (G->sdev[k]) (void)hipFree(G->sdev[k]);
   for (auto e : G->ev_start) (void)hipEventDestroy(e);
   for (auto e : G->ev_stop) (void)hipEventDestroy(e);
   if (G->done) (void)hipEventDestroy(G->done);
@@ -475,14 +558,20 @@ int dn_group_tune(dn_group* G, int* configs) {
   if (!G) return fail(DN_ERR_ARG, "null dn_group");
   DN_HIP(hipSetDevice(G->ctx->device));
   DN_TRY(join_systems(G));
+  // one eager build first: every stage's inputs (C planes and per-Q scales of split
+  // groups included) then hold real values while candidate launches are timed
+  DN_TRY(launch_build(G));
   for (int st = 0; st < kNumStages; ++st) {
     for (size_t li = 0; li < G->launches[st].size(); ++li) {
       // tile shape x workgroup count (1, 2 or 4 times the resident count, or one per tile)
       static const int grids[] = {0, -2, -4, -(1 << 20)};
       int best = G->launches[st][li].config, best_grid = 0;
       float best_ms = 1e30f;
-      for (int c = 0; c < kNumGemmConfigs; ++c) {
-        if (!gemm_config_ok(c, G->host[st][G->launches[st][li].first].g)) continue;
+      const bool sp = split_stage(G, st);
+      for (int c = 0; c < (sp ? (int)kNumSplitConfigs : (int)kNumGemmConfigs); ++c) {
+        if (sp ? !split_config_ok(c, st == kK1 ? 0 : 1)
+               : !gemm_config_ok(c, G->host[st][G->launches[st][li].first].g))
+          continue;
         DN_TRY(set_config(G, st, li, c));
         for (int gr : grids) {
           G->launches[st][li].grid = gr;
@@ -508,7 +597,7 @@ int dn_group_tune(dn_group* G, int* configs) {
       for (const auto& l : G->launches[st])
         if (l.flops > fmax) {
           fmax = l.flops;
-          configs[st] = l.config;
+          configs[st] = split_stage(G, st) ? 100 + l.config : l.config;
         }
     }
   return DN_OK;
@@ -518,6 +607,7 @@ int dn_group_plan_text(dn_group* G, char* buf, int len) {
   if (!G || !buf || len <= 0) return fail(DN_ERR_ARG, "bad dn_group_plan_text arguments");
   static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64",
                                 "lds64x128", "lds128x128", "lds128x256"};
+  static const char* snames[] = {"s128x128", "s64x128", "s64x64", "s32x64"};
   static const char* stages[] = {"K1", "K2", "J", "HC"};
   std::string t;
   for (int st = 0; st < kNumStages; ++st) {
@@ -528,7 +618,8 @@ int dn_group_plan_text(dn_group* G, char* buf, int len) {
       char grid[16];
       if (l.grid <= -(1 << 20)) std::snprintf(grid, sizeof(grid), "all");
       else std::snprintf(grid, sizeof(grid), "r%d", l.grid < 0 ? -l.grid : 1);
-      std::snprintf(one, sizeof(one), " %dx%s/%s(%.1fGF)", l.count, names[l.config], grid,
+      std::snprintf(one, sizeof(one), " %dx%s/%s(%.1fGF)", l.count,
+                    split_stage(G, st) ? snames[l.config] : names[l.config], grid,
                     l.flops * 1e-9);
       t += one;
     }

@@ -61,6 +61,69 @@ struct Gemm {
 };
 enum { kTriNone = 0, kTriB = 1, kTriA = 2 };
 
+// One split-precision GEMM (dn_split.hip): FP16 hi+lo operand planes, three MFMA products
+// accumulated in FP32.
+struct SplitGemm {
+  int M, N, K;
+  // A: either f16 planes [row][k] (ld = lda) or FP32 column-major A[m + k*lda]
+  const _Float16* ahi;
+  const _Float16* alo;
+  const float* af32;
+  long lda, sA;             // batch stride (FP32 source)
+  const float* a_scale;     // per batch: scale applied to the FP32 A before splitting
+  // B: f16 planes [col][k] (ld = ldb), batch stride sB (halves)
+  const _Float16* bhi;
+  const _Float16* blo;
+  long ldb, sB;
+  // C: FP32 column-major, batch stride sC
+  float* C;
+  long ldc, sC;
+  // out = acc * cs[1] * (per-Q col factor or per-batch factor); cs = C5 scale pair on
+  // the device (scale, 1/scale)
+  const float* cs;
+  const float* col_inv_scale;  // per Q = col / col_div (K1), nullable
+  int col_div;
+  const float* batch_inv_scale;  // per batch (K2, symmetric K1), nullable
+  float* qmax;                   // per Q max |out| (K1: by column group, or by batch when
+                                 // col_div == 0), nullable
+  int nbatch;
+};
+
+// One problem of a grouped split-precision launch (offsets as for GroupedGemm; nsplit is
+// the batch count).
+struct GroupedSplit {
+  SplitGemm g;
+  int tiles_m = 0, tiles_n = 0, nsplit = 0;
+  long tile_begin = 0, slot_begin = 0;
+};
+// Tile configurations of the grouped split launches; kind 0 is the symmetric-b first
+// transform (LDS-direct, all four), kind 1 the second transform (FP32 A split on load:
+// the first three).
+enum SplitConfig {
+  kSplit128x128 = 0,
+  kSplit64x128 = 1,
+  kSplit64x64 = 2,
+  kSplit32x64 = 3,
+  kNumSplitConfigs = 4
+};
+bool split_config_ok(int config, int kind);
+// Operands of one system's split transforms (the same for the single-system and the
+// grouped builds) and the per-system kernels that precede each launch.
+SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Kpad, int ld8,
+                            const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                            const _Float16* chi, const _Float16* clo, const float* cscale_dev,
+                            float* bpn, float* qmax);
+SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+                        const _Float16* clo, const float* cscale_dev, const float* bpn,
+                        float* bpq, const float* qs, const float* qinv);
+int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
+                    float* cscale, _Float16* hi, _Float16* lo);
+int split_k1_begin(hipStream_t s, float* qmax, int naux);
+int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float* qinv);
+long split_grouped_plan(GroupedSplit* desc, int n, int config);
+int split_grouped(hipStream_t s, const GroupedSplit* desc_host, const GroupedSplit* desc_dev,
+                  int n, long total, int config, int kind, int grid);
+
 int gemm_f64(hipStream_t s, const Gemm& g);
 
 // One problem of a grouped GEMM launch (tile list offsets filled by gemm_grouped_plan).

@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "dn_internal.h"
+#include "dn_sched.h"
 
 namespace dn {
 
@@ -130,32 +131,6 @@ __global__ void fill_f32_kernel(float* p, long n, float v) {
 
 // ---- the GEMM ----------------------------------------------------------------------------
 
-struct SplitGemm {
-  int M, N, K;
-  // A: either f16 planes [row][k] (ld = lda) or FP32 column-major A[m + k*lda]
-  const _Float16* ahi;
-  const _Float16* alo;
-  const float* af32;
-  long lda, sA;             // batch stride (FP32 source)
-  const float* a_scale;     // per batch: scale applied to the FP32 A before splitting
-  // B: f16 planes [col][k] (ld = ldb), batch stride sB (halves)
-  const _Float16* bhi;
-  const _Float16* blo;
-  long ldb, sB;
-  // C: FP32 column-major, batch stride sC
-  float* C;
-  long ldc, sC;
-  // out = acc * cs[1] * (per-Q col factor or per-batch factor); cs = C5 scale pair on
-  // the device (scale, 1/scale)
-  const float* cs;
-  const float* col_inv_scale;  // per Q = col / col_div (K1), nullable
-  int col_div;
-  const float* batch_inv_scale;  // per batch (K2, symmetric K1), nullable
-  float* qmax;                   // per Q max |out| (K1: by column group, or by batch when
-                                 // col_div == 0), nullable
-  int nbatch;
-};
-
 constexpr int SB_K = 32;  // k per MFMA / per LDS tile: four 16-byte chunks of 8 halves
 // LDS tiles are K-major in 16-byte chunks: chunk c (k = 8c..8c+7) of row r at
 // (c * ROWS + r) * 8 halves.  An MFMA fragment read (lane l: row l % 16, chunk l / 16) and
@@ -236,20 +211,24 @@ __device__ __forceinline__ void split_epilogue(const SplitGemm& g, v4f (&acc)[FM
 // TRIB: B is the symmetric slab b_Q (col n, k = m) and the tile multiplies by its lower
 // half Lh (m > n as is, m == n halved, m < n zero; b_Q = Lh + Lh^T), so the K loop of
 // column block n0 starts at m = n0.
-template <int BM, int BN, int WGM, int WGN, bool AF32, bool TRIB = false>
-__global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g, int tiles_m,
-                                                                   int tiles_n) {
+// LDS of split_tile: double-buffered hi and lo planes of A (BM rows) and B (BN rows)
+template <int BM, int BN>
+constexpr int split_tile_bytes() {
+  return 2 * 2 * (BM + BN) * SB_K * (int)sizeof(_Float16);
+}
+
+template <int BM, int BN, int WGM, int WGN, bool AF32, bool TRIB>
+__device__ __forceinline__ void split_tile(const SplitGemm& g, int mt, int nt, int z,
+                                           unsigned char* smem) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
-  __shared__ _Float16 Ah[2][BM * SB_K], Al[2][BM * SB_K];
-  __shared__ _Float16 Bh[2][BN * SB_K], Bl[2][BN * SB_K];
+  typedef _Float16 PlaneA[2][BM * SB_K];
+  typedef _Float16 PlaneB[2][BN * SB_K];
+  PlaneA& Ah = *reinterpret_cast<PlaneA*>(smem);
+  PlaneA& Al = *reinterpret_cast<PlaneA*>(smem + sizeof(PlaneA));
+  PlaneB& Bh = *reinterpret_cast<PlaneB*>(smem + 2 * sizeof(PlaneA));
+  PlaneB& Bl = *reinterpret_cast<PlaneB*>(smem + 2 * sizeof(PlaneA) + sizeof(PlaneB));
 
-  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
-  if (blockIdx.x >= total) return;
-  const unsigned lid = xcd_remap16(blockIdx.x, total);
-  const int mt = lid % tiles_m;
-  const int nt = (lid / tiles_m) % tiles_n;
-  const int z = lid / (tiles_m * tiles_n);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
@@ -411,6 +390,19 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g,
   split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
 }
 
+template <int BM, int BN, int WGM, int WGN, bool AF32, bool TRIB = false>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_gemm_kernel(SplitGemm g, int tiles_m,
+                                                                   int tiles_n) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[split_tile_bytes<BM, BN>()];
+  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
+  if (blockIdx.x >= total) return;
+  const unsigned lid = xcd_remap16(blockIdx.x, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  split_tile<BM, BN, WGM, WGN, AF32, TRIB>(g, mt, nt, z, smem);
+}
+
 // ---- f16-plane GEMM with LDS-direct loads ---------------------------------------------
 //
 // Both operands are FP16 planes (A [row][k], B [col][k]).  Every k tile of A and B goes
@@ -431,9 +423,14 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+template <int BM, int BN, int NST>
+constexpr int split_lds_bytes() {
+  return NST * (BM + BN) * 128;  // per stage: 2 planes x rows x 64 bytes
+}
+
 template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB>
-__global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, int tiles_m,
-                                                                  int tiles_n) {
+__device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int nt, int z,
+                                               unsigned char* smem) {
   constexpr int NW = WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
   constexpr int A_BYTES = BM * 128, B_BYTES = BN * 128;  // 2 planes x rows x 64 B
@@ -443,14 +440,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, 
   static_assert(BM % 16 == 0 && BN % 16 == 0 && (NA + NB) % NW == 0, "load split");
   static_assert(NST >= 2 && NST <= 4 && (NST - 2) * LW < 64, "pipeline depth");
   static_assert(FM >= 1 && FN >= 1, "wave tile");
-  __shared__ __attribute__((aligned(16))) unsigned char smem[NST * STAGE];
+  static_assert(split_lds_bytes<BM, BN, NST>() == NST * STAGE, "LDS size");
 
-  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
-  if (blockIdx.x >= total) return;
-  const unsigned lid = xcd_remap16(blockIdx.x, total);
-  const int mt = lid % tiles_m;
-  const int nt = (lid / tiles_m) % tiles_n;
-  const int z = lid / (tiles_m * tiles_n);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
@@ -566,6 +557,40 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, 
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
   }
   split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, int tiles_m,
+                                                                  int tiles_n) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[split_lds_bytes<BM, BN, NST>()];
+  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
+  if (blockIdx.x >= total) return;
+  const unsigned lid = xcd_remap16(blockIdx.x, total);
+  const int mt = lid % tiles_m;
+  const int nt = (lid / tiles_m) % tiles_n;
+  const int z = lid / (tiles_m * tiles_n);
+  split_tile_lds<BM, BN, WGM, WGN, NST, TRIB>(g, mt, nt, z, smem);
+}
+
+// Persistent grouped split GEMMs (the schedule of dn_sched.h).
+template <int BM, int BN, int WGM, int WGN, int NST>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_grouped_lds_kernel(
+    const GroupedSplit* __restrict__ desc, int nprob, long total_tiles, long total_slots) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[split_lds_bytes<BM, BN, NST>()];
+  grouped_schedule(desc, nprob, total_tiles, total_slots,
+                   [&](const GroupedSplit& d, int mt, int nt, int z) {
+                     split_tile_lds<BM, BN, WGM, WGN, NST, true>(d.g, mt, nt, z, smem);
+                   });
+}
+
+template <int BM, int BN, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void split_grouped_f32a_kernel(
+    const GroupedSplit* __restrict__ desc, int nprob, long total_tiles, long total_slots) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[split_tile_bytes<BM, BN>()];
+  grouped_schedule(desc, nprob, total_tiles, total_slots,
+                   [&](const GroupedSplit& d, int mt, int nt, int z) {
+                     split_tile<BM, BN, WGM, WGN, true, false>(d.g, mt, nt, z, smem);
+                   });
 }
 
 // ---- f16-plane GEMM, B streamed straight into registers ---------------------------
@@ -787,13 +812,12 @@ int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
 
 // Symmetric b: bpn[p,n,Q] = sum_{m >= n} C5[m,p] Lh[m,n,Q] (b_Q = Lh + Lh^T), batched
 // over Q so that column block n0 reads only rows m >= n0 of the slab; the second
-// transform then yields W with b_pq = W + W^T.
-int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
-                 const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
-                 const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
-                 float* qmax) {
-  hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_of(naux)), dim3(256), 0, s, qmax, (long)naux, 0.f);
-  DN_HIP(hipGetLastError());
+// transform then yields W with b_pq = W + W^T.  split_k1_begin clears the per-Q maxima
+// first.
+SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Kpad, int ld8,
+                            const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                            const _Float16* chi, const _Float16* clo, const float* cscale_dev,
+                            float* bpn, float* qmax) {
   SplitGemm g1{};
   g1.M = nbf5; g1.N = nbf; g1.K = nbf;
   g1.ahi = chi; g1.alo = clo; g1.lda = Kpad;
@@ -801,6 +825,22 @@ int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
   g1.C = bpn; g1.ldc = nbf5; g1.sC = (long)nbf5 * nbf;
   g1.cs = cscale_dev; g1.batch_inv_scale = b_inv_scale; g1.col_div = 0;
   g1.qmax = qmax; g1.nbatch = naux;
+  return g1;
+}
+
+int split_k1_begin(hipStream_t s, float* qmax, int naux) {
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_of(naux)), dim3(256), 0, s, qmax, (long)naux, 0.f);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+                 const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                 const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
+                 float* qmax) {
+  DN_TRY(split_k1_begin(s, qmax, naux));
+  const SplitGemm g1 = split_k1_sym_gemm(nbf, nbf5, naux, Kpad, ld8, bhi, blo, b_inv_scale, chi,
+                                         clo, cscale_dev, bpn, qmax);
   const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
   const long total = (long)tm * tn * g1.nbatch;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split gemm grid too large");
@@ -810,12 +850,11 @@ int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
   return DN_OK;
 }
 
-int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
-             const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
-             const float* qmax, float* qs, float* qinv) {
-  hipLaunchKernelGGL(bpn_scale_kernel, dim3((naux + 255) / 256), dim3(256), 0, s, qmax, naux, qs,
-                     qinv);
-  DN_HIP(hipGetLastError());
+// bpq[:,:,Q] = bpn[:,:,Q] C5 with bpn split on load under the per-Q scale that
+// split_k2_begin derives from the first transform's maxima.
+SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+                        const _Float16* clo, const float* cscale_dev, const float* bpn,
+                        float* bpq, const float* qs, const float* qinv) {
   SplitGemm g2{};
   g2.M = nbf5; g2.N = nbf5; g2.K = nbf;
   g2.af32 = bpn; g2.lda = nbf5; g2.sA = (long)nbf5 * nbf; g2.a_scale = qs;
@@ -823,7 +862,77 @@ int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float1
   g2.C = bpq; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
   g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.col_div = 0;
   g2.nbatch = naux;
-  return launch_split<128, 128, 2, 2, true>(s, g2);
+  return g2;
+}
+
+int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float* qinv) {
+  hipLaunchKernelGGL(bpn_scale_kernel, dim3((naux + 255) / 256), dim3(256), 0, s, qmax, naux, qs,
+                     qinv);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+             const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
+             const float* qmax, float* qs, float* qinv) {
+  DN_TRY(split_k2_begin(s, qmax, naux, qs, qinv));
+  return launch_split<128, 128, 2, 2, true>(
+      s, split_k2_gemm(nbf, nbf5, naux, Kpad, chi, clo, cscale_dev, bpn, bpq, qs, qinv));
+}
+
+// ---- grouped launches ------------------------------------------------------------------
+
+namespace {
+
+void split_config_tile(int config, int* bm, int* bn) {
+  switch (config) {
+    case kSplit64x128: *bm = 64; *bn = 128; break;
+    case kSplit64x64: *bm = 64; *bn = 64; break;
+    case kSplit32x64: *bm = 32; *bn = 64; break;
+    default: *bm = 128; *bn = 128; break;
+  }
+}
+
+template <int BM, int BN>
+int launch_split_grouped(hipStream_t s, int kind, const GroupedSplit* d, int n, long total,
+                         long slots, int grid) {
+  if (kind == 0)
+    return launch_persistent(s, split_grouped_lds_kernel<BM, BN, 2, 2, 2>, 256, d, n, total,
+                             slots, grid);
+  if constexpr (BM % 64 == 0)
+    return launch_persistent(s, split_grouped_f32a_kernel<BM, BN, 2, 2>, 256, d, n, total,
+                             slots, grid);
+  return fail(DN_ERR_ARG, "grouped split GEMM: configuration not available for this stage");
+}
+
+}  // namespace
+
+bool split_config_ok(int config, int kind) {
+  if (config < 0 || config >= kNumSplitConfigs) return false;
+  return kind == 0 || config != kSplit32x64;  // FP32 A staging covers 64-row groups
+}
+
+long split_grouped_plan(GroupedSplit* desc, int n, int config) {
+  int bm, bn;
+  split_config_tile(config, &bm, &bn);
+  for (int i = 0; i < n; ++i) desc[i].nsplit = desc[i].g.nbatch;
+  return grouped_offsets(desc, n, bm, bn);
+}
+
+// kind 0: symmetric-b first transforms (split_k1_sym_gemm), kind 1: second transforms
+// (split_k2_gemm).  The per-system set-up kernels (C planes, split_k1_begin,
+// split_k2_begin) run before the respective launch.
+int split_grouped(hipStream_t s, const GroupedSplit* dh, const GroupedSplit* dd, int n,
+                  long total, int config, int kind, int grid) {
+  if (n <= 0 || total <= 0) return DN_OK;
+  if (!split_config_ok(config, kind)) return fail(DN_ERR_ARG, "grouped split GEMM: bad configuration");
+  const long slots = grouped_slots(dh, n);
+  switch (config) {
+    case kSplit64x128: return launch_split_grouped<64, 128>(s, kind, dd, n, total, slots, grid);
+    case kSplit64x64: return launch_split_grouped<64, 64>(s, kind, dd, n, total, slots, grid);
+    case kSplit32x64: return launch_split_grouped<32, 64>(s, kind, dd, n, total, slots, grid);
+    default: return launch_split_grouped<128, 128>(s, kind, dd, n, total, slots, grid);
+  }
 }
 
 }  // namespace dn

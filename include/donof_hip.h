@@ -214,14 +214,17 @@ int dn_ogroup_step(dn_ogroup* group, int* active);
  * JKH_MO (integrals.jl:131-172) on its resident b, C (dn_sys_set_orbitals) and H
  * (dn_sys_set_hcore); the reference has no grouped form, it builds one system per call.
  * Results stay in each system (read them with dn_sys_get_jkh).  The systems must outlive
- * the group and share its context and precision. */
+ * the group and share its context and precision.  DN_SPLIT_F16 groups run the split
+ * transforms of the symmetric-b build (systems with a symmetric b and symmetry enabled)
+ * as grouped split launches; J and H_core stay FP64 as in the single-system build. */
 typedef struct dn_group dn_group;
 int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out);
 int dn_group_destroy(dn_group* group);
 /* The systems of each grouped GEMM stage (K1, K2, J, H C5) are split into classes by
  * operand layout, alignment and size; this times every tile configuration of every
  * class on this group, keeps the fastest, and captures the build graph.  configs
- * (4 ints, may be NULL) receives each stage's choice for its class with the most work. */
+ * (4 ints, may be NULL) receives each stage's choice for its class with the most work
+ * (FP64 tiles 0-8; split-precision transforms 100 + their split configuration). */
 int dn_group_tune(dn_group* group, int* configs);
 /* The launch plan as text: per stage, each class's system count, tile shape and GFLOP. */
 int dn_group_plan_text(dn_group* group, char* buf, int len);

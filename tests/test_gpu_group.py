@@ -75,6 +75,59 @@ def test_group_untuned_replay_and_timing(oracle):
         s.close()
 
 
+SPLIT_SHAPES = [(64, 256, 22), (37, 148, 13), (130, 520, 44), (20, 80, 7), (96, 384, 32),
+                (200, 800, 67)]
+
+
+def test_split_group_matches_oracle_on_every_replay(oracle):
+    """Split-precision grouped builds (grouped FP16 hi+lo transforms, FP64 J and H_core):
+    every system within the split bound of the FP64 oracle (5e-6 of max|J|, max|K|),
+    tuned, on every replay, with new orbitals between replays."""
+    systems, inputs = [], []
+    for i, (nbf, naux, nbf5) in enumerate(SPLIT_SHAPES):
+        p = synthetic.synthetic_param(nbf, naux, nbf5)
+        b = synthetic.make_b(nbf, naux, 50 + i)
+        s = api.System(p, precision="split").upload_b(b)
+        assert s.b_symmetric
+        C = synthetic.make_C(nbf, 150 + i)
+        H = synthetic.make_H(nbf, 250 + i)
+        s.set_orbitals(C)
+        s.set_hcore(H)
+        systems.append(s)
+        inputs.append((b, C, H, nbf5))
+    g = api.Group(systems)
+    cfg = g.tune()
+    print("split configs", cfg, g.plan(), flush=True)
+    assert cfg["K1"].startswith("s") and cfg["K2"].startswith("s")
+    for rep in range(3):
+        if rep == 2:
+            for i, s in enumerate(systems):
+                C = synthetic.make_C(s.nbf, 350 + i)
+                s.set_orbitals(C)
+                inputs[i] = (inputs[i][0], C, inputs[i][2], inputs[i][3])
+        g.jkh()
+        g.sync()
+        for s, (b, C, H, nbf5) in zip(systems, inputs):
+            J, K, Hc = s.get_jkh()
+            Jr, Kr, Hr = oracle.jkh_mo(C, H, b, nbf5)
+            ej = np.abs(J - Jr).max() / np.abs(Jr).max()
+            ek = np.abs(K - Kr).max() / np.abs(Kr).max()
+            assert ej < 5e-6 and ek < 5e-6, (rep, s.nbf, ej, ek)
+            assert np.abs(Hc - Hr).max() <= TOL, (rep, s.nbf)
+    g.close()
+    for s in systems:
+        s.close()
+
+
+def test_split_group_needs_symmetric_build():
+    p = synthetic.synthetic_param(24, 96, 8)
+    s = api.System(p, precision="split").upload_b(synthetic.make_b(24, 96, 3))
+    s.set_symmetry(False)
+    with pytest.raises(DnError, match="symmetric-b"):
+        api.Group([s])
+    s.close()
+
+
 def test_group_requires_ready_systems():
     p = synthetic.synthetic_param(20, 80, 7)
     s = api.System(p)                       # b never uploaded
