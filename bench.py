@@ -258,14 +258,39 @@ def serving_leg(api, serve, n=24, rate=4.0, seed=7, max_live=8):
         srv = serve.Server(hbm_budget=0, max_live=max_live, mode=mode)
         out["hbm_budget_bytes"] = srv.ctx.hbm_budget
         t0 = time.perf_counter()
-        res = srv.run(reqs)
+        res = srv.run(reqs, log=lambda m: log("serving (%s): %s" % (mode, m)))
         wall = time.perf_counter() - t0
         srv.close()
         st = serve.latency_stats(res, wall)
         log("serving (%s): %d solves in %.1f s, p50 %.3f s, p99 %.3f s"
             % (mode, st["solves"], wall, st["p50_s"] or 0.0, st["p99_s"] or 0.0))
-        out[mode] = {"wall_s": wall, "errors": sum(1 for r in res if r.error), **st}
+        errs = [r.error for r in res if r.error]
+        out[mode] = {"wall_s": wall, "errors": len(errs), **st}
+        if errs:
+            out[mode]["first_error"] = errs[0]
+            log("serving (%s): %d errors, first: %s" % (mode, len(errs), errs[0]))
+        if mode == "grouped":
+            out[mode]["phase_s"] = dict(srv.phases)
+            log("serving (grouped) phases: %s" % ", ".join(
+                "%s %.2f s" % kv for kv in sorted(srv.phases.items())))
     return out
+
+
+def served_in_child(timeout_s):
+    """The served-solves leg in a child process (own HIP context) under a time limit, so
+    that a stalled serving run cannot hold up the bench line."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--served-only"]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=timeout_s,
+                           text=True)
+    except subprocess.TimeoutExpired:
+        log("serving: no result within %.0f s" % timeout_s)
+        return {"error": "served-solves leg did not finish within %.0f s" % timeout_s}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "served-solves leg exited with status %d" % r.returncode}
+    return json.loads(lines[-1])
 
 
 def main():
@@ -284,7 +309,17 @@ def main():
                     help="arithmetic of the headline build (fp64 is the metric's)")
     ap.add_argument("--legs", default="800,grouped,served",
                     help="extra legs to run: any of 800, grouped, served (or none)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="run only the served-solves leg and print its JSON (the main run "
+                         "starts this in a child process under a time limit)")
+    ap.add_argument("--served-timeout", type=float, default=300.0)
     args = ap.parse_args()
+    if args.served_only:
+        import __graft_entry__
+        __graft_entry__.load_package()
+        from donof_amd import api, serve
+        print(json.dumps(serving_leg(api, serve)))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X and this "
               "build has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -325,7 +360,7 @@ def main():
         "build_tflops": F / (ms_step * 1e-3) / 1e12,
         "pct_fp64_peak": 100.0 * F / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "roofline": {"bound": "mfma",
-                     "kernel": ("dgemm_mfma" if args.precision == "fp64" else "split_lds_kernel")
+                     "kernel": ("dgemm_lds" if args.precision == "fp64" else "split_lds_kernel")
                                + " symmetric-b first transform (lower half of b)",
                      "achieved": k_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": k_tflops / peak, "traffic": traffic,
@@ -365,8 +400,7 @@ def main():
         out["extra_grouped_64"] = grouped_leg(api, synthetic, max(3, args.steps // 4),
                                               args.warmup, args.seed)
     if "served" in legs:
-        from donof_amd import serve
-        out["extra_served"] = serving_leg(api, serve)
+        out["extra_served"] = served_in_child(args.served_timeout)
     print(json.dumps(out))
     return 0
 

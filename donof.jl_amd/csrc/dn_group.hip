@@ -112,6 +112,10 @@ __global__ __launch_bounds__(64) void group_hdiag_kernel(const DiagDesc* __restr
 }
 
 enum Stage { kK1 = 0, kK2 = 1, kJ = 2, kHC = 3, kNumStages = 4 };
+// Parallel branches of a build: the classes of a stage (and the per-system set-up kernels
+// of split groups) run on these streams side by side, joined before the next stage; the
+// device serves four hardware queues per process.
+constexpr int kAux = 4;
 
 }  // namespace
 }  // namespace dn
@@ -124,6 +128,9 @@ struct dn_group {
   int precision = DN_FP64;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kAux] = {};
   // Problems of a stage sorted into classes by operand layout and alignment (16-byte
   // loads allowed for A, for B) and by nbf5 (<= 32, <= 64, <= 128, larger): one
   // persistent launch per class, each with the tile shape the autotuner found fastest
@@ -143,6 +150,8 @@ struct dn_group {
   // K1, 1 = K2) over these descriptors instead of host[kK1], host[kK2]
   std::vector<GroupedSplit> shost[2];
   GroupedSplit* sdev[2] = {};
+  SplitPrepDesc* sprep = nullptr;  // per-system set-up of the split transforms
+  long prep_blocks = 0, q_blocks = 0;
   KxDesc* kx = nullptr;
   long kx_blocks[3] = {0, 0, 0};
   DiagDesc* diag = nullptr;
@@ -331,33 +340,53 @@ int set_config(dn_group* G, int stage, size_t li, int config) {
   return DN_OK;
 }
 
-int launch_one(dn_group* G, int stage, size_t li) {
+int launch_one(dn_group* G, int stage, size_t li, hipStream_t st = nullptr) {
   const auto& l = G->launches[stage][li];
+  if (!st) st = G->stream;
   if (split_stage(G, stage))
-    return split_grouped(G->stream, G->shost[stage].data() + l.first, G->sdev[stage] + l.first,
-                         l.count, l.tiles, l.config, stage == kK1 ? 0 : 1, l.grid);
-  return gemm_f64_grouped(G->stream, G->host[stage].data() + l.first, G->dev[stage] + l.first,
-                          l.count, l.tiles, l.config, l.grid);
+    return split_grouped(st, G->shost[stage].data() + l.first, G->sdev[stage] + l.first, l.count,
+                         l.tiles, l.config, stage == kK1 ? 0 : 1, l.grid);
+  return gemm_f64_grouped(st, G->host[stage].data() + l.first, G->dev[stage] + l.first, l.count,
+                          l.tiles, l.config, l.grid);
+}
+
+// The first k branch streams start after everything issued on the group stream so far;
+// join makes the group stream wait for all of them (also inside a graph capture, where
+// this records parallel branches).
+int fork(dn_group* G, int k) {
+  DN_HIP(hipEventRecord(G->fork_ev, G->stream));
+  for (int i = 0; i < k; ++i) DN_HIP(hipStreamWaitEvent(G->aux[i], G->fork_ev, 0));
+  return DN_OK;
+}
+
+int join(dn_group* G, int k) {
+  for (int i = 0; i < k; ++i) {
+    DN_HIP(hipEventRecord(G->join_ev[i], G->aux[i]));
+    DN_HIP(hipStreamWaitEvent(G->stream, G->join_ev[i], 0));
+  }
+  return DN_OK;
 }
 
 int launch_stage(dn_group* G, int stage) {
-  for (size_t li = 0; li < G->launches[stage].size(); ++li) DN_TRY(launch_one(G, stage, li));
-  return DN_OK;
+  const int L = (int)G->launches[stage].size();
+  if (L <= 1) {
+    for (int li = 0; li < L; ++li) DN_TRY(launch_one(G, stage, li));
+    return DN_OK;
+  }
+  const int k = std::min(L, kAux);
+  DN_TRY(fork(G, k));
+  for (int li = 0; li < L; ++li) DN_TRY(launch_one(G, stage, li, G->aux[li % k]));
+  return join(G, k);
 }
 
 int launch_build(dn_group* G) {
   const int n = (int)G->sys.size();
   const bool split = G->precision == DN_SPLIT_F16;
-  if (split)  // C5 planes for the current orbitals, cleared per-Q maxima
-    for (dn_sys* s : G->sys) {
-      DN_TRY(split_prepare_c(G->stream, s->C, s->d.nbf, s->d.nbf5, s->Mpad, s->Kpad, s->cscale,
-                             s->chi, s->clo));
-      DN_TRY(split_k1_begin(G->stream, s->qmax, s->d.nbfaux));
-    }
+  // split groups: C5 planes for the current orbitals and cleared per-Q maxima, then the
+  // per-Q scales of the first transform's output, each one launch for all systems
+  if (split) DN_TRY(split_group_prep(G->stream, G->sprep, n, G->prep_blocks));
   DN_TRY(launch_stage(G, kK1));
-  if (split)  // per-Q scales of the first transform's output
-    for (dn_sys* s : G->sys)
-      DN_TRY(split_k2_begin(G->stream, s->qmax, s->d.nbfaux, s->qs, s->qinv));
+  if (split) DN_TRY(split_group_scales(G->stream, G->sprep, n, G->q_blocks));
   DN_TRY(launch_stage(G, kK2));
   DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red, split));
   if (G->nkx) {
@@ -464,6 +493,11 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
                                         "build (a symmetric b with symmetry enabled)"));
   hipError_t e = hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&G->done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&G->fork_ev, hipEventDisableTiming);
+  for (int i = 0; i < kAux && e == hipSuccess; ++i) {
+    e = hipStreamCreateWithFlags(&G->aux[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&G->join_ev[i], hipEventDisableTiming);
+  }
   for (int st = 0; st < kNumStages && e == hipSuccess; ++st)
     e = hipMalloc(&G->dev[st], nsys * sizeof(GroupedGemm));
   if (e == hipSuccess) e = hipMalloc(&G->kx, nsys * sizeof(KxDesc));
@@ -471,6 +505,29 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
   if (e == hipSuccess) e = hipMalloc(&G->symk, nsys * sizeof(SymKDesc));
   for (int k = 0; k < 2 && e == hipSuccess && G->precision == DN_SPLIT_F16; ++k)
     e = hipMalloc(&G->sdev[k], nsys * sizeof(GroupedSplit));
+  if (e == hipSuccess && G->precision == DN_SPLIT_F16) {
+    std::vector<SplitPrepDesc> sp(nsys);
+    for (int i = 0; i < nsys; ++i) {
+      dn_sys* s = G->sys[i];
+      SplitPrepDesc& q = sp[i];
+      q.C = s->C;
+      q.nbf = s->d.nbf;
+      q.nbf5 = s->d.nbf5;
+      q.Mpad = s->Mpad;
+      q.Kpad = s->Kpad;
+      q.naux = s->d.nbfaux;
+      q.cscale = s->cscale;
+      q.chi = s->chi;
+      q.clo = s->clo;
+      q.qmax = s->qmax;
+      q.qs = s->qs;
+      q.qinv = s->qinv;
+    }
+    split_prep_plan(sp.data(), nsys, &G->prep_blocks, &G->q_blocks);
+    e = hipMalloc(&G->sprep, nsys * sizeof(SplitPrepDesc));
+    if (e == hipSuccess)
+      e = hipMemcpy(G->sprep, sp.data(), nsys * sizeof(SplitPrepDesc), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, std::string("dn_group_create: ") +
                                                           hipGetErrorString(e)));
   std::vector<KxDesc> kx;
@@ -546,9 +603,16 @@ int dn_group_destroy(dn_group* G) {
   if (G->symk) (void)hipFree(G->symk);
   for (int k = 0; k < 2; ++k)
     if (G->sdev[k]) (void)hipFree(G->sdev[k]);
+  if (G->sprep) (void)hipFree(G->sprep);
   for (auto e : G->ev_start) (void)hipEventDestroy(e);
   for (auto e : G->ev_stop) (void)hipEventDestroy(e);
   if (G->done) (void)hipEventDestroy(G->done);
+  for (int i = 0; i < kAux; ++i) {
+    if (G->aux[i]) (void)hipStreamSynchronize(G->aux[i]);
+    if (G->aux[i]) (void)hipStreamDestroy(G->aux[i]);
+    if (G->join_ev[i]) (void)hipEventDestroy(G->join_ev[i]);
+  }
+  if (G->fork_ev) (void)hipEventDestroy(G->fork_ev);
   if (G->stream) (void)hipStreamDestroy(G->stream);
   delete G;
   return DN_OK;

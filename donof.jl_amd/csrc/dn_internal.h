@@ -118,6 +118,23 @@ SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Kpad, const _Float16* c
                         float* bpq, const float* qs, const float* qinv);
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
                     float* cscale, _Float16* hi, _Float16* lo);
+// Per-system set-up of a grouped split build in three launches for all systems (the
+// arithmetic of split_prepare_c, split_k1_begin and split_k2_begin).  plane_begin and
+// q_begin are block offsets filled by split_prep_plan.
+struct SplitPrepDesc {
+  const double* C;
+  int nbf = 0, nbf5 = 0, Mpad = 0, Kpad = 0, naux = 0;
+  float* cscale;
+  _Float16* chi;
+  _Float16* clo;
+  float* qmax;
+  float* qs;
+  float* qinv;
+  long plane_begin = 0, q_begin = 0;
+};
+void split_prep_plan(SplitPrepDesc* desc, int n, long* plane_blocks, long* q_blocks);
+int split_group_prep(hipStream_t s, const SplitPrepDesc* dev, int n, long plane_blocks);
+int split_group_scales(hipStream_t s, const SplitPrepDesc* dev, int n, long q_blocks);
 int split_k1_begin(hipStream_t s, float* qmax, int naux);
 int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float* qinv);
 long split_grouped_plan(GroupedSplit* desc, int n, int config);

@@ -749,7 +749,96 @@ __global__ void bpn_scale_kernel(const float* __restrict__ qmax, int naux,
   inv[Q] = 1.0f / s;
 }
 
+// ---- grouped set-up (all systems of a group per launch) ----------------------------------
+
+// index of the descriptor whose block range holds block b (ranges ascending)
+template <bool Q>
+__device__ __forceinline__ int find_prep(const SplitPrepDesc* __restrict__ d, int n, long b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((Q ? d[mid].q_begin : d[mid].plane_begin) <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Workgroup i: max |C5| of system i (the elements c_max_kernel reads), the scale pair,
+// and the cleared per-Q maxima of the first transform.
+__global__ __launch_bounds__(256) void group_c_scale_kernel(const SplitPrepDesc* __restrict__ desc) {
+  const SplitPrepDesc d = desc[blockIdx.x];
+  __shared__ float red[4];
+  const long n = (long)d.nbf * d.nbf5;
+  float m = 0.f;
+  for (long e = threadIdx.x; e < n; e += blockDim.x) m = fmaxf(m, (float)fabs(d.C[e]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  for (int Q = threadIdx.x; Q < d.naux; Q += blockDim.x) d.qmax[Q] = 0.f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    const float sc = pow2_scale(mx * 1.0001f);
+    d.cscale[2] = mx;
+    d.cscale[0] = sc;
+    d.cscale[1] = 1.0f / sc;
+  }
+}
+
+// The C5 planes of every system (c_planes_kernel's element map), 256 elements per block.
+__global__ __launch_bounds__(256) void group_c_planes_kernel(const SplitPrepDesc* __restrict__ desc,
+                                                             int n) {
+  const int i = __builtin_amdgcn_readfirstlane(find_prep<false>(desc, n, blockIdx.x));
+  const SplitPrepDesc d = desc[i];
+  const long e = (blockIdx.x - d.plane_begin) * 256 + threadIdx.x;
+  if (e >= (long)d.Mpad * d.Kpad) return;
+  const float sc = d.cscale[0];
+  const int p = e / d.Kpad, k = e % d.Kpad;
+  _Float16 h = 0, l = 0;
+  if (p < d.nbf5 && k < d.nbf) split_f16_d(d.C[k + (long)p * d.nbf] * (double)sc, &h, &l);
+  d.chi[e] = h;
+  d.clo[e] = l;
+}
+
+// The per-Q scales of every system's first-transform output (bpn_scale_kernel's map).
+__global__ __launch_bounds__(256) void group_q_scale_kernel(const SplitPrepDesc* __restrict__ desc,
+                                                            int n) {
+  const int i = __builtin_amdgcn_readfirstlane(find_prep<true>(desc, n, blockIdx.x));
+  const SplitPrepDesc d = desc[i];
+  const int Q = (int)((blockIdx.x - d.q_begin) * 256 + threadIdx.x);
+  if (Q >= d.naux) return;
+  const float s = pow2_scale(d.qmax[Q] * 1.0001f);
+  d.qs[Q] = s;
+  d.qinv[Q] = 1.0f / s;
+}
+
 // ---- host side ---------------------------------------------------------------------------
+
+void split_prep_plan(SplitPrepDesc* desc, int n, long* plane_blocks, long* q_blocks) {
+  long pb = 0, qb = 0;
+  for (int i = 0; i < n; ++i) {
+    desc[i].plane_begin = pb;
+    desc[i].q_begin = qb;
+    pb += ((long)desc[i].Mpad * desc[i].Kpad + 255) / 256;
+    qb += (desc[i].naux + 255) / 256;
+  }
+  *plane_blocks = pb;
+  *q_blocks = qb;
+}
+
+int split_group_prep(hipStream_t s, const SplitPrepDesc* dev, int n, long plane_blocks) {
+  if (n <= 0) return DN_OK;
+  hipLaunchKernelGGL(group_c_scale_kernel, dim3(n), dim3(256), 0, s, dev);
+  hipLaunchKernelGGL(group_c_planes_kernel, dim3((unsigned)plane_blocks), dim3(256), 0, s, dev, n);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int split_group_scales(hipStream_t s, const SplitPrepDesc* dev, int n, long q_blocks) {
+  if (n <= 0) return DN_OK;
+  hipLaunchKernelGGL(group_q_scale_kernel, dim3((unsigned)q_blocks), dim3(256), 0, s, dev, n);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
 
 static unsigned grid_of(long n) {
   long b = (n + 255) / 256;

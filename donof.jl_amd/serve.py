@@ -7,11 +7,13 @@ request gets its own resident system (b generated in HBM, C, H and the occupatio
 once) and runs the ADAM loop of minimization.jl:344-438 (maxloop = max_iter, stop when
 2 max|g| < gtol).
 
-mode "grouped" (default): one engine advances all live solves in lock step
-(dn_ogroup_step): per step every solve takes its ADAM step and rotation, and the orbital
-Lagrangians of all of them run as grouped persistent GEMMs in one captured HIP graph.
-When a solve finishes it leaves, and waiting requests join at the next step (the group is
-rebuilt).  mode "streams": one host thread per live solve, each running the whole loop
+mode "grouped" (default): lock-step engines (one per nbf size class, each on its own
+thread and stream) advance their live solves together (dn_ogroup_step): per step every
+solve takes its ADAM step and rotation, and the orbital Lagrangians of all of them run as
+grouped persistent GEMMs in one captured HIP graph.  Admitted requests are set up by a
+preparer thread (on their own streams) while the engines keep stepping; when a solve
+finishes it leaves, and prepared requests join at the next step (the group is rebuilt).
+Size classes keep a small solve from advancing at the pace of a large one.  mode "streams": one host thread per live solve, each running the whole loop
 on its own HIP stream (dn_orbopt_resident).
 
 Every kernel of a solve is deterministic, reads only its own system's buffers, and the
@@ -111,17 +113,116 @@ def start_solve(req, ctx):
     return dev, p
 
 
+class _LockStepEngine(threading.Thread):
+    """Advances its live solves in lock step (dn_ogroup_step) on its own thread: solves
+    arriving in the inbox join at the next step boundary (the group is rebuilt), finished
+    ones leave and are reported through `finished(result, need)`."""
+
+    def __init__(self, ctx, t0, phases, lock, finished, log):
+        super().__init__(daemon=True)
+        self.ctx, self.t0, self.phases, self.lock = ctx, t0, phases, lock
+        self.finished, self.log = finished, log
+        self.inbox = queue.Queue()
+
+    def _flush(self, ph):
+        with self.lock:
+            for k, v in ph.items():
+                self.phases[k] += v
+
+    def run(self):
+        live = {}            # rid -> (req, dev, p, need, admitted)
+        group = None
+        stop = False
+        ph = {"regroup": 0.0, "step": 0.0, "finish": 0.0}
+        try:
+            while not stop or live:
+                changed = False
+                while True:   # newcomers join at this step boundary
+                    try:
+                        item = self.inbox.get(timeout=0.05) if not live and not stop \
+                            else self.inbox.get_nowait()
+                    except queue.Empty:
+                        break
+                    if item is None:
+                        stop = True
+                        continue
+                    live[item[0].rid] = item
+                    changed = True
+                if not live:
+                    continue
+                try:
+                    group = self._advance(live, group, changed, ph)
+                except Exception as e:  # noqa: BLE001 - reported per solve, engine goes on
+                    # the live solves fail with the error; later arrivals are served anew
+                    if group is not None:
+                        try:
+                            group.close()
+                        except Exception:  # noqa: BLE001
+                            pass
+                        group = None
+                    t = time.perf_counter() - self.t0
+                    for rid, (req, dev, p, need, admitted) in list(live.items()):
+                        try:
+                            dev.close()
+                        except Exception:  # noqa: BLE001
+                            pass
+                        self.finished(SolveResult(rid, req.nbf, float("nan"), 0, False,
+                                                  req.arrival, admitted, t,
+                                                  "%s: %s" % (type(e).__name__, e)), need)
+                    live.clear()
+        finally:
+            if group is not None:
+                group.close()
+            for v in live.values():
+                v[1].close()
+            self._flush(ph)
+
+    def _advance(self, live, group, changed, ph):
+        """One step of the live solves (the group rebuilt first when membership changed);
+        finished solves leave and are reported.  Returns the group to keep."""
+        if changed and group is not None:   # the systems leave the group's stream first
+            group.close()
+            group = None
+        if group is None:
+            tg = time.perf_counter()
+            group = api.OGroup([v[1] for v in live.values()], self.ctx)
+            ph["regroup"] += time.perf_counter() - tg
+        ts = time.perf_counter()
+        group.step()
+        ph["step"] += time.perf_counter() - ts
+        done = [rid for rid, v in live.items() if v[1].orbopt_done()]
+        if not done:
+            return group
+        tf = time.perf_counter()
+        group.close()
+        t = time.perf_counter() - self.t0
+        for rid in done:
+            req, dev, p, need, admitted = live[rid]
+            E, nit, ok = dev.orbopt_finish(p)
+            dev.close()
+            del live[rid]
+            if self.log:
+                self.log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"
+                         % (rid, req.nbf, E, nit, t - req.arrival))
+            self.finished(SolveResult(rid, req.nbf, E, nit, ok, req.arrival, admitted, t), need)
+        ph["finish"] += time.perf_counter() - tf
+        return None
+
+
 class Server:
     """Admission by HBM budget and slots; live solves advanced in lock step by grouped
     kernels (mode "grouped") or by one thread and stream each (mode "streams")."""
 
-    def __init__(self, hbm_budget=0, max_live=8, device=0, mode="grouped"):
+    def __init__(self, hbm_budget=0, max_live=8, device=0, mode="grouped",
+                 size_classes=(160, 320)):
         if mode not in ("grouped", "streams"):
             raise ValueError("mode must be 'grouped' or 'streams'")
+        self.size_classes = tuple(sorted(size_classes))   # nbf edges of the engines
         self.ctx = api.Context(device, hbm_budget)
         self.max_live = max_live
         self.mode = mode
         self.results = []
+        self.phases = {}      # seconds spent per phase of the last run (grouped mode)
         self._lock = threading.Lock()
         self._live = 0
         self._reserved = 0
@@ -139,70 +240,102 @@ class Server:
 
     def _run_grouped(self, requests, log=None):
         t0 = time.perf_counter()
-        pending = sorted(requests, key=lambda r: r.arrival)
-        waiting, results = [], []
-        live = {}            # rid -> (req, dev, p, need, admitted)
-        reserved = 0
-        group = None
-        changed = False
-        while pending or waiting or live:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.pop(0))
-            while waiting:   # FIFO admission while slots and memory allow
-                req = waiting[0]
-                need = api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5))
-                if need > self.ctx.hbm_budget:
-                    waiting.pop(0)
-                    results.append(SolveResult(req.rid, req.nbf, float("nan"), 0, False,
-                                               req.arrival, now, now,
-                                               "larger than the HBM budget"))
-                    continue
-                if len(live) >= self.max_live or reserved + need > self.ctx.hbm_budget:
-                    break
-                waiting.pop(0)
-                if group is not None:       # the systems leave the group's stream first
-                    group.close()
-                    group = None
+        ph = {"admit": 0.0, "regroup": 0.0, "step": 0.0, "finish": 0.0}
+        self.phases = ph
+        lock = threading.Lock()
+        results = []
+        state = {"reserved": 0, "slots": 0}   # admitted and not finished
+        done_ev = threading.Event()
+
+        def finished(res, need):
+            with lock:
+                results.append(res)
+                state["reserved"] -= need
+                state["slots"] -= 1
+            done_ev.set()
+
+        # one lock-step engine per size class, each on its own thread and stream: solves of
+        # similar cost share grouped kernels, and small ones do not wait for large ones
+        engines = [_LockStepEngine(self.ctx, t0, ph, lock, finished, log)
+                   for _ in range(len(self.size_classes) + 1)]
+        for e in engines:
+            e.start()
+
+        def engine_for(nbf):
+            k = 0
+            while k < len(self.size_classes) and nbf >= self.size_classes[k]:
+                k += 1
+            return engines[k]
+
+        # admitted requests are set up (system created, b generated, run started) by a
+        # preparer thread on their own streams while the engines keep stepping
+        prep_q = queue.Queue()
+
+        def preparer():
+            while True:
+                item = prep_q.get()
+                if item is None:
+                    return
+                req, need, admitted = item
+                ta = time.perf_counter()
                 try:
                     dev, p = start_solve(req, self.ctx)
-                except DnError as e:
-                    results.append(SolveResult(req.rid, req.nbf, float("nan"), 0, False,
-                                               req.arrival, now, time.perf_counter() - t0,
-                                               str(e)))
+                except Exception as e:  # noqa: BLE001 - reported as the request's error
+                    finished(SolveResult(req.rid, req.nbf, float("nan"), 0, False, req.arrival,
+                                         admitted, time.perf_counter() - t0,
+                                         "%s: %s" % (type(e).__name__, e)), need)
                     continue
-                live[req.rid] = (req, dev, p, need, now)
-                reserved += need
-                changed = True
-                if log:
-                    log("admit %d nbf=%d at %.3fs" % (req.rid, req.nbf, now))
-            if not live:
-                if pending and not waiting:
-                    time.sleep(min(0.002, max(0.0, pending[0].arrival - now)))
-                continue
-            if changed or group is None:
-                if group is not None:
-                    group.close()
-                group = api.OGroup([v[1] for v in live.values()], self.ctx)
-                changed = False
-            group.step()
-            finished = [rid for rid, v in live.items() if v[1].orbopt_done()]
-            if finished:
-                group.close()
-                group = None
-                t = time.perf_counter() - t0
-                for rid in finished:
-                    req, dev, p, need, admitted = live.pop(rid)
-                    E, nit, ok = dev.orbopt_finish(p)
-                    dev.close()
-                    reserved -= need
-                    results.append(SolveResult(rid, req.nbf, E, nit, ok, req.arrival, admitted,
-                                               t))
+                finally:
+                    with lock:
+                        ph["admit"] += time.perf_counter() - ta
+                engine_for(req.nbf).inbox.put((req, dev, p, need, admitted))
+
+        worker = threading.Thread(target=preparer, daemon=True)
+        worker.start()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting = []
+        try:
+            while True:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    waiting.append(pending.pop(0))
+                while waiting:   # FIFO admission while slots and memory allow
+                    req = waiting[0]
+                    need = api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5))
+                    if need > self.ctx.hbm_budget:
+                        waiting.pop(0)
+                        with lock:
+                            results.append(SolveResult(req.rid, req.nbf, float("nan"), 0,
+                                                       False, req.arrival, now, now,
+                                                       "larger than the HBM budget"))
+                        continue
+                    with lock:
+                        ok = (state["slots"] < self.max_live
+                              and state["reserved"] + need <= self.ctx.hbm_budget)
+                        if ok:
+                            state["slots"] += 1
+                            state["reserved"] += need
+                    if not ok:
+                        break
+                    waiting.pop(0)
+                    prep_q.put((req, need, now))
                     if log:
-                        log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"
-                            % (rid, req.nbf, E, nit, t - req.arrival))
-        if group is not None:
-            group.close()
+                        log("admit %d nbf=%d at %.3fs" % (req.rid, req.nbf, now))
+                with lock:
+                    if len(results) == len(requests):
+                        break
+                if not worker.is_alive() or not all(e.is_alive() for e in engines):
+                    raise RuntimeError("a serving thread stopped with requests outstanding")
+                done_ev.wait(min(0.002, max(0.0, pending[0].arrival - now)) if pending
+                             else 0.002)
+                done_ev.clear()
+        finally:
+            prep_q.put(None)
+            worker.join()
+            for e in engines:
+                e.inbox.put(None)
+            for e in engines:
+                e.join()
         self.results = sorted(results, key=lambda r: r.rid)
         return self.results
 

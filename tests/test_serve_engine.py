@@ -1,0 +1,110 @@
+"""Host logic of the grouped serving engines (CPU): admission, size-class routing, join
+and leave at step boundaries, completion of every request, and error isolation (a step
+that fails reports its live solves and the engine goes on with later ones).  The device
+calls are replaced by fakes with the same interface."""
+import threading
+
+import pytest
+
+from donof_amd import serve
+from donof_amd._lib import DnError
+
+
+class FakeDev:
+    def __init__(self, req):
+        self.req = req
+        self.steps = 0
+        self.closed = False
+
+    def orbopt_done(self):
+        return self.steps >= self.req.max_iter
+
+    def orbopt_finish(self, p):
+        return -1.0 * self.req.nbf, self.steps, False
+
+    def close(self):
+        self.closed = True
+
+
+class FakeGroup:
+    fail_rids = set()
+    lock = threading.Lock()
+    sizes = []
+
+    def __init__(self, systems, ctx):
+        self.systems = list(systems)
+        with FakeGroup.lock:
+            FakeGroup.sizes.append(sorted(s.req.nbf for s in self.systems))
+
+    def step(self):
+        for s in self.systems:
+            if s.req.rid in FakeGroup.fail_rids:
+                raise DnError(6, "non-finite energy or orbital gradient")
+            s.steps += 1
+        return len(self.systems)
+
+    def close(self):
+        pass
+
+
+class FakeCtx:
+    hbm_budget = 10 ** 12
+
+
+@pytest.fixture
+def fake_device(monkeypatch):
+    monkeypatch.setattr(serve, "start_solve", lambda req, ctx: (FakeDev(req), object()))
+    monkeypatch.setattr(serve.api, "OGroup", FakeGroup)
+    monkeypatch.setattr(serve.api, "footprint", lambda p: 10 ** 9)
+    FakeGroup.fail_rids = set()
+    FakeGroup.sizes = []
+
+
+def _server(max_live=4, classes=(160, 320)):
+    srv = serve.Server.__new__(serve.Server)
+    srv.ctx = FakeCtx()
+    srv.max_live = max_live
+    srv.mode = "grouped"
+    srv.size_classes = tuple(classes)
+    srv.results = []
+    srv.phases = {}
+    srv._lock = threading.Lock()
+    srv._live = 0
+    srv._reserved = 0
+    return srv
+
+
+def _requests(n, iters=5):
+    sizes = [60, 500, 200, 80, 350, 150, 600, 100, 250, 90]
+    return [serve.SolveRequest(rid=i, nbf=sizes[i % len(sizes)], seed=i, arrival=0.001 * i,
+                               max_iter=iters) for i in range(n)]
+
+
+def test_every_request_completes_in_size_classes(fake_device):
+    srv = _server()
+    res = srv.run(_requests(10))
+    assert [r.rid for r in res] == list(range(10))
+    assert all(not r.error and r.nit == 5 for r in res)
+    # no group mixes size classes (< 160, [160, 320), >= 320)
+    cls = lambda n: 0 if n < 160 else (1 if n < 320 else 2)
+    assert all(len({cls(n) for n in g}) == 1 for g in FakeGroup.sizes)
+    assert set(srv.phases) == {"admit", "regroup", "step", "finish"}
+
+
+def test_failed_step_is_reported_and_engine_goes_on(fake_device):
+    FakeGroup.fail_rids = {1}          # nbf=500: its engine's step fails while it is live
+    srv = _server(max_live=2)
+    res = srv.run(_requests(10))
+    assert len(res) == 10
+    assert "non-finite" in res[1].error
+    # the same engine (nbf >= 320) serves later requests: 4 (350) and 6 (600)
+    assert not res[4].error and not res[6].error
+    assert sum(1 for r in res if not r.error) >= 8
+
+
+def test_admission_respects_slots(fake_device):
+    srv = _server(max_live=1)
+    res = srv.run(_requests(4, iters=3))
+    assert all(not r.error for r in res)
+    # one slot: every group holds exactly one solve
+    assert all(len(g) == 1 for g in FakeGroup.sizes)
