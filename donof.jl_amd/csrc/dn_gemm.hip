@@ -391,16 +391,24 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
           af[i][p].y = kl + 2 * p + 1 < K ? af[i][p].y : 0.0;
         }
     }
+    // symmetric-A mode: a fragment row whose first row lies past this k tile's last k is
+    // all zero after the mask; its MFMAs are skipped (a wavefront-uniform branch)
+    bool live_row[FM];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+      live_row[i] = TRI != kTriA || k0 + 15 >= m0 + wm * WM + 16 * i;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i) {
+        if (!live_row[i]) continue;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
           const double a = (s & 1) ? af[i][s >> 1].y : af[i][s >> 1].x;
           const double b = (s & 1) ? bf[j][s >> 1].y : bf[j][s >> 1].x;
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc[i][j], 0, 0, 0);
         }
+      }
   }
 
   // same accumulator layout and stores as gemm_tile

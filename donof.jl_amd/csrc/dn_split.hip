@@ -540,21 +540,30 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
       ah[i] = *reinterpret_cast<const v8h*>(A + o);
       al[i] = *reinterpret_cast<const v8h*>(A + BM * 64 + o);
     }
+    // TRIB: a fragment column whose first column lies past this k tile's last k is all
+    // zero after the mask; its MFMAs are skipped (a wavefront-uniform branch)
+    bool live_col[FN];
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+      live_col[j] = !TRIB || (kt0 + t) * SB_K + SB_K - 1 >= n0 + wn * WN + 16 * j;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        if (live_col[j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        if (live_col[j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        if (live_col[j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
   }
   split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
 }

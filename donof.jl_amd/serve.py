@@ -81,6 +81,12 @@ def synthetic_requests(n, rate, seed=0, nbf_min=50, nbf_max=600, max_iter=200, g
                          max_iter=max_iter, gtol=gtol) for i in range(n)]
 
 
+# Structural device operations of the grouped engines -- creating and destroying systems
+# (allocation, synchronous copies, frees) and lock-step groups (graph capture) -- are kept
+# apart from each other across threads; steps run concurrently.
+_STRUCT_LOCK = threading.Lock()
+
+
 def solve_one(req, ctx):
     """Run one request on its own resident system; returns (E, nit, success)."""
     sysd = synthetic.SyntheticSystem(req.nbf, req.naux, req.nbf5, req.seed)
@@ -100,7 +106,8 @@ def start_solve(req, ctx):
     sysd = synthetic.SyntheticSystem(req.nbf, req.naux, req.nbf5, req.seed)
     p = sysd.p
     p.maxloop, p.alpha, p.threshgorb = req.max_iter, req.alpha, req.gtol
-    dev = api.System(p, ctx=ctx)
+    with _STRUCT_LOCK:
+        dev = api.System(p, ctx=ctx)
     try:
         dev.generate_b(req.seed)
         dev.set_orbitals(sysd.C)
@@ -108,7 +115,8 @@ def start_solve(req, ctx):
         dev.set_gamma(sysd.gamma)
         dev.orbopt_start("ADAM", p)
     except Exception:
-        dev.close()
+        with _STRUCT_LOCK:
+            dev.close()
         raise
     return dev, p
 
@@ -154,16 +162,18 @@ class _LockStepEngine(threading.Thread):
                     group = self._advance(live, group, changed, ph)
                 except Exception as e:  # noqa: BLE001 - reported per solve, engine goes on
                     # the live solves fail with the error; later arrivals are served anew
-                    if group is not None:
-                        try:
-                            group.close()
-                        except Exception:  # noqa: BLE001
-                            pass
-                        group = None
+                    with _STRUCT_LOCK:
+                        if group is not None:
+                            try:
+                                group.close()
+                            except Exception:  # noqa: BLE001
+                                pass
+                            group = None
                     t = time.perf_counter() - self.t0
                     for rid, (req, dev, p, need, admitted) in list(live.items()):
                         try:
-                            dev.close()
+                            with _STRUCT_LOCK:
+                                dev.close()
                         except Exception:  # noqa: BLE001
                             pass
                         self.finished(SolveResult(rid, req.nbf, float("nan"), 0, False,
@@ -171,21 +181,24 @@ class _LockStepEngine(threading.Thread):
                                                   "%s: %s" % (type(e).__name__, e)), need)
                     live.clear()
         finally:
-            if group is not None:
-                group.close()
-            for v in live.values():
-                v[1].close()
+            with _STRUCT_LOCK:
+                if group is not None:
+                    group.close()
+                for v in live.values():
+                    v[1].close()
             self._flush(ph)
 
     def _advance(self, live, group, changed, ph):
         """One step of the live solves (the group rebuilt first when membership changed);
         finished solves leave and are reported.  Returns the group to keep."""
         if changed and group is not None:   # the systems leave the group's stream first
-            group.close()
+            with _STRUCT_LOCK:
+                group.close()
             group = None
         if group is None:
             tg = time.perf_counter()
-            group = api.OGroup([v[1] for v in live.values()], self.ctx)
+            with _STRUCT_LOCK:
+                group = api.OGroup([v[1] for v in live.values()], self.ctx)
             ph["regroup"] += time.perf_counter() - tg
         ts = time.perf_counter()
         group.step()
@@ -194,12 +207,14 @@ class _LockStepEngine(threading.Thread):
         if not done:
             return group
         tf = time.perf_counter()
-        group.close()
+        with _STRUCT_LOCK:
+            group.close()
         t = time.perf_counter() - self.t0
         for rid in done:
             req, dev, p, need, admitted = live[rid]
             E, nit, ok = dev.orbopt_finish(p)
-            dev.close()
+            with _STRUCT_LOCK:
+                dev.close()
             del live[rid]
             if self.log:
                 self.log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"

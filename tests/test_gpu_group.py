@@ -128,6 +128,41 @@ def test_split_group_needs_symmetric_build():
     s.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_chained_group_matches_per_system_builds(precision):
+    """Groups with enough work run as chains of leaf groups (parallel graph branches):
+    every system's J, K, H_core equal its own single-system build (to 1e-12 relative in
+    FP64, 1e-6 in split precision, where the two paths use different tile shapes)."""
+    shapes = [(300 + 6 * i, 4 * (300 + 6 * i), 100 + 2 * i) for i in range(6)]
+    systems, ref = [], []
+    for i, (nbf, naux, nbf5) in enumerate(shapes):
+        s = api.System(synthetic.synthetic_param(nbf, naux, nbf5), precision=precision)
+        s.generate_b(900 + i)
+        s.set_orbitals(synthetic.make_C(nbf, 910 + i))
+        s.set_hcore(synthetic.make_H(nbf, 920 + i))
+        s.jkh_mo_resident()
+        s.sync()
+        ref.append(s.get_jkh())
+        systems.append(s)
+    g = api.Group(systems)
+    g.tune()
+    plan = g.plan()
+    print(plan[:300], flush=True)
+    assert plan.startswith("chain 0")
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for rep in range(2):
+        g.jkh()
+        g.sync()
+        for s, (J, K, Hc) in zip(systems, ref):
+            Jg, Kg, Hg = s.get_jkh()
+            assert np.abs(Jg - J).max() <= tol * np.abs(J).max(), (rep, s.nbf)
+            assert np.abs(Kg - K).max() <= tol * np.abs(K).max(), (rep, s.nbf)
+            assert np.abs(Hg - Hc).max() <= 1e-12 * max(1.0, np.abs(Hc).max())
+    g.close()
+    for s in systems:
+        s.close()
+
+
 def test_group_requires_ready_systems():
     p = synthetic.synthetic_param(20, 80, 7)
     s = api.System(p)                       # b never uploaded

@@ -43,7 +43,7 @@ def test_alone_equals_in_the_mix(mode):
     res = _serve(reqs, mode, max_live=4)
     assert len(res) == len(reqs)
     for r in res:
-        assert r.error == ""
+        assert r.error == "", (r.rid, r.error)
         assert (r.E, r.nit, r.success) == alone[r.rid]       # bitwise the same
     st = serve.latency_stats(res, max(r.done for r in res))
     assert st["solves"] == len(reqs) and st["p99_s"] >= st["p50_s"] > 0
@@ -55,7 +55,7 @@ def test_admission_by_hbm_budget(mode):
     fp = {r.rid: api.footprint(synthetic.synthetic_param(r.nbf, r.naux, r.nbf5)) for r in reqs}
     budget = int(1.2 * max(fp.values()))
     res = _serve(reqs, mode, hbm_budget=budget, max_live=8)
-    assert all(r.error == "" for r in res)
+    assert all(r.error == "" for r in res), [(r.rid, r.error) for r in res if r.error]
     # at every admission, the footprints of the solves then resident fit in the budget
     for r in res:
         live = sum(fp[o.rid] for o in res if o.admitted <= r.admitted < o.done)
