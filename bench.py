@@ -169,14 +169,14 @@ def extra_legs(api, synthetic, steps, warmup, seed):
     return out
 
 
-def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
+def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64, nbf_range=(48, 400)):
     """64 systems, nbf ~ U[48, 400], naux = 4 nbf, nbf5 = ceil(nbf/3): one grouped build
     of all of them per step (graph replay), against per-system builds on their own
     streams; in FP64 and in split precision (the same tensors and orbitals, so the split
     results are compared with the FP64 ones)."""
-    shapes = synthetic.grouped_shapes(nsys, seed)
+    shapes = synthetic.grouped_shapes(nsys, seed, *nbf_range)
     F = sum(synthetic.build_flops(*sh) for sh in shapes)
-    out = {"config": {"nsys": nsys, "nbf_range": [48, 400], "seed": seed,
+    out = {"config": {"nsys": nsys, "nbf_range": list(nbf_range), "seed": seed,
                       "nbf": [sh[0] for sh in shapes],
                       "tflop_per_build_of_all": F / 1e12,
                       "b_gb": sum(8.0 * n * n * a for n, a, _ in shapes) / 1e9}}
@@ -189,8 +189,8 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64):
             s.set_orbitals(synthetic.make_C(nbf, seed + 1000 + i))
             s.set_hcore(synthetic.make_H(nbf, seed + 2000 + i))
             systems.append(s)
-        log("grouped %s: %d systems resident, %.2f TFLOP per build of all"
-            % (prec, nsys, F / 1e12))
+        log("grouped %s: %d systems (nbf %d-%d) resident, %.3f TFLOP per build of all"
+            % (prec, nsys, nbf_range[0], nbf_range[1], F / 1e12))
         for _ in range(warmup):
             for s in systems:
                 s.jkh_mo_resident()
@@ -399,6 +399,11 @@ def main():
     if "grouped" in legs:
         out["extra_grouped_64"] = grouped_leg(api, synthetic, max(3, args.steps // 4),
                                               args.warmup, args.seed)
+        # beside the configuration: many small systems, where per-system launches are
+        # latency-bound and grouping pays most
+        out["extra_grouped_64_small"] = grouped_leg(api, synthetic, max(5, args.steps // 2),
+                                                    args.warmup, args.seed,
+                                                    nbf_range=(16, 64))
     if "served" in legs:
         out["extra_served"] = served_in_child(args.served_timeout)
     print(json.dumps(out))

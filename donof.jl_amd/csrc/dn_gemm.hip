@@ -715,8 +715,8 @@ bool gemm_config_ok(int config, const Gemm& g) {
          vec_ok(g.A, g.lda, g.sA, g.nbatch, g.K) && vec_ok(g.B, g.ldb, g.sB, g.nbatch, g.K);
 }
 
-int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {
-  if (g.nbatch != 1) return gemm_f64(s, g);
+int splitk_count(const Gemm& g, size_t part_elems) {
+  if (g.nbatch != 1 || g.beta != 0.0) return 1;
   const long tiles = (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
   // largest split count (<= 16, dividing K, chunks >= 32) that keeps ~1024 workgroups
   int best = 1;
@@ -726,20 +726,29 @@ int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elem
     if (tiles * d > 2048) break;
     best = d;
   }
-  if (best == 1 || g.beta != 0.0) return gemm_f64(s, g);
-  const int kc = g.K / best;
+  return best;
+}
+
+Gemm splitk_partials(const Gemm& g, int nsplit, double* part) {
+  const int kc = g.K / nsplit;
   Gemm h = g;
   h.K = kc;
   h.sA = (long)kc * (g.a_kcontig ? 1 : g.lda);
   h.sB = (long)kc * (g.b_kcontig ? 1 : g.ldb);
-  h.nbatch = best;
+  h.nbatch = nsplit;
   h.bps = 1;
   h.C = part;
   h.ldc = g.M;
   h.sC = (long)g.M * g.N;
   h.alpha = 1.0;
   h.beta = 0.0;
-  DN_TRY(gemm_f64(s, h));
+  return h;
+}
+
+int gemm_f64_splitk(hipStream_t s, const Gemm& g, double* part, size_t part_elems) {
+  const int best = splitk_count(g, part_elems);
+  if (best == 1) return gemm_f64(s, g);
+  DN_TRY(gemm_f64(s, splitk_partials(g, best, part)));
   return reduce_splits(s, part, (long)g.M * g.N, best, g.M, g.N, g.M, g.C, g.ldc, 0.0, g.alpha);
 }
 

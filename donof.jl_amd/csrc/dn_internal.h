@@ -142,6 +142,11 @@ int split_grouped(hipStream_t s, const GroupedSplit* desc_host, const GroupedSpl
                   int n, long total, int config, int kind, int grid);
 
 int gemm_f64(hipStream_t s, const Gemm& g);
+// Split-K form of gemm_f64_splitk: the split count it uses for g with part_elems
+// doubles of partials (1 = no split), and the batched GEMM that writes the partials
+// (reduced in split order by reduce_splits, times g.alpha).
+int splitk_count(const Gemm& g, size_t part_elems);
+Gemm splitk_partials(const Gemm& g, int nsplit, double* part);
 
 // One problem of a grouped GEMM launch (tile list offsets filled by gemm_grouped_plan).
 struct GroupedGemm {
@@ -218,6 +223,37 @@ bool all_finite(const double* x, size_t n);
 int prepare_b(dn_sys* s);
 int begin_timing(dn_sys* s);  // HIP events around the dominant kernel
 int end_timing(dn_sys* s);
+}  // namespace dn
+
+namespace dn {
+// One system of a lock-step group for the grouped Lagrangian kernels (dn_lag.hip): the
+// operands of lag_w, lag_bmat, lag_init, the reduction of the C^J/C^K partials into elag,
+// energy_elec and pack_grad, and block offsets filled by lag_group_plan.
+struct LagSys {
+  const double* bt;
+  long slice;
+  int nbf, nbf5, naux, nalpha, nbeta;
+  const double* cj12;
+  const double* ck12;
+  const double* n;
+  double* X;
+  double* w;
+  double* bmat;
+  const double* Hmat;
+  double* elag;
+  const double* part;  // nsplit partials of nbf x nbf5
+  int nsplit;
+  double* scal;        // scal[0] = E, scal[1] = bits of max|g|
+  double* g;           // packed gradient
+  long w_begin, bmat_begin, sq_begin, rect_begin;
+};
+struct LagBlocks {
+  long w = 0, bmat = 0, sq = 0, rect = 0;
+  size_t lds_w = 0;
+};
+void lag_group_plan(LagSys* d, int n, LagBlocks* b);
+int lag_group_before(hipStream_t s, const LagSys* d, int n, const LagBlocks& b);
+int lag_group_after(hipStream_t s, const LagSys* d, int n, const LagBlocks& b);
 }  // namespace dn
 
 struct dn_ctx {

@@ -10,17 +10,18 @@
 // or b >= nalpha).  The first line is the C^K term (54-55), w carries C^J (50-51) and the
 // Coulomb terms; the one-electron term n[b] Hmat[a,b] (39) is added afterwards.
 #include "dn_internal.h"
+#include "dn_rot_dev.h"
 
 namespace dn {
 
-// w[b,Q] for all b < nbf5; one block per Q chunk, X read from bt's diagonal.
-__global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf5, int naux,
-                             const double* __restrict__ cj12, const double* __restrict__ n,
-                             int nalpha, int nbeta, double* __restrict__ X,
-                             double* __restrict__ w) {
-  extern __shared__ double xs[];  // nbf5 diagonal entries of slice Q
-  const int Q = blockIdx.x;
-  if (Q >= naux) return;
+// ---- per-block / per-element bodies (single-system and grouped kernels alike) -------------
+
+// X[:,Q] and w[:,Q] for all b < nbf5 (one workgroup per Q); xs: nbf5 doubles of LDS.
+__device__ __forceinline__ void lag_w_block(int Q, const double* __restrict__ bt, long slice,
+                                            int nbf5, const double* __restrict__ cj12,
+                                            const double* __restrict__ n, int nalpha, int nbeta,
+                                            double* __restrict__ X, double* __restrict__ w,
+                                            double* xs) {
   const double* s = bt + (long)Q * slice;
   for (int q = threadIdx.x; q < nbf5; q += blockDim.x) {
     const double v = s[(long)q * (nbf5 + 1)];
@@ -38,17 +39,17 @@ __global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf5
 }
 
 // bmat[q,b,Q] = -ck12[b,q] bt[q,b,Q] + [q==b] w[b,Q]
-// Block (tq, tb, zq): the 32 x 32 (q, b) tile at (32 tq, 32 tb) for Q in
+// Workgroup (tq, tb, zq): the 32 x 32 (q, b) tile at (32 tq, 32 tb) for Q in
 // [BMAT_Q zq, BMAT_Q (zq + 1)).  The ck12 tile is read once, along b, and kept transposed
-// in LDS, so that bt and bmat are read and written along q (coalesced).
+// in LDS (ct), so that bt and bmat are read and written along q (coalesced).
 constexpr int BMAT_Q = 16;
-__global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict__ bt, long slice,
-                                                       int nbf5, int naux,
-                                                       const double* __restrict__ ck12,
-                                                       const double* __restrict__ w,
-                                                       double* __restrict__ bmat) {
-  __shared__ double ct[32][33];  // ct[q][b] = ck12[b, q]
-  const int q0 = 32 * blockIdx.x, b0 = 32 * blockIdx.y;
+__device__ __forceinline__ void lag_bmat_block(int tq, int tb, int zq,
+                                               const double* __restrict__ bt, long slice,
+                                               int nbf5, int naux,
+                                               const double* __restrict__ ck12,
+                                               const double* __restrict__ w,
+                                               double* __restrict__ bmat, double (*ct)[33]) {
+  const int q0 = 32 * tq, b0 = 32 * tb;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int i = ty; i < 32; i += 8) {  // row q0 + i of the tile: ck12[b0 + tx, q0 + i]
     const int q = q0 + i, b = b0 + tx;
@@ -58,8 +59,8 @@ __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict_
   const long nn = (long)nbf5 * nbf5;
   const int q = q0 + tx;
   if (q >= nbf5) return;
-  const int Q1 = min(naux, BMAT_Q * ((int)blockIdx.z + 1));
-  for (int Q = BMAT_Q * blockIdx.z; Q < Q1; ++Q) {
+  const int Q1 = min(naux, BMAT_Q * (zq + 1));
+  for (int Q = BMAT_Q * zq; Q < Q1; ++Q) {
     const double* s = bt + (long)Q * slice;
     double* o = bmat + (long)Q * nn;
     for (int i = ty; i < 32; i += 8) {
@@ -73,21 +74,19 @@ __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict_
 }
 
 // elag[a,b] = n[b] Hmat[a,b] for b < nbf5, 0 for b >= nbf5 (utils.jl:22,39)
-__global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
-                                int nbf, int nbf5, double* __restrict__ elag) {
-  const long total = (long)nbf * nbf;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int a = e % nbf, b = e / nbf;
-    elag[e] = (b < nbf5) ? n[b] * Hmat[a + (long)b * nbf] : 0.0;
-  }
+__device__ __forceinline__ double lag_init_element(long e, const double* __restrict__ Hmat,
+                                                   const double* __restrict__ n, int nbf,
+                                                   int nbf5) {
+  const int a = e % nbf, b = e / nbf;
+  return (b < nbf5) ? n[b] * Hmat[a + (long)b * nbf] : 0.0;
 }
 
-// E = sum_{i<nbf5} elag_ii + sum_i n_i Hmat_ii  (utils.jl:216-227); single block.
-__global__ void energy_kernel(const double* __restrict__ elag, const double* __restrict__ Hmat,
-                              const double* __restrict__ n, int nbf, int nbf5,
-                              double* __restrict__ out) {
-  __shared__ double red[256];
+// E = sum_{i<nbf5} elag_ii + sum_i n_i Hmat_ii (utils.jl:216-227) by one 256-thread
+// workgroup; red: 256 doubles of LDS.
+__device__ __forceinline__ void energy_block(const double* __restrict__ elag,
+                                             const double* __restrict__ Hmat,
+                                             const double* __restrict__ n, int nbf, int nbf5,
+                                             double* __restrict__ out, double* red) {
   double s = 0.0;
   for (int i = threadIdx.x; i < nbf5; i += blockDim.x)
     s += elag[i + (long)i * nbf] + n[i] * Hmat[i + (long)i * nbf];
@@ -98,6 +97,121 @@ __global__ void energy_kernel(const double* __restrict__ elag, const double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---- single-system kernels --------------------------------------------------------------
+
+__global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf5, int naux,
+                             const double* __restrict__ cj12, const double* __restrict__ n,
+                             int nalpha, int nbeta, double* __restrict__ X,
+                             double* __restrict__ w) {
+  extern __shared__ double xs[];  // nbf5 diagonal entries of slice Q
+  const int Q = blockIdx.x;
+  if (Q >= naux) return;
+  lag_w_block(Q, bt, slice, nbf5, cj12, n, nalpha, nbeta, X, w, xs);
+}
+
+__global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict__ bt, long slice,
+                                                       int nbf5, int naux,
+                                                       const double* __restrict__ ck12,
+                                                       const double* __restrict__ w,
+                                                       double* __restrict__ bmat) {
+  __shared__ double ct[32][33];  // ct[q][b] = ck12[b, q]
+  lag_bmat_block(blockIdx.x, blockIdx.y, blockIdx.z, bt, slice, nbf5, naux, ck12, w, bmat, ct);
+}
+
+__global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
+                                int nbf, int nbf5, double* __restrict__ elag) {
+  const long total = (long)nbf * nbf;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x)
+    elag[e] = lag_init_element(e, Hmat, n, nbf, nbf5);
+}
+
+__global__ void energy_kernel(const double* __restrict__ elag, const double* __restrict__ Hmat,
+                              const double* __restrict__ n, int nbf, int nbf5,
+                              double* __restrict__ out) {
+  __shared__ double red[256];
+  energy_block(elag, Hmat, n, nbf, nbf5, out, red);
+}
+
+// ---- grouped kernels: every system of a lock-step group in one launch ------------------
+
+__device__ __forceinline__ int find_lag(const LagSys* __restrict__ d, int n, long b,
+                                        long LagSys::*begin) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].*begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ __launch_bounds__(128) void group_lag_w_kernel(const LagSys* __restrict__ d, int n) {
+  extern __shared__ double xs[];
+  const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::w_begin)];
+  lag_w_block((int)(blockIdx.x - r.w_begin), r.bt, r.slice, r.nbf5, r.cj12, r.n, r.nalpha,
+              r.nbeta, r.X, r.w, xs);
+}
+
+__global__ __launch_bounds__(256) void group_lag_bmat_kernel(const LagSys* __restrict__ d, int n) {
+  __shared__ double ct[32][33];
+  const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::bmat_begin)];
+  const int t = (r.nbf5 + 31) / 32;
+  const long local = blockIdx.x - r.bmat_begin;
+  const int zq = (int)(local / ((long)t * t)), rem = (int)(local % ((long)t * t));
+  lag_bmat_block(rem % t, rem / t, zq, r.bt, r.slice, r.nbf5, r.naux, r.ck12, r.w, r.bmat, ct);
+}
+
+__global__ __launch_bounds__(256) void group_lag_init_kernel(const LagSys* __restrict__ d, int n) {
+  const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::sq_begin)];
+  const long e = (blockIdx.x - r.sq_begin) * 256 + threadIdx.x;
+  if (e < (long)r.nbf * r.nbf) r.elag[e] = lag_init_element(e, r.Hmat, r.n, r.nbf, r.nbf5);
+}
+
+// elag += the split partials of the C^J/C^K contraction, summed in split order (the
+// arithmetic of reduce_splits with beta = scale = 1)
+__global__ __launch_bounds__(256) void group_lag_reduce_kernel(const LagSys* __restrict__ d,
+                                                               int n) {
+  const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::rect_begin)];
+  const long e = (blockIdx.x - r.rect_begin) * 256 + threadIdx.x;
+  const long sP = (long)r.nbf * r.nbf5;
+  if (e >= sP) return;
+  double s = 0.0;
+  for (int z = 0; z < r.nsplit; ++z) s += r.part[z * sP + e];
+  s *= 1.0;
+  r.elag[e] = s + 1.0 * r.elag[e];
+}
+
+__global__ __launch_bounds__(256) void group_energy_kernel(const LagSys* __restrict__ d) {
+  __shared__ double red[256];
+  const LagSys& r = d[blockIdx.x];
+  energy_block(r.elag, r.Hmat, r.n, r.nbf, r.nbf5, r.scal, red);
+}
+
+__global__ void group_zero_gmax_kernel(const LagSys* __restrict__ d, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) *reinterpret_cast<unsigned long long*>(d[i].scal + 1) = 0ull;
+}
+
+// g = pack(4 elag - 4 elag^T) and max|g| (the element map of pack_grad_kernel)
+__global__ __launch_bounds__(256) void group_pack_grad_kernel(const LagSys* __restrict__ d, int n) {
+  const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::rect_begin)];
+  const long e = (blockIdx.x - r.rect_begin) * 256 + threadIdx.x;
+  double mx = 0.0;
+  if (e < (long)r.nbf5 * r.nbf) {
+    const int j = e % r.nbf, i = e / r.nbf;
+    if (j > i) {
+      const double v = 4.0 * r.elag[i + (long)j * r.nbf] - 4.0 * r.elag[j + (long)i * r.nbf];
+      r.g[pair_index(i, j, r.nbf)] = v;
+      mx = fmax(mx, fabs(v));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(reinterpret_cast<unsigned long long*>(r.scal + 1),
+              (unsigned long long)__double_as_longlong(mx));
 }
 
 int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
@@ -127,6 +241,46 @@ int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nb
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
                 int nbf, int nbf5, double* out) {
   hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(256), 0, s, elag, Hmat, n, nbf, nbf5, out);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+void lag_group_plan(LagSys* d, int n, LagBlocks* b) {
+  long w = 0, bm = 0, sq = 0, rect = 0;
+  int maxq = 1;
+  for (int i = 0; i < n; ++i) {
+    LagSys& r = d[i];
+    r.w_begin = w;
+    r.bmat_begin = bm;
+    r.sq_begin = sq;
+    r.rect_begin = rect;
+    const long t = (r.nbf5 + 31) / 32;
+    w += r.naux;
+    bm += t * t * ((r.naux + BMAT_Q - 1) / BMAT_Q);
+    sq += ((long)r.nbf * r.nbf + 255) / 256;
+    rect += ((long)r.nbf * r.nbf5 + 255) / 256;
+    maxq = std::max(maxq, r.nbf5);
+  }
+  b->w = w;
+  b->bmat = bm;
+  b->sq = sq;
+  b->rect = rect;
+  b->lds_w = (size_t)maxq * sizeof(double);
+}
+
+int lag_group_before(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) {
+  hipLaunchKernelGGL(group_lag_w_kernel, dim3((unsigned)b.w), dim3(128), b.lds_w, s, d, n);
+  hipLaunchKernelGGL(group_lag_bmat_kernel, dim3((unsigned)b.bmat), dim3(256), 0, s, d, n);
+  hipLaunchKernelGGL(group_lag_init_kernel, dim3((unsigned)b.sq), dim3(256), 0, s, d, n);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int lag_group_after(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) {
+  hipLaunchKernelGGL(group_lag_reduce_kernel, dim3((unsigned)b.rect), dim3(256), 0, s, d, n);
+  hipLaunchKernelGGL(group_energy_kernel, dim3(n), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(group_zero_gmax_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d, n);
+  hipLaunchKernelGGL(group_pack_grad_kernel, dim3((unsigned)b.rect), dim3(256), 0, s, d, n);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

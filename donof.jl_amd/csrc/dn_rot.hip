@@ -9,14 +9,9 @@
 #include <cmath>
 
 #include "dn_internal.h"
+#include "dn_rot_dev.h"
 
 namespace dn {
-
-// packed index of the rotation variable (i, j), i < nbf5, j > i (row-wise order of
-// utils.jl:503-510)
-__device__ __forceinline__ long pair_index(int i, int j, int nbf) {
-  return (long)i * nbf - (long)i * (i + 1) / 2 + (j - i - 1);
-}
 
 // Y[a,b] from y (antisymmetric, zero outside rows/columns < nbf5)
 __global__ void build_y_kernel(const double* __restrict__ y, int nbf, int nbf5,
@@ -40,10 +35,7 @@ __global__ void norm1_kernel(const double* __restrict__ Y, int nbf,
   const int lane = threadIdx.x & 63;
   const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= nbf) return;
-  const double* col = Y + (long)j * nbf;
-  double s = 0.0;
-  for (int i = lane; i < nbf; i += 64) s += fabs(col[i]);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  const double s = column_abs_sum(Y + (long)j * nbf, nbf, lane);
   if (lane == 0) atomicMax(out, (unsigned long long)__double_as_longlong(s));
 }
 
@@ -56,13 +48,8 @@ __global__ void poly_block_kernel(const double* __restrict__ X, const double* __
                                   int nbf, double* __restrict__ out) {
   const long total = (long)nbf * nbf;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int a = e % nbf, b = e / nbf;
-    double v = c1 * X[e] + c2 * X2[e] + c3 * X3[e];
-    if (X4) v += c4 * X4[e];
-    if (a == b) v += c0;
-    out[e] = v;
-  }
+       e += (long)gridDim.x * blockDim.x)
+    out[e] = poly_element(e, nbf, X, X2, X3, X4, c0, c1, c2, c3, c4);
 }
 
 __global__ void scale_kernel(const double* __restrict__ in, double f, long n,
@@ -112,35 +99,8 @@ __global__ void adam_step_kernel(const double* __restrict__ g, double* __restric
                                  double* __restrict__ Y) {
   const long total = (long)nbf5 * nbf;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const int j = e % nbf, i = e / nbf;
-    if (j <= i) continue;
-    const long k = pair_index(i, j, nbf);
-    const double gk = g[k];
-    const double mk = beta1 * m[k] + (1.0 - beta1) * gk;
-    double vk;
-    if (method == DN_ADABELIEF) {
-      const double d = gk - mk;
-      vk = beta2 * v[k] + (1.0 - beta2) * (d * d);
-    } else if (method == DN_YOGI) {
-      const double g2 = gk * gk;
-      const double dv = v[k] - g2;
-      const double sg = (dv > 0.0) ? 1.0 : ((dv < 0.0) ? -1.0 : 0.0);
-      vk = v[k] - (1.0 - beta2) * sg * g2;
-    } else {
-      vk = beta2 * v[k] + (1.0 - beta2) * (gk * gk);
-    }
-    const double mhat = mk / bc1;
-    const double vhat = vk / bc2;
-    const double vm = fmax(vmax[k], vhat);
-    const double yk = -alpha * mhat / sqrt(vm + 1e-16);
-    m[k] = mk;
-    v[k] = vk;
-    vmax[k] = vm;
-    y[k] = yk;
-    Y[i + (long)j * nbf] = yk;
-    Y[j + (long)i * nbf] = -yk;
-  }
+       e += (long)gridDim.x * blockDim.x)
+    adam_update(e, g, m, v, vmax, y, method, beta1, beta2, bc1, bc2, alpha, nbf, Y);
 }
 
 namespace {
