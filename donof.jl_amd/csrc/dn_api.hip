@@ -128,16 +128,17 @@ Layout layout_of(const dn_dims& d, int precision) {
 int alloc(dn_sys* s, size_t elems, double** out) {
   const size_t bytes = std::max<size_t>(elems, 1) * sizeof(double);
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) return fail(DN_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  DN_TRY(dev_alloc(s->ctx, s->stream, bytes, &p));
   s->owned.push_back(DevBuf{p, bytes});
   *out = static_cast<double*>(p);
   return DN_OK;
 }
 
+// every buffer back to the pool once the system's queued work is done
 void release(dn_sys* s) {
-  for (auto& b : s->owned) (void)hipFree(b.p);
+  for (auto& b : s->owned) dev_free(s->ctx, s->stream, b.p);
   s->owned.clear();
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
 }
 
 
@@ -374,6 +375,55 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+int dev_alloc(dn_ctx* c, hipStream_t s, size_t bytes, void** p) {
+  *p = nullptr;
+  hipError_t e = hipMallocFromPoolAsync(p, bytes, c->pool, s);
+  if (e == hipErrorOutOfMemory) {  // hand the pool's idle blocks back and try once more
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    (void)hipMemPoolTrimTo(c->pool, 0);
+    e = hipMallocFromPoolAsync(p, bytes, c->pool, s);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(DN_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes) +
+                                  " bytes: " + hipGetErrorString(e));
+  }
+  return DN_OK;
+}
+
+void dev_free(dn_ctx* c, hipStream_t s, void* p) {
+  (void)c;
+  if (p) (void)hipFreeAsync(p, s);
+}
+
+int pinned_get(dn_ctx* c, size_t bytes, void** p, size_t* got) {
+  {
+    std::lock_guard<std::mutex> lock(c->pin_mu);
+    size_t best = c->pin_free.size();
+    for (size_t i = 0; i < c->pin_free.size(); ++i)
+      if (c->pin_free[i].second >= bytes &&
+          (best == c->pin_free.size() || c->pin_free[i].second < c->pin_free[best].second))
+        best = i;
+    if (best < c->pin_free.size()) {
+      *p = c->pin_free[best].first;
+      *got = c->pin_free[best].second;
+      c->pin_free.erase(c->pin_free.begin() + best);
+      return DN_OK;
+    }
+  }
+  DN_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+  *got = bytes;
+  return DN_OK;
+}
+
+void pinned_put(dn_ctx* c, void* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lock(c->pin_mu);
+  c->pin_free.emplace_back(p, bytes);
+}
+
 }  // namespace dn
 
 using namespace dn;
@@ -401,10 +451,24 @@ int dn_ctx_create(int device, uint64_t hbm_budget_bytes, dn_ctx** out) {
   DN_HIP(hipSetDevice(device));
   size_t free_b = 0, total_b = 0;
   DN_HIP(hipMemGetInfo(&free_b, &total_b));
+  hipMemPoolProps props{};
+  props.allocType = hipMemAllocationTypePinned;
+  props.handleTypes = hipMemHandleTypeNone;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = device;
+  hipMemPool_t pool = nullptr;
+  DN_HIP(hipMemPoolCreate(&pool, &props));
+  uint64_t keep = UINT64_MAX;  // freed blocks stay in the pool until trimmed
+  const hipError_t e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  if (e != hipSuccess) {
+    (void)hipMemPoolDestroy(pool);
+    return fail(DN_ERR_HIP, std::string("hipMemPoolSetAttribute: ") + hipGetErrorString(e));
+  }
   auto* c = new dn_ctx;
   c->device = device;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)free_b;
   c->used = 0;
+  c->pool = pool;
   *out = c;
   return DN_OK;
 }
@@ -412,6 +476,16 @@ int dn_ctx_create(int device, uint64_t hbm_budget_bytes, dn_ctx** out) {
 int dn_ctx_destroy(dn_ctx* ctx) {
   if (!ctx) return DN_OK;
   std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  (void)hipSetDevice(ctx->device);
+  {
+    std::lock_guard<std::mutex> pl(ctx->pin_mu);
+    for (auto& b : ctx->pin_free) (void)hipHostFree(b.first);
+    ctx->pin_free.clear();
+  }
+  if (ctx->pool) {
+    (void)hipMemPoolTrimTo(ctx->pool, 0);
+    (void)hipMemPoolDestroy(ctx->pool);
+  }
   delete ctx;
   return DN_OK;
 }
@@ -462,6 +536,13 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   s->precision = precision;
   s->bytes = need;
   int rc = DN_OK;
+  {
+    const hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      s->stream = nullptr;
+      rc = fail(DN_ERR_HIP, hipGetErrorString(e));
+    }
+  }
   auto A = [&](size_t n, double** p) {
     if (rc == DN_OK) rc = alloc(s, n, p);
   };
@@ -523,8 +604,8 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     A(2, &q); s->cscale = reinterpret_cast<float*>(q);  // scale, 1/scale, max bits
     if (rc == DN_OK) {
       // plane padding must read as zero
-      hipError_t e1 = hipMemset(s->bhi, 0, Layout::d8(L.bplane) * 8);
-      hipError_t e2 = hipMemset(s->blo, 0, Layout::d8(L.bplane) * 8);
+      hipError_t e1 = hipMemsetAsync(s->bhi, 0, Layout::d8(L.bplane) * 8, s->stream);
+      hipError_t e2 = hipMemsetAsync(s->blo, 0, Layout::d8(L.bplane) * 8, s->stream);
       if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(DN_ERR_HIP, "hipMemset of b planes");
     }
   }
@@ -546,17 +627,17 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     k32.W = nullptr;
     k32.Wf = s->bpq32;
     static_assert(sizeof(SymKDesc) % sizeof(double) == 0 && sizeof(SymKDesc) <= 128, "layout");
-    if (hipMemcpy(s->symk_dev, &k, sizeof(k), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->scal + 32, &k32, sizeof(k32), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(s->symk_dev, &k, sizeof(k), hipMemcpyHostToDevice, s->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(s->scal + 32, &k32, sizeof(k32), hipMemcpyHostToDevice, s->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)  // k, k32 live on this frame
       rc = fail(DN_ERR_HIP, "descriptor upload");
   }
-  hipError_t e = hipSuccess;
-  if (rc == DN_OK) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (rc == DN_OK && e != hipSuccess) rc = fail(DN_ERR_HIP, hipGetErrorString(e));
   if (rc != DN_OK) {
     std::string msg = g_last_error;
-    if (s->stream) (void)hipStreamDestroy(s->stream);
     release(s);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     {
       std::lock_guard<std::mutex> lock(g_ctx_mutex);
       ctx->used -= need;
@@ -571,13 +652,10 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
 int dn_sys_destroy(dn_sys* s) {
   if (!s) return DN_OK;
   (void)hipSetDevice(s->ctx->device);
-  if (s->stream) {
-    (void)hipStreamSynchronize(s->stream);
-    (void)hipStreamDestroy(s->stream);
-  }
+  release(s);  // ordered after the system's queued work
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   for (auto ev : s->ev_start) (void)hipEventDestroy(ev);
   for (auto ev : s->ev_stop) (void)hipEventDestroy(ev);
-  release(s);
   {
     std::lock_guard<std::mutex> lock(g_ctx_mutex);
     s->ctx->used -= s->bytes;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -260,7 +261,25 @@ struct dn_ctx {
   int device = 0;
   uint64_t budget = 0;
   uint64_t used = 0;
+  // Device memory of the context's systems and groups: a stream-ordered pool that keeps
+  // the blocks it is given back (hipFree waits for every stream of the device; a
+  // stream-ordered free waits for nothing), trimmed when an allocation does not fit.
+  hipMemPool_t pool = nullptr;
+  // pinned host staging blocks of lock-step groups, kept for reuse (hipHostFree waits
+  // for the device as hipFree does)
+  std::mutex pin_mu;
+  std::vector<std::pair<void*, size_t>> pin_free;
 };
+
+namespace dn {
+// bytes from the context's pool, ordered on stream s
+int dev_alloc(dn_ctx* c, hipStream_t s, size_t bytes, void** p);
+// give p back to the pool, ordered after the work queued on s
+void dev_free(dn_ctx* c, hipStream_t s, void* p);
+// a pinned host block of at least `bytes` (a cached one when one is large enough)
+int pinned_get(dn_ctx* c, size_t bytes, void** p, size_t* got);
+void pinned_put(dn_ctx* c, void* p, size_t bytes);
+}  // namespace dn
 
 struct dn_sys {
   dn_ctx* ctx = nullptr;

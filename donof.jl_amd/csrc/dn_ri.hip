@@ -166,7 +166,7 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   if (s->ctx->budget && s->ctx->used + bytes > s->ctx->budget)
     return fail(DN_ERR_BUDGET, "RI metric workspace exceeds the HBM budget");
   char* w = nullptr;
-  DN_HIP(hipMalloc(&w, bytes));
+  DN_TRY(dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&w)));
   double* dI = reinterpret_cast<double*>(w);
   double* dG = dI + nI;
   double* Dinv = dG + nG;
@@ -188,8 +188,8 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
     return DN_OK;
   };
   rc = run();
-  hipStreamSynchronize(s->stream);
-  hipFree(w);
+  dev_free(s->ctx, s->stream, w);
+  (void)hipStreamSynchronize(s->stream);
   if (rc == DN_OK) s->b_ready = true;
   return rc;
 }

@@ -13,6 +13,7 @@
 #include "dn_internal.h"
 #include "dn_occ.h"
 #include "dn_rot_dev.h"
+#include "dn_sched.h"
 
 namespace dn {
 
@@ -218,13 +219,14 @@ struct dn_ogroup {
   double** scal_ptrs = nullptr;  // device array of each system's scal
   double* gathered = nullptr;    // device: E, max|g| bits per system
   double* pinned = nullptr;      // host (pinned) copy
+  size_t pinned_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // grouped rotation: descriptors built on the host each step (pinned staging) and
   // copied to the device in one transfer per phase
   unsigned char* rot_h = nullptr;
   unsigned char* rot_d = nullptr;
-  size_t rot_bytes = 0;
+  size_t rot_bytes = 0, rot_h_bytes = 0;
   // grouped Lagrangian tail (lag_w, lag_bmat, lag_init, the partial sum, energy, gradient)
   dn::LagSys* lag_d = nullptr;
   dn::LagBlocks lag_b;
@@ -356,6 +358,13 @@ void rot_build_stage(dn_ogroup* G, const RotLayout& L, int kind, const RotSys* r
   }
 }
 
+// Grid of a lock-step group's grouped GEMM: one workgroup per tile slot.  Lock-step
+// engines run beside each other and beside the set-up of new solves; a persistent grid
+// would hold its CUs until its last tile and hold up every other engine's short kernels.
+int tile_grid(const GroupedGemm* h, int n) {
+  return (int)std::min(8 * grouped_slots(h, n), 0x7ffffff8L);
+}
+
 // Runs a stage for the problems with squaring count > level (all of them for level -1).
 int rot_run_stage(dn_ogroup* G, RotStage* st, int level) {
   for (const RotClass& c : st->classes) {
@@ -366,7 +375,8 @@ int rot_run_stage(dn_ogroup* G, RotStage* st, int level) {
     // area is not written again while its copy to the device may be pending)
     const GroupedGemm& last = st->h[c.first + p - 1];
     const long total = last.tile_begin + (long)last.tiles_m * last.tiles_n * last.nsplit;
-    DN_TRY(gemm_f64_grouped(G->stream, st->h + c.first, st->d + c.first, p, total, c.config, 0));
+    DN_TRY(gemm_f64_grouped(G->stream, st->h + c.first, st->d + c.first, p, total, c.config,
+                            tile_grid(st->h + c.first, p)));
   }
   int r = 0;
   while (r < st->nred && st->red_sq[r] > level) ++r;
@@ -421,7 +431,8 @@ int plan_lstage(dn_ogroup* G, int stage) {
 int launch_lstage(dn_ogroup* G, int stage) {
   for (const auto& l : G->launches[stage])
     DN_TRY(gemm_f64_grouped(G->stream, G->host[stage].data() + l.first, G->dev[stage] + l.first,
-                            l.count, l.tiles, l.config, 0));
+                            l.count, l.tiles, l.config,
+                            tile_grid(G->host[stage].data() + l.first, l.count)));
   return DN_OK;
 }
 
@@ -565,16 +576,30 @@ int dn_ogroup_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_ogroup** 
     G->saved_streams.push_back(s->stream);
     s->stream = G->stream;
   }
-  for (int st = 0; st < kNumL && e == hipSuccess; ++st)
-    e = hipMalloc(&G->dev[st], nsys * sizeof(GroupedGemm));
-  if (e == hipSuccess) e = hipMalloc(&G->scal_ptrs, nsys * sizeof(double*));
-  if (e == hipSuccess) e = hipMalloc(&G->gathered, 2 * nsys * sizeof(double));
-  if (e == hipSuccess) e = hipHostMalloc(&G->pinned, 2 * nsys * sizeof(double), hipHostMallocDefault);
-  G->rot_bytes = RotLayout(nsys).bytes;
-  if (e == hipSuccess) e = hipHostMalloc(&G->rot_h, G->rot_bytes, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc(&G->rot_d, G->rot_bytes);
-  if (e == hipSuccess) e = hipMalloc(&G->lag_d, nsys * sizeof(LagSys));
-  if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, std::string("dn_ogroup_create: ") + hipGetErrorString(e)));
+  if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
+  // descriptors and staging from the context's pools: a group is rebuilt whenever its
+  // membership changes, and neither may wait for other groups' work on the device
+  {
+    int rc = DN_OK;
+    auto D = [&](size_t bytes, void** p) {
+      if (rc == DN_OK) rc = dev_alloc(ctx, G->stream, bytes, p);
+    };
+    for (int st = 0; st < kNumL; ++st)
+      D(nsys * sizeof(GroupedGemm), reinterpret_cast<void**>(&G->dev[st]));
+    D(nsys * sizeof(double*), reinterpret_cast<void**>(&G->scal_ptrs));
+    D(2 * nsys * sizeof(double), reinterpret_cast<void**>(&G->gathered));
+    G->rot_bytes = RotLayout(nsys).bytes;
+    D(G->rot_bytes, reinterpret_cast<void**>(&G->rot_d));
+    D(nsys * sizeof(LagSys), reinterpret_cast<void**>(&G->lag_d));
+    if (rc == DN_OK)
+      rc = pinned_get(ctx, 2 * nsys * sizeof(double), reinterpret_cast<void**>(&G->pinned),
+                      &G->pinned_bytes);
+    if (rc == DN_OK)
+      rc = pinned_get(ctx, G->rot_bytes, reinterpret_cast<void**>(&G->rot_h), &G->rot_h_bytes);
+    if (rc != DN_OK) return cleanup(rc);
+    e = hipStreamSynchronize(G->stream);  // the blocks are usable from any stream now
+    if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
+  }
   std::vector<double*> sp(nsys);
   for (int i = 0; i < nsys; ++i) sp[i] = G->sys[i]->scal;
   e = hipMemcpy(G->scal_ptrs, sp.data(), nsys * sizeof(double*), hipMemcpyHostToDevice);
@@ -597,14 +622,16 @@ int dn_ogroup_destroy(dn_ogroup* G) {
   if (G->stream) (void)hipStreamSynchronize(G->stream);
   for (size_t i = 0; i < G->sys.size(); ++i) G->sys[i]->stream = G->saved_streams[i];
   drop_lgraph(G);
-  for (int st = 0; st < kNumL; ++st)
-    if (G->dev[st]) (void)hipFree(G->dev[st]);
-  if (G->scal_ptrs) (void)hipFree(G->scal_ptrs);
-  if (G->gathered) (void)hipFree(G->gathered);
-  if (G->pinned) (void)hipHostFree(G->pinned);
-  if (G->rot_h) (void)hipHostFree(G->rot_h);
-  if (G->rot_d) (void)hipFree(G->rot_d);
-  if (G->lag_d) (void)hipFree(G->lag_d);
+  if (G->stream) {
+    for (int st = 0; st < kNumL; ++st) dev_free(G->ctx, G->stream, G->dev[st]);
+    dev_free(G->ctx, G->stream, G->scal_ptrs);
+    dev_free(G->ctx, G->stream, G->gathered);
+    dev_free(G->ctx, G->stream, G->rot_d);
+    dev_free(G->ctx, G->stream, G->lag_d);
+    (void)hipStreamSynchronize(G->stream);
+  }
+  pinned_put(G->ctx, G->pinned, G->pinned_bytes);
+  pinned_put(G->ctx, G->rot_h, G->rot_h_bytes);
   if (G->stream) (void)hipStreamDestroy(G->stream);
   delete G;
   return DN_OK;

@@ -274,14 +274,15 @@ int dn_der_pnof_coefficients(dn_sys* s, const dn_functional* f, const double* n,
   DN_TRY(h2d(s, s->n, n, s->d.nbf5));
   DN_TRY(h2d(s, s->dn, dn_dgamma, (size_t)s->d.nbf5 * s->d.nv));
   double* tmp = nullptr;
-  DN_HIP(hipMalloc(&tmp, 2 * std::max<size_t>(total, 1) * sizeof(double)));
+  DN_TRY(dev_alloc(s->ctx, s->stream, 2 * std::max<size_t>(total, 1) * sizeof(double),
+                   reinterpret_cast<void**>(&tmp)));
   const OccWork w = work_of(s);
   int rc = occ_prepare(s->stream, P, w);
   if (rc == DN_OK) rc = occ_der_coefficients(s->stream, P, w, s->dn, tmp, tmp + total);
   if (rc == DN_OK) rc = d2h(s, Dcj12r, tmp, total);
   if (rc == DN_OK) rc = d2h(s, Dck12r, tmp + total, total);
+  dev_free(s->ctx, s->stream, tmp);
   hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(tmp);
   if (rc != DN_OK) return rc;
   if (e != hipSuccess) return fail(DN_ERR_HIP, hipGetErrorString(e));
   return DN_OK;
