@@ -17,9 +17,9 @@ int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nb
 int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
 int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
                double* dmax, double* bmax);
-int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
-          const double* n, int nalpha, int nbeta, double* X, double* w);
-int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf5, int naux,
+int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
+          const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w);
+int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat);
 int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nbf5,
              double* elag);
@@ -142,27 +142,23 @@ void release(dn_sys* s) {
 }
 
 
-// b_pn[p,(n,Q)] = sum_m C5[m,p] b[m,(n,Q)]: one GEMM with N = nbf*naux (integrals.jl:159).
-int transform1(dn_sys* s) {
-  const dn_dims& d = s->d;
-  Gemm g;
-  g.M = d.nbf5;
-  g.N = d.nbf * d.nbfaux;
-  g.K = d.nbf;
-  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-  g.B = s->b; g.ldb = d.nbf; g.b_kcontig = true;
-  g.C = s->bpn; g.ldc = d.nbf5;
-  DN_TRY(begin_timing(s));
-  DN_TRY(gemm_f64(s->stream, g));
-  return end_timing(s);
-}
+// The transforms of b (integrals.jl:159-160, utils.jl:19-20), as GEMM descriptors shared
+// by the single-system paths and the grouped builds and lock-step groups (the same
+// operands and tile family give the same arithmetic):
+//   t1: bpn[n,p,Q] = sum_m b[m,n,Q] C5[m,p], batched over Q (= (b_Q C5)[n,p], b_Q^T C5
+//       for a general b).  With `sym` (b_Q = Lh_Q + Lh_Q^T, Lh the strict lower triangle
+//       plus half the diagonal) the sum runs over m >= n with Lh: T_Q = Lh_Q^T C5, and
+//       row block n0 reads only rows m >= n0 of b, half of b;
+//   t2 flat: bt[q,(p,Q)] = sum_n C5[n,q] bpn[n,(p,Q)], one GEMM with (p, Q) flattened:
+//       W_Q^T = (C5^T Lh_Q C5)^T after the symmetric t1 (C5^T b_Q C5 = W_Q + W_Q^T);
+//   t2 batched: bt[q,a,Q] = sum_n bpn[n,q,Q] C[n,a] for a < ncols, (C5^T b_Q C)[q,a] after
+//       the general t1 (slice stride nbf5 * ncols): C5^T b_Q C5 for ncols = nbf5;
+//   t2 Lagrangian: bt[q,a,Q] = (C5^T b_Q C)[q,a] stored at a + nbf (q + nbf5 Q), one GEMM
+//       sum_n C[n,a] bpn[n,(q,Q)] with (q, Q) flattened (no tile rounding of nbf5).
+// All three read A and B along K, the layout of the LDS-direct tile.
+}  // namespace
 
-// Symmetric b (b_Q = Lh_Q + Lh_Q^T, Lh the strict lower triangle plus half the
-// diagonal): C5^T b_Q C5 = W_Q + W_Q^T with W_Q = C5^T Lh_Q C5.
-//   T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p]: one batched GEMM over Q whose row block
-//              n0 only reads rows m >= n0 of b, half of b;
-//   W^T      = C5^T T, one GEMM with N = nbf5 * naux (p, Q flattened).
-int transform1_sym(dn_sys* s) {
+Gemm t1_gemm(const dn_sys* s, bool sym) {
   const dn_dims& d = s->d;
   Gemm g;
   g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
@@ -170,14 +166,12 @@ int transform1_sym(dn_sys* s) {
   g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
   g.C = s->bpn; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
   g.nbatch = d.nbfaux; g.bps = 1;
-  g.tri = kTriA;
+  g.tri = sym ? kTriA : kTriNone;
   g.lds = true;
-  DN_TRY(begin_timing(s));
-  DN_TRY(gemm_f64(s->stream, g));
-  return end_timing(s);
+  return g;
 }
 
-int transform2_sym(dn_sys* s) {
+Gemm t2_flat_gemm(const dn_sys* s) {
   const dn_dims& d = s->d;
   Gemm g;
   g.M = d.nbf5; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
@@ -185,23 +179,53 @@ int transform2_sym(dn_sys* s) {
   g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
   g.C = s->bt; g.ldc = d.nbf5;
   g.lds = true;
-  return gemm_f64(s->stream, g);
+  return g;
 }
 
-// out[:, :, Q] = b_pn[:, :, Q] * C[:, 0:ncols]  (integrals.jl:160 for ncols = nbf5; the
-// transpose of b_MO of utils.jl:19-20 for ncols = nbf).  Slice stride nbf5*ncols.
-int transform2(dn_sys* s, int ncols, double* out) {
+Gemm t2_batched_gemm(const dn_sys* s, int ncols) {
   const dn_dims& d = s->d;
   Gemm g;
-  g.M = d.nbf5;
-  g.N = ncols;
-  g.K = d.nbf;
-  g.A = s->bpn; g.lda = d.nbf5; g.sA = (long)d.nbf5 * d.nbf; g.a_kcontig = false;
+  g.M = d.nbf5; g.N = ncols; g.K = d.nbf;
+  g.A = s->bpn; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf5; g.a_kcontig = true;
   g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
-  g.C = out; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * ncols;
+  g.C = s->bt; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * ncols;
+  g.nbatch = d.nbfaux; g.bps = 1;
+  g.lds = true;
+  return g;
+}
+
+Gemm t2_lag_gemm(const dn_sys* s) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
+  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
+  g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
+  g.C = s->bt; g.ldc = d.nbf;
+  g.lds = true;
+  return g;
+}
+
+// elag partials: part_z[a,b] = sum_{Q in split z} sum_q bt[q,a,Q] bmat[q,b,Q], with
+// bt[q,a,Q] at a + nbf (q + nbf5 Q) and bmat[q,b,Q] at b + nbf5 (q + nbf5 Q)
+Gemm lag_contract_gemm(const dn_sys* s) {
+  const dn_dims& d = s->d;
+  const int nsplit = (int)lag_splits(d);
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf5;
+  g.A = s->bt; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf5; g.a_kcontig = false;
+  g.B = s->bmat; g.ldb = d.nbf5; g.sB = (long)d.nbf5 * d.nbf5; g.b_kcontig = false;
+  g.C = s->part; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
   g.nbatch = d.nbfaux;
-  g.bps = 1;
-  return gemm_f64(s->stream, g);
+  g.bps = (d.nbfaux + nsplit - 1) / nsplit;
+  return g;
+}
+
+namespace {
+
+int transform1(dn_sys* s, bool sym) {
+  DN_TRY(begin_timing(s));
+  DN_TRY(gemm_f64(s->stream, t1_gemm(s, sym)));
+  return end_timing(s);
 }
 
 // HC = H C5 ; optionally Hmat = C^T HC.  The J/K build splits K for parallelism; the
@@ -301,16 +325,16 @@ int jkh_resident(dn_sys* s) {
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
   if (s->precision == DN_SPLIT_F16) return jkh_split(s);
   if (s->b_sym && s->use_sym) {
-    DN_TRY(transform1_sym(s));
-    DN_TRY(transform2_sym(s));
+    DN_TRY(transform1(s, true));
+    DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));
     DN_TRY(symk(s->stream, s->symk_dev, 1, s->symk_blocks, s->symk_red));
     DN_TRY(j_from_x(s->stream, s->X, d.nbf5, d.nbfaux, s->part2, s->part2_elems, s->J));
     DN_TRY(hc_products(s, false));
     DN_TRY(hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
     return DN_OK;
   }
-  DN_TRY(transform1(s));
-  DN_TRY(transform2(s, d.nbf5, s->bt));
+  DN_TRY(transform1(s, false));
+  DN_TRY(gemm_f64(s->stream, t2_batched_gemm(s, d.nbf5)));
   DN_TRY(jk_from_bpq(s->stream, s->bt, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X, s->part,
                      s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
@@ -324,24 +348,16 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   if (nbeta < 0 || nalpha < 0 || nbeta > d.nbf5 || nalpha > d.nbf5)
     return fail(DN_ERR_ARG, "nalpha/nbeta out of range");
   const long slice = (long)d.nbf5 * d.nbf;
-  DN_TRY(transform1(s));
-  DN_TRY(transform2(s, d.nbf, s->bt));
+  DN_TRY(transform1(s, false));
+  DN_TRY(gemm_f64(s->stream, t2_lag_gemm(s)));
   DN_TRY(hc_products(s, true, false));
-  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta, s->X,
-               s->w));
-  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
+  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
+               s->X, s->w));
+  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
   DN_TRY(lag_init(s->stream, s->Hmat, s->n, d.nbf, d.nbf5, s->elag));
   // elag[a,b] += sum_Q sum_q bt[q,a,Q] bmat[q,b,Q]
-  const int nsplit = (int)lag_splits(d);
-  const int bps = (d.nbfaux + nsplit - 1) / nsplit;
-  Gemm g;
-  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf5;
-  g.A = s->bt; g.lda = d.nbf5; g.sA = slice; g.a_kcontig = true;
-  g.B = s->bmat; g.ldb = d.nbf5; g.sB = (long)d.nbf5 * d.nbf5; g.b_kcontig = true;
-  g.C = s->part; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
-  g.nbatch = d.nbfaux;
-  g.bps = bps;
-  const int used = (d.nbfaux + bps - 1) / bps;
+  const Gemm g = lag_contract_gemm(s);
+  const int used = (g.nbatch + g.bps - 1) / g.bps;
   if ((size_t)used * d.nbf * d.nbf5 > s->part_elems) return fail(DN_ERR_ARG, "partials overflow");
   DN_TRY(gemm_f64(s->stream, g));
   DN_TRY(reduce_splits(s->stream, s->part, (long)d.nbf * d.nbf5, used, d.nbf, d.nbf5, d.nbf,

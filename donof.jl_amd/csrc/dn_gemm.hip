@@ -391,12 +391,17 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
           af[i][p].y = kl + 2 * p + 1 < K ? af[i][p].y : 0.0;
         }
     }
-    // symmetric-A mode: a fragment row whose first row lies past this k tile's last k is
-    // all zero after the mask; its MFMAs are skipped (a wavefront-uniform branch)
-    bool live_row[FM];
+    // MFMAs of fragments that contribute nothing are skipped (wavefront-uniform branches):
+    // fragment rows past M and columns past N (clamped copies, never stored), and in
+    // symmetric-A mode rows whose first row lies past this k tile's last k (all zero
+    // after the mask)
+    bool live_row[FM], live_col[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
-      live_row[i] = TRI != kTriA || k0 + 15 >= m0 + wm * WM + 16 * i;
+      live_row[i] = (TRI != kTriA || k0 + 15 >= m0 + wm * WM + 16 * i) &&
+                    m0 + wm * WM + 16 * i < M;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) live_col[j] = n0 + wn * WN + 16 * j < N;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -404,6 +409,7 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
         if (!live_row[i]) continue;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
+          if (!live_col[j]) continue;
           const double a = (s & 1) ? af[i][s >> 1].y : af[i][s >> 1].x;
           const double b = (s & 1) ? bf[j][s >> 1].y : bf[j][s >> 1].x;
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc[i][j], 0, 0, 0);
@@ -569,7 +575,11 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
     // 8 wavefronts per workgroup (2 per SIMD) beat 4 with twice the wave tile on every
     // transform shape measured on MI355X; 64-row blocks for the symmetric-A mode (short,
     // uneven K loops), 128 x 256 tiles for long N (the flattened second transform)
-    if (g.tri == kTriA || g.M <= 64) {
+    // (also for row counts whose last 128-row block would be less than half used)
+    if (g.tri == kTriA || g.M <= 64 || (g.M % 128 != 0 && g.M % 128 <= 64)) {
+      if (lds64_config(g.N) == kLds64x192)  // one 192-wide N tile instead of two
+        return g.tri == kTriA ? launch_lds<64, 192, 2, 4, 2, kTriA>(s, g)
+                              : launch_lds<64, 192, 2, 4, 2, kTriNone>(s, g);
       return g.tri == kTriA ? launch_lds<64, 128, 2, 4, 3, kTriA>(s, g)
                             : launch_lds<64, 128, 2, 4, 3, kTriNone>(s, g);
     }
@@ -611,6 +621,7 @@ void config_tile(int config, int* bm, int* bn) {
     case kTile32x64: *bm = 32; *bn = 64; break;
     case kLds64x128: *bm = 64; *bn = 128; break;
     case kLds128x256: *bm = 128; *bn = 256; break;
+    case kLds64x192: *bm = 64; *bn = 192; break;
     default: *bm = 128; *bn = 128; break;  // kTile128x128, kLds128x128
   }
 }
@@ -695,6 +706,7 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     switch (config) {
       case kLds64x128: return launch_grouped_lds<64, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
       case kLds128x128: return launch_grouped_lds<128, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
+      case kLds64x192: return launch_grouped_lds<64, 192, 2, 4, 2>(s, tri, dd, n, total, slots, grid);
       default: return launch_grouped_lds<128, 256, 2, 4, 2>(s, tri, dd, n, total, slots, grid);
     }
   }
@@ -707,6 +719,8 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
   }
 }
+
+int lds64_config(int N) { return N > 128 && N <= 192 ? kLds64x192 : kLds64x128; }
 
 bool gemm_config_ok(int config, const Gemm& g) {
   if (config < 0 || config >= kNumGemmConfigs) return false;

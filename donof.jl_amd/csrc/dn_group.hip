@@ -186,34 +186,11 @@ Gemm stage_gemm(dn_sys* s, int stage) {
   const dn_dims& d = s->d;
   Gemm g;
   switch (stage) {
-    case kK1:
-      if (s->b_sym && s->use_sym) {  // T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p]
-        g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
-        g.A = s->b; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf; g.a_kcontig = true;
-        g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
-        g.C = s->bpn; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
-        g.nbatch = d.nbfaux; g.bps = 1;
-        g.tri = kTriA;
-      } else {
-        g.M = d.nbf5; g.N = d.nbf * d.nbfaux; g.K = d.nbf;
-        g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-        g.B = s->b; g.ldb = d.nbf; g.b_kcontig = true;
-        g.C = s->bpn; g.ldc = d.nbf5;
-      }
+    case kK1:  // bpn = b_Q C5 (or Lh_Q^T C5 for the symmetric build), batched over Q
+      g = t1_gemm(s, s->b_sym && s->use_sym);
       break;
-    case kK2:
-      if (s->b_sym && s->use_sym) {  // W^T = C5^T T with (p, Q) flattened
-        g.M = d.nbf5; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
-        g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-        g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
-        g.C = s->bt; g.ldc = d.nbf5;
-      } else {
-        g.M = d.nbf5; g.N = d.nbf5; g.K = d.nbf;
-        g.A = s->bpn; g.lda = d.nbf5; g.sA = (long)d.nbf5 * d.nbf; g.a_kcontig = false;
-        g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
-        g.C = s->bt; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * d.nbf5;
-        g.nbatch = d.nbfaux; g.bps = 1;
-      }
+    case kK2:  // C5^T bpn with (p, Q) flattened (symmetric), bpn_Q^T C5 per Q (general)
+      g = (s->b_sym && s->use_sym) ? t2_flat_gemm(s) : t2_batched_gemm(s, d.nbf5);
       break;
     case kJ:  // J = X X^T, X[p + Q nbf5]
       g.M = d.nbf5; g.N = d.nbf5; g.K = d.nbfaux;
@@ -232,18 +209,19 @@ Gemm stage_gemm(dn_sys* s, int stage) {
 }
 
 int size_bucket(const Gemm& g, int stage) {
-  const int n5 = stage == kHC ? g.N : (g.tri ? g.N : g.M);
+  const int n5 = (stage == kHC || stage == kK1) ? g.N : g.M;
   return n5 <= 32 ? 0 : n5 <= 64 ? 1 : n5 <= 128 ? 2 : 3;
 }
 
 int default_config(int stage, int bucket, const Gemm& g) {
   if (bucket == 0) return kTile64x32;
-  // the symmetric-b transforms (K1 with the Lh mask, K2 flattened) take the LDS-direct
-  // tiles the single-system build uses
-  const bool sym = (stage == kK1 && g.tri == kTriA) || (stage == kK2 && g.a_kcontig && g.b_kcontig);
-  if (sym && gemm_config_ok(kLds64x128, g)) {
-    if (stage == kK1 || bucket == 1) return kLds64x128;
-    return kLds128x256;
+  // the transforms take the LDS-direct tiles the single-system build uses (K2 flattened
+  // over (p, Q) for the symmetric build, batched over Q for a general b)
+  const bool lds = (stage == kK1 || stage == kK2) && g.lds;
+  if (lds && gemm_config_ok(kLds64x128, g)) {
+    if (stage == kK1) return lds64_config(g.N);
+    if (bucket == 1) return kLds64x128;
+    return g.nbatch > 1 ? kLds128x128 : kLds128x256;
   }
   if (stage == kK2 && bucket >= 2) return kTile128x128;
   return stage == kK1 ? kTile64x128 : kTile64x64;

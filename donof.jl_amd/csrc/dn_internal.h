@@ -167,8 +167,12 @@ enum GemmConfig {
   kLds64x128 = 6,   // LDS-direct pipeline, K-contiguous A and B only
   kLds128x128 = 7,
   kLds128x256 = 8,
-  kNumGemmConfigs = 9
+  kLds64x192 = 9,   // for N in (128, 192]: one N tile, fragments past N skipped
+  kNumGemmConfigs = 10
 };
+// The 64-row LDS-direct configuration for N columns: 192-wide tiles for N in (128, 192],
+// 128-wide otherwise (dn_gemm.hip).
+int lds64_config(int N);
 // Whether a tile configuration can run a problem of the given layout and alignment.
 bool gemm_config_ok(int config, const Gemm& g);
 // Fill tile counts/offsets of desc[0..n) for a config; returns the total tile count.
@@ -214,6 +218,16 @@ struct DevBuf {
 }  // namespace dn
 
 namespace dn {
+// the transforms of b as GEMMs (dn_api.hip): t1 bpn[n,p,Q] = (b_Q^T C5)[n,p] (Lh_Q^T C5
+// with sym), t2 flat bt[q,(p,Q)] = (C5^T bpn)[q,(p,Q)], t2 batched
+// bt[q,a,Q] = (bpn_Q^T C)[q,a] for a < ncols
+Gemm t1_gemm(const dn_sys* s, bool sym);
+Gemm t2_flat_gemm(const dn_sys* s);
+Gemm t2_batched_gemm(const dn_sys* s, int ncols);
+// the Lagrangian's transform bt[q,a,Q] at a + nbf (q + nbf5 Q) and its contraction with
+// bmat into split partials (dn_api.hip)
+Gemm t2_lag_gemm(const dn_sys* s);
+Gemm lag_contract_gemm(const dn_sys* s);
 // resident building blocks (dn_api.hip)
 int check_sys(dn_sys* s);
 int h2d(dn_sys* s, double* dst, const double* src, size_t elems);

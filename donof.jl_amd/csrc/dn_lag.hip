@@ -9,6 +9,10 @@
 // where coul(b) marks the Coulomb self-terms of utils.jl:42-47 (nbeta > 0 and b < nbeta
 // or b >= nalpha).  The first line is the C^K term (54-55), w carries C^J (50-51) and the
 // Coulomb terms; the one-electron term n[b] Hmat[a,b] (39) is added afterwards.
+//
+// Layouts in HBM: bt[q,a,Q] at a + nbf q + nbf nbf5 Q (a fastest: the output of one flat
+// GEMM over (q, Q)) and bmat[q,b,Q] at b + nbf5 q + nbf5^2 Q, so that bmat is an
+// element-wise map of bt's first nbf5 rows with both read and written along b.
 #include "dn_internal.h"
 #include "dn_rot_dev.h"
 
@@ -17,14 +21,15 @@ namespace dn {
 // ---- per-block / per-element bodies (single-system and grouped kernels alike) -------------
 
 // X[:,Q] and w[:,Q] for all b < nbf5 (one workgroup per Q); xs: nbf5 doubles of LDS.
+// bt[q,q,Q] is at q (nbf + 1) + slice Q.
 __device__ __forceinline__ void lag_w_block(int Q, const double* __restrict__ bt, long slice,
-                                            int nbf5, const double* __restrict__ cj12,
+                                            int nbf, int nbf5, const double* __restrict__ cj12,
                                             const double* __restrict__ n, int nalpha, int nbeta,
                                             double* __restrict__ X, double* __restrict__ w,
                                             double* xs) {
   const double* s = bt + (long)Q * slice;
   for (int q = threadIdx.x; q < nbf5; q += blockDim.x) {
-    const double v = s[(long)q * (nbf5 + 1)];
+    const double v = s[(long)q * (nbf + 1)];
     xs[q] = v;
     X[q + (long)Q * nbf5] = v;
   }
@@ -38,39 +43,18 @@ __device__ __forceinline__ void lag_w_block(int Q, const double* __restrict__ bt
   }
 }
 
-// bmat[q,b,Q] = -ck12[b,q] bt[q,b,Q] + [q==b] w[b,Q]
-// Workgroup (tq, tb, zq): the 32 x 32 (q, b) tile at (32 tq, 32 tb) for Q in
-// [BMAT_Q zq, BMAT_Q (zq + 1)).  The ck12 tile is read once, along b, and kept transposed
-// in LDS (ct), so that bt and bmat are read and written along q (coalesced).
-constexpr int BMAT_Q = 16;
-__device__ __forceinline__ void lag_bmat_block(int tq, int tb, int zq,
-                                               const double* __restrict__ bt, long slice,
-                                               int nbf5, int naux,
-                                               const double* __restrict__ ck12,
-                                               const double* __restrict__ w,
-                                               double* __restrict__ bmat, double (*ct)[33]) {
-  const int q0 = 32 * tq, b0 = 32 * tb;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int i = ty; i < 32; i += 8) {  // row q0 + i of the tile: ck12[b0 + tx, q0 + i]
-    const int q = q0 + i, b = b0 + tx;
-    ct[i][tx] = (q < nbf5 && b < nbf5) ? ck12[b + (long)q * nbf5] : 0.0;
-  }
-  __syncthreads();
+// bmat[q,b,Q] = -ck12[b,q] bt[q,b,Q] + [q==b] w[b,Q] for element e = b + nbf5 q + nbf5^2 Q
+__device__ __forceinline__ double lag_bmat_element(long e, const double* __restrict__ bt,
+                                                   long slice, int nbf, int nbf5,
+                                                   const double* __restrict__ ck12,
+                                                   const double* __restrict__ w) {
   const long nn = (long)nbf5 * nbf5;
-  const int q = q0 + tx;
-  if (q >= nbf5) return;
-  const int Q1 = min(naux, BMAT_Q * (zq + 1));
-  for (int Q = BMAT_Q * zq; Q < Q1; ++Q) {
-    const double* s = bt + (long)Q * slice;
-    double* o = bmat + (long)Q * nn;
-    for (int i = ty; i < 32; i += 8) {
-      const int b = b0 + i;
-      if (b >= nbf5) break;
-      double v = -ct[tx][i] * s[q + (long)b * nbf5];
-      if (q == b) v += w[b + (long)Q * nbf5];
-      o[q + (long)b * nbf5] = v;
-    }
-  }
+  const long Q = e / nn;
+  const int r = (int)(e - Q * nn);
+  const int b = r % nbf5, q = r / nbf5;
+  double v = -ck12[r] * bt[b + (long)nbf * q + slice * Q];
+  if (q == b) v += w[b + (long)nbf5 * Q];
+  return v;
 }
 
 // elag[a,b] = n[b] Hmat[a,b] for b < nbf5, 0 for b >= nbf5 (utils.jl:22,39)
@@ -101,23 +85,25 @@ __device__ __forceinline__ void energy_block(const double* __restrict__ elag,
 
 // ---- single-system kernels --------------------------------------------------------------
 
-__global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf5, int naux,
-                             const double* __restrict__ cj12, const double* __restrict__ n,
-                             int nalpha, int nbeta, double* __restrict__ X,
-                             double* __restrict__ w) {
+__global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf, int nbf5,
+                             int naux, const double* __restrict__ cj12,
+                             const double* __restrict__ n, int nalpha, int nbeta,
+                             double* __restrict__ X, double* __restrict__ w) {
   extern __shared__ double xs[];  // nbf5 diagonal entries of slice Q
   const int Q = blockIdx.x;
   if (Q >= naux) return;
-  lag_w_block(Q, bt, slice, nbf5, cj12, n, nalpha, nbeta, X, w, xs);
+  lag_w_block(Q, bt, slice, nbf, nbf5, cj12, n, nalpha, nbeta, X, w, xs);
 }
 
 __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict__ bt, long slice,
-                                                       int nbf5, int naux,
+                                                       int nbf, int nbf5, int naux,
                                                        const double* __restrict__ ck12,
                                                        const double* __restrict__ w,
                                                        double* __restrict__ bmat) {
-  __shared__ double ct[32][33];  // ct[q][b] = ck12[b, q]
-  lag_bmat_block(blockIdx.x, blockIdx.y, blockIdx.z, bt, slice, nbf5, naux, ck12, w, bmat, ct);
+  const long total = (long)nbf5 * nbf5 * naux;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x)
+    bmat[e] = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
 }
 
 __global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
@@ -151,17 +137,15 @@ __device__ __forceinline__ int find_lag(const LagSys* __restrict__ d, int n, lon
 __global__ __launch_bounds__(128) void group_lag_w_kernel(const LagSys* __restrict__ d, int n) {
   extern __shared__ double xs[];
   const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::w_begin)];
-  lag_w_block((int)(blockIdx.x - r.w_begin), r.bt, r.slice, r.nbf5, r.cj12, r.n, r.nalpha,
-              r.nbeta, r.X, r.w, xs);
+  lag_w_block((int)(blockIdx.x - r.w_begin), r.bt, r.slice, r.nbf, r.nbf5, r.cj12, r.n,
+              r.nalpha, r.nbeta, r.X, r.w, xs);
 }
 
 __global__ __launch_bounds__(256) void group_lag_bmat_kernel(const LagSys* __restrict__ d, int n) {
-  __shared__ double ct[32][33];
   const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::bmat_begin)];
-  const int t = (r.nbf5 + 31) / 32;
-  const long local = blockIdx.x - r.bmat_begin;
-  const int zq = (int)(local / ((long)t * t)), rem = (int)(local % ((long)t * t));
-  lag_bmat_block(rem % t, rem / t, zq, r.bt, r.slice, r.nbf5, r.naux, r.ck12, r.w, r.bmat, ct);
+  const long e = (blockIdx.x - r.bmat_begin) * 256 + threadIdx.x;
+  if (e < (long)r.nbf5 * r.nbf5 * r.naux)
+    r.bmat[e] = lag_bmat_element(e, r.bt, r.slice, r.nbf, r.nbf5, r.ck12, r.w);
 }
 
 __global__ __launch_bounds__(256) void group_lag_init_kernel(const LagSys* __restrict__ d, int n) {
@@ -214,19 +198,19 @@ __global__ __launch_bounds__(256) void group_pack_grad_kernel(const LagSys* __re
               (unsigned long long)__double_as_longlong(mx));
 }
 
-int lag_w(hipStream_t s, const double* bt, long slice, int nbf5, int naux, const double* cj12,
-          const double* n, int nalpha, int nbeta, double* X, double* w) {
+int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
+          const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w) {
   hipLaunchKernelGGL(lag_w_kernel, dim3(naux), dim3(128), nbf5 * sizeof(double), s, bt, slice,
-                     nbf5, naux, cj12, n, nalpha, nbeta, X, w);
+                     nbf, nbf5, naux, cj12, n, nalpha, nbeta, X, w);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
 
-int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf5, int naux,
+int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat) {
-  const unsigned t = (nbf5 + 31) / 32;
-  hipLaunchKernelGGL(lag_bmat_kernel, dim3(t, t, (naux + BMAT_Q - 1) / BMAT_Q), dim3(256), 0, s,
-                     bt, slice, nbf5, naux, ck12, w, bmat);
+  const long blocks = ((long)nbf5 * nbf5 * naux + 255) / 256;
+  hipLaunchKernelGGL(lag_bmat_kernel, dim3((unsigned)std::min(blocks, 8192L)), dim3(256), 0, s,
+                     bt, slice, nbf, nbf5, naux, ck12, w, bmat);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -254,9 +238,8 @@ void lag_group_plan(LagSys* d, int n, LagBlocks* b) {
     r.bmat_begin = bm;
     r.sq_begin = sq;
     r.rect_begin = rect;
-    const long t = (r.nbf5 + 31) / 32;
     w += r.naux;
-    bm += t * t * ((r.naux + BMAT_Q - 1) / BMAT_Q);
+    bm += ((long)r.nbf5 * r.nbf5 * r.naux + 255) / 256;
     sq += ((long)r.nbf * r.nbf + 255) / 256;
     rect += ((long)r.nbf * r.nbf5 + 255) / 256;
     maxq = std::max(maxq, r.nbf5);

@@ -155,20 +155,13 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
   const dn_dims& d = s->d;
   Gemm g;
   switch (stage) {
-    case kT1:  // b_pn = C5^T b  (integrals.jl:159 / utils.jl:19)
-      g.M = d.nbf5; g.N = d.nbf * d.nbfaux; g.K = d.nbf;
-      g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-      g.B = s->b; g.ldb = d.nbf; g.b_kcontig = true;
-      g.C = s->bpn; g.ldc = d.nbf5;
-      *config = d.nbf5 > 64 ? kTile128x128 : kTile64x128;
+    case kT1:  // bpn = b_Q C5 (integrals.jl:159 / utils.jl:19)
+      g = t1_gemm(s, false);
+      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N) : kTile64x128;
       break;
-    case kT2:  // b_MO^T[:, :, Q] = b_pn[:, :, Q] C  (utils.jl:19-20)
-      g.M = d.nbf5; g.N = d.nbf; g.K = d.nbf;
-      g.A = s->bpn; g.lda = d.nbf5; g.sA = (long)d.nbf5 * d.nbf; g.a_kcontig = false;
-      g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
-      g.C = s->bt; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * d.nbf;
-      g.nbatch = d.nbfaux; g.bps = 1;
-      *config = d.nbf5 > 64 ? kTile128x128 : kTile64x64;
+    case kT2:  // b_MO^T = C5^T b_Q C, (q, Q) flattened  (utils.jl:19-20)
+      g = t2_lag_gemm(s);
+      *config = gemm_config_ok(kLds64x128, g) ? kLds64x128 : kTile128x128;
       break;
     case kHC:  // H C5
       g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
@@ -184,17 +177,10 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
       g.C = s->Hmat; g.ldc = d.nbf;
       *config = kTile64x64;
       break;
-    default: {  // elag partials: sum_Q sum_q bt[q,a,Q] bmat[q,b,Q] over Q ranges
-      const int nsplit = (int)lag_split_count(d);
-      const int bps = (d.nbfaux + nsplit - 1) / nsplit;
-      g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf5;
-      g.A = s->bt; g.lda = d.nbf5; g.sA = (long)d.nbf5 * d.nbf; g.a_kcontig = true;
-      g.B = s->bmat; g.ldb = d.nbf5; g.sB = (long)d.nbf5 * d.nbf5; g.b_kcontig = true;
-      g.C = s->part; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
-      g.nbatch = d.nbfaux; g.bps = bps;
+    default:  // elag partials: sum_Q sum_q bt[q,a,Q] bmat[q,b,Q] over Q ranges
+      g = lag_contract_gemm(s);
       *config = kTile64x64;
       break;
-    }
   }
   return g;
 }
