@@ -89,6 +89,8 @@ _SIGS = {
     "dn_ogroup_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
     "dn_ogroup_destroy": (_I, [_P]),
     "dn_ogroup_step": (_I, [_P, ct.POINTER(ct.c_int)]),
+    "dn_ogroup_run": (_I, [_P, _I, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                           ct.POINTER(ct.c_int)]),
     "dn_group_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
     "dn_group_destroy": (_I, [_P]),
     "dn_group_tune": (_I, [_P, ct.POINTER(ct.c_int)]),

@@ -299,6 +299,16 @@ class OGroup:
         check(self.lib.dn_ogroup_step(self.handle, ct.byref(n)))
         return n.value
 
+    def run(self, max_steps, stop=None):
+        """Steps in native code until a system finishes, none steps, `stop` (a ctypes
+        c_int another thread may set) is non-zero, or max_steps ran (dn_ogroup_run).
+        Returns (steps run, systems that stepped in the last one)."""
+        steps, active = ct.c_int(), ct.c_int()
+        check(self.lib.dn_ogroup_run(self.handle, int(max_steps),
+                                     ct.byref(stop) if stop is not None else None,
+                                     ct.byref(steps), ct.byref(active)))
+        return steps.value, active.value
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.dn_ogroup_destroy(self.handle)

@@ -243,6 +243,11 @@ struct GemmLds {
 // that half at (c % 4) * 256, row r at (r % 16) * 16.  One load instruction fills one half
 // of one block (lane l: row l % 16, chunk l / 16, 64 contiguous global bytes per row).
 //
+// Fragments are dealt to the wavefronts round-robin: fragment row i of wavefront row wm
+// covers tile rows 16 (i WGM + wm) .. + 15, fragment column j of wavefront column wn tile
+// columns 16 (j WGN + wn) .. + 15.  A tile cut short by M or N (whose fragments past the
+// edge are skipped) then leaves every wavefront, and so every SIMD, a similar share.
+//
 // MFMA k order: in step s (0..3) of a k tile, the lanes of fragment group g = lane / 16
 // contract k = 4 g + s (A and B alike, so the sum is the same; only its order differs
 // from the register-staged tile).  A lane's four k values are two consecutive chunks, read
@@ -359,10 +364,10 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
       const int o = (c >> 2) * 1024 + (c & 3) * 256 + fr * 16;
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        af[i][p] = *reinterpret_cast<const v2d*>(As + (wm * WM + 16 * i) * 128 + o);
+        af[i][p] = *reinterpret_cast<const v2d*>(As + 16 * (i * WGM + wm) * 128 + o);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        bf[j][p] = *reinterpret_cast<const v2d*>(Bs + (wn * WN + 16 * j) * 128 + o);
+        bf[j][p] = *reinterpret_cast<const v2d*>(Bs + 16 * (j * WGN + wn) * 128 + o);
     }
     const int k0 = (kt0 + ckt) * 16;
     if (++ckt == nkt) ckt = 0;
@@ -373,7 +378,7 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
       if (k0 < m0 + BM) {
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          const int d = kl - (m0 + wm * WM + 16 * i + fr);
+          const int d = kl - (m0 + 16 * (i * WGM + wm) + fr);
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
             af[i][p].x *= d + 2 * p > 0 ? 1.0 : (d + 2 * p == 0 ? 0.5 : 0.0);
@@ -398,10 +403,10 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
     bool live_row[FM], live_col[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
-      live_row[i] = (TRI != kTriA || k0 + 15 >= m0 + wm * WM + 16 * i) &&
-                    m0 + wm * WM + 16 * i < M;
+      live_row[i] = (TRI != kTriA || k0 + 15 >= m0 + 16 * (i * WGM + wm)) &&
+                    m0 + 16 * (i * WGM + wm) < M;
 #pragma unroll
-    for (int j = 0; j < FN; ++j) live_col[j] = n0 + wn * WN + 16 * j < N;
+    for (int j = 0; j < FN; ++j) live_col[j] = n0 + 16 * (j * WGN + wn) < N;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -417,17 +422,17 @@ __device__ __forceinline__ void gemm_tile_lds(const Gemm& g, int mt, int nt, int
       }
   }
 
-  // same accumulator layout and stores as gemm_tile
+  // the accumulator layout of gemm_tile, fragments placed round-robin as above
   double* Cz = g.C + (long)z * g.sC;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
-    const int row = m0 + wm * WM + i * 16 + fr;
+    const int row = m0 + 16 * (i * WGM + wm) + fr;
     if (row >= M) continue;
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int col = n0 + wn * WN + j * 16 + fk + 4 * rr;
+        const int col = n0 + 16 * (j * WGN + wn) + fk + 4 * rr;
         if (col < N) {
           double* c = Cz + row + (long)col * g.ldc;
           const double v = g.alpha * acc[i][j][rr];
@@ -720,7 +725,11 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
   }
 }
 
-int lds64_config(int N) { return N > 128 && N <= 192 ? kLds64x192 : kLds64x128; }
+int lds64_config(int N) {
+  // 192 columns: one N tile for N in (128, 192]; for long N (flattened transforms) the
+  // wider tile's operand reuse wins (measured on MI355X)
+  return N > 128 && (N <= 192 || N >= 1024) ? kLds64x192 : kLds64x128;
+}
 
 bool gemm_config_ok(int config, const Gemm& g) {
   if (config < 0 || config >= kNumGemmConfigs) return false;

@@ -170,8 +170,8 @@ enum GemmConfig {
   kLds64x192 = 9,   // for N in (128, 192]: one N tile, fragments past N skipped
   kNumGemmConfigs = 10
 };
-// The 64-row LDS-direct configuration for N columns: 192-wide tiles for N in (128, 192],
-// 128-wide otherwise (dn_gemm.hip).
+// The 64-row LDS-direct configuration for N columns: 192-wide tiles for N in (128, 192]
+// and for long N (>= 1024), 128-wide otherwise (dn_gemm.hip).
 int lds64_config(int N);
 // Whether a tile configuration can run a problem of the given layout and alignment.
 bool gemm_config_ok(int config, const Gemm& g);

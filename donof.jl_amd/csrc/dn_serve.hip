@@ -161,7 +161,7 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
       break;
     case kT2:  // b_MO^T = C5^T b_Q C, (q, Q) flattened  (utils.jl:19-20)
       g = t2_lag_gemm(s);
-      *config = gemm_config_ok(kLds64x128, g) ? kLds64x128 : kTile128x128;
+      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N) : kTile128x128;
       break;
     case kHC:  // H C5
       g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
@@ -767,6 +767,26 @@ int dn_ogroup_step(dn_ogroup* G, int* active) {
       o.improved = true;
     }
   }
+  return DN_OK;
+}
+
+int dn_ogroup_run(dn_ogroup* G, int max_steps, const volatile int* stop, int* steps,
+                  int* active) {
+  if (!G || max_steps < 0) return fail(DN_ERR_ARG, "bad dn_ogroup_run arguments");
+  auto finished = [&] {
+    int k = 0;
+    for (dn_sys* s : G->sys) k += s->opt.done ? 1 : 0;
+    return k;
+  };
+  int k = 0, act = 0;
+  while (k < max_steps && !(stop && *stop)) {
+    const int before = finished();
+    DN_TRY(dn_ogroup_step(G, &act));
+    ++k;
+    if (act == 0 || finished() != before) break;
+  }
+  if (steps) *steps = k;
+  if (active) *active = act;
   return DN_OK;
 }
 

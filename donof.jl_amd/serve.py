@@ -23,6 +23,7 @@ alone and in the mix.
 """
 from __future__ import annotations
 
+import ctypes as ct
 import math
 import queue
 import threading
@@ -126,11 +127,21 @@ class _LockStepEngine(threading.Thread):
     arriving in the inbox join at the next step boundary (the group is rebuilt), finished
     ones leave and are reported through `finished(result, need)`."""
 
+    # iterations per native run between checks of the inbox and of finished solves
+    RUN_STEPS = 64
+
     def __init__(self, ctx, t0, phases, lock, finished, log):
         super().__init__(daemon=True)
         self.ctx, self.t0, self.phases, self.lock = ctx, t0, phases, lock
         self.finished, self.log = finished, log
         self.inbox = queue.Queue()
+        # set when something waits in the inbox: the native loop ends at the next
+        # iteration boundary (dn_ogroup_run), the newcomers join there
+        self.stop = ct.c_int(0)
+
+    def submit(self, item):
+        self.inbox.put(item)
+        self.stop.value = 1
 
     def _flush(self, ph):
         with self.lock:
@@ -145,6 +156,7 @@ class _LockStepEngine(threading.Thread):
         try:
             while not stop or live:
                 changed = False
+                self.stop.value = 0   # before draining: a later submit sets it again
                 while True:   # newcomers join at this step boundary
                     try:
                         item = self.inbox.get(timeout=0.05) if not live and not stop \
@@ -201,7 +213,7 @@ class _LockStepEngine(threading.Thread):
                 group = api.OGroup([v[1] for v in live.values()], self.ctx)
             ph["regroup"] += time.perf_counter() - tg
         ts = time.perf_counter()
-        group.step()
+        group.run(self.RUN_STEPS, self.stop)
         ph["step"] += time.perf_counter() - ts
         done = [rid for rid, v in live.items() if v[1].orbopt_done()]
         if not done:
@@ -303,7 +315,7 @@ class Server:
                 finally:
                     with lock:
                         ph["admit"] += time.perf_counter() - ta
-                engine_for(req.nbf).inbox.put((req, dev, p, need, admitted))
+                engine_for(req.nbf).submit((req, dev, p, need, admitted))
 
         worker = threading.Thread(target=preparer, daemon=True)
         worker.start()
@@ -348,7 +360,7 @@ class Server:
             prep_q.put(None)
             worker.join()
             for e in engines:
-                e.inbox.put(None)
+                e.submit(None)
             for e in engines:
                 e.join()
         self.results = sorted(results, key=lambda r: r.rid)

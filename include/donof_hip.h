@@ -207,6 +207,13 @@ int dn_ogroup_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_ogroup** 
 int dn_ogroup_destroy(dn_ogroup* group);
 /* One iteration of every unfinished system; *active = systems that stepped. */
 int dn_ogroup_step(dn_ogroup* group, int* active);
+/* Repeated dn_ogroup_step without returning to the caller between iterations: stops after
+ * the iteration in which a system finished, when no system stepped, when *stop is non-zero
+ * at an iteration boundary (another thread may set it, e.g. when solves wait to join), or
+ * after max_steps.  *steps = iterations run, *active = systems that stepped in the last.
+ * (No reference counterpart: the serving loop's native inner loop.) */
+int dn_ogroup_run(dn_ogroup* group, int max_steps, const volatile int* stop, int* steps,
+                  int* active);
 
 /* ---- grouped builds --------------------------------------------------------------
  * Many resident systems of uneven size built in one sequence of grouped, persistent

@@ -2,6 +2,8 @@
 result the same request gets when it runs alone; admission honours the HBM budget.
 The lock-step grouped engine (dn_ogroup_step) must also reproduce the single-system
 device loop (dn_orbopt_resident) bit for bit."""
+import ctypes as ct
+
 import numpy as np
 import pytest
 
@@ -31,6 +33,36 @@ def test_grouped_engine_matches_single_system_loop():
         (g,) = _serve([r], "grouped", max_live=1)
         assert g.error == ""
         assert (g.E, g.nit, g.success) == resident[r.rid]     # bitwise the same
+
+
+def test_native_run_equals_single_steps():
+    """dn_ogroup_run (the engines' inner loop) advances exactly as repeated dn_ogroup_step
+    and honours its stop flag."""
+    reqs = requests(3)
+    ctx = api.Context(0)
+    a = [serve.start_solve(r, ctx) for r in reqs]
+    b = [serve.start_solve(r, ctx) for r in reqs]
+    ga = api.OGroup([d for d, _ in a], ctx)
+    gb = api.OGroup([d for d, _ in b], ctx)
+    stop = ct.c_int(1)
+    assert ga.run(5, stop) == (0, 0)          # stop set: no iteration
+    stop.value = 0
+    runs = 0
+    while not all(d.orbopt_done() for d, _ in a):
+        steps, _ = ga.run(4, stop)
+        assert 1 <= steps <= 4
+        runs += 1
+    while not all(d.orbopt_done() for d, _ in b):
+        gb.step()
+    ga.close()
+    gb.close()
+    for (da, pa), (db, pb) in zip(a, b):
+        assert da.orbopt_finish(pa) == db.orbopt_finish(pb)          # bitwise the same
+        assert np.array_equal(da.get_orbitals(), db.get_orbitals())
+        da.close()
+        db.close()
+    ctx.close()
+    assert runs >= 1
 
 
 @pytest.mark.parametrize("mode", ["grouped", "streams"])

@@ -43,6 +43,17 @@ class FakeGroup:
             s.steps += 1
         return len(self.systems)
 
+    def run(self, max_steps, stop=None):
+        # dn_ogroup_run: until a system finishes, the stop flag is set or max_steps ran
+        n = 0
+        while n < max_steps and not (stop is not None and stop.value):
+            before = sum(s.orbopt_done() for s in self.systems)
+            self.step()
+            n += 1
+            if sum(s.orbopt_done() for s in self.systems) != before:
+                break
+        return n, len(self.systems)
+
     def close(self):
         pass
 
