@@ -740,8 +740,10 @@ int dn_group_tune(dn_group* G, int* configs) {
 int dn_group_plan_text(dn_group* G, char* buf, int len) {
   if (!G || !buf || len <= 0) return fail(DN_ERR_ARG, "bad dn_group_plan_text arguments");
   static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64",
-                                "lds64x128", "lds128x128", "lds128x256"};
+                                "lds64x128", "lds128x128", "lds128x256", "lds64x192"};
   static const char* snames[] = {"s128x128", "s64x128", "s64x64", "s32x64"};
+  static_assert(sizeof(names) / sizeof(names[0]) == kNumGemmConfigs, "a name per config");
+  static_assert(sizeof(snames) / sizeof(snames[0]) == kNumSplitConfigs, "a name per config");
   static const char* stages[] = {"K1", "K2", "J", "HC"};
   std::string t;
   if (!G->chains.empty()) {
