@@ -225,6 +225,7 @@ class Group:
     STAGES = ("K1", "K2", "J", "HC")
     CONFIGS = {0: "128x128", 1: "64x64", 2: "64x128", 3: "128x64", 4: "64x32", 5: "32x64",
                6: "lds64x128", 7: "lds128x128", 8: "lds128x256", 9: "lds64x192",
+               10: "64x192",
                100: "s128x128", 101: "s64x128", 102: "s64x64", 103: "s32x64"}
 
     def __init__(self, systems, ctx=None):

@@ -181,6 +181,14 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
   }
   __syncthreads();
   const int fr = lane & 15, fk = lane >> 4;
+  // fragments dealt round-robin (fragment row i of wavefront row wm: tile rows
+  // 16 (i WGM + wm) ..; columns likewise) and those wholly past M or N skipped
+  // (wavefront-uniform): a tile cut short by the matrix edge keeps its SIMDs evenly busy
+  bool live_row[FM], live_col[FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) live_row[i] = m0 + 16 * (i * WGM + wm) < M;
+#pragma unroll
+  for (int j = 0; j < FN; ++j) live_col[j] = n0 + 16 * (j * WGN + wn) < N;
   for (int t = 0; t < T; ++t) {
     const int buf = t & 1;
     if (t + 1 < T) load_tile(t + 1);
@@ -188,14 +196,18 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
     for (int kk = 0; kk < BK; kk += 4) {
       double a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) a[i] = As[buf][LA::off(wm * WM + i * 16 + fr, kk + fk)];
+      for (int i = 0; i < FM; ++i) a[i] = As[buf][LA::off(16 * (i * WGM + wm) + fr, kk + fk)];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][LB::off(wn * WN + j * 16 + fr, kk + fk)];
+      for (int j = 0; j < FN; ++j) b[j] = Bs[buf][LB::off(16 * (j * WGN + wn) + fr, kk + fk)];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i) {
+        if (!live_row[i]) continue;
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
+        for (int j = 0; j < FN; ++j) {
+          if (!live_col[j]) continue;
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j], a[i], acc[i][j], 0, 0, 0);
+        }
+      }
     }
     if (t + 1 < T) store_tile(buf ^ 1, t + 1);
     __syncthreads();
@@ -209,13 +221,13 @@ __device__ __forceinline__ void gemm_tile(const Gemm& g, int mt, int nt, int z, 
   double* Cz = g.C + (long)z * g.sC;
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
-    const int row = m0 + wm * WM + i * 16 + fr;
+    const int row = m0 + 16 * (i * WGM + wm) + fr;
     if (row >= M) continue;
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int col = n0 + wn * WN + j * 16 + fk + 4 * rr;
+        const int col = n0 + 16 * (j * WGN + wn) + fk + 4 * rr;
         if (col < N) {
           double* c = Cz + row + (long)col * g.ldc;
           const double v = g.alpha * acc[i][j][rr];
@@ -594,6 +606,7 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   if (g.tri == kTriA) {
     // 64-row blocks (the K loop of row block m0 covers k >= m0 only): the short, uneven
     // K loops balance best over small workgroups (measured on MI355X at nbf = 400, 800)
+    if (g.N > 128 && g.N <= 192) return dispatch_vec<64, 192, 16, 2, 4, true, true, kTriA>(s, g);
     return dispatch_vec<64, 128, 16, 2, 4, true, true, kTriA>(s, g);
   }
   if (g.tri) {
@@ -604,10 +617,18 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
     // nbf = 400 and 800 against 128 x 64 and 128 x 128 tiles)
     return dispatch_vec<64, 64, 16, 2, 2, true, true, kTriB>(s, g);
   }
-  // Operands larger than 64 in both M and N get the 128 x 128 tile (the RI transforms
-  // have M = nbf5, N = nbf5 or nbf); everything else uses 64 x 64.
-  if (g.M > 64 && g.N > 64) return dispatch_layout<128, 128, 16, 2, 4>(s, g);
-  return dispatch_layout<64, 64, 16, 2, 2>(s, g);
+  switch (reg_config(g.M, g.N)) {
+    case kTile64x192: return dispatch_layout<64, 192, 16, 2, 4>(s, g);
+    case kTile64x128: return dispatch_layout<64, 128, 16, 2, 4>(s, g);
+    case kTile128x128: return dispatch_layout<128, 128, 16, 2, 4>(s, g);
+    default: return dispatch_layout<64, 64, 16, 2, 2>(s, g);
+  }
+}
+
+int reg_config(int M, int N) {
+  if (M <= 64 || N <= 64) return kTile64x64;
+  if (N <= 192 && N > 128) return kTile64x192;
+  return (M % 128 != 0 && M % 128 <= 64) ? kTile64x128 : kTile128x128;
 }
 
 namespace {
@@ -627,6 +648,7 @@ void config_tile(int config, int* bm, int* bn) {
     case kLds64x128: *bm = 64; *bn = 128; break;
     case kLds128x256: *bm = 128; *bn = 256; break;
     case kLds64x192: *bm = 64; *bn = 192; break;
+    case kTile64x192: *bm = 64; *bn = 192; break;
     default: *bm = 128; *bn = 128; break;  // kTile128x128, kLds128x128
   }
 }
@@ -704,7 +726,7 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     vb = vb && dh[i].g.vec_b;
   }
   const long slots = grouped_slots(dh, n);
-  if (config >= kLds64x128) {
+  if (is_lds_config(config)) {
     if (!ak || !bk || !va || !vb || tri == kTriB)
       return fail(DN_ERR_ARG, "grouped GEMM: LDS-direct configurations need K-contiguous, "
                               "16-byte aligned A and B");
@@ -721,6 +743,7 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
     case kTile128x64: return grouped_layout<128, 64, 16, 2, 2>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
     case kTile64x32: return grouped_layout<64, 32, 16, 2, 1>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
     case kTile32x64: return grouped_layout<32, 64, 16, 1, 2>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
+    case kTile64x192: return grouped_layout<64, 192, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
     default: return grouped_layout<128, 128, 16, 2, 4>(s, ak, bk, va, vb, tri, dd, n, total, slots, grid);
   }
 }
@@ -733,7 +756,7 @@ int lds64_config(int N) {
 
 bool gemm_config_ok(int config, const Gemm& g) {
   if (config < 0 || config >= kNumGemmConfigs) return false;
-  if (config < kLds64x128) return g.tri == kTriNone || g.tri == kTriA;
+  if (!is_lds_config(config)) return g.tri == kTriNone || g.tri == kTriA;
   return g.a_kcontig && g.b_kcontig && g.tri != kTriB &&
          vec_ok(g.A, g.lda, g.sA, g.nbatch, g.K) && vec_ok(g.B, g.ldb, g.sB, g.nbatch, g.K);
 }

@@ -224,7 +224,8 @@ int default_config(int stage, int bucket, const Gemm& g) {
     return g.nbatch > 1 ? kLds128x128 : kLds128x256;
   }
   if (stage == kK2 && bucket >= 2) return kTile128x128;
-  return stage == kK1 ? kTile64x128 : kTile64x64;
+  if (stage == kK1) return g.N > 128 && g.N <= 192 ? kTile64x192 : kTile64x128;
+  return kTile64x64;
 }
 
 // Classes of one stage with their default tile configurations; descriptors uploaded.
@@ -740,7 +741,8 @@ int dn_group_tune(dn_group* G, int* configs) {
 int dn_group_plan_text(dn_group* G, char* buf, int len) {
   if (!G || !buf || len <= 0) return fail(DN_ERR_ARG, "bad dn_group_plan_text arguments");
   static const char* names[] = {"128x128", "64x64", "64x128", "128x64", "64x32", "32x64",
-                                "lds64x128", "lds128x128", "lds128x256", "lds64x192"};
+                                "lds64x128", "lds128x128", "lds128x256", "lds64x192",
+                                "64x192"};
   static const char* snames[] = {"s128x128", "s64x128", "s64x64", "s32x64"};
   static_assert(sizeof(names) / sizeof(names[0]) == kNumGemmConfigs, "a name per config");
   static_assert(sizeof(snames) / sizeof(snames[0]) == kNumSplitConfigs, "a name per config");
