@@ -617,7 +617,7 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
     // nbf = 400 and 800 against 128 x 64 and 128 x 128 tiles)
     return dispatch_vec<64, 64, 16, 2, 2, true, true, kTriB>(s, g);
   }
-  switch (reg_config(g.M, g.N)) {
+  switch (reg_config(g.M, g.N, (g.nbatch + g.bps - 1) / g.bps)) {
     case kTile64x192: return dispatch_layout<64, 192, 16, 2, 4>(s, g);
     case kTile64x128: return dispatch_layout<64, 128, 16, 2, 4>(s, g);
     case kTile128x128: return dispatch_layout<128, 128, 16, 2, 4>(s, g);
@@ -625,10 +625,18 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   }
 }
 
-int reg_config(int M, int N) {
+int reg_config(int M, int N, long nsplit) {
   if (M <= 64 || N <= 64) return kTile64x64;
-  if (N <= 192 && N > 128) return kTile64x192;
-  return (M % 128 != 0 && M % 128 <= 64) ? kTile64x128 : kTile128x128;
+  // the shapes fitted to the edges only where the launch has tiles to spare: a GEMM with
+  // few output tiles (split-K, small squares) needs the parallelism of smaller tiles
+  constexpr long kMany = 1024;
+  const long rows64 = (M + 63) / 64;
+  if (N > 128 && N <= 192 && rows64 * nsplit >= kMany) return kTile64x192;
+  if (M % 128 != 0 && M % 128 <= 64 && rows64 * ((N + 127) / 128) * nsplit >= kMany)
+    return kTile64x128;
+  // fewer 128 x 128 tiles than two per CU: four times as many 64 x 64 ones
+  if ((long)((M + 127) / 128) * ((N + 127) / 128) * nsplit < 512) return kTile64x64;
+  return kTile128x128;
 }
 
 namespace {

@@ -172,10 +172,11 @@ enum GemmConfig {
   kNumGemmConfigs = 11
 };
 inline bool is_lds_config(int c) { return c >= kLds64x128 && c <= kLds64x192; }
-// The register-staged configuration for an M x N problem: 64 x 192 for N in (128, 192],
-// 64-row tiles when the last 128-row block would be less than half used, else 128 x 128;
-// 64 x 64 when M or N is at most 64 (dn_gemm.hip).
-int reg_config(int M, int N);
+// The register-staged configuration for an M x N problem in nsplit batches or splits:
+// 64 x 64 when M or N is at most 64; with at least 1024 tiles to run, 64 x 192 for N in
+// (128, 192] and 64-row tiles when the last 128-row block would be less than half used;
+// 64 x 64 when 128 x 128 would leave fewer than 512 tiles; else 128 x 128 (dn_gemm.hip).
+int reg_config(int M, int N, long nsplit);
 // The 64-row LDS-direct configuration for N columns: 192-wide tiles for N in (128, 192]
 // and for long N (>= 1024), 128-wide otherwise (dn_gemm.hip).
 int lds64_config(int N);

@@ -157,11 +157,13 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
   switch (stage) {
     case kT1:  // bpn = b_Q C5 (integrals.jl:159 / utils.jl:19)
       g = t1_gemm(s, false);
-      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N) : reg_config(g.M, g.N);
+      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N)
+                                              : reg_config(g.M, g.N, g.nbatch / g.bps);
       break;
     case kT2:  // b_MO^T = C5^T b_Q C, (q, Q) flattened  (utils.jl:19-20)
       g = t2_lag_gemm(s);
-      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N) : reg_config(g.M, g.N);
+      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N)
+                                              : reg_config(g.M, g.N, g.nbatch / g.bps);
       break;
     case kHC:  // H C5
       g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
