@@ -14,8 +14,8 @@ int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux
                 double* K);
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
                double* Hcore);
-int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed);
-int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
+int synth_b(hipStream_t s, double* b, int nbf, int ld, int naux, uint64_t seed);
+int b_symmetry(hipStream_t s, const double* b, int nbf, int ld, int naux, unsigned long long* dev2,
                double* dmax, double* bmax);
 int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
           const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w);
@@ -26,7 +26,7 @@ int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nb
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
                 int nbf, int nbf5, double* out);
 // dn_split.hip
-int split_prepare_b(hipStream_t s, const double* b, int nbf, int naux, int ld8, float* scale,
+int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int ld8, float* scale,
                     float* inv_scale, _Float16* hi, _Float16* lo);
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
                     float* cscale, _Float16* hi, _Float16* lo);
@@ -66,7 +66,7 @@ bool dims_valid(const dn_dims& d, std::string* why) {
 
 // Element counts of every device buffer of a system.
 struct Layout {
-  size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bmat, xw, part, scal;
+  size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bpn, cp, bmat, xw, part, scal;
   size_t nv, dnm, nvar, part2;
   // split precision (0 when FP64): half counts of the b and C planes, float counts
   size_t bplane, cplane, f_big, f_bpq, f_naux;
@@ -76,7 +76,7 @@ struct Layout {
     return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 5 * f8(f_naux) + 1;
   }
   size_t total_bytes() const {
-    return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + 2 * big + bmat + 2 * xw +
+    return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + big + bpn + cp + bmat + 2 * xw +
                 part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2 +
                 split_doubles());
   }
@@ -96,8 +96,8 @@ size_t lag_splits(const dn_dims& d) {
 
 Layout layout_of(const dn_dims& d, int precision) {
   Layout L{};
-  const size_t nbf = d.nbf, nbf5 = d.nbf5, naux = d.nbfaux;
-  L.b = nbf * nbf * naux;
+  const size_t nbf = d.nbf, nbf5 = d.nbf5, naux = d.nbfaux, ld = padded_ld(d.nbf);
+  L.b = ld * nbf * naux;
   L.C = nbf * nbf;
   L.H = nbf * nbf;
   L.n = nbf5;
@@ -105,6 +105,8 @@ Layout layout_of(const dn_dims& d, int precision) {
   L.nbf2 = nbf * nbf;
   L.nbf_nbf5 = nbf * nbf5;
   L.big = nbf5 * nbf * naux;
+  L.bpn = ld * nbf5 * naux;
+  L.cp = ld != nbf ? ld * nbf : 0;
   L.bmat = nbf5 * nbf5 * naux;
   L.xw = nbf5 * naux;
   const size_t jk_part = 2 * 64 * L.nn;
@@ -160,11 +162,12 @@ void release(dn_sys* s) {
 
 Gemm t1_gemm(const dn_sys* s, bool sym) {
   const dn_dims& d = s->d;
+  const int P = s->ldb;  // contraction over the padded rows (the pads are zero)
   Gemm g;
-  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
-  g.A = s->b; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf; g.a_kcontig = true;
-  g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
-  g.C = s->bpn; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
+  g.M = d.nbf; g.N = d.nbf5; g.K = P;
+  g.A = s->b; g.lda = P; g.sA = (long)P * d.nbf; g.a_kcontig = true;
+  g.B = s->Cp; g.ldb = P; g.sB = 0; g.b_kcontig = true;
+  g.C = s->bpn; g.ldc = P; g.sC = (long)P * d.nbf5;
   g.nbatch = d.nbfaux; g.bps = 1;
   g.tri = sym ? kTriA : kTriNone;
   g.lds = true;
@@ -173,10 +176,11 @@ Gemm t1_gemm(const dn_sys* s, bool sym) {
 
 Gemm t2_flat_gemm(const dn_sys* s) {
   const dn_dims& d = s->d;
+  const int P = s->ldb;
   Gemm g;
-  g.M = d.nbf5; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
-  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-  g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
+  g.M = d.nbf5; g.N = d.nbf5 * d.nbfaux; g.K = P;
+  g.A = s->Cp; g.lda = P; g.a_kcontig = true;
+  g.B = s->bpn; g.ldb = P; g.b_kcontig = true;
   g.C = s->bt; g.ldc = d.nbf5;
   g.lds = true;
   return g;
@@ -184,10 +188,11 @@ Gemm t2_flat_gemm(const dn_sys* s) {
 
 Gemm t2_batched_gemm(const dn_sys* s, int ncols) {
   const dn_dims& d = s->d;
+  const int P = s->ldb;
   Gemm g;
-  g.M = d.nbf5; g.N = ncols; g.K = d.nbf;
-  g.A = s->bpn; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf5; g.a_kcontig = true;
-  g.B = s->C; g.ldb = d.nbf; g.sB = 0; g.b_kcontig = true;
+  g.M = d.nbf5; g.N = ncols; g.K = P;
+  g.A = s->bpn; g.lda = P; g.sA = (long)P * d.nbf5; g.a_kcontig = true;
+  g.B = s->Cp; g.ldb = P; g.sB = 0; g.b_kcontig = true;
   g.C = s->bt; g.ldc = d.nbf5; g.sC = (long)d.nbf5 * ncols;
   g.nbatch = d.nbfaux; g.bps = 1;
   g.lds = true;
@@ -196,10 +201,11 @@ Gemm t2_batched_gemm(const dn_sys* s, int ncols) {
 
 Gemm t2_lag_gemm(const dn_sys* s) {
   const dn_dims& d = s->d;
+  const int P = s->ldb;
   Gemm g;
-  g.M = d.nbf; g.N = d.nbf5 * d.nbfaux; g.K = d.nbf;
-  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
-  g.B = s->bpn; g.ldb = d.nbf; g.b_kcontig = true;
+  g.M = d.nbf; g.N = d.nbf5 * d.nbfaux; g.K = P;
+  g.A = s->Cp; g.lda = P; g.a_kcontig = true;
+  g.B = s->bpn; g.ldb = P; g.b_kcontig = true;
   g.C = s->bt; g.ldc = d.nbf;
   g.lds = true;
   return g;
@@ -218,6 +224,11 @@ Gemm lag_contract_gemm(const dn_sys* s) {
   g.nbatch = d.nbfaux;
   g.bps = (d.nbfaux + nsplit - 1) / nsplit;
   return g;
+}
+
+int pad_c(dn_sys* s) {
+  if (s->Cp == s->C) return DN_OK;
+  return pad_c_launch(s->stream, s->C, s->Cp, s->d.nbf, s->ldb);
 }
 
 namespace {
@@ -263,11 +274,11 @@ int check_sys(dn_sys* s) {
 int prepare_b(dn_sys* s) {
   // b[m,n,Q] = b[n,m,Q] (to 2^-50 of max|b|) lets the J/K build read half of b
   double dmax = 0.0, bmax = 0.0;
-  DN_TRY(b_symmetry(s->stream, s->b, s->d.nbf, s->d.nbfaux,
+  DN_TRY(b_symmetry(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux,
                     reinterpret_cast<unsigned long long*>(s->scal + 62), &dmax, &bmax));
   s->b_sym = dmax <= std::ldexp(bmax, -50);
   if (s->precision != DN_SPLIT_F16) return DN_OK;
-  return split_prepare_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, s->ld8, s->bscale, s->binv,
+  return split_prepare_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->ld8, s->bscale, s->binv,
                          s->bhi, s->blo);
 }
 
@@ -324,6 +335,7 @@ int jkh_resident(dn_sys* s) {
   const dn_dims& d = s->d;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
   if (s->precision == DN_SPLIT_F16) return jkh_split(s);
+  DN_TRY(pad_c(s));
   if (s->b_sym && s->use_sym) {
     DN_TRY(transform1(s, true));
     DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));
@@ -348,6 +360,7 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   if (nbeta < 0 || nalpha < 0 || nbeta > d.nbf5 || nalpha > d.nbf5)
     return fail(DN_ERR_ARG, "nalpha/nbeta out of range");
   const long slice = (long)d.nbf5 * d.nbf;
+  DN_TRY(pad_c(s));
   DN_TRY(transform1(s, false));
   DN_TRY(gemm_f64(s->stream, t2_lag_gemm(s)));
   DN_TRY(hc_products(s, true, false));
@@ -562,8 +575,11 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   auto A = [&](size_t n, double** p) {
     if (rc == DN_OK) rc = alloc(s, n, p);
   };
+  s->ldb = padded_ld(dims->nbf);
   A(L.b, &s->b);
   A(L.C, &s->C);
+  if (L.cp) A(L.cp, &s->Cp);
+  else s->Cp = s->C;
   A(L.H, &s->H);
   A(L.n, &s->n);
   A(L.nn, &s->cj12);
@@ -574,7 +590,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf2, &s->elag);
   A(L.nbf_nbf5, &s->Hmat);
   A(L.nbf_nbf5, &s->HC);
-  A(L.big, &s->bpn);
+  A(L.bpn, &s->bpn);
   A(L.big, &s->bt);
   A(L.bmat, &s->bmat);
   A(L.xw, &s->X);
@@ -626,6 +642,13 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     }
   }
   s->part_elems = L.part;
+  if (rc == DN_OK && s->ldb != dims->nbf) {  // the pad rows of b, bpn and Cp read as zero
+    hipError_t e1 = hipMemsetAsync(s->b, 0, L.b * sizeof(double), s->stream);
+    hipError_t e2 = hipMemsetAsync(s->bpn, 0, L.bpn * sizeof(double), s->stream);
+    hipError_t e3 = hipMemsetAsync(s->Cp, 0, L.cp * sizeof(double), s->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+      rc = fail(DN_ERR_HIP, "hipMemset of the padded buffers");
+  }
   if (rc == DN_OK) {  // descriptor of the symmetric build's K/X pass, kept in scal[16..31]
     static_assert(sizeof(SymKDesc) <= 16 * sizeof(double), "SymKDesc fits scal[16..31]");
     SymKDesc k;
@@ -682,8 +705,12 @@ int dn_sys_destroy(dn_sys* s) {
 
 int dn_sys_upload_b(dn_sys* s, const double* b) {
   DN_TRY(check_sys(s));
-  const size_t n = (size_t)s->d.nbf * s->d.nbf * s->d.nbfaux;
-  DN_TRY(h2d(s, s->b, b, n));
+  if (!b) return fail(DN_ERR_ARG, "null host pointer");
+  const size_t nbf = s->d.nbf;
+  // into rows of ldb doubles (the pad rows keep their zeros)
+  DN_HIP(hipMemcpy2DAsync(s->b, s->ldb * sizeof(double), b, nbf * sizeof(double),
+                          nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyHostToDevice,
+                          s->stream));
   DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;
@@ -693,15 +720,17 @@ int dn_sys_upload_b(dn_sys* s, const double* b) {
 int dn_sys_download_b(dn_sys* s, double* b) {
   DN_TRY(check_sys(s));
   if (!b) return fail(DN_ERR_ARG, "null host pointer");
-  const size_t n = (size_t)s->d.nbf * s->d.nbf * s->d.nbfaux;
-  DN_TRY(d2h(s, b, s->b, n));
+  const size_t nbf = s->d.nbf;
+  DN_HIP(hipMemcpy2DAsync(b, nbf * sizeof(double), s->b, s->ldb * sizeof(double),
+                          nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyDeviceToHost,
+                          s->stream));
   DN_HIP(hipStreamSynchronize(s->stream));
   return DN_OK;
 }
 
 int dn_sys_generate_b(dn_sys* s, uint64_t seed) {
   DN_TRY(check_sys(s));
-  DN_TRY(synth_b(s->stream, s->b, s->d.nbf, s->d.nbfaux, seed));
+  DN_TRY(synth_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, seed));
   DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;

@@ -586,8 +586,9 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
   };
   g.vec_a = aligned(g.A, g.lda, g.sA, g.nbatch, g.a_kcontig ? g.K : g.M);
   g.vec_b = aligned(g.B, g.ldb, g.sB, g.nbatch, g.b_kcontig ? g.K : g.N);
-  if (g.tri == kTriA && (!g.a_kcontig || !g.b_kcontig || g.M != g.K))
-    return fail(DN_ERR_ARG, "gemm_f64: symmetric-A mode needs K-contiguous A and B, M == K");
+  // (K may exceed M by zero pad rows of A's slabs)
+  if (g.tri == kTriA && (!g.a_kcontig || !g.b_kcontig || g.M > g.K))
+    return fail(DN_ERR_ARG, "gemm_f64: symmetric-A mode needs K-contiguous A and B, M <= K");
   if (g.lds && g.a_kcontig && g.b_kcontig && g.vec_a && g.vec_b && g.tri != kTriB) {
     // 8 wavefronts per workgroup (2 per SIMD) beat 4 with twice the wave tile on every
     // transform shape measured on MI355X; 64-row blocks for the symmetric-A mode (short,

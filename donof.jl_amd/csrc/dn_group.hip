@@ -139,6 +139,10 @@ struct dn_group {
   // another's second transform and K/X as independent streams would, while every chain
   // still builds its systems with grouped launches.
   std::vector<dn_group*> chains;
+  // FP64 systems with odd nbf: C copied into its padded form before the transforms
+  PadDesc* pad = nullptr;
+  int npad = 0;
+  long pad_blocks = 0;
   // Problems of a stage sorted into classes by operand layout and alignment (16-byte
   // loads allowed for A, for B) and by nbf5 (<= 32, <= 64, <= 128, larger): one
   // persistent launch per class, each with the tile shape the autotuner found fastest
@@ -387,6 +391,7 @@ int launch_build(dn_group* G) {
   // split groups: C5 planes for the current orbitals and cleared per-Q maxima, then the
   // per-Q scales of the first transform's output, each one launch for all systems
   if (split) DN_TRY(split_group_prep(G->stream, G->sprep, n, G->prep_blocks));
+  if (!split) DN_TRY(pad_c_group(G->stream, G->pad, G->npad, G->pad_blocks));
   DN_TRY(launch_stage(G, kK1));
   if (split) DN_TRY(split_group_scales(G->stream, G->sprep, n, G->q_blocks));
   DN_TRY(launch_stage(G, kK2));
@@ -550,6 +555,18 @@ static int group_create_leaf(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_g
   if (e == hipSuccess) e = hipMalloc(&G->kx, nsys * sizeof(KxDesc));
   if (e == hipSuccess) e = hipMalloc(&G->diag, nsys * sizeof(DiagDesc));
   if (e == hipSuccess) e = hipMalloc(&G->symk, nsys * sizeof(SymKDesc));
+  if (e == hipSuccess && G->precision == DN_FP64) {
+    std::vector<PadDesc> pd;
+    for (dn_sys* s : G->sys)
+      if (s->Cp != s->C) pd.push_back(PadDesc{s->C, s->Cp, s->d.nbf, s->ldb, 0});
+    G->npad = (int)pd.size();
+    if (G->npad) {
+      G->pad_blocks = pad_plan(pd.data(), G->npad);
+      e = hipMalloc(&G->pad, pd.size() * sizeof(PadDesc));
+      if (e == hipSuccess)
+        e = hipMemcpy(G->pad, pd.data(), pd.size() * sizeof(PadDesc), hipMemcpyHostToDevice);
+    }
+  }
   for (int k = 0; k < 2 && e == hipSuccess && G->precision == DN_SPLIT_F16; ++k)
     e = hipMalloc(&G->sdev[k], nsys * sizeof(GroupedSplit));
   if (e == hipSuccess && G->precision == DN_SPLIT_F16) {
@@ -650,6 +667,7 @@ int dn_group_destroy(dn_group* G) {
   if (G->kx) (void)hipFree(G->kx);
   if (G->diag) (void)hipFree(G->diag);
   if (G->symk) (void)hipFree(G->symk);
+  if (G->pad) (void)hipFree(G->pad);
   for (int k = 0; k < 2; ++k)
     if (G->sdev[k]) (void)hipFree(G->sdev[k]);
   if (G->sprep) (void)hipFree(G->sprep);

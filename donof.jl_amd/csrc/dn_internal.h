@@ -225,6 +225,19 @@ struct DevBuf {
 }  // namespace dn
 
 namespace dn {
+inline int padded_ld(int nbf) { return nbf + (nbf & 1); }
+// padded copies C -> Cp (dn_jk.hip): one system, or every system of a group in one launch
+struct PadDesc {
+  const double* C;
+  double* Cp;
+  int nbf, ld;
+  long begin;  // first block (pad_plan)
+};
+int pad_c_launch(hipStream_t s, const double* C, double* Cp, int nbf, int ld);
+long pad_plan(PadDesc* d, int n);
+int pad_c_group(hipStream_t s, const PadDesc* d, int n, long blocks);
+// Cp = C with the zero pad row (a no-op when ldb == nbf), before the transforms read it
+int pad_c(dn_sys* s);
 // the transforms of b as GEMMs (dn_api.hip): t1 bpn[n,p,Q] = (b_Q^T C5)[n,p] (Lh_Q^T C5
 // with sym), t2 flat bt[q,(p,Q)] = (C5^T bpn)[q,(p,Q)], t2 batched
 // bt[q,a,Q] = (bpn_Q^T C)[q,a] for a < ncols
@@ -310,8 +323,14 @@ struct dn_sys {
   std::vector<dn::DevBuf> owned;
   uint64_t bytes = 0;
   // resident state (column-major)
-  double* b = nullptr;      // nbf*nbf*naux
+  // b[m,n,Q] at m + ldb (n + nbf Q); ldb = nbf rounded up to even, the pad row zero, so
+  // that every row of b (and of bpn, Cp) starts 16-byte aligned and the transforms take
+  // the LDS-direct tiles for odd nbf too (contracting over ldb: the pads add zeros)
+  int ldb = 0;
+  double* b = nullptr;      // ldb*nbf*naux
   double* C = nullptr;      // nbf*nbf
+  double* Cp = nullptr;     // ldb*nbf: C with the zero pad row (C itself when ldb == nbf),
+                            // refreshed from C before the transforms (pad_c)
   double* H = nullptr;      // nbf*nbf
   double* n = nullptr;      // nbf5
   double* cj12 = nullptr;   // nbf5*nbf5
@@ -322,7 +341,8 @@ struct dn_sys {
   double* elag = nullptr;   // nbf*nbf
   double* Hmat = nullptr;   // nbf*nbf5
   // workspaces
-  double* bpn = nullptr;    // nbf5*nbf*naux : b_pn[p,n,Q] = sum_m C5[m,p] b[m,n,Q]
+  double* bpn = nullptr;    // ldb*nbf5*naux : [n,p,Q] = (b_Q^T C5)[n,p] (Lh^T C5 symmetric);
+                            //                 pad row zero
   double* bt = nullptr;     // nbf5*nbf*naux : b_MO^T[q,a,Q]; also b_pq for the J/K build
   double* bmat = nullptr;   // nbf5*nbf5*naux: C^K/C^J right operand of the Lagrangian GEMM
   double* X = nullptr;      // nbf5*naux     : b_pp diagonal per Q

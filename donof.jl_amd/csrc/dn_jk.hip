@@ -215,7 +215,8 @@ int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks
 // out[0] = max |b[m,n,Q] - b[n,m,Q]|, out[1] = max |b| (as the bit patterns of
 // non-negative doubles, which order like the values).
 __global__ __launch_bounds__(256) void sym_check_kernel(const double* __restrict__ b, int nbf,
-                                                        int naux, int ntile, int q_per_block,
+                                                        int ld, int naux, int ntile,
+                                                        int q_per_block,
                                                         unsigned long long* __restrict__ out) {
   // workgroup: 32 x 32 tile pair (I, J), I <= J, of a range of slabs; both tiles staged in
   // LDS so that b[m,n] and b[n,m] are each read once, coalesced
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void sym_check_kernel(const double* __restrict
     ++I;
   }
   const int J = I + pidx;
-  const long nn = (long)nbf * nbf;
+  const long nn = (long)ld * nbf;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int mI = I * 32 + tx, mJ = J * 32 + tx;
   double dmax = 0.0, bmax = 0.0;
@@ -235,8 +236,8 @@ __global__ __launch_bounds__(256) void sym_check_kernel(const double* __restrict
     const double* s = b + (long)Q * nn;
     for (int r = ty; r < 32; r += 8) {
       const int nJ = J * 32 + r, nI = I * 32 + r;
-      a[tx][r] = (mI < nbf && nJ < nbf) ? s[mI + (long)nJ * nbf] : 0.0;  // tile (I, J)
-      t[tx][r] = (mJ < nbf && nI < nbf) ? s[mJ + (long)nI * nbf] : 0.0;  // tile (J, I)
+      a[tx][r] = (mI < nbf && nJ < nbf) ? s[mI + (long)nJ * ld] : 0.0;  // tile (I, J)
+      t[tx][r] = (mJ < nbf && nI < nbf) ? s[mJ + (long)nI * ld] : 0.0;  // tile (J, I)
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
@@ -261,15 +262,15 @@ __global__ void clear_u64_kernel(unsigned long long* p, int n) {
 }
 }  // namespace
 
-int b_symmetry(hipStream_t s, const double* b, int nbf, int naux, unsigned long long* dev2,
-               double* dmax, double* bmax) {
+int b_symmetry(hipStream_t s, const double* b, int nbf, int ld, int naux,
+               unsigned long long* dev2, double* dmax, double* bmax) {
   hipLaunchKernelGGL(clear_u64_kernel, dim3(1), dim3(64), 0, s, dev2, 2);
   const int ntile = (nbf + 31) / 32;
   const int pairs = ntile * (ntile + 1) / 2;
   const int qpb = (int)std::max<long>(1, (long)naux * pairs / 8192);
   const int nq = (naux + qpb - 1) / qpb;
-  hipLaunchKernelGGL(sym_check_kernel, dim3(pairs, nq), dim3(256), 0, s, b, nbf, naux, ntile, qpb,
-                     dev2);
+  hipLaunchKernelGGL(sym_check_kernel, dim3(pairs, nq), dim3(256), 0, s, b, nbf, ld, naux, ntile,
+                     qpb, dev2);
   DN_HIP(hipGetLastError());
   unsigned long long h[2] = {0, 0};
   DN_HIP(hipMemcpyAsync(h, dev2, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -327,8 +328,8 @@ __device__ __forceinline__ double philox_normal(uint64_t e, uint32_t stream, uin
 }
 
 // b[m,n,Q] = (A_Q[m,n] + A_Q[n,m]) exp(-(Q+1)/400) / (2 sqrt(nbf)),
-// A_Q[m,n] = z(m + nbf n + nbf^2 Q) of stream 0.
-__global__ void synth_b_kernel(double* __restrict__ b, int nbf, int naux, uint64_t seed) {
+// A_Q[m,n] = z(m + nbf n + nbf^2 Q) of stream 0; stored at m + ld (n + nbf Q).
+__global__ void synth_b_kernel(double* __restrict__ b, int nbf, int ld, int naux, uint64_t seed) {
   const uint64_t nb2 = (uint64_t)nbf * nbf;
   const uint64_t total = nb2 * (uint64_t)naux;
   const double inv = 0.5 / sqrt((double)nbf);
@@ -339,12 +340,65 @@ __global__ void synth_b_kernel(double* __restrict__ b, int nbf, int naux, uint64
     const uint64_t m = r % nbf, n = r / nbf;
     const double a1 = philox_normal(m + nbf * n + nb2 * Q, 0u, seed);
     const double a2 = philox_normal(n + nbf * m + nb2 * Q, 0u, seed);
-    b[e] = (a1 + a2) * (exp(-(double)(Q + 1) / 400.0) * inv);
+    b[m + (uint64_t)ld * (n + (uint64_t)nbf * Q)] =
+        (a1 + a2) * (exp(-(double)(Q + 1) / 400.0) * inv);
   }
 }
 
-int synth_b(hipStream_t s, double* b, int nbf, int naux, uint64_t seed) {
-  hipLaunchKernelGGL(synth_b_kernel, dim3(16384), dim3(256), 0, s, b, nbf, naux, seed);
+int synth_b(hipStream_t s, double* b, int nbf, int ld, int naux, uint64_t seed) {
+  hipLaunchKernelGGL(synth_b_kernel, dim3(16384), dim3(256), 0, s, b, nbf, ld, naux, seed);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+// Cp[m,n] = C[m,n] for m < nbf, 0 for the pad row m = nbf (ld = nbf + 1)
+__device__ __forceinline__ double pad_element(long e, const double* __restrict__ C, int nbf,
+                                              int ld) {
+  const int m = (int)(e % ld);
+  const long n = e / ld;
+  return m < nbf ? C[m + (long)nbf * n] : 0.0;
+}
+
+__global__ void pad_c_kernel(const double* __restrict__ C, double* __restrict__ Cp, int nbf,
+                             int ld) {
+  const long total = (long)ld * nbf;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x)
+    Cp[e] = pad_element(e, C, nbf, ld);
+}
+
+__global__ __launch_bounds__(256) void pad_c_group_kernel(const PadDesc* __restrict__ d, int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].begin <= (long)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const PadDesc& r = d[__builtin_amdgcn_readfirstlane(lo)];
+  const long e = ((long)blockIdx.x - r.begin) * 256 + threadIdx.x;
+  if (e < (long)r.ld * r.nbf) r.Cp[e] = pad_element(e, r.C, r.nbf, r.ld);
+}
+
+int pad_c_launch(hipStream_t s, const double* C, double* Cp, int nbf, int ld) {
+  const long blocks = ((long)ld * nbf + 255) / 256;
+  hipLaunchKernelGGL(pad_c_kernel, dim3((unsigned)std::min(blocks, 4096L)), dim3(256), 0, s, C,
+                     Cp, nbf, ld);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+long pad_plan(PadDesc* d, int n) {
+  long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    d[i].begin = blocks;
+    blocks += ((long)d[i].ld * d[i].nbf + 255) / 256;
+  }
+  return blocks;
+}
+
+int pad_c_group(hipStream_t s, const PadDesc* d, int n, long blocks) {
+  if (n <= 0) return DN_OK;
+  hipLaunchKernelGGL(pad_c_group_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, n);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

@@ -160,7 +160,9 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   DN_TRY(check_sys(s));
   if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
   const int naux = s->d.nbfaux;
-  const long rows = (long)s->d.nbf * s->d.nbf;
+  // rows of b in its resident layout: ldb per column of each slab, the pad rows zero (the
+  // solve maps zero rows of I3c to zero rows of b)
+  const long rows = (long)s->ldb * s->d.nbf;
   const size_t nI = (size_t)rows * naux, nG = (size_t)naux * naux, nP = (size_t)naux * NB;
   const size_t bytes = (nI + nG + 2 * nP) * sizeof(double) + 256;
   if (s->ctx->budget && s->ctx->used + bytes > s->ctx->budget)
@@ -174,7 +176,10 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   int* flag = reinterpret_cast<int*>(T + nP);
   int rc = DN_OK, host_flag = 0;
   auto run = [&]() -> int {
-    DN_TRY(h2d(s, dI, I3c, nI));
+    if (s->ldb != s->d.nbf) DN_HIP(hipMemsetAsync(dI, 0, nI * sizeof(double), s->stream));
+    const size_t nbf = s->d.nbf;
+    DN_HIP(hipMemcpy2DAsync(dI, s->ldb * sizeof(double), I3c, nbf * sizeof(double),
+                            nbf * sizeof(double), nbf * naux, hipMemcpyHostToDevice, s->stream));
     DN_TRY(h2d(s, dG, G, nG));
     DN_HIP(hipMemsetAsync(flag, 0, sizeof(int), s->stream));
     DN_TRY(cholesky_f64(s->stream, dG, naux, Dinv, T, flag));

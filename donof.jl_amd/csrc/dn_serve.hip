@@ -218,6 +218,10 @@ struct dn_ogroup {
   // grouped Lagrangian tail (lag_w, lag_bmat, lag_init, the partial sum, energy, gradient)
   dn::LagSys* lag_d = nullptr;
   dn::LagBlocks lag_b;
+  // systems with odd nbf: C copied into its padded form before the transforms
+  dn::PadDesc* pad_d = nullptr;
+  int npad = 0;
+  long pad_blocks = 0;
 };
 
 namespace dn {
@@ -426,6 +430,7 @@ int launch_lstage(dn_ogroup* G, int stage) {
 
 // The Lagrangian, energy and packed gradient of every system (the graph body).
 int lagrangian_all(dn_ogroup* G) {
+  DN_TRY(pad_c_group(G->stream, G->pad_d, G->npad, G->pad_blocks));
   DN_TRY(launch_lstage(G, kT1));
   DN_TRY(launch_lstage(G, kT2));
   DN_TRY(launch_lstage(G, kHC));
@@ -472,6 +477,14 @@ int plan_lag_tail(dn_ogroup* G) {
   }
   lag_group_plan(h.data(), n, &G->lag_b);
   DN_HIP(hipMemcpy(G->lag_d, h.data(), n * sizeof(LagSys), hipMemcpyHostToDevice));
+  std::vector<PadDesc> pd;
+  for (dn_sys* s : G->sys)
+    if (s->Cp != s->C) pd.push_back(PadDesc{s->C, s->Cp, s->d.nbf, s->ldb, 0});
+  G->npad = (int)pd.size();
+  if (G->npad) {
+    G->pad_blocks = pad_plan(pd.data(), G->npad);
+    DN_HIP(hipMemcpy(G->pad_d, pd.data(), pd.size() * sizeof(PadDesc), hipMemcpyHostToDevice));
+  }
   return DN_OK;
 }
 
@@ -579,6 +592,7 @@ int dn_ogroup_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_ogroup** 
     G->rot_bytes = RotLayout(nsys).bytes;
     D(G->rot_bytes, reinterpret_cast<void**>(&G->rot_d));
     D(nsys * sizeof(LagSys), reinterpret_cast<void**>(&G->lag_d));
+    D(nsys * sizeof(PadDesc), reinterpret_cast<void**>(&G->pad_d));
     if (rc == DN_OK)
       rc = pinned_get(ctx, 2 * nsys * sizeof(double), reinterpret_cast<void**>(&G->pinned),
                       &G->pinned_bytes);
@@ -616,6 +630,7 @@ int dn_ogroup_destroy(dn_ogroup* G) {
     dev_free(G->ctx, G->stream, G->gathered);
     dev_free(G->ctx, G->stream, G->rot_d);
     dev_free(G->ctx, G->stream, G->lag_d);
+    dev_free(G->ctx, G->stream, G->pad_d);
     (void)hipStreamSynchronize(G->stream);
   }
   pinned_put(G->ctx, G->pinned, G->pinned_bytes);

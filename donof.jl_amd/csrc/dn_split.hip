@@ -78,16 +78,16 @@ __global__ void b_slab_max_kernel(const double* __restrict__ b, long slab, int n
   }
 }
 
-// b planes: hi/lo of b * scale[Q] as [(n,Q)][m] with leading dimension ld8 >= nbf
-// (16-byte aligned rows; the padding stays zero)
-__global__ void b_split_kernel(const double* __restrict__ b, int nbf, long slab, long total,
+// b planes: hi/lo of b * scale[Q] as [(n,Q)][m] with leading dimension ld8 >= ld >= nbf
+// (16-byte aligned rows; the padding stays zero: b's own pad row ld > nbf holds zeros)
+__global__ void b_split_kernel(const double* __restrict__ b, int ld, long slab, long total,
                                int ld8, const float* __restrict__ scale,
                                _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
     const double x = b[e] * (double)scale[e / slab];
-    const long col = e / nbf;
-    const long o = col * ld8 + (e - col * nbf);
+    const long col = e / ld;
+    const long o = col * ld8 + (e - col * ld);
     split_f16_d(x, &hi[o], &lo[o]);
   }
 }
@@ -854,13 +854,13 @@ static unsigned grid_of(long n) {
   return (unsigned)std::max<long>(1, std::min<long>(b, 8192));
 }
 
-int split_prepare_b(hipStream_t s, const double* b, int nbf, int naux, int ld8, float* scale,
-                    float* inv_scale, _Float16* hi, _Float16* lo) {
-  const long slab = (long)nbf * nbf;
+int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int ld8,
+                    float* scale, float* inv_scale, _Float16* hi, _Float16* lo) {
+  const long slab = (long)ld * nbf;  // pad rows hold zeros: they change no maximum
   hipLaunchKernelGGL(b_slab_max_kernel, dim3(naux), dim3(256), 0, s, b, slab, naux, scale,
                      inv_scale);
   DN_HIP(hipGetLastError());
-  hipLaunchKernelGGL(b_split_kernel, dim3(grid_of(slab * naux)), dim3(256), 0, s, b, nbf, slab,
+  hipLaunchKernelGGL(b_split_kernel, dim3(grid_of(slab * naux)), dim3(256), 0, s, b, ld, slab,
                      slab * naux, ld8, scale, hi, lo);
   DN_HIP(hipGetLastError());
   return DN_OK;
