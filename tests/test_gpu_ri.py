@@ -10,8 +10,9 @@ from donof_amd._lib import DnError
 
 pytestmark = pytest.mark.gpu
 
-# naux below, at and across the 64-wide Cholesky blocks; one with a ragged last block
-SHAPES = [(6, 20, 4), (12, 64, 8), (20, 150, 7), (16, 300, 9), (40, 513, 14)]
+# naux below, at and across the 64-wide Cholesky blocks; one with a ragged last block; odd
+# nbf (b resident with a padded leading dimension)
+SHAPES = [(6, 20, 4), (12, 64, 8), (20, 150, 7), (16, 300, 9), (40, 513, 14), (15, 130, 6)]
 
 
 @pytest.mark.parametrize("nbf,naux,nbf5", SHAPES)
