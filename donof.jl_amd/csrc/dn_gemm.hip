@@ -742,7 +742,9 @@ int gemm_f64_grouped(hipStream_t s, const GroupedGemm* dh, const GroupedGemm* dd
       return fail(DN_ERR_ARG, "grouped GEMM: LDS-direct configurations need K-contiguous, "
                               "16-byte aligned A and B");
     switch (config) {
-      case kLds64x128: return launch_grouped_lds<64, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
+      case kLds64x128:  // symmetric-A mode on a two-stage ring, as in gemm_f64
+        return tri == kTriA ? launch_grouped_lds<64, 128, 2, 4, 2>(s, tri, dd, n, total, slots, grid)
+                            : launch_grouped_lds<64, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
       case kLds128x128: return launch_grouped_lds<128, 128, 2, 4, 3>(s, tri, dd, n, total, slots, grid);
       case kLds64x192: return launch_grouped_lds<64, 192, 2, 4, 2>(s, tri, dd, n, total, slots, grid);
       default: return launch_grouped_lds<128, 256, 2, 4, 2>(s, tri, dd, n, total, slots, grid);
