@@ -603,7 +603,9 @@ int gemm_f64(hipStream_t s, const Gemm& g_in) {
       return g.tri == kTriA ? launch_lds<64, 128, 2, 4, 2, kTriA>(s, g)
                             : launch_lds<64, 128, 2, 4, 3, kTriNone>(s, g);
     }
-    return g.N > 128 ? launch_lds<128, 256, 2, 4, 2, kTriNone>(s, g)
+    // (4 x 2 wavefronts of 32 x 128 for the long-N tile: 352-365 against 361-375 us for
+    // 2 x 4 on the flattened second transform at 400/1600/120)
+    return g.N > 128 ? launch_lds<128, 256, 4, 2, 2, kTriNone>(s, g)
                      : launch_lds<128, 128, 2, 4, 3, kTriNone>(s, g);
   }
   if (g.tri == kTriA) {
