@@ -284,7 +284,7 @@ class Group:
 
 class OGroup:
     """Started systems advanced together, one iteration per step (dn_ogroup_*): the
-    Lagrangian of all of them runs as grouped persistent GEMMs in one HIP graph."""
+    Lagrangian of all of them runs as grouped GEMMs in one HIP graph."""
 
     def __init__(self, systems, ctx=None):
         self.systems = list(systems)

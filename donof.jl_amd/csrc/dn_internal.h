@@ -343,8 +343,10 @@ struct dn_sys {
   // workspaces
   double* bpn = nullptr;    // ldb*nbf5*naux : [n,p,Q] = (b_Q^T C5)[n,p] (Lh^T C5 symmetric);
                             //                 pad row zero
-  double* bt = nullptr;     // nbf5*nbf*naux : b_MO^T[q,a,Q]; also b_pq for the J/K build
-  double* bmat = nullptr;   // nbf5*nbf5*naux: C^K/C^J right operand of the Lagrangian GEMM
+  double* bt = nullptr;     // nbf5*nbf*naux : Lagrangian b_MO^T[q,a,Q] at a + nbf (q + nbf5 Q);
+                            //   J/K build: b_pq[q,p,Q] (general b) or W^T (symmetric b)
+  double* bmat = nullptr;   // nbf5*nbf5*naux: [q,b,Q] at b + nbf5 (q + nbf5 Q), the C^K/C^J
+                            //   right operand of the Lagrangian GEMM
   double* X = nullptr;      // nbf5*naux     : b_pp diagonal per Q
   double* w = nullptr;      // nbf5*naux     : Coulomb + C^J weights
   double* HC = nullptr;     // nbf*nbf5      : H C5

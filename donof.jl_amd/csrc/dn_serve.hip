@@ -1,6 +1,7 @@
 // Lock-step ADAM-family optimisation of several resident systems: every step advances all
 // live solves together, the orbital Lagrangian of all of them (the dominant work,
-// utils.jl:1-59) running as grouped persistent GEMMs captured once in a HIP graph.
+// utils.jl:1-59) running as grouped GEMMs (one workgroup per tile) captured once in a HIP
+// graph.
 //
 // Per system the recurrence is the one of the single-system loop (minimization.jl:344-438,
 // dn_orbopt_resident): the same convergence test, moments, step, rotation and best-energy

@@ -8,18 +8,20 @@ once) and runs the ADAM loop of minimization.jl:344-438 (maxloop = max_iter, sto
 2 max|g| < gtol).
 
 mode "grouped" (default): lock-step engines (one per nbf size class, each on its own
-thread and stream) advance their live solves together (dn_ogroup_step): per step every
-solve takes its ADAM step and rotation, and the orbital Lagrangians of all of them run as
-grouped persistent GEMMs in one captured HIP graph.  Admitted requests are set up by a
-preparer thread (on their own streams) while the engines keep stepping; when a solve
-finishes it leaves, and prepared requests join at the next step (the group is rebuilt).
-Size classes keep a small solve from advancing at the pace of a large one.  mode "streams": one host thread per live solve, each running the whole loop
-on its own HIP stream (dn_orbopt_resident).
+thread and stream) advance their live solves together (dn_ogroup_run, a native loop of
+dn_ogroup_step): per step every solve takes its ADAM step and rotation as grouped
+launches, and the orbital Lagrangians of all of them run as grouped GEMMs in one captured
+HIP graph.  Admitted requests are set up by a preparer thread (on their own streams)
+while the engines keep stepping; when a solve finishes it leaves, and prepared requests
+join at the next step (the group is rebuilt).  Size classes keep a small solve from
+advancing at the pace of a large one.  mode "streams": one host thread per live solve,
+each running the whole loop on its own HIP stream (dn_orbopt_resident).
 
-Every kernel of a solve is deterministic, reads only its own system's buffers, and the
-tile shapes of the grouped GEMMs depend only on that system's sizes, so a result does
-not depend on what else is running: tests check that each result is bitwise the same
-alone and in the mix.
+Every kernel of a solve is deterministic, reads only its own system's buffers, and each
+GEMM of a system uses one tile family whose summation order does not depend on the tile
+shape (split-K boundaries depend only on the system's sizes), so a result does not depend
+on what else is running: tests check that each result is bitwise the same alone and in
+the mix.
 """
 from __future__ import annotations
 

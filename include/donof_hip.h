@@ -191,7 +191,7 @@ int dn_orbopt_resident(dn_sys* sys, const dn_functional* f, dn_orbopt_state* st)
  * The loop of dn_orbopt_resident (orbopt_adam & co., minimization.jl:242-630) split into
  * start / steps / finish, so that several systems can advance together: a dn_ogroup
  * steps every started system by one iteration per dn_ogroup_step, the orbital Lagrangian
- * of all of them (utils.jl:1-59) running as grouped persistent GEMMs in one HIP graph.
+ * of all of them (utils.jl:1-59) running as grouped GEMMs in one HIP graph.
  * The reference has no grouped form (it optimises one system per call); per system the
  * recurrence and its decisions are the single-system loop's, with the same arithmetic.
  * dn_orbopt_start takes gamma, C and H already set on the system (dn_sys_set_gamma,
