@@ -461,10 +461,21 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_lds_kernel(Gemm g, int t
   const unsigned total = (unsigned)tiles_m * tiles_n * nsplit;
   const unsigned hw = blockIdx.x;
   if (hw >= total) return;
-  const unsigned lid = xcd_remap(hw, total);
-  const int mt = lid % tiles_m;
-  const int nt = (lid / tiles_m) % tiles_n;
-  const int z = lid / (tiles_m * tiles_n);
+  int mt, nt, z;
+  if constexpr (TRI == kTriA) {
+    // row block mt runs k tiles from its own row on: dispatch the longest first (row
+    // blocks in order, each over all (nt, z)), so the run ends on short tiles; blocks of
+    // one slab read disjoint rows of it, and consecutive workgroups go to different XCDs
+    const unsigned per = (unsigned)tiles_n * nsplit;
+    mt = hw / per;
+    nt = (hw % per) % tiles_n;
+    z = (hw % per) / tiles_n;
+  } else {
+    const unsigned lid = xcd_remap(hw, total);
+    mt = lid % tiles_m;
+    nt = (lid / tiles_m) % tiles_n;
+    z = lid / (tiles_m * tiles_n);
+  }
   gemm_tile_lds<BM, BN, WGM, WGN, NST, TRI>(g, mt, nt, z, smem);
 }
 
