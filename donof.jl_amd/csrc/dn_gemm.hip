@@ -462,14 +462,18 @@ __global__ __launch_bounds__(64 * WGM * WGN) void dgemm_lds_kernel(Gemm g, int t
   const unsigned hw = blockIdx.x;
   if (hw >= total) return;
   int mt, nt, z;
-  if constexpr (TRI == kTriA) {
-    // row block mt runs k tiles from its own row on: dispatch the longest first (row
-    // blocks in order, each over all (nt, z)), so the run ends on short tiles; blocks of
-    // one slab read disjoint rows of it, and consecutive workgroups go to different XCDs
-    const unsigned per = (unsigned)tiles_n * nsplit;
-    mt = hw / per;
-    nt = (hw % per) % tiles_n;
-    z = (hw % per) / tiles_n;
+  const unsigned pairs = (unsigned)tiles_m * nsplit;
+  if (TRI == kTriA && pairs % 8u == 0) {
+    // row block mt runs k tiles from its own row on: dispatch the longest first, so the
+    // run ends on short tiles.  Work items are the (mt, z) pairs in order of mt, dealt
+    // round-robin to the XCDs (workgroup hw runs on XCD hw % 8); the tiles_n tiles of a
+    // pair, which read the same rows of b, run back to back on one XCD and share its L2.
+    // (Blocks of one slab read disjoint rows of it: no reuse is lost between pairs.)
+    const unsigned k = hw / 8u, x = hw % 8u;
+    const unsigned p = 8u * (k / (unsigned)tiles_n) + x;
+    nt = k % (unsigned)tiles_n;
+    mt = p / (unsigned)nsplit;
+    z = p % (unsigned)nsplit;
   } else {
     const unsigned lid = xcd_remap(hw, total);
     mt = lid % tiles_m;
