@@ -76,7 +76,7 @@ struct Layout {
     return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 5 * f8(f_naux) + 1;
   }
   size_t total_bytes() const {
-    return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 2 + n + big + bpn + cp + bmat + 2 * xw +
+    return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 3 + n + big + bpn + cp + bmat + 2 * xw +
                 part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2 +
                 split_doubles());
   }
@@ -157,9 +157,7 @@ void release(dn_sys* s) {
 //       W_Q^T = (C5^T Lh_Q C5)^T after the symmetric t1 (C5^T b_Q C5 = W_Q + W_Q^T);
 //   t2 batched: bt[q,a,Q] = sum_n bpn[n,q,Q] C[n,a] for a < ncols, (C5^T b_Q C)[q,a] after
 //       the general t1 (slice stride nbf5 * ncols): C5^T b_Q C5 for ncols = nbf5;
-//   t2 Lagrangian: bt[q,a,Q] = (C5^T b_Q C)[q,a] stored at a + nbf (q + nbf5 Q), one GEMM
-//       sum_n C[n,a] bpn[n,(q,Q)] with (q, Q) flattened (no tile rounding of nbf5).
-// All three read A and B along K, the layout of the LDS-direct tile.
+// All read A and B along K, the layout of the LDS-direct tile.
 }  // namespace
 
 Gemm t1_gemm(const dn_sys* s, bool sym) {
@@ -201,30 +199,35 @@ Gemm t2_batched_gemm(const dn_sys* s, int ncols) {
   return g;
 }
 
-Gemm t2_lag_gemm(const dn_sys* s) {
-  const dn_dims& d = s->d;
-  const int P = s->ldb;
-  Gemm g;
-  g.M = d.nbf; g.N = d.nbf5 * d.nbfaux; g.K = P;
-  g.A = s->Cp; g.lda = P; g.a_kcontig = true;
-  g.B = s->bpn; g.ldb = P; g.b_kcontig = true;
-  g.C = s->bt; g.ldc = d.nbf;
-  g.lds = true;
-  return g;
-}
-
-// elag partials: part_z[a,b] = sum_{Q in split z} sum_q bt[q,a,Q] bmat[q,b,Q], with
-// bt[q,a,Q] at a + nbf (q + nbf5 Q) and bmat[q,b,Q] at b + nbf5 (q + nbf5 Q)
+// The Lagrangian's two-electron terms (utils.jl:19-55) without the full transform b_MO:
+// elag[a,b] += sum_{q,Q} b_MO^T[q,a,Q] bmat[q,b,Q] with b_MO^T[:,:,Q] = bpn_Q^T C, so
+//   Z[n,b]     = sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q]   (split partials over Q ranges),
+//   elag[a,b] += sum_n C[n,a] Z[n,b];
+// bmat needs b_MO only on its first nbf5 columns, the flattened second transform C5^T bpn
+// (t2_flat_gemm: b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)).  The full transform (nbf columns,
+// 2 nbf^2 nbf5 naux FLOP) becomes two of 2 nbf nbf5^2 naux.
 Gemm lag_contract_gemm(const dn_sys* s) {
   const dn_dims& d = s->d;
   const int nsplit = (int)lag_splits(d);
+  const int P = s->ldb;
   Gemm g;
   g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf5;
-  g.A = s->bt; g.lda = d.nbf; g.sA = (long)d.nbf * d.nbf5; g.a_kcontig = false;
+  g.A = s->bpn; g.lda = P; g.sA = (long)P * d.nbf5; g.a_kcontig = false;
   g.B = s->bmat; g.ldb = d.nbf5; g.sB = (long)d.nbf5 * d.nbf5; g.b_kcontig = false;
   g.C = s->part; g.ldc = d.nbf; g.sC = (long)d.nbf * d.nbf5;
   g.nbatch = d.nbfaux;
   g.bps = (d.nbfaux + nsplit - 1) / nsplit;
+  return g;
+}
+
+Gemm lag_final_gemm(const dn_sys* s) {
+  const dn_dims& d = s->d;
+  Gemm g;
+  g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
+  g.A = s->C; g.lda = d.nbf; g.a_kcontig = true;
+  g.B = s->Z; g.ldb = d.nbf; g.b_kcontig = true;
+  g.C = s->elag; g.ldc = d.nbf;
+  g.beta = 1.0;  // onto the one-electron term n[b] Hmat[a,b]
   return g;
 }
 
@@ -361,22 +364,23 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
   if (nbeta < 0 || nalpha < 0 || nbeta > d.nbf5 || nalpha > d.nbf5)
     return fail(DN_ERR_ARG, "nalpha/nbeta out of range");
-  const long slice = (long)d.nbf5 * d.nbf;
+  const long slice = (long)d.nbf5 * d.nbf5;
   DN_TRY(pad_c(s));
   DN_TRY(transform1(s, false));
-  DN_TRY(gemm_f64(s->stream, t2_lag_gemm(s)));
+  DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));  // b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)
   DN_TRY(hc_products(s, true, false));
-  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
+  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
                s->X, s->w));
-  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
+  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
   DN_TRY(lag_init(s->stream, s->Hmat, s->n, d.nbf, d.nbf5, s->elag));
-  // elag[a,b] += sum_Q sum_q bt[q,a,Q] bmat[q,b,Q]
+  // Z[n,b] = sum_Q sum_q bpn[n,q,Q] bmat[q,b,Q], then elag += C^T Z
   const Gemm g = lag_contract_gemm(s);
   const int used = (g.nbatch + g.bps - 1) / g.bps;
   if ((size_t)used * d.nbf * d.nbf5 > s->part_elems) return fail(DN_ERR_ARG, "partials overflow");
   DN_TRY(gemm_f64(s->stream, g));
   DN_TRY(reduce_splits(s->stream, s->part, (long)d.nbf * d.nbf5, used, d.nbf, d.nbf5, d.nbf,
-                       s->elag, d.nbf, 1.0));
+                       s->Z, d.nbf, 0.0));
+  DN_TRY(gemm_f64(s->stream, lag_final_gemm(s)));
   DN_TRY(energy_elec(s->stream, s->elag, s->Hmat, s->n, d.nbf, d.nbf5, s->scal));
   return DN_OK;
 }
@@ -592,6 +596,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf2, &s->elag);
   A(L.nbf_nbf5, &s->Hmat);
   A(L.nbf_nbf5, &s->HC);
+  A(L.nbf_nbf5, &s->Z);
   A(L.bpn, &s->bpn);
   A(L.big, &s->bt);
   A(L.bmat, &s->bmat);

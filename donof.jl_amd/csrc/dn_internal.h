@@ -244,10 +244,10 @@ int pad_c(dn_sys* s);
 Gemm t1_gemm(const dn_sys* s, bool sym);
 Gemm t2_flat_gemm(const dn_sys* s);
 Gemm t2_batched_gemm(const dn_sys* s, int ncols);
-// the Lagrangian's transform bt[q,a,Q] at a + nbf (q + nbf5 Q) and its contraction with
-// bmat into split partials (dn_api.hip)
-Gemm t2_lag_gemm(const dn_sys* s);
+// the Lagrangian's contraction Z[n,b] = sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q] in split partials
+// and elag += C^T Z (dn_api.hip)
 Gemm lag_contract_gemm(const dn_sys* s);
+Gemm lag_final_gemm(const dn_sys* s);
 // resident building blocks (dn_api.hip)
 int check_sys(dn_sys* s);
 int h2d(dn_sys* s, double* dst, const double* src, size_t elems);
@@ -265,8 +265,9 @@ namespace dn {
 // operands of lag_w, lag_bmat, lag_init, the reduction of the C^J/C^K partials into elag,
 // energy_elec and pack_grad, and block offsets filled by lag_group_plan.
 struct LagSys {
-  const double* bt;
-  long slice;
+  const double* bt;    // b_pq[q,p,Q] at p + ldt (q + nbf5 Q)
+  long slice;          // its slab stride (ldt nbf5)
+  int ldt;
   int nbf, nbf5, naux, nalpha, nbeta;
   const double* cj12;
   const double* ck12;
@@ -278,6 +279,7 @@ struct LagSys {
   double* elag;
   const double* part;  // nsplit partials of nbf x nbf5
   int nsplit;
+  double* Z;           // their sum (nbf x nbf5), contracted with C into elag afterwards
   double* scal;        // scal[0] = E, scal[1] = bits of max|g|
   double* g;           // packed gradient
   long w_begin, bmat_begin, sq_begin, rect_begin;
@@ -288,6 +290,8 @@ struct LagBlocks {
 };
 void lag_group_plan(LagSys* d, int n, LagBlocks* b);
 int lag_group_before(hipStream_t s, const LagSys* d, int n, const LagBlocks& b);
+// the partials summed into Z; then (after elag += C^T Z) energy, gmax and packed gradient
+int lag_group_reduce(hipStream_t s, const LagSys* d, int n, const LagBlocks& b);
 int lag_group_after(hipStream_t s, const LagSys* d, int n, const LagBlocks& b);
 }  // namespace dn
 
@@ -350,6 +354,7 @@ struct dn_sys {
   double* X = nullptr;      // nbf5*naux     : b_pp diagonal per Q
   double* w = nullptr;      // nbf5*naux     : Coulomb + C^J weights
   double* HC = nullptr;     // nbf*nbf5      : H C5
+  double* Z = nullptr;      // nbf*nbf5      : sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q] (Lagrangian)
   double* part = nullptr;   // split partials
   size_t part_elems = 0;
   double* scal = nullptr;   // small scalars (energy, norms)

@@ -10,9 +10,11 @@
 // or b >= nalpha).  The first line is the C^K term (54-55), w carries C^J (50-51) and the
 // Coulomb terms; the one-electron term n[b] Hmat[a,b] (39) is added afterwards.
 //
-// Layouts in HBM: bt[q,a,Q] at a + nbf q + nbf nbf5 Q (a fastest: the output of one flat
-// GEMM over (q, Q)) and bmat[q,b,Q] at b + nbf5 q + nbf5^2 Q, so that bmat is an
-// element-wise map of bt's first nbf5 rows with both read and written along b.
+// Only bt's first nbf5 columns enter bmat and w: b_pq[q,p,Q] = (C5^T b_Q C5)[q,p], stored at
+// p + nbf5 (q + nbf5 Q) by the flattened second transform; bmat[q,b,Q] is kept at the same
+// index (an element-wise map, read and written along b).  The contraction over (q, Q) then
+// runs against the first transform's output instead of the full b_MO (dn_api.hip):
+//   Z[n,b] = sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q],   elag[a,b] += sum_n C[n,a] Z[n,b].
 #include "dn_internal.h"
 #include "dn_rot_dev.h"
 
@@ -137,7 +139,7 @@ __device__ __forceinline__ int find_lag(const LagSys* __restrict__ d, int n, lon
 __global__ __launch_bounds__(128) void group_lag_w_kernel(const LagSys* __restrict__ d, int n) {
   extern __shared__ double xs[];
   const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::w_begin)];
-  lag_w_block((int)(blockIdx.x - r.w_begin), r.bt, r.slice, r.nbf, r.nbf5, r.cj12, r.n,
+  lag_w_block((int)(blockIdx.x - r.w_begin), r.bt, r.slice, r.ldt, r.nbf5, r.cj12, r.n,
               r.nalpha, r.nbeta, r.X, r.w, xs);
 }
 
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void group_lag_bmat_kernel(const LagSys* __res
   const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::bmat_begin)];
   const long e = (blockIdx.x - r.bmat_begin) * 256 + threadIdx.x;
   if (e < (long)r.nbf5 * r.nbf5 * r.naux)
-    r.bmat[e] = lag_bmat_element(e, r.bt, r.slice, r.nbf, r.nbf5, r.ck12, r.w);
+    r.bmat[e] = lag_bmat_element(e, r.bt, r.slice, r.ldt, r.nbf5, r.ck12, r.w);
 }
 
 __global__ __launch_bounds__(256) void group_lag_init_kernel(const LagSys* __restrict__ d, int n) {
@@ -154,8 +156,8 @@ __global__ __launch_bounds__(256) void group_lag_init_kernel(const LagSys* __res
   if (e < (long)r.nbf * r.nbf) r.elag[e] = lag_init_element(e, r.Hmat, r.n, r.nbf, r.nbf5);
 }
 
-// elag += the split partials of the C^J/C^K contraction, summed in split order (the
-// arithmetic of reduce_splits with beta = scale = 1)
+// Z = the split partials of the C^J/C^K contraction, summed in split order (the
+// arithmetic of reduce_splits with scale 1 and beta 0)
 __global__ __launch_bounds__(256) void group_lag_reduce_kernel(const LagSys* __restrict__ d,
                                                                int n) {
   const LagSys& r = d[find_lag(d, n, blockIdx.x, &LagSys::rect_begin)];
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void group_lag_reduce_kernel(const LagSys* __r
   double s = 0.0;
   for (int z = 0; z < r.nsplit; ++z) s += r.part[z * sP + e];
   s *= 1.0;
-  r.elag[e] = s + 1.0 * r.elag[e];
+  r.Z[e] = s;
 }
 
 __global__ __launch_bounds__(256) void group_energy_kernel(const LagSys* __restrict__ d) {
@@ -259,8 +261,13 @@ int lag_group_before(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) 
   return DN_OK;
 }
 
-int lag_group_after(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) {
+int lag_group_reduce(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) {
   hipLaunchKernelGGL(group_lag_reduce_kernel, dim3((unsigned)b.rect), dim3(256), 0, s, d, n);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int lag_group_after(hipStream_t s, const LagSys* d, int n, const LagBlocks& b) {
   hipLaunchKernelGGL(group_energy_kernel, dim3(n), dim3(256), 0, s, d);
   hipLaunchKernelGGL(group_zero_gmax_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d, n);
   hipLaunchKernelGGL(group_pack_grad_kernel, dim3((unsigned)b.rect), dim3(256), 0, s, d, n);

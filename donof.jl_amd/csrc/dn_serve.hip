@@ -27,7 +27,7 @@ int squarings_for(double nrm);
 
 namespace {
 
-enum LStage { kT1 = 0, kT2 = 1, kHC = 2, kHM = 3, kLG = 4, kNumL = 5 };
+enum LStage { kT1 = 0, kT2 = 1, kHC = 2, kHM = 3, kLG = 4, kLZ = 5, kNumL = 6 };
 
 // ---- grouped rotation: the per-system work of a step for all live systems at once ------
 //
@@ -161,10 +161,11 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
       *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N)
                                               : reg_config(g.M, g.N, g.nbatch / g.bps);
       break;
-    case kT2:  // b_MO^T = C5^T b_Q C, (q, Q) flattened  (utils.jl:19-20)
-      g = t2_lag_gemm(s);
-      *config = gemm_config_ok(kLds64x128, g) ? lds64_config(g.N)
-                                              : reg_config(g.M, g.N, g.nbatch / g.bps);
+    case kT2:  // b_pq = C5^T bpn, (q, Q) flattened (the first nbf5 columns of b_MO)
+      g = t2_flat_gemm(s);
+      *config = !gemm_config_ok(kLds64x128, g) ? reg_config(g.M, g.N, 1)
+                : (g.M <= 64 || (g.M % 128 != 0 && g.M % 128 <= 64)) ? lds64_config(g.N)
+                                                                    : kLds128x256;
       break;
     case kHC:  // H C5
       g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
@@ -180,8 +181,12 @@ Gemm lag_gemm(dn_sys* s, int stage, int* config) {
       g.C = s->Hmat; g.ldc = d.nbf;
       *config = kTile64x64;
       break;
-    default:  // elag partials: sum_Q sum_q bt[q,a,Q] bmat[q,b,Q] over Q ranges
+    case kLG:  // Z partials: sum_Q sum_q bpn[n,q,Q] bmat[q,b,Q] over Q ranges
       g = lag_contract_gemm(s);
+      *config = kTile64x64;
+      break;
+    default:  // elag += C^T Z
+      g = lag_final_gemm(s);
       *config = kTile64x64;
       break;
   }
@@ -439,6 +444,8 @@ int lagrangian_all(dn_ogroup* G) {
   const int n = (int)G->sys.size();
   DN_TRY(lag_group_before(G->stream, G->lag_d, n, G->lag_b));
   DN_TRY(launch_lstage(G, kLG));
+  DN_TRY(lag_group_reduce(G->stream, G->lag_d, n, G->lag_b));
+  DN_TRY(launch_lstage(G, kLZ));
   DN_TRY(lag_group_after(G->stream, G->lag_d, n, G->lag_b));
   hipLaunchKernelGGL(gather2_kernel, dim3((n + 63) / 64), dim3(64), 0, G->stream,
                      (const double* const*)G->scal_ptrs, n, G->gathered);
@@ -457,7 +464,8 @@ int plan_lag_tail(dn_ogroup* G) {
     const int bps = (d.nbfaux + nsplit - 1) / nsplit;
     LagSys& r = h[i];
     r.bt = s->bt;
-    r.slice = (long)d.nbf5 * d.nbf;
+    r.ldt = d.nbf5;
+    r.slice = (long)d.nbf5 * d.nbf5;
     r.nbf = d.nbf;
     r.nbf5 = d.nbf5;
     r.naux = d.nbfaux;
@@ -475,6 +483,7 @@ int plan_lag_tail(dn_ogroup* G) {
     r.nsplit = (d.nbfaux + bps - 1) / bps;
     r.scal = s->scal;
     r.g = s->gv;
+    r.Z = s->Z;
   }
   lag_group_plan(h.data(), n, &G->lag_b);
   DN_HIP(hipMemcpy(G->lag_d, h.data(), n * sizeof(LagSys), hipMemcpyHostToDevice));
