@@ -76,19 +76,24 @@ def cpu_baseline(C, H, b, nbf5, budget_s=20.0):
 
 
 def executed_flops_sym_k1(nbf, naux, nbf5, precision="fp64"):
-    """MFMA flops the symmetric-b first transform really performs.
-    fp64: 64-row blocks of T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p] start their K loop at
-    the 16-aligned k below the block's first row; the 128-wide p tiles are padded.
-    split: 128-column blocks of T[p,n,Q] start at the 32-aligned k below the block's first
-    column; 128-row p tiles are padded; every tile issues three products (hi*hi, hi*lo,
-    lo*hi)."""
+    """MFMA flops the symmetric-b first transform really issues (the count of its MFMA
+    instructions times their flops; SQ_INSTS_MFMA agrees exactly at 400/1600/120 in FP64
+    and 800/3200/240 split).
+    fp64: T[n,p,Q] = sum_{m >= n} Lh[m,n,Q] C5[m,p]; a 16-row fragment row starting at n0
+    runs the 16-wide k tiles from n0 to the end (earlier ones are masked to zero and
+    skipped), rows past nbf and 16-column fragments past nbf5 are skipped.
+    split: T[p,n,Q] with 128-column blocks; a 16-column fragment starting at c0 runs the
+    32-wide k tiles from the one holding c0 to the end; the p rows are padded to 128 and
+    every fragment pair issues three products (hi*hi, hi*lo, lo*hi)."""
     if precision == "fp64":
-        tiles = -(-nbf5 // 128)
-        rows = sum(nbf - (m0 // 16) * 16 for m0 in range(0, nbf, 64))
-        return 2.0 * 64 * 128 * tiles * rows * naux
+        nkt = -(-nbf // 16)
+        frag_cols = -(-nbf5 // 16)
+        ktiles = sum(nkt - r0 // 16 for r0 in range(0, nbf, 16))
+        return 2.0 * 16 * 16 * 16 * frag_cols * ktiles * naux
     mpad = -(-nbf5 // 128) * 128
-    cols = sum(nbf - (n0 // 32) * 32 for n0 in range(0, nbf, 128))
-    return 3 * 2.0 * mpad * 128 * cols * naux
+    nkt = -(-nbf // 32)
+    ktiles = sum(max(0, nkt - c0 // 32) for c0 in range(0, -(-nbf // 128) * 128, 16))
+    return 3 * 2.0 * mpad * 16 * 32 * ktiles * naux
 
 
 def profile_value(name, key):
