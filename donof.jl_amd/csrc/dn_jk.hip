@@ -186,8 +186,15 @@ void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks) {
     const long nn = (long)d.nbf5 * d.nbf5;
     const int ntile = (d.nbf5 + 31) / 32;
     const int pairs = ntile * (ntile + 1) / 2;
+    // Q ranges of 8 slabs or more, doubled until the system has at most ~2048 workgroups
+    // (8 per CU): longer ranges amortise the tile set-up, fewer partials shorten pass 2
+    // (800/3200/240: 459 us with 8 slabs per range, 311 us with 64; 400/1600/120 keeps 8)
     d.qps = 8;
     d.splits = (d.naux + d.qps - 1) / d.qps;
+    while ((long)pairs * d.splits > 2048 && d.qps < d.naux) {
+      d.qps *= 2;
+      d.splits = (d.naux + d.qps - 1) / d.qps;
+    }
     while ((size_t)d.splits * nn > d.part_elems) {
       d.qps *= 2;
       d.splits = (d.naux + d.qps - 1) / d.qps;
