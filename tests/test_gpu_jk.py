@@ -18,9 +18,11 @@ def _system(nbf, naux, nbf5, seed, ndoc=None):
     return s, b
 
 
-# ragged shapes exercise every tile edge of the MFMA kernels
+# ragged shapes exercise every tile edge of the MFMA kernels; (150, 1200, 130) has more
+# (tile pair, Q range) workgroups than the K/X pass allows at 8 slabs per range, so its
+# ranges are doubled
 SHAPES = [(8, 4, 3, 1), (24, 96, 20, 5), (37, 19, 9, 3), (65, 130, 33, 3), (130, 70, 66, 3),
-          (200, 300, 60, 5)]
+          (200, 300, 60, 5), (150, 1200, 130, 5)]
 
 
 @pytest.mark.parametrize("nbf,naux,nbf5,ndoc", SHAPES)
