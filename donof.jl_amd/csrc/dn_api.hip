@@ -667,7 +667,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     k.part_elems = s->part_elems;
     k.nbf5 = dims->nbf5;
     k.naux = dims->nbfaux;
-    symk_plan(&k, 1, &s->symk_blocks, &s->symk_red);
+    symk_plan(&k, 1, &s->symk_blocks, &s->symk_red, s->precision == DN_SPLIT_F16);
     s->symk_dev = reinterpret_cast<SymKDesc*>(s->scal + 16);
     SymKDesc k32 = k;  // the split-precision build's W (FP32), at scal[32..47]
     k32.W = nullptr;

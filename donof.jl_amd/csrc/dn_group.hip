@@ -640,7 +640,7 @@ static int group_create_leaf(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_g
   G->ncols = col;
   G->nsym = (int)sk.size();
   G->nkx = (int)kx.size();
-  symk_plan(sk.data(), G->nsym, &G->symk_blocks, &G->symk_red);
+  symk_plan(sk.data(), G->nsym, &G->symk_blocks, &G->symk_red, G->precision == DN_SPLIT_F16);
   if (G->nsym)
     e = hipMemcpy(G->symk, sk.data(), sk.size() * sizeof(SymKDesc), hipMemcpyHostToDevice);
   if (e == hipSuccess && G->nkx)

@@ -210,7 +210,8 @@ struct SymKDesc {
   int qps = 1, splits = 1;  // set by symk_plan
   long blk_begin = 0, red_begin = 0;
 };
-void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks);
+// f32: the plan is for the FP32 W of the split-precision build
+void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks, bool f32 = false);
 int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
              size_t part2_elems, double* J);
 int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks,

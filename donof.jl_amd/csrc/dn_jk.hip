@@ -131,22 +131,35 @@ __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __res
   const int Q0 = z * d.qps, Q1 = min(Q0 + d.qps, d.naux);
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int pI = I * 32 + tx, pJ = J * 32 + tx;
+  // a diagonal tile pair is one tile: (J, I) = (I, I) is read once and mirrored in LDS
+  const bool diag = I == J;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int Q = Q0; Q < Q1; ++Q) {
+  // the next slice's elements are loaded into registers while this slice is reduced
+  double ra[4], rb[4];
+  auto load = [&](int Q) {
     const T* W = Wall + (long)Q * nn;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int r = ty + 8 * k, qJ = J * 32 + r, qI = I * 32 + r;
-      a[tx][r] = (pI < n5 && qJ < n5) ? (double)W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
-      b[tx][r] = (pJ < n5 && qI < n5) ? (double)W[pJ + (long)qI * n5] : 0.0;  // tile (J, I)
+      ra[k] = (pI < n5 && qJ < n5) ? (double)W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
+      rb[k] = (!diag && pJ < n5 && qI < n5) ? (double)W[pJ + (long)qI * n5] : 0.0;  // (J, I)
     }
+  };
+  if (Q0 < Q1) load(Q0);
+  for (int Q = Q0; Q < Q1; ++Q) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[tx][ty + 8 * k] = ra[k];
+      if (!diag) b[tx][ty + 8 * k] = rb[k];
+    }
+    if (Q + 1 < Q1) load(Q + 1);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int r = ty + 8 * k;
-      const double v = a[tx][r] + b[r][tx];  // b_pq at p = I*32 + tx, q = J*32 + r
+      const double v = a[tx][r] + (diag ? a[r][tx] : b[r][tx]);  // b_pq, p = I*32+tx, q = J*32+r
       acc[k] = fma(v, v, acc[k]);
-      if (I == J && r == tx && pI < n5) d.X[pI + (long)Q * n5] = v;
+      if (diag && r == tx && pI < n5) d.X[pI + (long)Q * n5] = v;
     }
     __syncthreads();
   }
@@ -158,42 +171,59 @@ __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __res
   }
 }
 
-// 64 elements per workgroup, the four wavefronts summing interleaved Q ranges.
-__global__ __launch_bounds__(256) void symk_reduce_kernel(const SymKDesc* __restrict__ desc,
+// 64 elements per workgroup, the eight wavefronts summing interleaved Q ranges (two
+// accumulators each, so that two loads are in flight per lane).
+__global__ __launch_bounds__(512) void symk_reduce_kernel(const SymKDesc* __restrict__ desc,
                                                           int n) {
-  __shared__ double red[4][64];
+  __shared__ double red[8][64];
   const SymKDesc d = desc[find_symk(desc, n, blockIdx.x, true)];
   const int n5 = d.nbf5;
   const long nn = (long)n5 * n5;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long e = (blockIdx.x - d.red_begin) * 64 + lane;
-  double k = 0.0;
+  double k0 = 0.0, k1 = 0.0;
   if (e < nn) {
     const int p = (int)(e % n5), q = (int)(e / n5);
     // partials exist for tile(p) <= tile(q); K is symmetric
     const long src = (p / 32 <= q / 32) ? e : q + (long)p * n5;
-    for (int z = w; z < d.splits; z += 4) k += d.part[z * nn + src];
+    int z = w;
+    for (; z + 8 < d.splits; z += 16) {
+      k0 += d.part[z * nn + src];
+      k1 += d.part[(z + 8) * nn + src];
+    }
+    if (z < d.splits) k0 += d.part[z * nn + src];
   }
-  red[w][lane] = k;
+  red[w][lane] = k0 + k1;
   __syncthreads();
-  if (w == 0 && e < nn) d.K[e] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  if (w == 0 && e < nn)
+    d.K[e] = ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])) +
+             ((red[4][lane] + red[5][lane]) + (red[6][lane] + red[7][lane]));
 }
 
-void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks) {
+void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks, bool f32) {
   long b = 0, r = 0;
   for (int i = 0; i < n; ++i) {
     SymKDesc& d = desc[i];
     const long nn = (long)d.nbf5 * d.nbf5;
     const int ntile = (d.nbf5 + 31) / 32;
     const int pairs = ntile * (ntile + 1) / 2;
-    // Q ranges of 8 slabs or more, doubled until the system has at most ~2048 workgroups
-    // (8 per CU): longer ranges amortise the tile set-up, fewer partials shorten pass 2
-    // (800/3200/240: 459 us with 8 slabs per range, 311 us with 64; 400/1600/120 keeps 8)
+    // Q ranges of 8 slabs or more.  FP64 W: the longest ranges that leave >= 360
+    // workgroups (each loads the next slice while it reduces this one): 120/1600 41.6 us
+    // with 32 slabs per range against 55.8 us with 8, 240/3200 293 against 478 us.  FP32
+    // W: doubled only until <= 2048 workgroups (240/3200: 317 us with 64 slabs, 528 us
+    // with 128).
     d.qps = 8;
     d.splits = (d.naux + d.qps - 1) / d.qps;
-    while ((long)pairs * d.splits > 2048 && d.qps < d.naux) {
-      d.qps *= 2;
-      d.splits = (d.naux + d.qps - 1) / d.qps;
+    if (f32) {
+      while ((long)pairs * d.splits > 2048 && d.qps < d.naux) {
+        d.qps *= 2;
+        d.splits = (d.naux + d.qps - 1) / d.qps;
+      }
+    } else {
+      while (d.qps < d.naux && (long)pairs * ((d.naux + 2 * d.qps - 1) / (2 * d.qps)) >= 360) {
+        d.qps *= 2;
+        d.splits = (d.naux + d.qps - 1) / d.qps;
+      }
     }
     while ((size_t)d.splits * nn > d.part_elems) {
       d.qps *= 2;
@@ -214,7 +244,7 @@ int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks
     hipLaunchKernelGGL(symk_partial_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
   else
     hipLaunchKernelGGL(symk_partial_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
-  hipLaunchKernelGGL(symk_reduce_kernel, dim3((unsigned)red_blocks), dim3(256), 0, s, dev, n);
+  hipLaunchKernelGGL(symk_reduce_kernel, dim3((unsigned)red_blocks), dim3(512), 0, s, dev, n);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
