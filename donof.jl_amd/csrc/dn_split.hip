@@ -94,14 +94,18 @@ __global__ void b_split_kernel(const double* __restrict__ b, int ld, long slab, 
 
 // C5^T planes [Mpad][Kpad] (row = orbital p, k = AO index m), zero padded, in two
 // launches over the whole GPU: max |C5| into cscale[2] (bits of a non-negative float, which
-// order like the value; cleared first), then the planes with the power-of-two scale.
-__global__ void c_max_kernel(const double* __restrict__ C, long n, float* __restrict__ cscale) {
+// order like the value; cleared first; one atomic per workgroup, at most 128 workgroups:
+// one per wavefront on a single address took 36 us at nbf 800), then the planes with the
+// power-of-two scale.
+__global__ __launch_bounds__(256) void c_max_kernel(const double* __restrict__ C, long n,
+                                                    float* __restrict__ cscale) {
+  __shared__ float red[16];
   float m = 0.f;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x)
     m = fmaxf(m, (float)fabs(C[e]));
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned*>(cscale + 2), __float_as_uint(m));
+  m = block_max(m, red);
+  if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned*>(cscale + 2), __float_as_uint(m));
 }
 
 __global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5, int Mpad,
@@ -870,7 +874,8 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
                     float* cscale, _Float16* hi, _Float16* lo) {
   const long n = (long)nbf * nbf5;
   hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, cscale + 2, 1L, 0.f);
-  hipLaunchKernelGGL(c_max_kernel, dim3(grid_of(n)), dim3(256), 0, s, C, n, cscale);
+  hipLaunchKernelGGL(c_max_kernel, dim3(std::min(grid_of(n), 128u)), dim3(256), 0, s, C, n,
+                     cscale);
   hipLaunchKernelGGL(c_planes_kernel, dim3(grid_of((long)Mpad * Kpad)), dim3(256), 0, s, C, nbf,
                      nbf5, Mpad, Kpad, cscale, hi, lo);
   DN_HIP(hipGetLastError());
