@@ -99,6 +99,7 @@ _SIGS = {
     "dn_group_sync": (_I, [_P]),
     "dn_group_set_timing": (_I, [_P, _I]),
     "dn_group_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),
+    "dn_jkh_mo_noalloc": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

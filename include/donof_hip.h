@@ -215,6 +215,25 @@ int dn_ogroup_step(dn_ogroup* group, int* active);
 int dn_ogroup_run(dn_ogroup* group, int max_steps, const volatile int* stop, int* steps,
                   int* active);
 
+/* ---- J/K-MO build without allocation ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * build of dn_jkh_mo (computeJKH_MO, integrals.jl:131-172) on caller-owned DEVICE arrays,
+ * b (nbf x nbf x nbfaux), C and H (nbf x nbf) in, J, K (nbf5 x nbf5) and Hcore (nbf5) out,
+ * all column-major with leading dimension nbf, enqueued on `stream` (a hipStream_t; NULL
+ * is the null stream) without synchronising.  It calls no hipMalloc/hipFree or any
+ * allocating routine: its intermediates live in one static device array of
+ * DN_NOALLOC_SCRATCH_BYTES per GPU, held from the library's first use of that GPU until
+ * the process ends.  The intermediates take 8 * (nbf*nbf5*nbfaux + nbf5^2*nbfaux +
+ * nbf5*nbfaux + nbf*nbf5 + 128*nbf5^2 + 16*nbf^2) bytes (each part rounded up to 256
+ * bytes); larger sizes return DN_ERR_ARG.  Every call on a GPU uses that GPU's one array:
+ * calls that overlap on one GPU (other streams, other threads) race, so the caller
+ * orders them.  b need not be symmetric (the general contraction runs). */
+#define DN_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int dn_jkh_mo_noalloc(int device, int nbf, int nbfaux, int nbf5, const double* b,
+                      const double* C, const double* H, double* J, double* K, double* Hcore,
+                      void* stream);
+
 /* ---- grouped builds --------------------------------------------------------------
  * Many resident systems of uneven size built in one sequence of grouped, persistent
  * launches, captured once into a HIP graph.  Each system's build is computeJKH_MO /
