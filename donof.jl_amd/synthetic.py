@@ -194,6 +194,12 @@ def build_flops(nbf, naux, nbf5):
             + 4.0 * nbf5 * nbf5 * naux + 2.0 * nbf * nbf * nbf5 + 2.0 * nbf * nbf5)
 
 
+def build_bytes(nbf, naux, nbf5, e=8):
+    """Minimum HBM bytes of one J/K-MO build: b read once (e bytes per element: 8 for FP64,
+    4 for a hi+lo pair of 16-bit planes) plus C5, H, J, K and H_core in FP64."""
+    return e * nbf * nbf * naux + 8 * (nbf * nbf5 + nbf * nbf + 2 * nbf5 * nbf5 + nbf5)
+
+
 class SyntheticSystem:
     """Inputs of one synthetic solve: C, H, n, gamma and the Param; b is generated
     on the device (or on the host with make_b for small sizes)."""
