@@ -16,8 +16,17 @@ How each part is pinned (tests/test_oracle_*.py):
                     central finite differences of E(rotate(y)) vs calcorbg.
   rotate_orbital    scipy.linalg.expm (Al-Mohy & Higham 2009) vs an independent
                     eigen-route, U^T U = I, y = 0 -> C.
-  pnof_coefficients closed forms (HF limit), symmetry; der_* checked by finite
-                    differences of cj12/ck12 in gamma through the occupations.
+  pnof_coefficients HF limit (integer n); one electron pair (ndoc = 1) reduces every
+                    functional to the Loewdin-Shull two-electron energy; the
+                    cumulant sum rule sum_q Delta_qp = n_p (1 - n_p); GNOF with the
+                    dynamic factor quenched equals PNOF7; the dynamic factor of a pair
+                    depends on that pair only; symmetry.  der_* checked by finite
+                    differences of cj12/ck12 in gamma through the occupations; PNOF4's
+                    by finite differences of the factors of Pi combined with the
+                    denominator pnof.jl:255 writes.  Parity unpinned: the signs of
+                    GNOF's dynamic beta-cwo / cwo-cwo terms and the block pattern of
+                    its ista = 0 static term (pnof.jl:540-578) are pinned only by this
+                    transcription and by finite-difference consistency.
   ocupacion_*       pair sums == 1 (softmax), finite-difference Jacobians.
   calcoccg          finite differences of calcocce.
   orbopt_*          deterministic recurrence; replayed step by step.

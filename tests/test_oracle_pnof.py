@@ -130,3 +130,139 @@ def test_gnof_static_and_dynamic_terms(oracle):
     nd0 = n[0] * F1
     expect = n[0] * n[b2] + fi[0] * fi[b2] + np.sqrt(nd0 * ndb) - nd0 * ndb
     assert ck[0, b2] == pytest.approx(expect, abs=1e-15)
+
+
+# ---- pins that do not restate the coefficient formulas --------------------------------
+
+def _two_electron_energy(n, J, K, Hc):
+    """Energy of the two-electron Loewdin-Shull wave function sum_i c_i |i alpha i beta|
+    in natural orbitals, c_strong = +sqrt(n), c_weak = -sqrt(n):
+    E = sum_i n_i (2 h_ii + (ii|ii)) + sum_{i != j} c_i c_j (ij|ij)."""
+    c = -np.sqrt(n)
+    c[0] = np.sqrt(n[0])
+    E = float(np.sum(n * (2.0 * Hc + np.diag(J))))
+    cc = np.outer(c, c)
+    np.fill_diagonal(cc, 0.0)
+    return E + float(np.sum(cc * K))
+
+
+@pytest.mark.parametrize("ipnof,ista", [(4, 0), (5, 0), (7, 0), (7, 1), (8, 0), (8, 1),
+                                         (8, 2), (8, 3), (8, 4), (8, 5), (9, 0)])
+def test_one_pair_reduces_to_loewdin_shull(oracle, ipnof, ista):
+    """With one electron pair (ndoc = 1) every functional is exact: its energy is the
+    Loewdin-Shull two-electron energy built from (ij|ij) = K_ij alone."""
+    s, p = system(ipnof, ista, nbf=14, naux=30, nbf5=9, ndoc=1)
+    J, K, Hc = jk(oracle, s)
+    rng = np.random.default_rng(11 + ipnof)
+    n, _ = oracle.ocupacion(rng.standard_normal(p.nv) - 1.5, p)
+    assert abs(n.sum() - 1.0) < 1e-14
+    cj, ck = oracle.pnof_selector(n, p)
+    E = oracle.calce(n, cj, ck, J, K, Hc, p)
+    assert abs(E - _two_electron_energy(n, J, K, Hc)) < 1e-12 * max(1.0, abs(E))
+
+
+@pytest.mark.parametrize("ipnof,ista", [(4, 0), (5, 0), (7, 0), (8, 0), (8, 3), (9, 0)])
+def test_cumulant_sum_rule(oracle, ipnof, ista):
+    """The cumulant part Delta = n n^T - cj12/2 of the reconstructed two-matrix obeys the
+    sum rule sum_{q != p} Delta_qp = n_p (1 - n_p) for every p (the pairing of the
+    occupations and of the coefficients agree)."""
+    s, p = system(ipnof, ista, ndoc=3)
+    rng = np.random.default_rng(21)
+    n, _ = oracle.ocupacion(rng.standard_normal(p.nv) - 2.0, p)
+    cj, _ = oracle.pnof_selector(n, p)
+    D = np.outer(n, n) - cj / 2.0
+    np.fill_diagonal(D, 0.0)
+    assert np.abs(D.sum(axis=0) - n * (1.0 - n)).max() < 1e-14
+
+
+@pytest.mark.parametrize("ista", [1, 5])
+def test_gnof_quenched_dynamics_is_pnof7(oracle, ista):
+    """GNOF with the interpair static term on every block (ista 1, 5) and its dynamic
+    factor quenched (1 - n_g >= 10 h_cut for every pair) is PNOF7 (ista 0)."""
+    s, p = system(8, ista, ndoc=3)
+    p.h_cut = 1e-3
+    rng = np.random.default_rng(5)
+    n, _ = oracle.ocupacion(rng.standard_normal(p.nv) - 1.0, p)
+    assert (1.0 - n[:p.ndoc]).min() >= 10 * p.h_cut
+    cj8, ck8 = oracle.pnof_selector(n, p)
+    p.ipnof, p.ista = 7, 0
+    cj7, ck7 = oracle.pnof_selector(n, p)
+    assert np.abs(cj8 - cj7).max() == 0.0
+    assert np.abs(ck8 - ck7).max() < 1e-15
+
+
+@pytest.mark.parametrize("ipnof", [8, 9])
+@pytest.mark.parametrize("moved", [1, 2, 3])
+def test_dynamic_factor_is_local_to_its_pair(oracle, ipnof, moved):
+    """The dynamic occupations of pair g depend on pair g's occupations only: moving
+    weight inside one pair leaves every coefficient between orbitals of the two other
+    pairs unchanged, and changes those between the moved pair and the others."""
+    s, p = system(ipnof, 0, ndoc=3)
+    rng = np.random.default_rng(8)
+    # strong occupations near 1 - h_cut, where the dynamic factor is of order one
+    g = 0.3 * rng.standard_normal(p.nv) - 5.0
+    n1, _ = oracle.ocupacion(g, p)
+    F = np.exp(-((1.0 - n1[:p.ndoc]) / p.h_cut) ** 2)
+    assert F.min() > 0.05
+    g2 = g.copy()
+    g2[oracle.pair_gamma(p, moved)] += 0.7
+    n2, _ = oracle.ocupacion(g2, p)
+    members = {l: np.r_[p.no1 + l - 1, oracle.pair_weak(p, l)] for l in (1, 2, 3)}
+    a, c = [members[l] for l in (1, 2, 3) if l != moved]
+    b = members[moved]
+    assert np.abs(n1[a] - n2[a]).max() == 0.0 and np.abs(n1[c] - n2[c]).max() == 0.0
+    assert np.abs(n1[b] - n2[b]).max() > 1e-3
+    _, ck1 = oracle.pnof_selector(n1, p)
+    _, ck2 = oracle.pnof_selector(n2, p)
+    assert np.abs(ck1[np.ix_(a, c)] - ck2[np.ix_(a, c)]).max() == 0.0
+    assert np.abs(ck1[np.ix_(a, b)] - ck2[np.ix_(a, b)]).max() > 1e-6
+
+
+def test_pnof4_derivative_against_its_factors(oracle):
+    """der_CJCKD4 (pnof.jl:175-272) against finite differences.  cj12's derivative is the
+    total derivative.  In ck12 the occupied-virtual block of Pi = -sqrt(R) sqrt(n_p - n_q
+    + R), R = h_p n_q / S_F (pnof.jl:128-139) is differentiated with the denominator
+    2 sqrt(n_p n_q + R) where the chain rule gives 2 sqrt(n_p - n_q + R) (pnof.jl:255 as
+    written); every other entry is the total derivative.  The finite differences of the
+    two factors sqrt(R) and n_p - n_q + R, combined with that denominator, predict the
+    difference entry by entry."""
+    s, p = system(4, 0, nbf=14, naux=20, nbf5=12, ndoc=3)
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal(p.nv) - 2.0
+    n, dn = oracle.ocupacion(g, p)
+    Dcj, Dck = oracle.der_pnof_selector(n, dn, p)
+    nd = p.ndoc
+    occ, vir = np.arange(nd), np.arange(nd, p.nbf5)
+
+    def factors(nn):
+        h = 1.0 - nn
+        R = np.outer(h[occ], nn[vir]) / h[occ].sum()
+        return np.sqrt(R), nn[occ][:, None] - nn[vir][None, :] + R, R
+
+    sR, arg, R = factors(n)
+    prod = np.outer(n[occ], n[vir]) + R
+    hstep = 1e-6
+    for k in range(p.nv):
+        e = np.zeros(p.nv)
+        e[k] = hstep
+        np_, _ = oracle.ocupacion(g + e, p)
+        nm_, _ = oracle.ocupacion(g - e, p)
+        cjp, ckp = oracle.pnof_selector(np_, p)
+        cjm, ckm = oracle.pnof_selector(nm_, p)
+        fd_cj = (cjp - cjm) / (2 * hstep)
+        fd_ck = (ckp - ckm) / (2 * hstep)
+        assert np.abs(fd_cj - (Dcj[:, :, k] + Dcj[:, :, k].T)).max() < 1e-7
+        sRp, argp, _ = factors(np_)
+        sRm, argm, _ = factors(nm_)
+        dsR = (sRp - sRm) / (2 * hstep)
+        darg = (argp - argm) / (2 * hstep)
+        # Pi_pq's chain rule against the written denominator, on the occupied-virtual block
+        dpi_true = -(dsR * np.sqrt(arg) + sR * darg / (2 * np.sqrt(arg)))
+        dpi_code = -(dsR * np.sqrt(arg) + sR * darg / np.maximum(2 * np.sqrt(prod), 1e-7))
+        expect = np.zeros((p.nbf5, p.nbf5))
+        expect[np.ix_(occ, vir)] = dpi_true - dpi_code
+        expect[np.ix_(vir, occ)] = (dpi_true - dpi_code).T
+        got = (Dck[:, :, k] + Dck[:, :, k].T) - fd_ck
+        assert np.abs(got - expect).max() < 1e-6
+    # the denominators differ here, so the quirk is visible
+    assert np.abs(np.sqrt(prod) - np.sqrt(arg)).max() > 1e-3
