@@ -65,6 +65,7 @@ _SIGS = {
     "dn_sys_b_symmetric": (_I, [_P, ct.POINTER(ct.c_int)]),
     "dn_jkh_mo": (_I, [_P, _D, _D, _D, _D, _D]),
     "dn_lagrange2": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _D, _D, _D]),
+    "dn_lagrange2_jkh": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _D, _D, _D, _D, _D, _D]),
     "dn_sys_set_orbitals": (_I, [_P, _D]),
     "dn_sys_set_hcore": (_I, [_P, _D]),
     "dn_jkh_mo_resident": (_I, [_P]),

@@ -403,6 +403,29 @@ def compute_Lagrange2(C, n, H, b, cj12, ck12, nalpha, nbeta, return_energy=False
     return elag, Hmat
 
 
+def compute_Lagrange2_JKH(C, n, H, b, cj12, ck12, nalpha, nbeta):
+    """(elag, H_mat, E, J_MO, K_MO, H_core): compute_Lagrange2 (utils.jl:1-59) and
+    computeJKH_MO (integrals.jl:131-172) at the same orbitals from one transform of b,
+    as the diagnostics of an outer iteration need them (energy.jl:172-190)."""
+    s = _require_system(b)
+    nbf, nbf5 = s.nbf, s.nbf5
+    C = _colmajor(C, (nbf, nbf), "C")
+    H = _colmajor(H, (nbf, nbf), "H")
+    n = _vec(n, nbf5, "n")
+    cj12 = _colmajor(cj12, (nbf5, nbf5), "cj12")
+    ck12 = _colmajor(ck12, (nbf5, nbf5), "ck12")
+    elag = np.empty((nbf, nbf), order="F")
+    Hmat = np.empty((nbf, nbf5), order="F")
+    J = np.empty((nbf5, nbf5), order="F")
+    K = np.empty((nbf5, nbf5), order="F")
+    Hc = np.empty(nbf5)
+    E = ct.c_double()
+    check(s.lib.dn_lagrange2_jkh(s.handle, dptr(C), dptr(n), dptr(H), dptr(cj12), dptr(ck12),
+                                 int(nalpha), int(nbeta), dptr(elag), dptr(Hmat), ct.byref(E),
+                                 dptr(J), dptr(K), dptr(Hc)))
+    return elag, Hmat, E.value, J, K, Hc
+
+
 # --------------------------------------------------------------------------------------
 # pnof.jl: occupations, coefficients, occupation energy and gradient
 # --------------------------------------------------------------------------------------

@@ -14,6 +14,7 @@ int jk_from_bpq(hipStream_t s, const double* bpq, long slice, int nbf5, int naux
                 double* K);
 int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nbf5,
                double* Hcore);
+int hcore_from_hmat(hipStream_t s, const double* Hmat, int nbf, int nbf5, double* Hcore);
 int synth_b(hipStream_t s, double* b, int nbf, int ld, int naux, uint64_t seed);
 int b_symmetry(hipStream_t s, const double* b, int nbf, int ld, int naux, unsigned long long* dev2,
                double* dmax, double* bmax);
@@ -820,6 +821,46 @@ int dn_lagrange2(dn_sys* s, const double* C, const double* n, const double* H,
   if (E) *E = e;
   if (!std::isfinite(e) || (elag && !all_finite(elag, (size_t)d.nbf * d.nbf)))
     return fail(DN_ERR_NONFINITE, "non-finite value in the Lagrangian");
+  return DN_OK;
+}
+
+int dn_lagrange2_jkh(dn_sys* s, const double* C, const double* n, const double* H,
+                     const double* cj12, const double* ck12, int nalpha, int nbeta,
+                     double* elag, double* Hmat, double* E, double* J, double* K,
+                     double* Hcore) {
+  DN_TRY(check_sys(s));
+  if (!J || !K || !Hcore) return fail(DN_ERR_ARG, "null J/K/Hcore output");
+  const dn_dims& d = s->d;
+  const size_t nn = (size_t)d.nbf5 * d.nbf5;
+  DN_TRY(h2d(s, s->C, C, (size_t)d.nbf * d.nbf));
+  DN_TRY(h2d(s, s->H, H, (size_t)d.nbf * d.nbf));
+  DN_TRY(h2d(s, s->n, n, d.nbf5));
+  DN_TRY(h2d(s, s->cj12, cj12, nn));
+  DN_TRY(h2d(s, s->ck12, ck12, nn));
+  DN_TRY(lagrange_resident(s, nalpha, nbeta));
+  // b_pq[q,p,Q] of the Lagrangian's second transform gives K and X = diag(b_pq), and
+  // J = X X^T; H_core is the diagonal of Hmat's leading nbf5 block (C5^T H C5)
+  DN_TRY(jk_from_bpq(s->stream, s->bt, (long)nn, d.nbf5, d.nbfaux, s->X, s->part,
+                     s->part_elems, s->part2, s->part2_elems, s->J, s->K));
+  DN_TRY(hcore_from_hmat(s->stream, s->Hmat, d.nbf, d.nbf5, s->Hcore));
+  std::vector<double> hm;
+  double* hm_out = Hmat;
+  if (!hm_out) {
+    hm.resize((size_t)d.nbf * d.nbf5);
+    hm_out = hm.data();
+  }
+  DN_TRY(d2h(s, elag, s->elag, (size_t)d.nbf * d.nbf));
+  DN_TRY(d2h(s, hm_out, s->Hmat, (size_t)d.nbf * d.nbf5));
+  DN_TRY(d2h(s, J, s->J, nn));
+  DN_TRY(d2h(s, K, s->K, nn));
+  DN_TRY(d2h(s, Hcore, s->Hcore, d.nbf5));
+  double e = 0.0;
+  DN_TRY(d2h(s, &e, s->scal, 1));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  if (E) *E = e;
+  if (!std::isfinite(e) || (elag && !all_finite(elag, (size_t)d.nbf * d.nbf)) ||
+      !all_finite(J, nn) || !all_finite(K, nn) || !all_finite(Hcore, d.nbf5))
+    return fail(DN_ERR_NONFINITE, "non-finite value in the Lagrangian or J/K/H_core");
   return DN_OK;
 }
 

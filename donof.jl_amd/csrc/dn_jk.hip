@@ -340,6 +340,21 @@ int hcore_diag(hipStream_t s, const double* C, const double* HC, int nbf, int nb
   return DN_OK;
 }
 
+// H_core[p] = Hmat[p,p] (Hmat = C^T H C5, nbf x nbf5): the diagonal of its leading
+// nbf5 block is (C5^T H C5)_pp (integrals.jl:156,169)
+__global__ void hcore_from_hmat_kernel(const double* __restrict__ Hmat, int nbf, int nbf5,
+                                       double* __restrict__ Hcore) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nbf5) Hcore[p] = Hmat[(long)p * nbf + p];
+}
+
+int hcore_from_hmat(hipStream_t s, const double* Hmat, int nbf, int nbf5, double* Hcore) {
+  hipLaunchKernelGGL(hcore_from_hmat_kernel, dim3((nbf5 + 255) / 256), dim3(256), 0, s, Hmat,
+                     nbf, nbf5, Hcore);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
 // ---- Philox4x32-10 normal stream (same definition as the host generator) -----------
 __device__ __forceinline__ double philox_normal(uint64_t e, uint32_t stream, uint64_t seed) {
   const uint64_t ctr = e >> 1;

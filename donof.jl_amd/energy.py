@@ -6,7 +6,15 @@ supplies the RI tensor (a `System` with b uploaded or generated), H_core, starti
 orbitals and the nuclear repulsion.  Each outer iteration runs, as the reference does:
 the orbital optimiser (p.orb_method) on the device, the occupation optimiser, the
 Softmax re-ordering, and the diagnostics (ENERGY1r, calcorbg, J/K and calcoccg), then
-the convergence test max|grad_orb| < threshgorb and ||grad_occ|| < threshgocc.
+the convergence test max|grad_orb| < threshgorb and ||grad_occ|| < threshgocc.  After
+the loop the pairs are sorted (order_subspaces, guess.jl:58-106) for ipnof > 4.
+
+Checkpoints: the reference saves (E, C, n) to `<title>.jld2` after every outer iteration
+and (C, n) at the end (energy.jl:209-213, 226-232), and resumes through read_C / read_n
+(guess.jl:1-56) whose results the caller passes back to energy(C=..., n=...).  JLD2 is an
+HDF5 file and no HDF5 library is available to this package, so the same records are
+written as NumPy `.npz` files (`<title>.npz`, arrays E, C, n; column-major C as given)
+and read back by read_C / read_n / read_all.
 """
 from __future__ import annotations
 
@@ -32,19 +40,94 @@ def computeE_elec_hc(Hmat, n, elag, p):
     return E
 
 
-def ENERGY1r(C, n, H, I, cj12, ck12, p):
-    """(E, elag, sumdiff, maxdiff) of utils.jl:117-136 (no1 = 0 branch)."""
+def ENERGY1r(C, n, H, I, cj12, ck12, p, want_jkh=False):
+    """(E, elag, sumdiff, maxdiff) of utils.jl:117-136 (no1 = 0 branch).  With want_jkh
+    also (J_MO, K_MO, H_core) at the same orbitals, taken from the same transform of b."""
     if p.no1 != 0:
         raise NotImplementedError("the no1 > 0 branch (computeJKj/computeF_RC) is out of scope")
-    elag, Hmat = api.compute_Lagrange2(C, n, H, I, cj12, ck12, p.nalpha, p.nbeta)
+    if want_jkh:
+        elag, Hmat, _, J, K, Hc = api.compute_Lagrange2_JKH(C, n, H, I, cj12, ck12, p.nalpha,
+                                                           p.nbeta)
+    else:
+        elag, Hmat = api.compute_Lagrange2(C, n, H, I, cj12, ck12, p.nalpha, p.nbeta)
     E = computeE_elec_hc(Hmat, n, elag, p)
     d = np.abs(elag - elag.T)
+    if want_jkh:
+        return E, elag, float(d.sum()), float(d.max()), (J, K, Hc)
     return E, elag, float(d.sum()), float(d.max())
+
+
+def order_subspaces(old_C, old_n, elag, H, I, p):
+    """guess.jl:58-106: the strong orbitals sorted by increasing elag_ii (stable, as
+    Julia's sortperm), each pair's weak orbitals (columns ll:ndoc:ul) and occupations
+    moving with their strong orbital; the singly occupied block sorted the same way;
+    frozen-core columns and columns past nbf5 kept.  Returns (C, n, elag) with elag from
+    ENERGY1r at the sorted orbitals."""
+    C = np.zeros_like(np.asarray(old_C))
+    n = np.zeros_like(np.asarray(old_n))
+    d = np.diag(elag)
+    C[:, :p.no1] = old_C[:, :p.no1]
+    n[:p.no1] = old_n[:p.no1]
+    for i, i_old in enumerate(np.argsort(d[p.no1:p.nbeta], kind="stable"), start=1):
+        i_old = int(i_old) + 1
+        C[:, p.no1 + i - 1] = old_C[:, p.no1 + i_old - 1]
+        n[p.no1 + i - 1] = old_n[p.no1 + i_old - 1]
+        C[:, mini.pair_weak(p, i)] = old_C[:, mini.pair_weak(p, i_old)]
+        n[mini.pair_weak(p, i)] = old_n[mini.pair_weak(p, i_old)]
+    s0 = p.no1 + p.ndoc
+    for i, i_old in enumerate(np.argsort(d[s0:p.no1 + p.ndns], kind="stable")):
+        C[:, s0 + i] = old_C[:, s0 + int(i_old)]
+        n[s0 + i] = old_n[s0 + int(i_old)]
+    C[:, p.nbf5:] = old_C[:, p.nbf5:]
+    C = np.asfortranarray(C)
+    cj12, ck12 = api.PNOFi_selector(n, p, I)
+    _, elag, _, _ = ENERGY1r(C, n, H, I, cj12, ck12, p)
+    return C, n, elag
+
+
+def _checkpoint_path(title):
+    return title if title.endswith(".npz") else title + ".npz"
+
+
+def save_checkpoint(title, C, n, E=None):
+    """(E, C, n) as energy.jl:209-213 saves them, or (C, n) as energy.jl:226-232 does."""
+    arrays = {"C": np.asarray(C), "n": np.asarray(n)}
+    if E is not None:
+        arrays["E"] = np.asarray(E)
+    np.savez(_checkpoint_path(title), **arrays)
+
+
+def _read(title, key):
+    path = _checkpoint_path(title)
+    try:
+        with np.load(path, allow_pickle=False) as f:
+            return np.array(f[key], order="F")
+    except (OSError, KeyError):
+        print("%s: no %s found" % (path, key))
+        return None
+
+
+def read_C(title="donof"):
+    """guess.jl:1-15: the orbitals of a checkpoint, or None."""
+    return _read(title, "C")
+
+
+def read_n(title="donof"):
+    """guess.jl:17-31: the occupations of a checkpoint, or None."""
+    return _read(title, "n")
+
+
+def read_all(title="donof"):
+    """guess.jl:49-56 without fmiug0 (the ID orbital method is not supported): (C, n)."""
+    return read_C(title), read_n(title)
 
 
 def energy(I, H, C, p, E_nuc=0.0, n=None, freeze_occ=False, printmode=True, checkpoint=False,
            log=print):
-    """Outer loop of energy.jl:140-220.  Returns (E_nuc + E, C, n, elag, history)."""
+    """Outer loop of energy.jl:140-224.  Returns (E_nuc + E, C, n, elag, history); C, n
+    and elag are sorted by order_subspaces for ipnof > 4.  checkpoint writes
+    `<p.title>.npz` every iteration and at the end; a later call resumes with
+    energy(..., C=read_C(title), n=read_n(title))."""
     if p.orb_method not in mini.ORBOPT:
         raise NotImplementedError("orb_method %r (ID and Rotations are broken in the "
                                   "reference)" % p.orb_method)
@@ -73,22 +156,34 @@ def energy(I, H, C, p, E_nuc=0.0, n=None, freeze_occ=False, printmode=True, chec
         E = E_orb
         E_diff = E - E_old
         E_old = E
-        Etmp, elag, _, maxdiff = ENERGY1r(C, n, H, I, cj12, ck12, p)
-        n, _ = api.ocupacion(gamma, p, I)
-        cj12, ck12 = api.PNOFi_selector(n, p, I)
-        grad_orb = api.calcorbg(np.zeros(p.nvar), n, cj12, ck12, C, H, I, p)
-        J, K, Hc = api.computeJKH_MO(C, H, I, p)
+        # diagnostics (energy.jl:172-190) from one transform of b: ENERGY1r's Lagrangian
+        # also gives J/K/H_core at these orbitals, and when the re-ordering left the
+        # occupations as they were, calcorbg(0) (pnof.jl:1389-1407) is the packed
+        # 4 (elag - elag^T) of that same Lagrangian
+        Etmp, elag, sumdiff, maxdiff, (J, K, Hc) = ENERGY1r(C, n, H, I, cj12, ck12, p,
+                                                            want_jkh=True)
+        n_new, _ = api.ocupacion(gamma, p, I)
+        if np.array_equal(n_new, n):
+            grad_orb = api.pack_grad(elag, p.nbf5)
+        else:
+            n = n_new
+            cj12, ck12 = api.PNOFi_selector(n, p, I)
+            grad_orb = api.calcorbg(np.zeros(p.nvar), n, cj12, ck12, C, H, I, p)
         grad_occ = api.calcoccg(gamma, J, K, Hc, p, I)
         gmax, gocc = float(np.max(np.abs(grad_orb))), float(np.linalg.norm(grad_occ))
         history.append({"i_ext": i_ext, "E": E, "Etot": E + E_nuc, "nit_orb": nit_orb,
-                        "nit_occ": nit_occ, "maxdiff": maxdiff, "grad_orb": gmax,
-                        "grad_occ": gocc})
+                        "nit_occ": nit_occ, "E1r": Etmp, "sumdiff": sumdiff,
+                        "maxdiff": maxdiff, "grad_orb": gmax, "grad_occ": gocc})
         if printmode:
             log("%6i %7i %10.1e %4i %10.1e %14.8f %14.8f %14.8f %10.6f   %4.1e   %4.1e"
                 % (i_ext, nit_orb, t2 - t1, nit_occ, t3 - t2, E, E + E_nuc, E_diff, maxdiff,
                    gmax, gocc))
         if checkpoint:  # energy.jl:209-213 writes (E, C, n) every iteration
-            np.savez(p.title + ".npz", E=Etmp, C=C, n=n)
+            save_checkpoint(p.title, C, n, Etmp)
         if gmax < p.threshgorb and gocc < p.threshgocc:
             break
+    if p.ipnof > 4 and elag is not None:  # energy.jl:222-224
+        C, n, elag = order_subspaces(C, n, elag, H, I, p)
+    if checkpoint:  # energy.jl:226-232 rewrites the file with (C, n)
+        save_checkpoint(p.title, C, n)
     return E_nuc + E, C, n, elag, history

@@ -37,6 +37,12 @@ def _weak(p, l):
     return p.no1 + p.ndns + (p.ndoc - l) + p.ndoc * np.arange(p.ncwo)
 
 
+def pair_weak(p, l):
+    """0-based columns of the weak orbitals of strong orbital l (1-based):
+    ll:ndoc:ul with ll = no1 + ndns + (ndoc - l) + 1 (pnof.jl:509-510)."""
+    return _weak(p, l)
+
+
 def n_to_gammas_softmax(n, p):
     """utils.jl:539-561 (reads n[i] as the reference does)."""
     g = np.zeros(p.nv)

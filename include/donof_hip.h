@@ -114,6 +114,16 @@ int dn_lagrange2(dn_sys* sys, const double* C, const double* n, const double* H,
                  const double* cj12, const double* ck12, int nalpha, int nbeta,
                  double* elag, double* Hmat, double* E);
 
+/* The diagnostics of one outer iteration of energy (energy.jl:172-190: ENERGY1r's
+ * Lagrangian, and computeJKH_MO at the same orbitals) from one pass over b: dn_lagrange2
+ * plus J, K (nbf5 x nbf5) and Hcore (nbf5) taken from the Lagrangian's own transform
+ * b_pq = C5^T b C5 and from H_mat, instead of a second J/K-MO build
+ * (integrals.jl:131-172).  Hmat may be NULL. */
+int dn_lagrange2_jkh(dn_sys* sys, const double* C, const double* n, const double* H,
+                     const double* cj12, const double* ck12, int nalpha, int nbeta,
+                     double* elag, double* Hmat, double* E, double* J, double* K,
+                     double* Hcore);
+
 /* Resident variants: inputs already in HBM (set once), outputs stay in HBM.  They
  * enqueue on the system's stream and return without synchronising. */
 int dn_sys_set_orbitals(dn_sys* sys, const double* C);

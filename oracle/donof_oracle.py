@@ -846,8 +846,47 @@ def order_occupations_softmax(old_C, old_gamma, p):
     return C, gamma
 
 
+def energy1r(C, n, H, b, cj12, ck12, p):
+    """ENERGY1r (utils.jl:117-136), no1 = 0 branch: the Lagrangian, E by the (H, C) form
+    of computeE_elec (utils.jl:191-214) and sumdiff/maxdiff of |elag - elag^T|."""
+    elag, _ = compute_lagrange2(C, n, H, b, cj12, ck12, p.nalpha, p.nbeta)
+    E = computeE_elec_hc(H, C, n, elag, p)
+    sumdiff, maxdiff = lagrange_convergency(elag)
+    return E, elag, sumdiff, maxdiff
+
+
+def order_subspaces(old_C, old_n, elag, H, b, p):
+    """guess.jl:58-106: the strong orbitals sorted by increasing elag_ii (stable, as
+    Julia's sortperm), each carrying its weak orbitals (pair columns ll:ndoc:ul) and
+    occupations along; the singly occupied block sorted the same way; frozen-core columns
+    kept (the reference sorts them and then copies them back); columns past nbf5 kept.
+    Then ENERGY1r at the sorted (C, n) gives the returned elag."""
+    C = np.zeros_like(old_C)
+    n = np.zeros_like(old_n)
+    d = np.diag(elag)
+    C[:, :p.no1] = old_C[:, :p.no1]
+    n[:p.no1] = old_n[:p.no1]
+    order = np.argsort(d[p.no1:p.nbeta], kind="stable")
+    for i, i_old in enumerate(order, start=1):
+        i_old = int(i_old) + 1
+        C[:, p.no1 + i - 1] = old_C[:, p.no1 + i_old - 1]
+        n[p.no1 + i - 1] = old_n[p.no1 + i_old - 1]
+        C[:, pair_weak(p, i)] = old_C[:, pair_weak(p, i_old)]
+        n[pair_weak(p, i)] = old_n[pair_weak(p, i_old)]
+    s0 = p.no1 + p.ndoc
+    order = np.argsort(d[s0:p.no1 + p.ndns], kind="stable")
+    for i, i_old in enumerate(order):
+        C[:, s0 + i] = old_C[:, s0 + int(i_old)]
+        n[s0 + i] = old_n[s0 + int(i_old)]
+    C[:, p.nbf5:] = old_C[:, p.nbf5:]
+    cj12, ck12 = pnof_selector(n, p)
+    _, elag_new, _, _ = energy1r(C, n, H, b, cj12, ck12, p)
+    return C, n, elag_new
+
+
 def energy_loop(b, H, C, p, n=None, freeze_occ=False):
-    """energy.jl:140-220 for given integrals (no1 = 0).  Returns (E, C, n, history)."""
+    """energy.jl:140-224 for given integrals (no1 = 0).  Returns (E, C, n, hist, elag);
+    hist rows are (E, max|grad_orb|, ||grad_occ||, ENERGY1r E, maxdiff)."""
     if n is None:
         gamma = (guess_gamma_softmax(p.ndoc, p.ncwo) if p.occ_method == "Softmax"
                  else guess_gamma_trigonometric(p.ndoc, p.ncwo))
@@ -855,6 +894,7 @@ def energy_loop(b, H, C, p, n=None, freeze_occ=False):
         gamma = n_to_gammas_softmax(n, p)
     E = 9999.0
     hist = []
+    elag = None
     for i_ext in range(1, p.maxit + 1):
         E_orb, C, nit_orb, _ = orbopt(p.orb_method, gamma, C, H, b, p)
         if p.ndoc > 0 and not freeze_occ:
@@ -863,15 +903,22 @@ def energy_loop(b, H, C, p, n=None, freeze_occ=False):
             if nit_occ < 1:
                 p.maxloop += 10
         n, _ = ocupacion(gamma, p)
+        cj12, ck12 = pnof_selector(n, p)
         if p.occ_method == "Softmax":
             C, gamma = order_occupations_softmax(C, gamma, p)
         E = E_orb
+        # ENERGY1r with the occupations of occoptr and the re-ordered orbitals
+        # (energy.jl:157-172)
+        Etmp, elag, _, maxdiff = energy1r(C, n, H, b, cj12, ck12, p)
         n, _ = ocupacion(gamma, p)
         cj12, ck12 = pnof_selector(n, p)
         g_orb = calcorbg(np.zeros(p.nvar), n, cj12, ck12, C, H, b, p)
         J, K, Hc = jkh_mo(C, H, b, p.nbf5)
         g_occ = calcoccg(gamma, J, K, Hc, p)
-        hist.append((E, float(np.max(np.abs(g_orb))), float(np.linalg.norm(g_occ))))
+        hist.append((E, float(np.max(np.abs(g_orb))), float(np.linalg.norm(g_occ)), Etmp,
+                     maxdiff))
         if np.max(np.abs(g_orb)) < p.threshgorb and np.linalg.norm(g_occ) < p.threshgocc:
             break
-    return E, C, n, hist
+    if p.ipnof > 4:                                             # energy.jl:222-224
+        C, n, elag = order_subspaces(C, n, elag, H, b, p)
+    return E, C, n, hist, elag

@@ -195,3 +195,26 @@ def test_errors_are_reported():
         api.computeJKH_MO(C, s.H, dev, s.p)
     assert e.value.code == DN_ERR_NONFINITE
     dev.close()
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5,ndoc", [SHAPES[1], SHAPES[2], SHAPES[5]])
+def test_lagrange2_jkh_one_transform(oracle, nbf, naux, nbf5, ndoc):
+    """dn_lagrange2_jkh: the Lagrangian and J/K/H_core of the same orbitals from one
+    transform of b, each against the oracle (utils.jl:1-59, integrals.jl:144-172)."""
+    print("case lag+jkh", nbf, naux, nbf5, flush=True)
+    s, b = _system(nbf, naux, nbf5, 300 + nbf, ndoc)
+    p = s.p
+    dev = api.System(p)
+    dev.upload_b(b)
+    cj12, ck12 = oracle.pnof_selector(s.n, p)
+    elag, Hmat, E, J, K, Hc = api.compute_Lagrange2_JKH(s.C, s.n, s.H, dev, cj12, ck12,
+                                                        p.nalpha, p.nbeta)
+    er, Hr = oracle.compute_lagrange2(s.C, s.n, s.H, b, cj12, ck12, p.nalpha, p.nbeta)
+    Jr, Kr, Hcr = oracle.jkh_mo(s.C, s.H, b, nbf5)
+    assert np.abs(elag - er).max() <= TOL
+    assert np.abs(Hmat - Hr).max() <= TOL
+    assert abs(E - oracle.computeE_elec_hmat(Hr, s.n, er, nbf5)) <= 1e-9
+    assert np.abs(J - Jr).max() <= TOL
+    assert np.abs(K - Kr).max() <= TOL
+    assert np.abs(Hc - Hcr).max() <= TOL
+    dev.close()
