@@ -206,8 +206,9 @@ class System:
         return ms.value, n.value
 
     def close(self):
+        """Frees the system; refused (DnError, DN_ERR_STATE) while a group holds it."""
         if getattr(self, "handle", None):
-            self.lib.dn_sys_destroy(self.handle)
+            check(self.lib.dn_sys_destroy(self.handle))
             self.handle = None
 
     def __del__(self):

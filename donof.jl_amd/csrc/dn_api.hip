@@ -27,19 +27,19 @@ int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nb
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
                 int nbf, int nbf5, double* out);
 // dn_split.hip
-int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int ld8, float* scale,
+int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int Kpad, float* scale,
                     float* inv_scale, _Float16* hi, _Float16* lo);
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
                     float* cscale, _Float16* hi, _Float16* lo);
-int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
              const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
              const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
              float* qmax);
-int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                  const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                  const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
                  float* qmax);
-int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
              const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
              const float* qmax, float* qs, float* qinv);
 int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
@@ -121,7 +121,7 @@ Layout layout_of(const dn_dims& d, int precision) {
   L.nvar = nbf * nbf5 - nbf5 * (nbf5 + 1) / 2;
   L.part2 = 16 * nbf * nbf;
   if (precision == DN_SPLIT_F16) {
-    L.bplane = (size_t)round_up(d.nbf, 8) * nbf * naux + 64;
+    L.bplane = (size_t)round_up(d.nbf, 32) * nbf * naux + 64;
     L.cplane = (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32);
     L.f_big = nbf5 * nbf * naux;
     L.f_bpq = nbf5 * nbf5 * naux;
@@ -284,7 +284,7 @@ int prepare_b(dn_sys* s) {
                     reinterpret_cast<unsigned long long*>(s->scal + 62), &dmax, &bmax));
   s->b_sym = dmax <= std::ldexp(bmax, -50);
   if (s->precision != DN_SPLIT_F16) return DN_OK;
-  return split_prepare_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->ld8, s->bscale, s->binv,
+  return split_prepare_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->Kpad, s->bscale, s->binv,
                          s->bhi, s->blo);
 }
 
@@ -315,13 +315,13 @@ int jkh_split(dn_sys* s) {
   const bool sym = s->b_sym && s->use_sym;
   DN_TRY(begin_timing(s));
   if (sym)
-    DN_TRY(split_k1_sym(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo,
+    DN_TRY(split_k1_sym(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
                         s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   else
-    DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo,
+    DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
                     s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   DN_TRY(end_timing(s));
-  DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale,
+  DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
                   s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
   if (sym) {  // bpq32 holds W with b_pq = W + W^T
     DN_TRY(symk(s->stream, reinterpret_cast<const SymKDesc*>(s->scal + 32), 1, s->symk_blocks,
@@ -405,6 +405,31 @@ bool all_finite(const double* x, size_t n) {
 }
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+int claim_lockstep(dn_sys* const* systems, int n, void* owner) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  for (int i = 0; i < n; ++i) {
+    if (!systems[i]) return fail(DN_ERR_ARG, "null system");
+    if (systems[i]->lockstep)
+      return fail(DN_ERR_STATE, "a system already belongs to a lock-step group");
+    for (int j = 0; j < i; ++j)
+      if (systems[j] == systems[i]) return fail(DN_ERR_ARG, "a system is listed twice");
+  }
+  for (int i = 0; i < n; ++i) systems[i]->lockstep = owner;
+  return DN_OK;
+}
+
+void release_lockstep(dn_sys* const* systems, int n, void* owner) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  for (int i = 0; i < n; ++i)
+    if (systems[i] && systems[i]->lockstep == owner) systems[i]->lockstep = nullptr;
+}
+
+void count_build_group(dn_sys* const* systems, int n, int delta) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  for (int i = 0; i < n; ++i)
+    if (systems[i]) systems[i]->build_groups += delta;
+}
 
 int fail(int code, const std::string& msg) {
   set_error(msg);
@@ -626,7 +651,6 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.part2, &s->part2);
   s->part2_elems = L.part2;
   if (precision == DN_SPLIT_F16) {
-    s->ld8 = round_up(dims->nbf, 8);
     s->Kpad = round_up(dims->nbf, 32);
     s->Mpad = round_up(dims->nbf5, 128);
     double* q = nullptr;
@@ -698,6 +722,11 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
 
 int dn_sys_destroy(dn_sys* s) {
   if (!s) return DN_OK;
+  {
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    if (s->lockstep || s->build_groups > 0)
+      return fail(DN_ERR_STATE, "the system belongs to a group; destroy the group first");
+  }
   (void)hipSetDevice(s->ctx->device);
   release(s);  // ordered after the system's queued work
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -127,6 +127,7 @@ using namespace dn;
 struct dn_group {
   dn_ctx* ctx = nullptr;
   std::vector<dn_sys*> sys;
+  bool counted = false;  // the top-level group: holds its systems' build_groups count
   int precision = DN_FP64;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -241,9 +242,9 @@ bool split_stage(const dn_group* G, int stage) {
 SplitGemm split_stage_gemm(dn_sys* s, int stage) {
   const dn_dims& d = s->d;
   if (stage == kK1)
-    return split_k1_sym_gemm(d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->ld8, s->bhi, s->blo, s->binv,
+    return split_k1_sym_gemm(d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo, s->binv,
                              s->chi, s->clo, s->cscale, s->bpn32, s->qmax);
-  return split_k2_gemm(d.nbf, d.nbf5, d.nbfaux, s->Kpad, s->chi, s->clo, s->cscale, s->bpn32,
+  return split_k2_gemm(d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale, s->bpn32,
                        s->bpq32, s->qs, s->qinv);
 }
 
@@ -491,7 +492,12 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
   // chains pay when their persistent launches are long enough to overlap; for small
   // systems the extra graph nodes cost more than the overlap gains (measured on MI355X:
   // 64 systems of nbf 16-64 build in 1.0 ms as one chain, 1.2 ms as four)
-  if (nsys < kChainMin || total < kChainWork) return group_create_leaf(ctx, systems, nsys, out);
+  if (nsys < kChainMin || total < kChainWork) {
+    DN_TRY(group_create_leaf(ctx, systems, nsys, out));
+    count_build_group(systems, nsys, +1);  // the systems may not be destroyed meanwhile
+    (*out)->counted = true;
+    return DN_OK;
+  }
   // a parent over chains: the systems dealt by build work, largest first, to the
   // least-loaded chain
   const int nch = std::min(kAux, nsys / 2);
@@ -514,6 +520,8 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
     }
     G->chains.push_back(leaf);
   }
+  count_build_group(systems, nsys, +1);
+  G->counted = true;
   *out = G;
   return DN_OK;
 }
@@ -659,6 +667,7 @@ int dn_group_destroy(dn_group* G) {
   if (!G) return DN_OK;
   (void)hipSetDevice(G->ctx->device);
   if (G->stream) (void)hipStreamSynchronize(G->stream);
+  if (G->counted) count_build_group(G->sys.data(), (int)G->sys.size(), -1);
   drop_graph(G);
   for (dn_group* c : G->chains) dn_group_destroy(c);
   G->chains.clear();

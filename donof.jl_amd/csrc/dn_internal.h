@@ -66,13 +66,15 @@ enum { kTriNone = 0, kTriB = 1, kTriA = 2 };
 // accumulated in FP32.
 struct SplitGemm {
   int M, N, K;
-  // A: either f16 planes [row][k] (ld = lda) or FP32 column-major A[m + k*lda]
+  // A: either tile-major f16 planes (row r, k) at (k/32) lda + 32 r + k%32 (lda = rows x 32)
+  // or FP32 column-major A[m + k*lda]
   const _Float16* ahi;
   const _Float16* alo;
   const float* af32;
   long lda, sA;             // batch stride (FP32 source)
   const float* a_scale;     // per batch: scale applied to the FP32 A before splitting
-  // B: f16 planes [col][k] (ld = ldb), batch stride sB (halves)
+  // B: tile-major f16 planes (col c, k) at (k/32) ldb + 32 c + k%32 (ldb = cols x 32),
+  // batch stride sB (halves)
   const _Float16* bhi;
   const _Float16* blo;
   long ldb, sB;
@@ -110,11 +112,11 @@ enum SplitConfig {
 bool split_config_ok(int config, int kind);
 // Operands of one system's split transforms (the same for the single-system and the
 // grouped builds) and the per-system kernels that precede each launch.
-SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Kpad, int ld8,
+SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Mpad, int Kpad,
                             const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                             const _Float16* chi, const _Float16* clo, const float* cscale_dev,
                             float* bpn, float* qmax);
-SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
                         const _Float16* clo, const float* cscale_dev, const float* bpn,
                         float* bpq, const float* qs, const float* qinv);
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
@@ -315,6 +317,11 @@ namespace dn {
 int dev_alloc(dn_ctx* c, hipStream_t s, size_t bytes, void** p);
 // give p back to the pool, ordered after the work queued on s
 void dev_free(dn_ctx* c, hipStream_t s, void* p);
+// group membership (dn_api.hip): a lock-step group claims its systems (none may already
+// belong to one, none listed twice) and releases them; grouped builds count their systems
+int claim_lockstep(dn_sys* const* systems, int n, void* owner);
+void release_lockstep(dn_sys* const* systems, int n, void* owner);
+void count_build_group(dn_sys* const* systems, int n, int delta);
 // a pinned host block of at least `bytes` (a cached one when one is large enough)
 int pinned_get(dn_ctx* c, size_t bytes, void** p, size_t* got);
 void pinned_put(dn_ctx* c, void* p, size_t bytes);
@@ -380,10 +387,10 @@ struct dn_sys {
   double* part2 = nullptr;  // 16*nbf*nbf split-K partials
   size_t part2_elems = 0;
   // split-precision J/K build (precision == DN_SPLIT_F16)
-  int ld8 = 0, Kpad = 0, Mpad = 0;
-  _Float16* bhi = nullptr;  // [(n,Q)][m] rows of ld8 halves: hi of b*scale[Q]
+  int Kpad = 0, Mpad = 0;
+  _Float16* bhi = nullptr;  // tile-major per Q (nbf columns n, k = m < Kpad): hi of b*scale[Q]
   _Float16* blo = nullptr;
-  _Float16* chi = nullptr;  // [Mpad][Kpad]: hi of C5^T*cscale
+  _Float16* chi = nullptr;  // tile-major Mpad rows, k < Kpad: hi of C5^T*cscale
   _Float16* clo = nullptr;
   float* bscale = nullptr;  // naux scales and inverses of b
   float* binv = nullptr;
@@ -401,6 +408,11 @@ struct dn_sys {
     double alpha = 0, beta1 = 0, binit = 0, beta2 = 0, threshgorb = 0;
     double E = 0, gmax = 0, best_E = 0, nrm = 0;
   } opt;
+  // group membership: the lock-step group driving the system (it runs on that group's
+  // stream meanwhile) and the number of grouped builds that read its buffers; a system
+  // that belongs to either is not destroyed
+  void* lockstep = nullptr;
+  int build_groups = 0;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
   dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
                                      // the split-precision one at scal + 32)

@@ -202,6 +202,7 @@ struct dn_ogroup {
   dn_ctx* ctx = nullptr;
   std::vector<dn_sys*> sys;
   std::vector<hipStream_t> saved_streams;
+  std::vector<dn_sys*> claimed;  // the systems whose membership this group holds
   hipStream_t stream = nullptr;
   struct Launch {
     int first, count, config;
@@ -573,14 +574,24 @@ int dn_ogroup_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_ogroup** 
     dn_ogroup_destroy(G);
     return rc;
   };
-  hipError_t e = hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
   for (int i = 0; i < nsys; ++i) {
     dn_sys* s = systems[i];
     if (!s || s->ctx != ctx) return cleanup(fail(DN_ERR_ARG, "systems must share the context"));
     if (!s->opt.started || s->opt.done)
       return cleanup(fail(DN_ERR_STATE, "every system needs a started, unfinished optimisation"));
     if (s->precision != DN_FP64) return cleanup(fail(DN_ERR_ARG, "lock-step groups are FP64"));
+  }
+  // a system runs on one group's stream at a time: none may belong to another group, and
+  // none may be listed twice
+  {
+    const int rc = claim_lockstep(systems, nsys, G);
+    if (rc != DN_OK) return cleanup(rc);
+    G->claimed.assign(systems, systems + nsys);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return cleanup(fail(DN_ERR_HIP, hipGetErrorString(e)));
+  for (int i = 0; i < nsys; ++i) {
+    dn_sys* s = systems[i];
     // the group drives the system on its own stream until it is destroyed
     (void)hipStreamSynchronize(s->stream);
     G->sys.push_back(s);
@@ -633,6 +644,7 @@ int dn_ogroup_destroy(dn_ogroup* G) {
   (void)hipSetDevice(G->ctx->device);
   if (G->stream) (void)hipStreamSynchronize(G->stream);
   for (size_t i = 0; i < G->sys.size(); ++i) G->sys[i]->stream = G->saved_streams[i];
+  release_lockstep(G->claimed.data(), (int)G->claimed.size(), G);
   drop_lgraph(G);
   if (G->stream) {
     for (int st = 0; st < kNumL; ++st) dev_free(G->ctx, G->stream, G->dev[st]);

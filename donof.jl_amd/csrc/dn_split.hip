@@ -9,6 +9,15 @@
 //   bpn[p,(n,Q)] = sum_m C5[m,p] b[m,(n,Q)]       A: C5^T planes, B: b planes
 //   bpq[:,:,Q]   = bpn[:,:,Q] C5                   A: bpn (FP32, split on load), B: C5 planes
 // and J, K (integrals.jl:163,166) are reduced from the FP32 bpq in FP64.
+//
+// Plane layout ("tile-major"): the contraction index k (the AO index m of b and C5) is cut
+// into tiles of 32; element (row r, k) of a plane with R rows sits at
+// (k / 32) * R * 32 + r * 32 + k % 32, so one 32-wide k tile of R rows is R * 64
+// contiguous bytes and the 16-row x 64-byte block that one load instruction moves is 1 KB
+// contiguous (row-major planes spread it over 16 rows of stride 2 nbf bytes; the
+// first transform at 800/3200/240 takes 2.9 ms with those and 2.3 ms with these).  The
+// per-k-tile stride R * 32 is the operand's lda / ldb; b planes hold one such block per
+// Q (batch stride sB = nbf * Kpad), with k padded to Kpad = 32 * ceil(nbf / 32) by zeros.
 #include <cmath>
 
 #include "dn_internal.h"
@@ -78,21 +87,25 @@ __global__ void b_slab_max_kernel(const double* __restrict__ b, long slab, int n
   }
 }
 
-// b planes: hi/lo of b * scale[Q] as [(n,Q)][m] with leading dimension ld8 >= ld >= nbf
-// (16-byte aligned rows; the padding stays zero: b's own pad row ld > nbf holds zeros)
-__global__ void b_split_kernel(const double* __restrict__ b, int ld, long slab, long total,
-                               int ld8, const float* __restrict__ scale,
+// b planes: hi/lo of b * scale[Q], tile-major per slab Q: (column n, k = m) at
+// Q nbf Kpad + (m / 32) nbf 32 + n 32 + m % 32 (k past nbf stays zero: the planes are
+// cleared at creation and b's own pad row ld > nbf holds zeros)
+__global__ void b_split_kernel(const double* __restrict__ b, int ld, int nbf, long slab,
+                               long total, int Kpad, const float* __restrict__ scale,
                                _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+  const long pslab = (long)nbf * Kpad;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
-    const double x = b[e] * (double)scale[e / slab];
-    const long col = e / ld;
-    const long o = col * ld8 + (e - col * ld);
+    const long Q = e / slab;
+    const double x = b[e] * (double)scale[Q];
+    const long r = e - Q * slab;
+    const int n = (int)(r / ld), m = (int)(r - (long)n * ld);
+    const long o = Q * pslab + (long)(m >> 5) * nbf * 32 + (long)n * 32 + (m & 31);
     split_f16_d(x, &hi[o], &lo[o]);
   }
 }
 
-// C5^T planes [Mpad][Kpad] (row = orbital p, k = AO index m), zero padded, in two
+// C5^T planes (row = orbital p < Mpad, k = AO index m < Kpad), tile-major, zero padded, in two
 // launches over the whole GPU: max |C5| into cscale[2] (bits of a non-negative float, which
 // order like the value; cleared first; one atomic per workgroup, at most 128 workgroups:
 // one per wavefront on a single address took 36 us at nbf 800), then the planes with the
@@ -119,7 +132,9 @@ __global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5,
   const long total = (long)Mpad * Kpad;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x) {
-    const int p = e / Kpad, k = e % Kpad;
+    // e runs over the tile-major plane: k tile, row p, k within the tile
+    const int kt = e / ((long)Mpad * 32), rem = e % ((long)Mpad * 32);
+    const int p = rem >> 5, k = kt * 32 + (rem & 31);
     _Float16 h = 0, l = 0;
     if (p < nbf5 && k < nbf) split_f16_d(C[k + (long)p * nbf] * (double)sc, &h, &l);
     hi[e] = h;
@@ -280,7 +295,8 @@ __device__ __forceinline__ void split_tile(const SplitGemm& g, int mt, int nt, i
       for (int i = 0; i < AV; ++i) {
         int r, c;
         stage_rc(tid + i * NT, &r, &c);
-        const long off = (long)(m0 + r) * g.lda + k0 + 8 * c;  // planes are row/K padded
+        // tile-major planes (rows and k padded): k tile kt0 + t at (kt0 + t) lda
+        const long off = (long)(kt0 + t) * g.lda + (long)(m0 + r) * 32 + 8 * c;
         rah[i] = *reinterpret_cast<const v8h*>(g.ahi + off);
         ral[i] = *reinterpret_cast<const v8h*>(g.alo + off);
       }
@@ -290,7 +306,7 @@ __device__ __forceinline__ void split_tile(const SplitGemm& g, int mt, int nt, i
       int r, c;
       stage_rc(tid + i * NT, &r, &c);
       const int col = min(n0 + r, N - 1);
-      const long off = (long)col * g.ldb + k0 + 8 * c;
+      const long off = (long)(kt0 + t) * g.ldb + (long)col * 32 + 8 * c;  // tile-major
       rbh[i] = *reinterpret_cast<const v8h*>(Bhi + off);
       rbl[i] = *reinterpret_cast<const v8h*>(Blo + off);
     }
@@ -464,23 +480,26 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
     const int rows = isA ? BM : BN;
     const int p = qq / (rows / 16), grp = qq % (rows / 16);
     const int r = 16 * grp + (lane & 15), c = lane >> 4;
+    // tile-major planes: the block of 16 rows x 64 bytes is 1 KB contiguous
     if (isA) {
-      src[i] = (p ? g.alo : g.ahi) + (long)(m0 + r) * g.lda + 8 * c;  // planes row-padded
+      src[i] = (p ? g.alo : g.ahi) + (long)(m0 + r) * 32 + 8 * c;
     } else {
       const int col = min(n0 + r, g.N - 1);
-      src[i] = (p ? g.blo : g.bhi) + (long)z * g.sB + (long)col * g.ldb + 8 * c;
+      src[i] = (p ? g.blo : g.bhi) + (long)z * g.sB + (long)col * 32 + 8 * c;
     }
     dst[i] = (isA ? 0 : A_BYTES) + p * rows * 64 + grp * 1024;
   }
   auto issue = [&](int t) {
-    const int kofs = (kt0 + t) * SB_K;
+    const long kofsA = (long)(kt0 + t) * g.lda, kofsB = (long)(kt0 + t) * g.ldb;
     unsigned char* base = smem + (t % NST) * STAGE;
 #if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: the host pass only needs the stub
 #pragma unroll
     for (int i = 0; i < LW; ++i)
-      __builtin_amdgcn_global_load_lds(src[i] + kofs, base + dst[i], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(src[i] + (wave + NW * i < NA ? kofsA : kofsB),
+                                       base + dst[i], 16, 0, 0);
 #else
-    (void)kofs;
+    (void)kofsA;
+    (void)kofsB;
     (void)base;
 #endif
   };
@@ -606,152 +625,6 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_grouped_f32a_kernel(
                    });
 }
 
-// ---- f16-plane GEMM, B streamed straight into registers ---------------------------
-//
-// For the first transform, where B (the b planes) is the large stream and A (C5^T
-// planes) is small and shared: every wavefront owns all BM = 128 rows of the tile and FN
-// 16-column fragments, loads its B fragments from global memory directly in MFMA layout
-// (lane l: column l % 16, k = 8 (l / 16)..+7, 16 contiguous bytes), one k tile ahead, and
-// reads only A from LDS.  LDS traffic per MFMA is then set by the A fragments alone
-// (FN MFMA columns share each), a quarter of what a tile staging both operands needs.
-template <int FN, int NWAVE, bool TRIB>
-__global__ __launch_bounds__(64 * NWAVE) void split_bdirect_kernel(SplitGemm g, int tiles_m,
-                                                                  int tiles_n) {
-  constexpr int BM = 128, FM = BM / 16, WN = 16 * FN, BN = WN * NWAVE, NT = 64 * NWAVE;
-  constexpr int AV = BM * 4 / NT;  // A chunks per thread and plane
-  static_assert(BM * 4 % NT == 0 && AV >= 1, "A staging");
-  __shared__ _Float16 Ah[2][BM * SB_K], Al[2][BM * SB_K];
-
-  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
-  if (blockIdx.x >= total) return;
-  const unsigned lid = xcd_remap16(blockIdx.x, total);
-  const int mt = lid % tiles_m;
-  const int nt = (lid / tiles_m) % tiles_n;
-  const int z = lid / (tiles_m * tiles_n);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = mt * BM, n0 = nt * BN;
-  const int kt0 = TRIB ? n0 / SB_K : 0;
-  const int nkt = (g.K + SB_K - 1) / SB_K - kt0;
-  const int fr = lane & 15, fc = lane >> 4;
-
-  // B: this lane's column of each fragment j (clamped; columns past N are not stored)
-  const _Float16* bh_p[FN];
-  const _Float16* bl_p[FN];
-#pragma unroll
-  for (int j = 0; j < FN; ++j) {
-    const int col = min(n0 + wave * WN + j * 16 + fr, g.N - 1);
-    const long off = (long)z * g.sB + (long)col * g.ldb + 8 * fc;
-    bh_p[j] = g.bhi + off;
-    bl_p[j] = g.blo + off;
-  }
-  // A staging: chunk (r, c) of the K-major LDS tile per thread and plane
-  const _Float16* ah_p[AV];
-  const _Float16* al_p[AV];
-  int a_lds[AV];
-#pragma unroll
-  for (int i = 0; i < AV; ++i) {
-    int r, c;
-    stage_rc(tid + i * NT, &r, &c);
-    const long off = (long)(m0 + r) * g.lda + 8 * c;  // planes are row/K padded
-    ah_p[i] = g.ahi + off;
-    al_p[i] = g.alo + off;
-    a_lds[i] = chunk_off(BM, r, c);
-  }
-
-  v4f acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-
-  v8h rah[AV], ral[AV];
-  auto load_a = [&](int t) {
-    const int k0 = (kt0 + t) * SB_K;
-#pragma unroll
-    for (int i = 0; i < AV; ++i) {
-      rah[i] = *reinterpret_cast<const v8h*>(ah_p[i] + k0);
-      ral[i] = *reinterpret_cast<const v8h*>(al_p[i] + k0);
-    }
-  };
-  auto store_a = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < AV; ++i) {
-      *reinterpret_cast<v8h*>(&Ah[buf][a_lds[i]]) = rah[i];
-      *reinterpret_cast<v8h*>(&Al[buf][a_lds[i]]) = ral[i];
-    }
-  };
-  auto load_b = [&](int t, v8h (&bh)[FN], v8h (&bl)[FN]) {
-    const int k0 = (kt0 + t) * SB_K;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      bh[j] = *reinterpret_cast<const v8h*>(bh_p[j] + k0);
-      bl[j] = *reinterpret_cast<const v8h*>(bl_p[j] + k0);
-    }
-  };
-  auto step = [&](int t, int buf, v8h (&bh)[FN], v8h (&bl)[FN]) {
-    if constexpr (TRIB) {
-      // Lh: m = kk + 8 fc + e against column n; only k tiles reaching into the block's
-      // diagonal need it (a uniform branch)
-      const int kk = (kt0 + t) * SB_K;
-      if (kk < n0 + BN) {
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          const int d = kk + 8 * fc - (n0 + wave * WN + j * 16 + fr);
-          v8h f;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {  // bit patterns of 1.0, 0.5, 0.0 in FP16
-            const unsigned short bits = d + e > 0 ? 0x3C00 : (d + e == 0 ? 0x3800 : 0);
-            f[e] = __builtin_bit_cast(_Float16, bits);
-          }
-          bh[j] *= f;
-          bl[j] *= f;
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-      const v8h ah = *reinterpret_cast<const v8h*>(&Ah[buf][chunk_off(BM, i * 16 + fr, fc)]);
-      const v8h al = *reinterpret_cast<const v8h*>(&Al[buf][chunk_off(BM, i * 16 + fr, fc)]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
-    }
-  };
-
-  // two B register sets, ping-pong: tile t + 1's fragments load while tile t multiplies
-  v8h bh0[FN], bl0[FN], bh1[FN], bl1[FN];
-  if (nkt > 0) {
-    load_a(0);
-    load_b(0, bh0, bl0);
-    store_a(0);
-  }
-  __syncthreads();
-  for (int t = 0; t < nkt; t += 2) {
-    if (t + 1 < nkt) {
-      load_a(t + 1);
-      load_b(t + 1, bh1, bl1);
-    }
-    step(t, 0, bh0, bl0);
-    if (t + 1 < nkt) store_a(1);
-    __syncthreads();
-    if (t + 1 >= nkt) break;
-    if (t + 2 < nkt) {
-      load_a(t + 2);
-      load_b(t + 2, bh0, bl0);
-    }
-    step(t + 1, 1, bh1, bl1);
-    if (t + 2 < nkt) store_a(0);
-    __syncthreads();
-  }
-  split_epilogue<FM, FN>(g, acc, m0, n0 + wave * WN, n0, z, lane);
-}
-
 // per-batch scale for the FP32 A of the second transform
 __global__ void bpn_scale_kernel(const float* __restrict__ qmax, int naux,
                                  float* __restrict__ scale, float* __restrict__ inv) {
@@ -805,7 +678,8 @@ __global__ __launch_bounds__(256) void group_c_planes_kernel(const SplitPrepDesc
   const long e = (blockIdx.x - d.plane_begin) * 256 + threadIdx.x;
   if (e >= (long)d.Mpad * d.Kpad) return;
   const float sc = d.cscale[0];
-  const int p = e / d.Kpad, k = e % d.Kpad;
+  const int kt = e / ((long)d.Mpad * 32), rem = e % ((long)d.Mpad * 32);
+  const int p = rem >> 5, k = kt * 32 + (rem & 31);  // tile-major, as c_planes_kernel
   _Float16 h = 0, l = 0;
   if (p < d.nbf5 && k < d.nbf) split_f16_d(d.C[k + (long)p * d.nbf] * (double)sc, &h, &l);
   d.chi[e] = h;
@@ -858,14 +732,14 @@ static unsigned grid_of(long n) {
   return (unsigned)std::max<long>(1, std::min<long>(b, 8192));
 }
 
-int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int ld8,
+int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int Kpad,
                     float* scale, float* inv_scale, _Float16* hi, _Float16* lo) {
   const long slab = (long)ld * nbf;  // pad rows hold zeros: they change no maximum
   hipLaunchKernelGGL(b_slab_max_kernel, dim3(naux), dim3(256), 0, s, b, slab, naux, scale,
                      inv_scale);
   DN_HIP(hipGetLastError());
-  hipLaunchKernelGGL(b_split_kernel, dim3(grid_of(slab * naux)), dim3(256), 0, s, b, ld, slab,
-                     slab * naux, ld8, scale, hi, lo);
+  hipLaunchKernelGGL(b_split_kernel, dim3(grid_of(slab * naux)), dim3(256), 0, s, b, ld, nbf,
+                     slab, slab * naux, Kpad, scale, hi, lo);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -895,36 +769,44 @@ static int launch_split(hipStream_t s, const SplitGemm& g) {
 
 // The split-precision J/K build: C planes must be prepared (split_prepare_c) for the
 // current orbitals; bpn (nbf5*nbf*naux floats), bpq (nbf5*nbf5*naux floats), qmax,
-// qs/qinv (naux floats each) are workspaces.  b planes have rows of ld8 halves; the
-// C planes are [Mpad][Kpad] with Mpad, Kpad multiples of 128 and 32.
-int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+// qs/qinv (naux floats each) are workspaces.  The planes are tile-major: b per slab with
+// nbf columns (k tile stride nbf * 32, slab stride nbf * Kpad), C5^T with Mpad rows (k tile
+// stride Mpad * 32); Mpad and Kpad are multiples of 128 and 32.
+// General b: bpn[p,n,Q] = sum_m C5[m,p] b[m,n,Q], batched over Q (the LDS-direct tile).
+int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
              const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
              const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
              float* qmax) {
   hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_of(naux)), dim3(256), 0, s, qmax, (long)naux, 0.f);
   DN_HIP(hipGetLastError());
   SplitGemm g1{};
-  g1.M = nbf5; g1.N = nbf * naux; g1.K = nbf;
-  g1.ahi = chi; g1.alo = clo; g1.lda = Kpad;
-  g1.bhi = bhi; g1.blo = blo; g1.ldb = ld8;
-  g1.C = bpn; g1.ldc = nbf5; g1.sC = 0;
-  g1.cs = cscale_dev; g1.col_inv_scale = b_inv_scale; g1.col_div = nbf;
-  g1.qmax = qmax; g1.nbatch = 1;
-  return launch_split<128, 128, 2, 2, false>(s, g1);
+  g1.M = nbf5; g1.N = nbf; g1.K = nbf;
+  g1.ahi = chi; g1.alo = clo; g1.lda = (long)Mpad * 32;
+  g1.bhi = bhi; g1.blo = blo; g1.ldb = (long)nbf * 32; g1.sB = (long)nbf * Kpad;
+  g1.C = bpn; g1.ldc = nbf5; g1.sC = (long)nbf5 * nbf;
+  g1.cs = cscale_dev; g1.batch_inv_scale = b_inv_scale; g1.col_div = 0;
+  g1.qmax = qmax; g1.nbatch = naux;
+  const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
+  const long total = (long)tm * tn * g1.nbatch;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split gemm grid too large");
+  hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 2, 2, false>), dim3((unsigned)total),
+                     dim3(256), 0, s, g1, tm, tn);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
 }
 
 // Symmetric b: bpn[p,n,Q] = sum_{m >= n} C5[m,p] Lh[m,n,Q] (b_Q = Lh + Lh^T), batched
 // over Q so that column block n0 reads only rows m >= n0 of the slab; the second
 // transform then yields W with b_pq = W + W^T.  split_k1_begin clears the per-Q maxima
 // first.
-SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Kpad, int ld8,
+SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Mpad, int Kpad,
                             const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                             const _Float16* chi, const _Float16* clo, const float* cscale_dev,
                             float* bpn, float* qmax) {
   SplitGemm g1{};
   g1.M = nbf5; g1.N = nbf; g1.K = nbf;
-  g1.ahi = chi; g1.alo = clo; g1.lda = Kpad;
-  g1.bhi = bhi; g1.blo = blo; g1.ldb = ld8; g1.sB = (long)nbf * ld8;
+  g1.ahi = chi; g1.alo = clo; g1.lda = (long)Mpad * 32;
+  g1.bhi = bhi; g1.blo = blo; g1.ldb = (long)nbf * 32; g1.sB = (long)nbf * Kpad;
   g1.C = bpn; g1.ldc = nbf5; g1.sC = (long)nbf5 * nbf;
   g1.cs = cscale_dev; g1.batch_inv_scale = b_inv_scale; g1.col_div = 0;
   g1.qmax = qmax; g1.nbatch = naux;
@@ -937,12 +819,12 @@ int split_k1_begin(hipStream_t s, float* qmax, int naux) {
   return DN_OK;
 }
 
-int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
+int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                  const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                  const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
                  float* qmax) {
   DN_TRY(split_k1_begin(s, qmax, naux));
-  const SplitGemm g1 = split_k1_sym_gemm(nbf, nbf5, naux, Kpad, ld8, bhi, blo, b_inv_scale, chi,
+  const SplitGemm g1 = split_k1_sym_gemm(nbf, nbf5, naux, Mpad, Kpad, bhi, blo, b_inv_scale, chi,
                                          clo, cscale_dev, bpn, qmax);
   const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
   const long total = (long)tm * tn * g1.nbatch;
@@ -955,13 +837,13 @@ int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, int ld8,
 
 // bpq[:,:,Q] = bpn[:,:,Q] C5 with bpn split on load under the per-Q scale that
 // split_k2_begin derives from the first transform's maxima.
-SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
                         const _Float16* clo, const float* cscale_dev, const float* bpn,
                         float* bpq, const float* qs, const float* qinv) {
   SplitGemm g2{};
   g2.M = nbf5; g2.N = nbf5; g2.K = nbf;
   g2.af32 = bpn; g2.lda = nbf5; g2.sA = (long)nbf5 * nbf; g2.a_scale = qs;
-  g2.bhi = chi; g2.blo = clo; g2.ldb = Kpad;
+  g2.bhi = chi; g2.blo = clo; g2.ldb = (long)Mpad * 32;
   g2.C = bpq; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
   g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.col_div = 0;
   g2.nbatch = naux;
@@ -975,11 +857,11 @@ int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float*
   return DN_OK;
 }
 
-int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Kpad, const _Float16* chi,
+int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
              const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
              const float* qmax, float* qs, float* qinv) {
   DN_TRY(split_k2_begin(s, qmax, naux, qs, qinv));
-  const SplitGemm g = split_k2_gemm(nbf, nbf5, naux, Kpad, chi, clo, cscale_dev, bpn, bpq, qs, qinv);
+  const SplitGemm g = split_k2_gemm(nbf, nbf5, naux, Mpad, chi, clo, cscale_dev, bpn, bpq, qs, qinv);
   // nbf5 in (128, 256]: one 256-wide N tile reads each FP32 slab of T once (1.37 against
   // 1.47 ms at 800/3200/240 on MI355X, same sums)
   if (nbf5 > 128 && nbf5 <= 256) return launch_split<128, 256, 2, 4, true>(s, g);

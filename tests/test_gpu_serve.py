@@ -97,3 +97,36 @@ def test_admission_by_hbm_budget(mode):
 def test_request_larger_than_budget_is_rejected():
     res = _serve(requests(2, rate=0.0), "grouped", hbm_budget=1 << 20, max_live=2)
     assert all("budget" in r.error for r in res)
+
+
+def test_group_membership_misuse_is_refused():
+    """A system belongs to at most one lock-step group, is listed once, and is not
+    destroyed while a group (lock-step or grouped build) holds it."""
+    from donof_amd._lib import DN_ERR_ARG, DN_ERR_STATE
+    reqs = requests(2)
+    ctx = api.Context(0)
+    (a, pa), (b, pb) = [serve.start_solve(r, ctx) for r in reqs]
+    g = api.OGroup([a, b], ctx)
+    with pytest.raises(api.DnError) as e:
+        api.OGroup([a], ctx)                      # already in a lock-step group
+    assert e.value.code == DN_ERR_STATE
+    with pytest.raises(api.DnError) as e:
+        a.close()                                 # destroy while grouped
+    assert e.value.code == DN_ERR_STATE and a.handle
+    g.step()                                      # the group still works
+    g.close()
+    with pytest.raises(api.DnError) as e:
+        api.OGroup([a, a], ctx)                   # listed twice
+    assert e.value.code == DN_ERR_ARG
+    g2 = api.OGroup([a], ctx)                     # membership released by close
+    g2.close()
+    bg = api.Group([b], ctx)
+    with pytest.raises(api.DnError) as e:
+        b.close()                                 # held by a grouped build
+    assert e.value.code == DN_ERR_STATE
+    bg.close()
+    a.orbopt_finish(pa)
+    b.orbopt_finish(pb)
+    a.close()
+    b.close()
+    ctx.close()
