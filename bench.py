@@ -83,14 +83,14 @@ def executed_flops_sym_k1(nbf, naux, nbf5, precision="fp64"):
     runs the 16-wide k tiles from n0 to the end (earlier ones are masked to zero and
     skipped), rows past nbf and 16-column fragments past nbf5 are skipped.
     split: T[p,n,Q] with 128-column blocks; a 16-column fragment starting at c0 runs the
-    32-wide k tiles from the one holding c0 to the end; the p rows are padded to 128 and
-    every fragment pair issues three products (hi*hi, hi*lo, lo*hi)."""
+    32-wide k tiles from the one holding c0 to the end; 16-row fragments past nbf5 are
+    skipped; every fragment pair issues three products (hi*hi, hi*lo, lo*hi)."""
     if precision == "fp64":
         nkt = -(-nbf // 16)
         frag_cols = -(-nbf5 // 16)
         ktiles = sum(nkt - r0 // 16 for r0 in range(0, nbf, 16))
         return 2.0 * 16 * 16 * 16 * frag_cols * ktiles * naux
-    mpad = -(-nbf5 // 128) * 128
+    mpad = -(-nbf5 // 16) * 16        # fragment rows past nbf5 are skipped too
     nkt = -(-nbf // 32)
     ktiles = sum(max(0, nkt - c0 // 32) for c0 in range(0, -(-nbf // 128) * 128, 16))
     return 3 * 2.0 * mpad * 16 * 32 * ktiles * naux

@@ -39,6 +39,13 @@ int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                  const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                  const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
                  float* qmax);
+int split_lhnorm(hipStream_t s, const double* b, int nbf, int ld, int naux, float* lhmax);
+int split_sym_scales(hipStream_t s, int naux, const float* cscale_dev, const float* lhmax,
+                     float* qs, float* qinv);
+int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
+                     const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                     const _Float16* chi, const _Float16* clo, const float* cscale_dev,
+                     const float* qs, const float* qinv, _Float16* tp, float* wt, int part);
 int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
              const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
              const float* qmax, float* qs, float* qinv);
@@ -74,7 +81,7 @@ struct Layout {
   static size_t d8(size_t halves) { return (halves + 3) / 4; }   // halves -> doubles
   static size_t f8(size_t floats) { return (floats + 1) / 2; }   // floats -> doubles
   size_t split_doubles() const {
-    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 5 * f8(f_naux) + 1;
+    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 1;
   }
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 3 + n + big + bpn + cp + bmat + 2 * xw +
@@ -123,7 +130,8 @@ Layout layout_of(const dn_dims& d, int precision) {
   if (precision == DN_SPLIT_F16) {
     L.bplane = (size_t)round_up(d.nbf, 32) * nbf * naux + 64;
     L.cplane = (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32);
-    L.f_big = nbf5 * nbf * naux;
+    // FP32 T (general b) or the two FP16 planes of T (symmetric b), whichever is larger
+    L.f_big = std::max(nbf5 * nbf, (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32)) * naux;
     L.f_bpq = nbf5 * nbf5 * naux;
     L.f_naux = naux;
   }
@@ -137,6 +145,18 @@ int alloc(dn_sys* s, size_t elems, double** out) {
   s->owned.push_back(DevBuf{p, bytes});
   *out = static_cast<double*>(p);
   return DN_OK;
+}
+
+// the second stream and its events, once its queued work is done
+void drop_aux(dn_sys* s) {
+  if (s->aux) {
+    (void)hipStreamSynchronize(s->aux);
+    (void)hipStreamDestroy(s->aux);
+  }
+  if (s->fork_ev) (void)hipEventDestroy(s->fork_ev);
+  if (s->join_ev) (void)hipEventDestroy(s->join_ev);
+  s->aux = nullptr;
+  s->fork_ev = s->join_ev = nullptr;
 }
 
 // every buffer back to the pool once the system's queued work is done
@@ -284,6 +304,7 @@ int prepare_b(dn_sys* s) {
                     reinterpret_cast<unsigned long long*>(s->scal + 62), &dmax, &bmax));
   s->b_sym = dmax <= std::ldexp(bmax, -50);
   if (s->precision != DN_SPLIT_F16) return DN_OK;
+  DN_TRY(split_lhnorm(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->lhmax));
   return split_prepare_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->Kpad, s->bscale, s->binv,
                          s->bhi, s->blo);
 }
@@ -313,23 +334,44 @@ int jkh_split(dn_sys* s) {
   DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
                          s->clo));
   const bool sym = s->b_sym && s->use_sym;
+  if (sym) {
+    // H C5 and H_core need only C and H: on the second stream, beside the transforms (a
+    // plain GEMM: its split-K workspace is the J GEMM's)
+    DN_HIP(hipEventRecord(s->fork_ev, s->stream));
+    DN_HIP(hipStreamWaitEvent(s->aux, s->fork_ev, 0));
+    {
+      Gemm g;
+      g.M = d.nbf; g.N = d.nbf5; g.K = d.nbf;
+      g.A = s->H; g.lda = d.nbf; g.a_kcontig = false;
+      g.B = s->C; g.ldb = d.nbf; g.b_kcontig = true;
+      g.C = s->HC; g.ldc = d.nbf;
+      DN_TRY(gemm_f64(s->aux, g));
+      DN_TRY(hcore_diag(s->aux, s->C, s->HC, d.nbf, d.nbf5, s->Hcore));
+    }
+    DN_HIP(hipEventRecord(s->join_ev, s->aux));
+    // both transforms on FP16 planes; bpq32 then holds W^T with b_pq = W + W^T
+    _Float16* tp = reinterpret_cast<_Float16*>(s->bpn32);
+    DN_TRY(split_sym_scales(s->stream, d.nbfaux, s->cscale, s->lhmax, s->qs, s->qinv));
+    DN_TRY(begin_timing(s));
+    DN_TRY(split_sym_planes(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi,
+                            s->blo, s->binv, s->chi, s->clo, s->cscale, s->qs, s->qinv, tp,
+                            s->bpq32, 1));
+    DN_TRY(end_timing(s));
+    DN_TRY(split_sym_planes(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi,
+                            s->blo, s->binv, s->chi, s->clo, s->cscale, s->qs, s->qinv, tp,
+                            s->bpq32, 2));
+    DN_TRY(symk(s->stream, reinterpret_cast<const SymKDesc*>(s->scal + 32), 1, s->symk_blocks,
+                s->symk_red, true, d.nbf5 % 4 == 0));
+    DN_TRY(j_from_x(s->stream, s->X, d.nbf5, d.nbfaux, s->part2, s->part2_elems, s->J));
+    DN_HIP(hipStreamWaitEvent(s->stream, s->join_ev, 0));
+    return DN_OK;
+  }
   DN_TRY(begin_timing(s));
-  if (sym)
-    DN_TRY(split_k1_sym(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
-                        s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
-  else
-    DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
-                    s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
+  DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
+                  s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   DN_TRY(end_timing(s));
   DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
                   s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
-  if (sym) {  // bpq32 holds W with b_pq = W + W^T
-    DN_TRY(symk(s->stream, reinterpret_cast<const SymKDesc*>(s->scal + 32), 1, s->symk_blocks,
-                s->symk_red, true));
-    DN_TRY(j_from_x(s->stream, s->X, d.nbf5, d.nbfaux, s->part2, s->part2_elems, s->J));
-    DN_TRY(hc_products(s, false));
-    return hcore_diag(s->stream, s->C, s->HC, d.nbf, d.nbf5, s->Hcore);
-  }
   DN_TRY(jk_from_bpq_f32(s->stream, s->bpq32, (long)d.nbf5 * d.nbf5, d.nbf5, d.nbfaux, s->X,
                          s->part, s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hc_products(s, false));
@@ -650,6 +692,12 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf2, &s->bestC);
   A(L.part2, &s->part2);
   s->part2_elems = L.part2;
+  if (precision == DN_SPLIT_F16 && rc == DN_OK) {
+    if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming) != hipSuccess)
+      rc = fail(DN_ERR_HIP, "second stream of a split-precision system");
+  }
   if (precision == DN_SPLIT_F16) {
     s->Kpad = round_up(dims->nbf, 32);
     s->Mpad = round_up(dims->nbf5, 128);
@@ -665,6 +713,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     A(Layout::f8(L.f_naux), &q); s->qmax = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->qs = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->qinv = reinterpret_cast<float*>(q);
+    A(Layout::f8(L.f_naux), &q); s->lhmax = reinterpret_cast<float*>(q);
     A(2, &q); s->cscale = reinterpret_cast<float*>(q);  // scale, 1/scale, max bits
     if (rc == DN_OK) {
       // plane padding must read as zero
@@ -707,6 +756,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   }
   if (rc != DN_OK) {
     std::string msg = g_last_error;
+    drop_aux(s);
     release(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     {
@@ -728,6 +778,7 @@ int dn_sys_destroy(dn_sys* s) {
       return fail(DN_ERR_STATE, "the system belongs to a group; destroy the group first");
   }
   (void)hipSetDevice(s->ctx->device);
+  drop_aux(s);
   release(s);  // ordered after the system's queued work
   if (s->stream) (void)hipStreamDestroy(s->stream);
   for (auto ev : s->ev_start) (void)hipEventDestroy(ev);

@@ -170,6 +170,7 @@ struct dn_group {
   DiagDesc* diag = nullptr;
   SymKDesc* symk = nullptr;  // systems with the symmetric build
   int nsym = 0;
+  bool symk_vec4 = false;
   long symk_blocks = 0, symk_red = 0;
   int nkx = 0;               // systems with the full build
   long ncols = 0;
@@ -396,7 +397,8 @@ int launch_build(dn_group* G) {
   DN_TRY(launch_stage(G, kK1));
   if (split) DN_TRY(split_group_scales(G->stream, G->sprep, n, G->q_blocks));
   DN_TRY(launch_stage(G, kK2));
-  DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red, split));
+  DN_TRY(symk(G->stream, G->symk, G->nsym, G->symk_blocks, G->symk_red, split,
+              split && G->symk_vec4));
   if (G->nkx) {
     const int m = G->nkx;
     hipLaunchKernelGGL(group_kpart_kernel, dim3(G->kx_blocks[0]), dim3(256), 0, G->stream, G->kx, m);
@@ -648,6 +650,8 @@ static int group_create_leaf(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_g
   G->ncols = col;
   G->nsym = (int)sk.size();
   G->nkx = (int)kx.size();
+  G->symk_vec4 = true;  // 16-byte loads of the FP32 W when every nbf5 is a multiple of 4
+  for (const SymKDesc& k : sk) G->symk_vec4 = G->symk_vec4 && k.nbf5 % 4 == 0;
   symk_plan(sk.data(), G->nsym, &G->symk_blocks, &G->symk_red, G->precision == DN_SPLIT_F16);
   if (G->nsym)
     e = hipMemcpy(G->symk, sk.data(), sk.size() * sizeof(SymKDesc), hipMemcpyHostToDevice);

@@ -90,6 +90,15 @@ struct SplitGemm {
   float* qmax;                   // per Q max |out| (K1: by column group, or by batch when
                                  // col_div == 0), nullable
   int nbatch;
+  // planes output of the symmetric first transform (kernels instantiated for it): hi/lo
+  // FP16 of out * out_scale[z], tile-major with C's rows as plane columns and C's columns
+  // as k: element (r, c) of batch z at z out_sb + (c / 32) out_ld + 32 r + c % 32, written
+  // for c < 32 out_kt (zero for c >= N); the second transform reads them as its B planes
+  _Float16* ohi;
+  _Float16* olo;
+  long out_ld, out_sb;
+  int out_kt;
+  const float* out_scale;
 };
 
 // One problem of a grouped split-precision launch (offsets as for GroupedGemm; nsplit is
@@ -216,8 +225,9 @@ struct SymKDesc {
 void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks, bool f32 = false);
 int j_from_x(hipStream_t s, const double* X, int nbf5, int naux, double* part2,
              size_t part2_elems, double* J);
+// vec4: FP32 W whose nbf5 is a multiple of 4 for every system (16-byte loads)
 int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks,
-         bool f32 = false);
+         bool f32 = false, bool vec4 = false);
 
 // ---- device buffers -----------------------------------------------------------------
 struct DevBuf {
@@ -332,6 +342,10 @@ struct dn_sys {
   dn_dims d{};
   int precision = DN_FP64;
   hipStream_t stream = nullptr;
+  // split-precision systems: H C5 and H_core run on a second stream beside the transforms
+  // (forked and joined with the two events)
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::vector<dn::DevBuf> owned;
   uint64_t bytes = 0;
   // resident state (column-major)
@@ -395,7 +409,9 @@ struct dn_sys {
   float* bscale = nullptr;  // naux scales and inverses of b
   float* binv = nullptr;
   float* cscale = nullptr;  // (scale, 1/scale) of C5
-  float* bpn32 = nullptr;   // nbf5*nbf*naux
+  float* bpn32 = nullptr;   // max(nbf5*nbf, Mpad*Kpad)*naux: FP32 T (general b) or the
+                            // hi/lo planes of T (symmetric b), tile-major Mpad x Kpad per Q
+  float* lhmax = nullptr;   // naux: max_n ||Lh_Q[:,n]|| (the symmetric T's scale bound)
   float* bpq32 = nullptr;   // nbf5*nbf5*naux
   float* qmax = nullptr;    // naux
   float* qs = nullptr;      // naux

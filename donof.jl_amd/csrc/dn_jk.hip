@@ -109,7 +109,10 @@ __device__ __forceinline__ int find_symk(const SymKDesc* __restrict__ d, int n, 
   return __builtin_amdgcn_readfirstlane(lo);  // uniform: scalar descriptor loads
 }
 
-template <typename T>
+// VEC (FP32 W with nbf5 % 4 == 0): each thread moves four consecutive rows p of one column
+// q of each tile with one 16-byte load, 32 x 32 tiles in 256 loads, instead of four 4-byte
+// loads per thread and tile.
+template <typename T, bool VEC = false>
 __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __restrict__ desc,
                                                            int n) {
   __shared__ double a[32][33], b[32][33];
@@ -136,21 +139,45 @@ __global__ __launch_bounds__(256) void symk_partial_kernel(const SymKDesc* __res
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // the next slice's elements are loaded into registers while this slice is reduced
   double ra[4], rb[4];
+  // VEC: rows p4 .. p4 + 3 (local) of local column c4
+  const int p4 = 4 * (threadIdx.x & 7), c4 = threadIdx.x >> 3;
   auto load = [&](int Q) {
     const T* W = Wall + (long)Q * nn;
+    if constexpr (VEC) {
+      typedef float v4 __attribute__((ext_vector_type(4)));
+      const int pa = I * 32 + p4, qa = J * 32 + c4;  // tile (I, J)
+      const int pb = J * 32 + p4, qb = I * 32 + c4;  // tile (J, I)
+      v4 va = {0.f, 0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f, 0.f};
+      if (pa < n5 && qa < n5) va = *reinterpret_cast<const v4*>(W + pa + (long)qa * n5);
+      if (!diag && pb < n5 && qb < n5) vb = *reinterpret_cast<const v4*>(W + pb + (long)qb * n5);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int r = ty + 8 * k, qJ = J * 32 + r, qI = I * 32 + r;
-      ra[k] = (pI < n5 && qJ < n5) ? (double)W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
-      rb[k] = (!diag && pJ < n5 && qI < n5) ? (double)W[pJ + (long)qI * n5] : 0.0;  // (J, I)
+      for (int e = 0; e < 4; ++e) {
+        ra[e] = (double)va[e];
+        rb[e] = (double)vb[e];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = ty + 8 * k, qJ = J * 32 + r, qI = I * 32 + r;
+        ra[k] = (pI < n5 && qJ < n5) ? (double)W[pI + (long)qJ * n5] : 0.0;  // tile (I, J)
+        rb[k] = (!diag && pJ < n5 && qI < n5) ? (double)W[pJ + (long)qI * n5] : 0.0;  // (J, I)
+      }
     }
   };
   if (Q0 < Q1) load(Q0);
   for (int Q = Q0; Q < Q1; ++Q) {
+    if constexpr (VEC) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      a[tx][ty + 8 * k] = ra[k];
-      if (!diag) b[tx][ty + 8 * k] = rb[k];
+      for (int e = 0; e < 4; ++e) {
+        a[p4 + e][c4] = ra[e];
+        if (!diag) b[p4 + e][c4] = rb[e];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        a[tx][ty + 8 * k] = ra[k];
+        if (!diag) b[tx][ty + 8 * k] = rb[k];
+      }
     }
     if (Q + 1 < Q1) load(Q + 1);
     __syncthreads();
@@ -210,8 +237,9 @@ void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks, bool f32) 
     // Q ranges of 8 slabs or more.  FP64 W: the longest ranges that leave >= 360
     // workgroups (each loads the next slice while it reduces this one): 120/1600 41.6 us
     // with 32 slabs per range against 55.8 us with 8, 240/3200 293 against 478 us.  FP32
-    // W: doubled only until <= 2048 workgroups (240/3200: 317 us with 64 slabs, 528 us
-    // with 128).
+    // W: doubled only until <= 2048 workgroups (240/3200 with 4-byte loads: 317 us with
+    // 64 slabs, 528 us with 128; with 16-byte loads 4096 workgroups doubled the partials'
+    // reduction for 11 us less in the partial pass).
     d.qps = 8;
     d.splits = (d.naux + d.qps - 1) / d.qps;
     if (f32) {
@@ -238,9 +266,13 @@ void symk_plan(SymKDesc* desc, int n, long* blocks, long* red_blocks, bool f32) 
   *red_blocks = r;
 }
 
-int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks, bool f32) {
+int symk(hipStream_t s, const SymKDesc* dev, int n, long blocks, long red_blocks, bool f32,
+         bool vec4) {
   if (n <= 0) return DN_OK;
-  if (f32)
+  if (f32 && vec4)
+    hipLaunchKernelGGL((symk_partial_kernel<float, true>), dim3((unsigned)blocks), dim3(256), 0, s,
+                       dev, n);
+  else if (f32)
     hipLaunchKernelGGL(symk_partial_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);
   else
     hipLaunchKernelGGL(symk_partial_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, dev, n);

@@ -27,6 +27,7 @@ namespace dn {
 
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+typedef _Float16 v4h __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -140,6 +141,63 @@ __global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5,
     hi[e] = h;
     lo[e] = l;
   }
+}
+
+// max_p ||C5[:,p]|| (one block per column) into cscale[3] as float bits (cleared first)
+__global__ __launch_bounds__(256) void c_colnorm_kernel(const double* __restrict__ C, int nbf,
+                                                        float* __restrict__ cscale) {
+  __shared__ double red[4];
+  const int p = blockIdx.x;
+  double s2 = 0.0;
+  for (int m = threadIdx.x; m < nbf; m += 256) {
+    const double x = C[m + (long)p * nbf];
+    s2 = fma(x, x, s2);
+  }
+  for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float nrm = (float)sqrt((red[0] + red[1]) + (red[2] + red[3])) * 1.000001f;
+    atomicMax(reinterpret_cast<unsigned*>(cscale + 3), __float_as_uint(nrm));
+  }
+}
+
+// Per slab Q: max_n ||Lh_Q[:,n]|| with Lh the strict lower triangle of b_Q plus half its
+// diagonal (b_Q = Lh + Lh^T).  With ||C5[:,p]|| it bounds |T[p,n]| = |sum_{m>=n} C5[m,p]
+// Lh[m,n]| by Cauchy-Schwarz, which sets the scale of T's FP16 planes before T exists.
+// One block per Q, a wavefront per column.
+__global__ __launch_bounds__(256) void b_lhnorm_kernel(const double* __restrict__ b, int nbf,
+                                                       int ld, float* __restrict__ lhmax) {
+  __shared__ double red[4];
+  const double* sl = b + (long)blockIdx.x * ld * nbf;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double best = 0.0;
+  for (int n = w; n < nbf; n += 4) {
+    double s2 = 0.0;
+    for (int m = n + lane; m < nbf; m += 64) {
+      double x = sl[m + (long)n * ld];
+      if (m == n) x *= 0.5;
+      s2 = fma(x, x, s2);
+    }
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    best = fmax(best, s2);  // lane 0 holds the column's sum
+  }
+  if (lane == 0) red[w] = best;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    lhmax[blockIdx.x] =
+        (float)sqrt(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))) * 1.000001f;
+}
+
+// The per-Q scale of T's planes from the bound ||C5 col|| max * lhmax[Q]
+__global__ void tplane_scale_kernel(const float* __restrict__ lhmax,
+                                    const float* __restrict__ cscale, int naux,
+                                    float* __restrict__ qs, float* __restrict__ qinv) {
+  const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Q >= naux) return;
+  const float sc = pow2_scale(lhmax[Q] * cscale[3] * 1.0001f);
+  qs[Q] = sc;
+  qinv[Q] = 1.0f / sc;
 }
 
 __global__ void fill_f32_kernel(float* p, long n, float v) {
@@ -370,6 +428,12 @@ __device__ __forceinline__ void split_tile(const SplitGemm& g, int mt, int nt, i
   }
   __syncthreads();
   const int fr = lane & 15, fc = lane >> 4;
+  bool live[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+      live[i][j] = m0 + wm * WM + 16 * i < M && n0 + wn * WN + 16 * j < N;
   for (int t = 0; t < nkt; ++t) {
     const int buf = t & 1;
     if (t + 1 < nkt) load(t + 1);
@@ -387,22 +451,26 @@ __device__ __forceinline__ void split_tile(const SplitGemm& g, int mt, int nt, i
       al[i] = *reinterpret_cast<const v8h*>(&Al[buf][chunk_off(BM, r, fc)]);
     }
     // the three products, each over all FM x FN accumulators in turn: consecutive MFMAs
-    // write different accumulators, so none waits for the previous one's result
+    // write different accumulators, so none waits for the previous one's result;
+    // fragments that lie wholly past M or N are skipped (wavefront-uniform branches)
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        if (live[i][j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        if (live[i][j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        if (live[i][j])
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     if (t + 1 < nkt) store(buf ^ 1, t + 1);
     __syncthreads();
   }
@@ -448,7 +516,68 @@ constexpr int split_lds_bytes() {
   return NST * (BM + BN) * 128;  // per stage: 2 planes x rows x 64 bytes
 }
 
-template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB>
+// Epilogue of the symmetric first transform when the second transform takes its output
+// as FP16 planes: out = acc * cs[1] * batch_inv_scale[z] * out_scale[z] (columns past N
+// written as zero: they are the k padding of the second transform), split into hi/lo and
+// stored tile-major.  A lane holds four rows of one column of each fragment; a 4 x 4
+// transpose inside each quad of lanes (two xor-shuffle rounds) gives lane q of the quad
+// row q of the quad's four columns, four consecutive halves of the plane: one 8-byte store
+// per plane, and the 16 lanes of a row write its 32 bytes of the fragment.
+template <int BM, int BN, int WGM, int WGN, int FM, int FN>
+__device__ __forceinline__ void split_epilogue_planes(const SplitGemm& g, v4f (&acc)[FM][FN],
+                                                      int m0, int n0, int wm, int wn, int z,
+                                                      int lane) {
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  const float f = g.cs[1] * g.batch_inv_scale[z] * g.out_scale[z];
+  const int q = lane & 3;
+  _Float16* ohi = g.ohi + (long)z * g.out_sb;
+  _Float16* olo = g.olo + (long)z * g.out_sb;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      float v0 = acc[i][j][0], v1 = acc[i][j][1], v2 = acc[i][j][2], v3 = acc[i][j][3];
+      {  // round 1 (partner lane ^ 1): keep the rows whose bit 0 is this lane's
+        const bool odd = q & 1;
+        const float s0 = odd ? v0 : v1, s1 = odd ? v2 : v3;
+        const float r0 = __shfl_xor(s0, 1, 64), r1 = __shfl_xor(s1, 1, 64);
+        const float k0 = odd ? v1 : v0, k1 = odd ? v3 : v2;
+        v0 = odd ? r0 : k0;
+        v1 = odd ? k0 : r0;
+        v2 = odd ? r1 : k1;
+        v3 = odd ? k1 : r1;
+      }
+      {  // round 2 (partner lane ^ 2): the lower row's pair stays in lanes q < 2
+        const bool hi2 = q & 2;
+        const float s0 = hi2 ? v0 : v2, s1 = hi2 ? v1 : v3;
+        const float r0 = __shfl_xor(s0, 2, 64), r1 = __shfl_xor(s1, 2, 64);
+        const float k0 = hi2 ? v2 : v0, k1 = hi2 ? v3 : v1;
+        v0 = hi2 ? r0 : k0;
+        v1 = hi2 ? r1 : k1;
+        v2 = hi2 ? k0 : r0;
+        v3 = hi2 ? k1 : r1;
+      }
+      // row m0 + lr, columns n0 + lc0 .. + 3
+      const int lc0 = wn * WN + j * 16 + (lane & 12);
+      const int lr = wm * WM + i * 16 + 4 * (lane >> 4) + q;
+      const int col = n0 + lc0;
+      if (col / 32 >= g.out_kt) continue;
+      const float vals[4] = {v0, v1, v2, v3};
+      v4h h4, l4;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        _Float16 h, l;
+        split_f16(col + c < g.N ? vals[c] * f : 0.f, &h, &l);
+        h4[c] = h;
+        l4[c] = l;
+      }
+      const long o = (long)(col >> 5) * g.out_ld + (long)(m0 + lr) * 32 + (col & 31);
+      *reinterpret_cast<v4h*>(ohi + o) = h4;
+      *reinterpret_cast<v4h*>(olo + o) = l4;
+    }
+}
+
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false>
 __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int nt, int z,
                                                unsigned char* smem) {
   constexpr int NW = WGM * WGN;
@@ -565,33 +694,40 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
     }
     // TRIB: a fragment column whose first column lies past this k tile's last k is all
     // zero after the mask; its MFMAs are skipped (a wavefront-uniform branch)
-    bool live_col[FN];
+    // and fragment rows past M (the padding of nbf5 to the tile) likewise
+    bool live_col[FN], live_row[FM];
 #pragma unroll
     for (int j = 0; j < FN; ++j)
       live_col[j] = !TRIB || (kt0 + t) * SB_K + SB_K - 1 >= n0 + wn * WN + 16 * j;
 #pragma unroll
+    for (int i = 0; i < FM; ++i) live_row[i] = m0 + wm * WM + 16 * i < g.M;
+#pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        if (live_col[j])
+        if (live_col[j] && live_row[i])
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        if (live_col[j])
+        if (live_col[j] && live_row[i])
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        if (live_col[j])
+        if (live_col[j] && live_row[i])
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
   }
-  split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
+  if constexpr (OUTP) {
+    split_epilogue_planes<BM, BN, WGM, WGN, FM, FN>(g, acc, m0, n0, wm, wn, z, lane);
+  } else {
+    split_epilogue<FM, FN>(g, acc, m0 + wm * WM, n0 + wn * WN, n0, z, lane);
+  }
 }
 
-template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB>
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false>
 __global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, int tiles_m,
                                                                   int tiles_n) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[split_lds_bytes<BM, BN, NST>()];
@@ -601,7 +737,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, 
   const int mt = lid % tiles_m;
   const int nt = (lid / tiles_m) % tiles_n;
   const int z = lid / (tiles_m * tiles_n);
-  split_tile_lds<BM, BN, WGM, WGN, NST, TRIB>(g, mt, nt, z, smem);
+  split_tile_lds<BM, BN, WGM, WGN, NST, TRIB, OUTP>(g, mt, nt, z, smem);
 }
 
 // Persistent grouped split GEMMs (the schedule of dn_sched.h).
@@ -747,11 +883,70 @@ int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, i
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
                     float* cscale, _Float16* hi, _Float16* lo) {
   const long n = (long)nbf * nbf5;
-  hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, cscale + 2, 1L, 0.f);
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, cscale + 2, 2L, 0.f);
   hipLaunchKernelGGL(c_max_kernel, dim3(std::min(grid_of(n), 128u)), dim3(256), 0, s, C, n,
                      cscale);
   hipLaunchKernelGGL(c_planes_kernel, dim3(grid_of((long)Mpad * Kpad)), dim3(256), 0, s, C, nbf,
                      nbf5, Mpad, Kpad, cscale, hi, lo);
+  hipLaunchKernelGGL(c_colnorm_kernel, dim3(nbf5), dim3(256), 0, s, C, nbf, cscale);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int split_lhnorm(hipStream_t s, const double* b, int nbf, int ld, int naux, float* lhmax) {
+  hipLaunchKernelGGL(b_lhnorm_kernel, dim3(naux), dim3(256), 0, s, b, nbf, ld, lhmax);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+// The symmetric-b split build with T as FP16 planes (one system):
+//   K1: T[p,n,Q] = sum_{m>=n} C5[m,p] Lh[m,n,Q] written as hi/lo planes of T * qs[Q], the
+//       scale set before T exists from ||C5[:,p]|| ||Lh_Q[:,n]|| >= |T[p,n]|;
+//   K2: W^T[q,p,Q] = sum_n C5[n,q] T[p,n,Q], both operands planes (the LDS-direct tile),
+//       out = acc * cs[1] * qinv[Q].
+// W^T instead of W: K = sum_Q (W + W^T)^2 and X = 2 diag W read either.  tp holds the two
+// planes (Mpad x Kpad halves per Q each).  part 1 launches K1, part 2 K2; split_sym_scales
+// sets qs, qinv first.
+int split_sym_scales(hipStream_t s, int naux, const float* cscale_dev, const float* lhmax,
+                     float* qs, float* qinv) {
+  hipLaunchKernelGGL(tplane_scale_kernel, dim3((naux + 255) / 256), dim3(256), 0, s, lhmax,
+                     cscale_dev, naux, qs, qinv);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
+                     const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
+                     const _Float16* chi, const _Float16* clo, const float* cscale_dev,
+                     const float* qs, const float* qinv, _Float16* tp, float* wt, int part) {
+  const long pl = (long)Mpad * Kpad * naux;
+  SplitGemm g1{};
+  g1.M = nbf5; g1.N = nbf; g1.K = nbf;
+  g1.ahi = chi; g1.alo = clo; g1.lda = (long)Mpad * 32;
+  g1.bhi = bhi; g1.blo = blo; g1.ldb = (long)nbf * 32; g1.sB = (long)nbf * Kpad;
+  g1.cs = cscale_dev; g1.batch_inv_scale = b_inv_scale; g1.nbatch = naux;
+  g1.ohi = tp; g1.olo = tp + pl; g1.out_ld = (long)Mpad * 32; g1.out_sb = (long)Mpad * Kpad;
+  g1.out_kt = Kpad / 32; g1.out_scale = qs;
+  if (part == 1) {
+    const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
+    const long total = (long)tm * tn * naux;
+    if (total > 0x7fffffffL || tm * 128 > Mpad) return fail(DN_ERR_ARG, "split K1 grid");
+    hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 2, 2, true, true>), dim3((unsigned)total),
+                       dim3(256), 0, s, g1, tm, tn);
+    DN_HIP(hipGetLastError());
+    return DN_OK;
+  }
+  SplitGemm g2{};
+  g2.M = nbf5; g2.N = nbf5; g2.K = nbf;
+  g2.ahi = chi; g2.alo = clo; g2.lda = (long)Mpad * 32;
+  g2.bhi = tp; g2.blo = tp + pl; g2.ldb = (long)Mpad * 32; g2.sB = (long)Mpad * Kpad;
+  g2.C = wt; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
+  g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.nbatch = naux;
+  const int tm = (g2.M + 127) / 128, tn = (g2.N + 127) / 128;
+  const long total = (long)tm * tn * naux;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split K2 grid");
+  hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 2, 2, false>), dim3((unsigned)total),
+                     dim3(256), 0, s, g2, tm, tn);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

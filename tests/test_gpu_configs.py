@@ -75,7 +75,9 @@ def test_split_800_3200_240_full_size(oracle):
     # the GNOF energy to 1e-6 Ha
     assert _rel(J16, J64) <= 5e-6 and _rel(K16, K64) <= 5e-6
     assert _rel(J16[np.ix_(P, P)], Jb) <= 5e-6 and _rel(K16[np.ix_(P, P)], Kb) <= 5e-6
-    assert np.array_equal(H16, H64)                 # H_core stays FP64
+    # H_core stays FP64 (the split build runs H C5 unsplit on its second stream, so the
+    # summation order, not the precision, differs)
+    assert np.abs(H16 - H64).max() <= 1e-12 * max(1.0, np.abs(H64).max())
     E16 = oracle.calce(sd.n, cj, ck, J16, K16, H16, p)
     print("split 800: |dJ|/|J| %.2e |dK|/|K| %.2e |dE| %.2e Ha"
           % (_rel(J16, J64), _rel(K16, K64), abs(E16 - E64)), flush=True)
@@ -102,12 +104,17 @@ def test_grouped_64_systems_full_size(oracle):
             single.append(s.get_jkh())
         g = api.Group(systems)
         g.tune()
+        # FP64: the same arithmetic as the single-system build; split: the grouped build
+        # keeps the FP32 intermediate T with its exact per-Q scale, the single-system build
+        # splits T into FP16 planes under a Cauchy-Schwarz scale, so they agree to the
+        # split rounding
+        tol = 1e-12 if prec == "fp64" else 1e-6
         for rep in range(2):
             g.jkh()
             g.sync()
             got = [s.get_jkh() for s in systems]
             for i, ((J, K, Hc), (Js, Ks, Hs)) in enumerate(zip(got, single)):
-                assert _rel(J, Js) <= 1e-12 and _rel(K, Ks) <= 1e-12, (prec, rep, i)
+                assert _rel(J, Js) <= tol and _rel(K, Ks) <= tol, (prec, rep, i)
                 assert np.abs(Hc - Hs).max() <= 1e-12 * max(1.0, np.abs(Hs).max())
         g.close()
         if prec == "fp64":
