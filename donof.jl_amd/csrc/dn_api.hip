@@ -29,8 +29,6 @@ int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const dou
 // dn_split.hip
 int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, int Kpad, float* scale,
                     float* inv_scale, _Float16* hi, _Float16* lo);
-int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
-                    float* cscale, _Float16* hi, _Float16* lo);
 int split_k1(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
              const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
              const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
@@ -40,8 +38,6 @@ int split_k1_sym(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                  const _Float16* chi, const _Float16* clo, const float* cscale_dev, float* bpn,
                  float* qmax);
 int split_lhnorm(hipStream_t s, const double* b, int nbf, int ld, int naux, float* lhmax);
-int split_sym_scales(hipStream_t s, int naux, const float* cscale_dev, const float* lhmax,
-                     float* qs, float* qinv);
 int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                      const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                      const _Float16* chi, const _Float16* clo, const float* cscale_dev,
@@ -331,9 +327,9 @@ int end_timing(dn_sys* s) {
 
 int jkh_split(dn_sys* s) {
   const dn_dims& d = s->d;
-  DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
-                         s->clo));
   const bool sym = s->b_sym && s->use_sym;
+  DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
+                         s->clo, sym ? s->lhmax : nullptr, d.nbfaux, s->qs, s->qinv));
   if (sym) {
     // H C5 and H_core need only C and H: on the second stream, beside the transforms (a
     // plain GEMM: its split-K workspace is the J GEMM's)
@@ -351,7 +347,6 @@ int jkh_split(dn_sys* s) {
     DN_HIP(hipEventRecord(s->join_ev, s->aux));
     // both transforms on FP16 planes; bpq32 then holds W^T with b_pq = W + W^T
     _Float16* tp = reinterpret_cast<_Float16*>(s->bpn32);
-    DN_TRY(split_sym_scales(s->stream, d.nbfaux, s->cscale, s->lhmax, s->qs, s->qinv));
     DN_TRY(begin_timing(s));
     DN_TRY(split_sym_planes(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi,
                             s->blo, s->binv, s->chi, s->clo, s->cscale, s->qs, s->qinv, tp,

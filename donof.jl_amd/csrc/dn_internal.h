@@ -128,8 +128,11 @@ SplitGemm split_k1_sym_gemm(int nbf, int nbf5, int naux, int Mpad, int Kpad,
 SplitGemm split_k2_gemm(int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
                         const _Float16* clo, const float* cscale_dev, const float* bpn,
                         float* bpq, const float* qs, const float* qinv);
+// C5 planes and scale; with lhmax (else nullptr) also the per-Q scales qs, qinv of the
+// symmetric build's T planes
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
-                    float* cscale, _Float16* hi, _Float16* lo);
+                    float* cscale, _Float16* hi, _Float16* lo, const float* lhmax, int naux,
+                    float* qs, float* qinv);
 // Per-system set-up of a grouped split build in three launches for all systems (the
 // arithmetic of split_prepare_c, split_k1_begin and split_k2_begin).  plane_begin and
 // q_begin are block offsets filled by split_prep_plan.

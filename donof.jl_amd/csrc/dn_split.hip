@@ -106,29 +106,57 @@ __global__ void b_split_kernel(const double* __restrict__ b, int ld, int nbf, lo
   }
 }
 
-// C5^T planes (row = orbital p < Mpad, k = AO index m < Kpad), tile-major, zero padded, in two
-// launches over the whole GPU: max |C5| into cscale[2] (bits of a non-negative float, which
-// order like the value; cleared first; one atomic per workgroup, at most 128 workgroups:
-// one per wavefront on a single address took 36 us at nbf 800), then the planes with the
-// power-of-two scale.
-__global__ __launch_bounds__(256) void c_max_kernel(const double* __restrict__ C, long n,
-                                                    float* __restrict__ cscale) {
-  __shared__ float red[16];
+// C5^T planes (row = orbital p < Mpad, k = AO index m < Kpad), tile-major, zero padded.
+// Per column p of C5 (one workgroup each): max |C5[:,p]| and ||C5[:,p]|| into cscale[2] and
+// cscale[3] (bits of non-negative floats, which order like the values; cleared first).
+__global__ __launch_bounds__(256) void c_stats_kernel(const double* __restrict__ C, int nbf,
+                                                     float* __restrict__ cscale) {
+  __shared__ double red_n[4];
+  __shared__ float red_m[4];
+  const int p = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double s2 = 0.0;
   float m = 0.f;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
-       e += (long)gridDim.x * blockDim.x)
-    m = fmaxf(m, (float)fabs(C[e]));
-  m = block_max(m, red);
-  if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned*>(cscale + 2), __float_as_uint(m));
+  for (int k = threadIdx.x; k < nbf; k += 256) {
+    const double x = C[k + (long)p * nbf];
+    s2 = fma(x, x, s2);
+    m = fmaxf(m, (float)fabs(x));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s2 += __shfl_xor(s2, off, 64);
+    m = fmaxf(m, __shfl_xor(m, off, 64));
+  }
+  if (lane == 0) {
+    red_n[w] = s2;
+    red_m[w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float n = (float)sqrt((red_n[0] + red_n[1]) + (red_n[2] + red_n[3])) * 1.000001f;
+    const float mm = fmaxf(fmaxf(red_m[0], red_m[1]), fmaxf(red_m[2], red_m[3]));
+    atomicMax(reinterpret_cast<unsigned*>(cscale + 2), __float_as_uint(mm));
+    atomicMax(reinterpret_cast<unsigned*>(cscale + 3), __float_as_uint(n));
+  }
 }
 
+// The C5 planes under the power-of-two scale of max |C5| (block 0 records the scale pair),
+// and, with lhmax, the per-Q scales of the symmetric build's T planes from the
+// Cauchy-Schwarz bound ||C5[:,p]|| ||Lh_Q[:,n]|| >= |T[p,n]| (the first blocks).
 __global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5, int Mpad,
                                 int Kpad, float* __restrict__ cscale, _Float16* __restrict__ hi,
-                                _Float16* __restrict__ lo) {
+                                _Float16* __restrict__ lo, const float* __restrict__ lhmax,
+                                int naux, float* __restrict__ qs, float* __restrict__ qinv) {
   const float sc = pow2_scale(cscale[2] * 1.0001f);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cscale[0] = sc;
     cscale[1] = 1.0f / sc;
+  }
+  if (lhmax) {
+    const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (Q < naux) {
+      const float sq = pow2_scale(lhmax[Q] * cscale[3] * 1.0001f);
+      qs[Q] = sq;
+      qinv[Q] = 1.0f / sq;
+    }
   }
   const long total = (long)Mpad * Kpad;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
@@ -140,25 +168,6 @@ __global__ void c_planes_kernel(const double* __restrict__ C, int nbf, int nbf5,
     if (p < nbf5 && k < nbf) split_f16_d(C[k + (long)p * nbf] * (double)sc, &h, &l);
     hi[e] = h;
     lo[e] = l;
-  }
-}
-
-// max_p ||C5[:,p]|| (one block per column) into cscale[3] as float bits (cleared first)
-__global__ __launch_bounds__(256) void c_colnorm_kernel(const double* __restrict__ C, int nbf,
-                                                        float* __restrict__ cscale) {
-  __shared__ double red[4];
-  const int p = blockIdx.x;
-  double s2 = 0.0;
-  for (int m = threadIdx.x; m < nbf; m += 256) {
-    const double x = C[m + (long)p * nbf];
-    s2 = fma(x, x, s2);
-  }
-  for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s2;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const float nrm = (float)sqrt((red[0] + red[1]) + (red[2] + red[3])) * 1.000001f;
-    atomicMax(reinterpret_cast<unsigned*>(cscale + 3), __float_as_uint(nrm));
   }
 }
 
@@ -187,17 +196,6 @@ __global__ __launch_bounds__(256) void b_lhnorm_kernel(const double* __restrict_
   if (threadIdx.x == 0)
     lhmax[blockIdx.x] =
         (float)sqrt(fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))) * 1.000001f;
-}
-
-// The per-Q scale of T's planes from the bound ||C5 col|| max * lhmax[Q]
-__global__ void tplane_scale_kernel(const float* __restrict__ lhmax,
-                                    const float* __restrict__ cscale, int naux,
-                                    float* __restrict__ qs, float* __restrict__ qinv) {
-  const int Q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (Q >= naux) return;
-  const float sc = pow2_scale(lhmax[Q] * cscale[3] * 1.0001f);
-  qs[Q] = sc;
-  qinv[Q] = 1.0f / sc;
 }
 
 __global__ void fill_f32_kernel(float* p, long n, float v) {
@@ -881,14 +879,13 @@ int split_prepare_b(hipStream_t s, const double* b, int nbf, int ld, int naux, i
 }
 
 int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad, int Kpad,
-                    float* cscale, _Float16* hi, _Float16* lo) {
-  const long n = (long)nbf * nbf5;
+                    float* cscale, _Float16* hi, _Float16* lo, const float* lhmax, int naux,
+                    float* qs, float* qinv) {
   hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, cscale + 2, 2L, 0.f);
-  hipLaunchKernelGGL(c_max_kernel, dim3(std::min(grid_of(n), 128u)), dim3(256), 0, s, C, n,
-                     cscale);
-  hipLaunchKernelGGL(c_planes_kernel, dim3(grid_of((long)Mpad * Kpad)), dim3(256), 0, s, C, nbf,
-                     nbf5, Mpad, Kpad, cscale, hi, lo);
-  hipLaunchKernelGGL(c_colnorm_kernel, dim3(nbf5), dim3(256), 0, s, C, nbf, cscale);
+  hipLaunchKernelGGL(c_stats_kernel, dim3(nbf5), dim3(256), 0, s, C, nbf, cscale);
+  const unsigned grid = std::max(grid_of((long)Mpad * Kpad), (unsigned)((naux + 255) / 256));
+  hipLaunchKernelGGL(c_planes_kernel, dim3(grid), dim3(256), 0, s, C, nbf, nbf5, Mpad, Kpad,
+                     cscale, hi, lo, lhmax, naux, qs, qinv);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -905,16 +902,8 @@ int split_lhnorm(hipStream_t s, const double* b, int nbf, int ld, int naux, floa
 //   K2: W^T[q,p,Q] = sum_n C5[n,q] T[p,n,Q], both operands planes (the LDS-direct tile),
 //       out = acc * cs[1] * qinv[Q].
 // W^T instead of W: K = sum_Q (W + W^T)^2 and X = 2 diag W read either.  tp holds the two
-// planes (Mpad x Kpad halves per Q each).  part 1 launches K1, part 2 K2; split_sym_scales
+// planes (Mpad x Kpad halves per Q each).  part 1 launches K1, part 2 K2; split_prepare_c
 // sets qs, qinv first.
-int split_sym_scales(hipStream_t s, int naux, const float* cscale_dev, const float* lhmax,
-                     float* qs, float* qinv) {
-  hipLaunchKernelGGL(tplane_scale_kernel, dim3((naux + 255) / 256), dim3(256), 0, s, lhmax,
-                     cscale_dev, naux, qs, qinv);
-  DN_HIP(hipGetLastError());
-  return DN_OK;
-}
-
 int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int Kpad,
                      const _Float16* bhi, const _Float16* blo, const float* b_inv_scale,
                      const _Float16* chi, const _Float16* clo, const float* cscale_dev,
