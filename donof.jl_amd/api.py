@@ -5,6 +5,7 @@ Names and argument meaning follow the reference so calling code reads the same:
   computeJKH_MO(C, H, I, p)                    integrals.jl:131-140
   JKH_MO(C, H, b, nbf, nbf5, nbfaux)           integrals.jl:144-172
   compute_Lagrange2(C, n, H, b, cj12, ck12, nalpha, nbeta)   utils.jl:1-59
+  compute_geom_gradients(bset, n, C, cj12, ck12, elag, p, I)  grads.jl:1-100 (RI)
 
 `I`/`b` is a `System`: the device-resident three-index tensor of one solve, the analogue
 of the CuArray the reference's GPU extension dispatches on (ext/DoNOFGPUExt.jl:52,63,150).
@@ -425,6 +426,189 @@ def compute_Lagrange2_JKH(C, n, H, b, cj12, ck12, nalpha, nbeta):
                                  int(nalpha), int(nbeta), dptr(elag), dptr(Hmat), ct.byref(E),
                                  dptr(J), dptr(K), dptr(Hc)))
     return elag, Hmat, E.value, J, K, Hc
+
+
+# --------------------------------------------------------------------------------------
+# grads.jl:1-100 (RI branch), utils.jl:315-336
+# --------------------------------------------------------------------------------------
+
+BOHR_PER_ANGSTROM = 1.88973      # the conversion utils.jl:305,320 use
+
+
+class DeviceArray:
+    """A column-major float64 array in HBM, allocated with the HIP runtime libdonofhip.so
+    links (hipMalloc / hipMemcpy / hipFree of libamdhip64.so.7): inputs such as derivative
+    integrals kept resident and read in place.  (A second HIP runtime in the process, as
+    torch wheels bundle, does not share devices with this one.)"""
+    _rt = None
+
+    @classmethod
+    def _runtime(cls):
+        if cls._rt is None:
+            _lib.load_library()
+            rt = ct.CDLL("libamdhip64.so.7")
+            rt.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+            rt.hipFree.argtypes = [ct.c_void_p]
+            rt.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+            rt.hipMemset.argtypes = [ct.c_void_p, ct.c_int, ct.c_size_t]
+            rt.hipSetDevice.argtypes = [ct.c_int]
+            cls._rt = rt
+        return cls._rt
+
+    def __init__(self, shape, device=0):
+        rt = self._runtime()
+        self.shape = tuple(int(x) for x in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        self.device = device
+        self.ptr = ct.c_void_p()
+        if rt.hipSetDevice(int(device)) != 0 or \
+                rt.hipMalloc(ct.byref(self.ptr), max(self.nbytes, 8)) != 0:
+            self.ptr = None
+            raise DnError(_lib.DN_ERR_NOMEM, "hipMalloc of %d bytes" % self.nbytes)
+        if rt.hipMemset(self.ptr, 0, max(self.nbytes, 8)) != 0:
+            raise DnError(_lib.DN_ERR_HIP, "hipMemset")
+
+    @classmethod
+    def from_host(cls, a, device=0):
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        d = cls(a.shape, device)
+        d.upload(a)
+        return d
+
+    def upload(self, a, offset_bytes=0):
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        if offset_bytes + a.nbytes > self.nbytes:
+            raise ValueError("upload past the end of the device array")
+        dst = ct.c_void_p(self.ptr.value + offset_bytes)
+        if self._runtime().hipMemcpy(dst, a.ctypes.data_as(ct.c_void_p), a.nbytes, 1) != 0:
+            raise DnError(_lib.DN_ERR_HIP, "hipMemcpy host to device")
+
+    def download(self):
+        out = np.empty(self.shape, order="F")
+        if self._runtime().hipMemcpy(out.ctypes.data_as(ct.c_void_p), self.ptr, self.nbytes,
+                                     2) != 0:
+            raise DnError(_lib.DN_ERR_HIP, "hipMemcpy device to host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _host_or_device(a, shape, name):
+    """A derivative-integral array as the ABI takes it: a DeviceArray is read in place,
+    anything else becomes a column-major float64 host array."""
+    if isinstance(a, DeviceArray):
+        if a.shape != tuple(shape):
+            raise ValueError("%s must have shape %s, got %s" % (name, shape, a.shape))
+        return a, a.ptr
+    h = _colmajor(a, shape, name)
+    return h, h.ctypes.data_as(ct.c_void_p)
+
+
+class Gradient:
+    """The densities of compute_geom_gradients' RI branch (grads.jl:1-100), formed on the
+    device from the resident b: RDM1, lag, the three-index density d and the metric
+    density m.  `atom(...)` contracts them with one atom's derivative integrals.
+
+    Wt: None when b was set by System.ri_metric (b = I3c L^{-T}); otherwise the matrix
+    with I3c G^{-1} = b Wt (G^{-1/2} for the reference's eigen-route b)."""
+
+    def __init__(self, b, n, C, cj12, ck12, elag, Wt=None):
+        s = _require_system(b)
+        self.system = s
+        nbf, nbf5, naux = s.nbf, s.nbf5, s.nbfaux
+        C = _colmajor(C, (nbf, nbf), "C")
+        n = _vec(np.asarray(n)[:nbf5], nbf5, "n")
+        cj12 = _colmajor(cj12, (nbf5, nbf5), "cj12")
+        ck12 = _colmajor(ck12, (nbf5, nbf5), "ck12")
+        elag = _colmajor(elag, (nbf, nbf), "elag")
+        W = None if Wt is None else _colmajor(Wt, (naux, naux), "Wt")
+        h = ct.c_void_p()
+        check(s.lib.dn_grad_create(s.handle, dptr(C), dptr(n), dptr(cj12), dptr(ck12),
+                                   dptr(elag), dptr(W), ct.byref(h)))
+        self.handle = h
+
+    def atom(self, dS, dT, dV, geri3c, geri2c):
+        """-sum lag dS + sum RDM1 (dT + dV) + 2 sum d geri3c - sum m geri2c per Cartesian
+        component (grads.jl:13-21, 91-96)."""
+        s = self.system
+        nbf, naux = s.nbf, s.nbfaux
+        keep = [_host_or_device(dS, (nbf, nbf, 3), "dS"),
+                _host_or_device(dT, (nbf, nbf, 3), "dT"),
+                _host_or_device(dV, (nbf, nbf, 3), "dV"),
+                _host_or_device(geri3c, (nbf, nbf, naux, 3), "geri3c"),
+                _host_or_device(geri2c, (naux, naux, 3), "geri2c")]
+        out = np.zeros(3)
+        check(s.lib.dn_grad_atom(self.handle, *[k[1] for k in keep], dptr(out)))
+        return out
+
+    def densities(self, with_d=False):
+        """(RDM1, lag, m[, d]) copied to the host."""
+        s = self.system
+        nbf, naux = s.nbf, s.nbfaux
+        R = np.empty((nbf, nbf), order="F")
+        L = np.empty((nbf, nbf), order="F")
+        m = np.empty((naux, naux), order="F")
+        d = np.empty((nbf, nbf, naux), order="F") if with_d else None
+        check(s.lib.dn_grad_get_densities(self.handle, dptr(R), dptr(L), dptr(m), dptr(d)))
+        return (R, L, m, d) if with_d else (R, L, m)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(self.system.lib.dn_grad_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compute_grad_nuc(atoms):
+    """Nuclear-repulsion gradient (utils.jl:315-336) in Hartree/Bohr, natoms x 3; each
+    atom has .Z and .xyz (Angstrom)."""
+    Z = np.array([float(a.Z) for a in atoms])
+    R = np.array([np.asarray(a.xyz, dtype=np.float64) for a in atoms]) * BOHR_PER_ANGSTROM
+    g = np.zeros((len(atoms), 3))
+    for k in range(len(atoms)):
+        for j in range(len(atoms)):
+            if j != k:
+                r = R[k] - R[j]
+                g[k] -= Z[k] * Z[j] * r / np.linalg.norm(r) ** 3
+    return g
+
+
+def compute_geom_gradients(bset, n, C, cj12, ck12, elag, p, I, Wt=None):
+    """grads.jl:1-100 with p.RI: natoms x 3 nuclear gradient.
+
+    `I` is the System holding b (as everywhere in this module); `bset` supplies the atoms
+    and the derivative integrals the reference takes from GaussianBasis: bset.atoms
+    (each with .Z, .xyz in Angstrom) and, for atom i (0-based), bset.overlap_deriv(i),
+    kinetic_deriv(i), nuclear_deriv(i) (nbf x nbf x 3), eri_2e3c_deriv(i)
+    (nbf x nbf x naux x 3) and eri_2e2c_deriv(i) (naux x naux x 3), as NumPy arrays or
+    DeviceArrays (read in place).  Unlike the reference, cj12 and ck12 are not modified
+    (grads.jl:24-25 zero their diagonals in place; the device ignores them)."""
+    if not p.RI:
+        raise NotImplementedError("only the RI branch (p.RI = true) is implemented")
+    grad = compute_grad_nuc(bset.atoms)
+    g = Gradient(I, n, C, cj12, ck12, elag, Wt)
+    try:
+        for i in range(len(bset.atoms)):
+            grad[i] += g.atom(bset.overlap_deriv(i), bset.kinetic_deriv(i),
+                              bset.nuclear_deriv(i), bset.eri_2e3c_deriv(i),
+                              bset.eri_2e2c_deriv(i))
+    finally:
+        g.close()
+    return grad
 
 
 # --------------------------------------------------------------------------------------

@@ -71,17 +71,19 @@ bool dims_valid(const dn_dims& d, std::string* why) {
 // Element counts of every device buffer of a system.
 struct Layout {
   size_t b, C, H, n, nn, nbf2, nbf_nbf5, big, bpn, cp, bmat, xw, part, scal;
-  size_t nv, dnm, nvar, part2;
+  size_t nv, dnm, nvar, part2, riw;
   // split precision (0 when FP64): half counts of the b and C planes, float counts
   size_t bplane, cplane, f_big, f_bpq, f_naux;
   static size_t d8(size_t halves) { return (halves + 3) / 4; }   // halves -> doubles
   static size_t f8(size_t floats) { return (floats + 1) / 2; }   // floats -> doubles
   size_t split_doubles() const {
-    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 1;
+    if (!bplane) return 0;  // FP64 system
+    // + the two doubles of the C5 scale pair (cscale)
+    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 2;
   }
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 3 + n + big + bpn + cp + bmat + 2 * xw +
-                part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2 +
+                part + scal + 2 * nv + dnm + 9 * n + 5 * nvar + 9 * nbf2 + part2 + riw +
                 split_doubles());
   }
 };
@@ -123,6 +125,7 @@ Layout layout_of(const dn_dims& d, int precision) {
   L.dnm = std::max<size_t>(nbf5 * d.nv, 1);
   L.nvar = nbf * nbf5 - nbf5 * (nbf5 + 1) / 2;
   L.part2 = 16 * nbf * nbf;
+  L.riw = naux * naux;
   if (precision == DN_SPLIT_F16) {
     L.bplane = (size_t)round_up(d.nbf, 32) * nbf * naux + 64;
     L.cplane = (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32);
@@ -468,6 +471,25 @@ void count_build_group(dn_sys* const* systems, int n, int delta) {
     if (systems[i]) systems[i]->build_groups += delta;
 }
 
+int ctx_reserve(dn_ctx* c, uint64_t bytes) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  if (c->used + bytes > c->budget)
+    return fail(DN_ERR_BUDGET, "needs " + std::to_string(bytes) + " bytes; budget left " +
+                                   std::to_string(c->budget - c->used));
+  c->used += bytes;
+  return DN_OK;
+}
+
+void ctx_unreserve(dn_ctx* c, uint64_t bytes) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  c->used -= bytes;
+}
+
+void count_grad(dn_sys* s, int delta) {
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  s->grad_objs += delta;
+}
+
 int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;
@@ -687,6 +709,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf2, &s->bestC);
   A(L.part2, &s->part2);
   s->part2_elems = L.part2;
+  A(L.riw, &s->riw);
   if (precision == DN_SPLIT_F16 && rc == DN_OK) {
     if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming) != hipSuccess ||
@@ -771,6 +794,8 @@ int dn_sys_destroy(dn_sys* s) {
     std::lock_guard<std::mutex> lock(g_ctx_mutex);
     if (s->lockstep || s->build_groups > 0)
       return fail(DN_ERR_STATE, "the system belongs to a group; destroy the group first");
+    if (s->grad_objs > 0)
+      return fail(DN_ERR_STATE, "a gradient object holds the system; destroy it first");
   }
   (void)hipSetDevice(s->ctx->device);
   drop_aux(s);
@@ -797,6 +822,7 @@ int dn_sys_upload_b(dn_sys* s, const double* b) {
   DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;
+  s->riw_set = false;  // b no longer comes from dn_sys_ri_metric
   return DN_OK;
 }
 
@@ -817,6 +843,7 @@ int dn_sys_generate_b(dn_sys* s, uint64_t seed) {
   DN_TRY(prepare_b(s));
   DN_HIP(hipStreamSynchronize(s->stream));
   s->b_ready = true;
+  s->riw_set = false;  // b no longer comes from dn_sys_ri_metric
   return DN_OK;
 }
 

@@ -333,6 +333,12 @@ void dev_free(dn_ctx* c, hipStream_t s, void* p);
 // group membership (dn_api.hip): a lock-step group claims its systems (none may already
 // belong to one, none listed twice) and releases them; grouped builds count their systems
 int claim_lockstep(dn_sys* const* systems, int n, void* owner);
+// HBM budget of a context: reserve (DN_ERR_BUDGET past the budget) and give back bytes
+// held outside the systems' own footprints
+int ctx_reserve(dn_ctx* c, uint64_t bytes);
+void ctx_unreserve(dn_ctx* c, uint64_t bytes);
+// a dn_grad holds (+1) or lets go of (-1) its system
+void count_grad(dn_sys* s, int delta);
 void release_lockstep(dn_sys* const* systems, int n, void* owner);
 void count_build_group(dn_sys* const* systems, int n, int delta);
 // a pinned host block of at least `bytes` (a cached one when one is large enough)
@@ -432,6 +438,11 @@ struct dn_sys {
   // that belongs to either is not destroyed
   void* lockstep = nullptr;
   int build_groups = 0;
+  int grad_objs = 0;     // live dn_grad objects (they run on the system's stream)
+  // the auxiliary-index factor of the RI metric kept by dn_sys_ri_metric: riw = L^{-T}
+  // with G = L L^T, so that I3c G^{-1} = b L^{-1} = b riw^T (nuclear gradients)
+  double* riw = nullptr;  // naux*naux
+  bool riw_set = false;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
   dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
                                      // the split-precision one at scal + 32)

@@ -91,6 +91,13 @@ int copy_block(hipStream_t s, double* dst, long ldd, const double* src, long lds
   return DN_OK;
 }
 
+// the n x n identity, column-major
+__global__ void identity_kernel(double* X, int n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)n * n) return;
+  X[i] = (i % n == i / n) ? 1.0 : 0.0;
+}
+
 }  // namespace
 
 // In place: the lower triangle of G (naux x naux, ld = naux) becomes L with G = L L^T;
@@ -163,7 +170,8 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   // rows of b in its resident layout: ldb per column of each slab, the pad rows zero (the
   // solve maps zero rows of I3c to zero rows of b)
   const long rows = (long)s->ldb * s->d.nbf;
-  const size_t nI = (size_t)rows * naux, nG = (size_t)naux * naux, nP = (size_t)naux * NB;
+  const size_t nG = (size_t)naux * naux, nP = (size_t)naux * NB;
+  const size_t nI = std::max((size_t)rows * naux, nG);  // I3c, then the identity
   const size_t bytes = (nI + nG + 2 * nP) * sizeof(double) + 256;
   if (s->ctx->budget && s->ctx->used + bytes > s->ctx->budget)
     return fail(DN_ERR_BUDGET, "RI metric workspace exceeds the HBM budget");
@@ -188,13 +196,19 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
     if (host_flag)
       return fail(DN_ERR_ARG, "RI metric is not positive definite (Cholesky pivot <= 0)");
     DN_TRY(trsm_right_lt(s->stream, dI, s->b, rows, naux, dG, Dinv));
+    // riw = 1 L^{-T}: I3c G^{-1} = b L^{-1} = b riw^T for the nuclear gradients
+    hipLaunchKernelGGL(identity_kernel, dim3((nG + 255) / 256), dim3(256), 0, s->stream, dI,
+                       naux);
+    DN_HIP(hipGetLastError());
+    DN_TRY(trsm_right_lt(s->stream, dI, s->riw, naux, naux, dG, Dinv));
     DN_TRY(prepare_b(s));
     DN_HIP(hipStreamSynchronize(s->stream));
     return DN_OK;
   };
+  s->riw_set = false;
   rc = run();
   dev_free(s->ctx, s->stream, w);
   (void)hipStreamSynchronize(s->stream);
-  if (rc == DN_OK) s->b_ready = true;
+  if (rc == DN_OK) s->b_ready = s->riw_set = true;
   return rc;
 }

@@ -94,6 +94,33 @@ int dn_sys_generate_b(dn_sys* sys, uint64_t seed);
  * not positive definite returns DN_ERR_ARG.  Host arrays are not modified; a transient
  * device workspace of the size of b is taken for the call. */
 int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
+/* RI nuclear gradients: replaces compute_geom_gradients' RI branch (grads.jl:1-100).
+ * dn_grad_create forms on the device, from the resident b at orbitals C (nbf x nbf),
+ * occupations n (nbf5), cj12, ck12 (nbf5 x nbf5; their diagonals are ignored, as
+ * grads.jl:24-25 zero them; the caller's arrays are not modified) and the Lagrangian
+ * elag (nbf x nbf): RDM1 = 2 C5 diag(n) C5^T (grads.jl:5), lag = 2 C elag C^T (6), the
+ * three-index density d[l,s,Q] (66-84) and the metric density m[Q,R] (89).  IG =
+ * I3c G^{-1} is taken as b Wt: Wt = NULL uses the factor kept by dn_sys_ri_metric
+ * (b = I3c L^{-T}, so Wt = L^{-1}); for any other b = I3c W pass Wt = W^{-1} G^{-1}
+ * (naux x naux, column-major; G^{-1/2} for the reference's eigen-route b of
+ * integrals.jl:68-91).  The object holds nbf^2 naux + naux^2 + 2 nbf^2 doubles of HBM
+ * (charged to the context's budget) and keeps the system from being destroyed.
+ * dn_grad_atom returns one atom's electronic gradient (grads.jl:13-21, 91-96)
+ *   grad3[k] = -<lag, dS_k> + <RDM1, dT_k + dV_k> + 2 <d, geri3c_k> - <m, geri2c_k>
+ * from the derivative integrals of that atom (dS, dT, dV: nbf x nbf x 3; geri3c:
+ * nbf x nbf x naux x 3; geri2c: naux x naux x 3; column-major, Cartesian index last),
+ * each a host or a device pointer (device arrays are read in place, host arrays streamed
+ * through a staging buffer).  The nuclear-repulsion term compute_grad_nuc
+ * (utils.jl:315-336) is the caller's.  dn_grad_get_densities copies out any of RDM1,
+ * lag, m, d (NULL skips). */
+typedef struct dn_grad dn_grad;
+int dn_grad_create(dn_sys* sys, const double* C, const double* n, const double* cj12,
+                   const double* ck12, const double* elag, const double* Wt, dn_grad** out);
+int dn_grad_atom(dn_grad* grad, const double* dS, const double* dT, const double* dV,
+                 const double* geri3c, const double* geri2c, double* grad3);
+int dn_grad_get_densities(dn_grad* grad, double* RDM1, double* lag, double* m, double* d);
+int dn_grad_destroy(dn_grad* grad);
+
 /* Whenever b is set, the library checks b[m,n,Q] == b[n,m,Q] (to 2^-50 of max|b|; the
  * RI tensor is symmetric in m, n, integrals.jl:86-88).  A symmetric b lets the FP64
  * J/K build contract only its lower half: C5^T b_Q C5 = W + W^T with W = C5^T Lh_Q C5,

@@ -30,6 +30,12 @@ How each part is pinned (tests/test_oracle_*.py):
   ocupacion_*       pair sums == 1 (softmax), finite-difference Jacobians.
   calcoccg          finite differences of calcocce.
   orbopt_*          deterministic recurrence; replayed step by step.
+  geom_gradient_*   the RI nuclear gradient equals the derivative of the energy
+                    (iaux, jkh_mo, calce) along S, H, I3c and G varied linearly with
+                    Loewdin-orthonormalised orbitals, each term alone and all together,
+                    closed and open shells, PNOF5/7 and GNOF; the coefficient diagonals
+                    are ignored (grads.jl:24-25).  grad_nuc: finite differences of e_nuc
+                    and translation invariance.
   occoptr           parity unpinned for the iterates (the reference uses Optim.jl
                     ConjugateGradient/HagerZhang, not available here); only the
                     stationary point |calcoccg| < threshen is pinned.
@@ -922,3 +928,88 @@ def energy_loop(b, H, C, p, n=None, freeze_occ=False):
     if p.ipnof > 4:                                             # energy.jl:222-224
         C, n, elag = order_subspaces(C, n, elag, H, b, p)
     return E, C, n, hist, elag
+
+
+# --------------------------------------------------------------------------------------
+# Nuclear gradients, RI branch (grads.jl:1-100; utils.jl:299-336)
+# --------------------------------------------------------------------------------------
+
+BOHR_PER_ANGSTROM = 1.88973      # the factor utils.jl:305,320 uses
+
+
+def e_nuc(Z, xyz):
+    """compute_E_nuc (utils.jl:299-313): sum_{i<j} Z_i Z_j / |r_i - r_j|, xyz in Angstrom."""
+    E = 0.0
+    for i in range(len(Z)):
+        for j in range(i + 1, len(Z)):
+            E += Z[i] * Z[j] / (np.linalg.norm(xyz[i] - xyz[j]) * BOHR_PER_ANGSTROM)
+    return E
+
+
+def grad_nuc(Z, xyz):
+    """compute_grad_nuc (utils.jl:315-336): dE_nuc/dR_k in Hartree/Bohr, natoms x 3."""
+    na = len(Z)
+    g = np.zeros((na, 3))
+    for k in range(na):
+        for j in range(na):
+            if j != k:
+                r = (xyz[k] - xyz[j]) * BOHR_PER_ANGSTROM
+                g[k] -= Z[k] * Z[j] * r / np.linalg.norm(r) ** 3
+    return g
+
+
+def geom_gradient_densities(n, C, cj12, ck12, elag, p, G, I3c):
+    """The densities of compute_geom_gradients' RI branch (grads.jl:1-100):
+    RDM1[m,l] = 2 sum_p n_p C5[m,p] C5[l,p] (5); lag = 2 C elag C^T (6);
+    cj12, ck12 with zero diagonals (24-25; the reference zeroes the caller's arrays in
+    place, this copy does not); G symmetrised (55) and IG = I3c inv(G) (61);
+    I2 = C5^T IG C5 per Q (66); d[l,s,Q] = sum_pq C5[l,p] C5[s,q] D[p,q,Q] with
+    D = -ck12 I2 (68-70) plus, on the diagonal, bj = cj12^T diag(I2) (72-74) and
+    n_p diag(I2) for p < nbeta (76-79) and for nalpha <= p < nbf5 (81-84);
+    m[Q,R] = sum_{l,s} d[l,s,Q] IG[s,l,R] (89).  Returns (RDM1, lag, d, m)."""
+    nbf5 = p.nbf5
+    C5 = C[:, :nbf5]
+    RDM1 = 2.0 * (C5 * n[None, :nbf5]) @ C5.T
+    lag = 2.0 * C @ elag @ C.T
+    cj = np.array(cj12, dtype=float)
+    ck = np.array(ck12, dtype=float)
+    np.fill_diagonal(cj, 0.0)
+    np.fill_diagonal(ck, 0.0)
+    Gs = 0.5 * (G + G.T)
+    nbf, _, naux = I3c.shape
+    IG = (I3c.reshape(nbf * nbf, naux, order="F") @ np.linalg.inv(Gs)).reshape(
+        nbf, nbf, naux, order="F")
+    _, I2 = mo_transform(C5, IG)                                         # [p,q,Q]
+    D = -ck[:, :, None] * I2
+    X = np.einsum("ppQ->pQ", I2)
+    diag = cj.T @ X                                                      # bj[q,Q]
+    diag[:p.nbeta] += n[:p.nbeta, None] * X[:p.nbeta]
+    diag[p.nalpha:nbf5] += n[p.nalpha:nbf5, None] * X[p.nalpha:nbf5]
+    idx = np.arange(nbf5)
+    D[idx, idx, :] += diag
+    # d_Q = C5 D_Q C5^T
+    T = np.einsum("lp,pqQ->lqQ", C5, D, optimize=True)
+    d = np.einsum("lqQ,sq->lsQ", T, C5, optimize=True)
+    m = np.einsum("lsQ,slR->QR", d, IG, optimize=True)
+    return RDM1, lag, d, m
+
+
+def geom_gradient_atom(RDM1, lag, d, m, dS, dT, dV, geri3c, geri2c):
+    """One atom's electronic terms (grads.jl:13-21, 94-95):
+    -sum lag dS + sum RDM1 (dT + dV) + 2 sum d geri3c - sum m geri2c, per Cartesian k."""
+    g = -np.einsum("mn,mnk->k", lag, dS) + np.einsum("mn,mnk->k", RDM1, dT + dV)
+    g += 2.0 * np.einsum("slQ,slQk->k", d, geri3c, optimize=True)
+    g -= np.einsum("QR,QRk->k", m, geri2c)
+    return g
+
+
+def compute_geom_gradients(n, C, cj12, ck12, elag, p, G, I3c, atoms, Z, xyz):
+    """compute_geom_gradients (grads.jl:1-100) with RI: the nuclear repulsion gradient
+    (utils.jl:315-336) plus each atom's electronic terms; `atoms` yields per atom the
+    derivative integrals (dS, dT, dV: nbf x nbf x 3; geri3c: nbf x nbf x naux x 3;
+    geri2c: naux x naux x 3).  Returns natoms x 3."""
+    RDM1, lag, d, m = geom_gradient_densities(n, C, cj12, ck12, elag, p, G, I3c)
+    grad = grad_nuc(Z, xyz)
+    for i, (dS, dT, dV, g3, g2) in enumerate(atoms):
+        grad[i] += geom_gradient_atom(RDM1, lag, d, m, dS, dT, dV, g3, g2)
+    return grad
