@@ -20,10 +20,12 @@ Extra legs (--legs, default all; --no-extra for none):
           GNOF-energy error against the FP64 device path on the same tensor and orbitals;
   grouped 64 systems, nbf ~ U[48, 400], built together by grouped persistent kernels in
           one captured HIP graph, against per-system builds on their own streams;
-  served  Poisson arrivals of orbital-optimisation solves, p50/p99 latency and solves/s.
+  served  Poisson arrivals of orbital-optimisation solves, p50/p99 latency and solves/s;
+  grad    RI nuclear gradients: the densities at 400/1600/120 and 800/3200/240 and one
+          atom's contractions with derivative integrals in HBM (and from host arrays).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
-                    [--legs 800,grouped,served] [--no-cpu-baseline] [--no-extra]
+                    [--legs 800,grouped,served,grad] [--no-cpu-baseline] [--no-extra]
 """
 from __future__ import annotations
 
@@ -186,6 +188,69 @@ def extra_legs(api, synthetic, steps, warmup, seed):
     return out
 
 
+def grad_flops(nbf, naux, nbf5):
+    """Algorithmic FLOP of the gradient densities as built here, and of the reference's
+    order of operations (grads.jl:61-89: IG = I3c inv(G), I2, d, m = d IG over (l, s))."""
+    t = 2.0 * (nbf * nbf * nbf5 + nbf * nbf5 * nbf5) * naux   # one two-sided transform
+    ours = 2 * t + 2 * (2.0 * nbf5 * nbf5 * naux * naux)        # b_pq, d; I2 and m GEMMs
+    ref = 2 * t + 2 * (2.0 * nbf * nbf * naux * naux)           # I2, d; IG and m
+    return ours, ref
+
+
+def grad_leg(api, synthetic, seed, shapes=((400, 1600, 120), (800, 3200, 240)), reps=3):
+    """RI nuclear gradients (grads.jl:1-100): the densities (dn_grad_create) and one
+    atom's contractions (dn_grad_atom) with derivative integrals resident in HBM, and
+    from host arrays at the first shape (PCIe-inclusive)."""
+    out = {}
+    for nbf, naux, nbf5 in shapes:
+        sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "fp64")
+        p = sysd.p
+        cj, ck = api.PNOFi_selector(sysd.n, p, dev)
+        elag, _ = api.compute_Lagrange2(sysd.C, sysd.n, sysd.H, dev, cj, ck, p.nalpha,
+                                        p.nbeta)
+        Wt = np.eye(naux) + np.tril(np.full((naux, naux), 1e-3), -1)
+        api.Gradient(dev, sysd.n, sysd.C, cj, ck, elag, Wt=Wt).close()      # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            api.Gradient(dev, sysd.n, sysd.C, cj, ck, elag, Wt=Wt).close()
+        ms_dens = (time.perf_counter() - t0) * 1e3 / reps
+        g = api.Gradient(dev, sysd.n, sysd.C, cj, ck, elag, Wt=Wt)
+        dS = api.DeviceArray((nbf, nbf, 3))
+        g3 = api.DeviceArray((nbf, nbf, naux, 3))
+        g2 = api.DeviceArray((naux, naux, 3))
+        g.atom(dS, dS, dS, g3, g2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.atom(dS, dS, dS, g3, g2)
+        ms_atom = (time.perf_counter() - t0) * 1e3 / reps
+        # bytes one atom reads: d once and geri3c (3 components), m and geri2c, the
+        # nbf^2 densities against dS, dT, dV
+        atom_bytes = 8.0 * (4 * nbf * nbf * naux + 4 * naux * naux + 3 * 4 * nbf * nbf)
+        ours, ref = grad_flops(nbf, naux, nbf5)
+        rec = {"densities_ms": ms_dens,
+               "densities_tflops": ours / (ms_dens * 1e-3) / 1e12,
+               "densities_flops": ours, "reference_order_flops": ref,
+               "reference_order_tflops_equiv": ref / (ms_dens * 1e-3) / 1e12,
+               "atom_ms_hbm_inputs": ms_atom,
+               "atom_gbs": atom_bytes / (ms_atom * 1e-3) / 1e9}
+        if nbf == shapes[0][0]:
+            h3 = np.zeros((nbf, nbf, naux, 3), order="F")
+            h2 = np.zeros((naux, naux, 3), order="F")
+            hS = np.zeros((nbf, nbf, 3), order="F")
+            t0 = time.perf_counter()
+            g.atom(hS, hS, hS, h3, h2)
+            rec["atom_ms_host_inputs"] = (time.perf_counter() - t0) * 1e3
+            del h3
+        for a in (dS, g3, g2):
+            a.free()
+        g.close()
+        dev.close()
+        out["%d_%d_%d" % (nbf, naux, nbf5)] = rec
+        log("gradient %d/%d/%d: densities %.2f ms (%.1f TFLOP/s), atom %.3f ms (%.0f GB/s)"
+            % (nbf, naux, nbf5, ms_dens, rec["densities_tflops"], ms_atom, rec["atom_gbs"]))
+    return out
+
+
 def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64, nbf_range=(48, 400)):
     """64 systems, nbf ~ U[48, 400], naux = 4 nbf, nbf5 = ceil(nbf/3): one grouped build
     of all of them per step (graph replay), against per-system builds on their own
@@ -324,8 +389,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--precision", choices=("fp64", "split"), default="fp64",
                     help="arithmetic of the headline build (fp64 is the metric's)")
-    ap.add_argument("--legs", default="800,grouped,served",
-                    help="extra legs to run: any of 800, grouped, served (or none)")
+    ap.add_argument("--legs", default="800,grouped,served,grad",
+                    help="extra legs to run: any of 800, grouped, served, grad (or none)")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served-solves leg and print its JSON (the main run "
                          "starts this in a child process under a time limit)")
@@ -438,6 +503,8 @@ def main():
         out["extra_grouped_64_small"] = grouped_leg(api, synthetic, max(5, args.steps // 2),
                                                     args.warmup, args.seed,
                                                     nbf_range=(16, 64))
+    if "grad" in legs:
+        out["extra_gradients"] = grad_leg(api, synthetic, args.seed)
     if "served" in legs:
         out["extra_served"] = served_in_child(args.served_timeout)
     print(json.dumps(out))

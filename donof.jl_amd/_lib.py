@@ -58,7 +58,8 @@ _SIGS = {
     "dn_sys_destroy": (_I, [_P]),
     "dn_sys_footprint": (_I, [ct.POINTER(dn_dims), _I, ct.POINTER(ct.c_uint64)]),
     "dn_sys_upload_b": (_I, [_P, _D]),
-    "dn_grad_create": (_I, [_P, _D, _D, _D, _D, _D, _D, ct.POINTER(_P)]),
+    "dn_sys_set_ri_factor": (_I, [_P, _D]),
+    "dn_grad_create": (_I, [_P, _D, _D, _D, _D, _D, ct.POINTER(_P)]),
     "dn_grad_atom": (_I, [_P, _P, _P, _P, _P, _P, _D]),
     "dn_grad_get_densities": (_I, [_P, _D, _D, _D, _D]),
     "dn_grad_destroy": (_I, [_P]),

@@ -122,6 +122,14 @@ class System:
         check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))
         return self
 
+    def set_ri_factor(self, Wt):
+        """The RI factor of an uploaded b = I3c W for the nuclear gradients:
+        Wt = W^{-1} G^{-1} (naux x naux), so that I3c G^{-1} = b Wt; G^{-1/2} for the
+        reference's eigen-route b (integrals.jl:68-91)."""
+        Wt = _colmajor(Wt, (self.nbfaux, self.nbfaux), "Wt")
+        check(self.lib.dn_sys_set_ri_factor(self.handle, dptr(Wt)))
+        return self
+
     def set_gamma(self, gamma):
         g = np.ascontiguousarray(gamma, dtype=np.float64)
         check(self.lib.dn_sys_set_gamma(self.handle, dptr(g)))
@@ -518,8 +526,9 @@ class Gradient:
     device from the resident b: RDM1, lag, the three-index density d and the metric
     density m.  `atom(...)` contracts them with one atom's derivative integrals.
 
-    Wt: None when b was set by System.ri_metric (b = I3c L^{-T}); otherwise the matrix
-    with I3c G^{-1} = b Wt (G^{-1/2} for the reference's eigen-route b)."""
+    Wt: None when b was set by System.ri_metric (b = I3c L^{-T}) or its factor was given
+    with System.set_ri_factor; otherwise the matrix with I3c G^{-1} = b Wt (G^{-1/2} for
+    the reference's eigen-route b), kept on the system."""
 
     def __init__(self, b, n, C, cj12, ck12, elag, Wt=None):
         s = _require_system(b)
@@ -530,10 +539,11 @@ class Gradient:
         cj12 = _colmajor(cj12, (nbf5, nbf5), "cj12")
         ck12 = _colmajor(ck12, (nbf5, nbf5), "ck12")
         elag = _colmajor(elag, (nbf, nbf), "elag")
-        W = None if Wt is None else _colmajor(Wt, (naux, naux), "Wt")
+        if Wt is not None:
+            s.set_ri_factor(Wt)
         h = ct.c_void_p()
         check(s.lib.dn_grad_create(s.handle, dptr(C), dptr(n), dptr(cj12), dptr(ck12),
-                                   dptr(elag), dptr(W), ct.byref(h)))
+                                   dptr(elag), ct.byref(h)))
         self.handle = h
 
     def atom(self, dS, dT, dV, geri3c, geri2c):

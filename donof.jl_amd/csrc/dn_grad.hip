@@ -39,7 +39,7 @@ struct dn_grad {
 namespace dn {
 namespace {
 
-constexpr int kDotBlocks = 256;
+constexpr int kDotBlocks = 1024;  // 16 wavefronts per CU for the HBM-bound dots
 constexpr int kDotThreads = 256;
 constexpr long kStageLen = 1L << 24;  // doubles per component and chunk (128 MiB)
 
@@ -82,6 +82,19 @@ __global__ __launch_bounds__(256) void grad_dens_kernel(const double* __restrict
   D[i] = p == q ? dg[p + (long)nbf5 * Q] : -ck12[j] * I2[Q * slab + q + (long)nbf5 * p];
 }
 
+// B[j] = W[j] + W[j^T] within each nbf5 x nbf5 slab: b_pq = W + W^T of the half
+// contraction
+__global__ __launch_bounds__(256) void grad_sym_kernel(const double* __restrict__ W, int nbf5,
+                                                       int naux, double* __restrict__ B) {
+  const long slab = (long)nbf5 * nbf5;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= slab * naux) return;
+  const long Q = i / slab;
+  const int j = (int)(i - Q * slab);
+  const int a = j % nbf5, c = j / nbf5;
+  B[i] = W[i] + W[Q * slab + c + (long)nbf5 * a];
+}
+
 // CT[p + nbf5 l] = C[l,p], CnT[p + nbf5 l] = n_p C[l,p] for p < nbf5
 __global__ void grad_ct_kernel(const double* __restrict__ C, const double* __restrict__ n,
                                int nbf, int nbf5, double* __restrict__ CT,
@@ -108,7 +121,9 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
 }
 
 // part[blockIdx.x + kDotBlocks k] = this block's share of sum_i x[i] y[i + k ystride],
-// k < 3, over i < len.  Grid-stride with a fixed grid: the order is fixed by len.
+// k < 3, over i < len.  Grid-stride with a fixed grid: the order is fixed by len.  VEC:
+// 16-byte loads of element pairs (x, y 16-byte aligned, ystride and len even).
+template <bool VEC>
 __global__ __launch_bounds__(kDotThreads) void dot3_kernel(const double* __restrict__ x,
                                                            const double* __restrict__ y,
                                                            long ystride, long len,
@@ -116,11 +131,24 @@ __global__ __launch_bounds__(kDotThreads) void dot3_kernel(const double* __restr
   __shared__ double red[kDotThreads / 64];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   const long step = (long)gridDim.x * kDotThreads;
-  for (long i = blockIdx.x * (long)kDotThreads + threadIdx.x; i < len; i += step) {
-    const double v = x[i];
-    a0 += v * y[i];
-    a1 += v * y[i + ystride];
-    a2 += v * y[i + 2 * ystride];
+  if constexpr (VEC) {
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    const double2* y0 = reinterpret_cast<const double2*>(y);
+    const double2* y1 = reinterpret_cast<const double2*>(y + ystride);
+    const double2* y2 = reinterpret_cast<const double2*>(y + 2 * ystride);
+    for (long i = blockIdx.x * (long)kDotThreads + threadIdx.x; i < len / 2; i += step) {
+      const double2 v = x2[i], u0 = y0[i], u1 = y1[i], u2 = y2[i];
+      a0 += v.x * u0.x + v.y * u0.y;
+      a1 += v.x * u1.x + v.y * u1.y;
+      a2 += v.x * u2.x + v.y * u2.y;
+    }
+  } else {
+    for (long i = blockIdx.x * (long)kDotThreads + threadIdx.x; i < len; i += step) {
+      const double v = x[i];
+      a0 += v * y[i];
+      a1 += v * y[i + ystride];
+      a2 += v * y[i + 2 * ystride];
+    }
   }
   a0 = block_sum256(a0, red);
   a1 = block_sum256(a1, red);
@@ -140,12 +168,27 @@ __global__ __launch_bounds__(kDotThreads) void dot3_reduce_kernel(const double* 
   for (int k = 0; k < 3; ++k) {
     double a = 0.0;
     for (int c = 0; c < nchunk; ++c)
-      a += part[(3L * c + k) * kDotBlocks + threadIdx.x];  // kDotThreads == kDotBlocks
+      for (int b = threadIdx.x; b < kDotBlocks; b += kDotThreads)
+        a += part[(3L * c + k) * kDotBlocks + b];
     a = block_sum256(a, red);
     if (threadIdx.x == 0) res[k] = a;
   }
 }
-static_assert(kDotThreads == kDotBlocks, "one reduce thread per dot block");
+
+// one chunk's partials: the vector kernel where the operands allow 16-byte loads
+int launch_dot3(hipStream_t st, const double* x, const double* y, long ystride, long len,
+                double* part) {
+  const bool vec = (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0 && ystride % 2 == 0 &&
+                   len % 2 == 0;
+  if (vec)
+    hipLaunchKernelGGL(dot3_kernel<true>, dim3(kDotBlocks), dim3(kDotThreads), 0, st, x, y,
+                       ystride, len, part);
+  else
+    hipLaunchKernelGGL(dot3_kernel<false>, dim3(kDotBlocks), dim3(kDotThreads), 0, st, x, y,
+                       ystride, len, part);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
 
 bool on_device(const void* p, int device) {
   hipPointerAttribute_t a{};
@@ -160,9 +203,7 @@ bool on_device(const void* p, int device) {
 int dot3(dn_grad* g, const double* x, const double* y, long len, double* res) {
   hipStream_t st = g->s->stream;
   if (on_device(y, g->s->ctx->device)) {
-    hipLaunchKernelGGL(dot3_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, st, x, y, len, len,
-                       g->part);
-    DN_HIP(hipGetLastError());
+    DN_TRY(launch_dot3(st, x, y, len, len, g->part));
     hipLaunchKernelGGL(dot3_reduce_kernel, dim3(1), dim3(kDotThreads), 0, st, g->part, 1, res);
     DN_HIP(hipGetLastError());
     return DN_OK;
@@ -174,9 +215,7 @@ int dot3(dn_grad* g, const double* x, const double* y, long len, double* res) {
     for (int k = 0; k < 3; ++k)
       DN_HIP(hipMemcpyAsync(g->stage + k * cl, y + k * len + off, cl * sizeof(double),
                             hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(dot3_kernel, dim3(kDotBlocks), dim3(kDotThreads), 0, st, x + off,
-                       g->stage, cl, cl, g->part + c * 3 * kDotBlocks);
-    DN_HIP(hipGetLastError());
+    DN_TRY(launch_dot3(st, x + off, g->stage, cl, cl, g->part + c * 3 * kDotBlocks));
   }
   hipLaunchKernelGGL(dot3_reduce_kernel, dim3(1), dim3(kDotThreads), 0, st, g->part,
                      (int)nchunk, res);
@@ -194,13 +233,14 @@ using namespace dn;
 extern "C" {
 
 int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj12,
-                   const double* ck12, const double* elag, const double* Wt, dn_grad** out) {
+                   const double* ck12, const double* elag, dn_grad** out) {
   DN_TRY(check_sys(s));
   if (!C || !n || !cj12 || !ck12 || !elag || !out) return fail(DN_ERR_ARG, "null argument");
   *out = nullptr;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been set");
-  if (!Wt && !s->riw_set)
-    return fail(DN_ERR_STATE, "b was not set by dn_sys_ri_metric: pass its RI factor Wt");
+  if (!s->riw_set)
+    return fail(DN_ERR_STATE, "no RI factor for b: set b with dn_sys_ri_metric, or give "
+                              "the factor of an uploaded b with dn_sys_set_ri_factor");
   if (s->lockstep) return fail(DN_ERR_STATE, "the system is driven by a lock-step group");
   const dn_dims& dd = s->d;
   const size_t nbf = dd.nbf, naux = dd.nbfaux, nbf5 = dd.nbf5, nn = nbf5 * nbf5;
@@ -209,11 +249,14 @@ int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj
   const long maxlen = std::max<long>(len3, (long)std::max(naux * naux, nbf * nbf));
   const long stage_len = std::min<long>(kStageLen, maxlen);
   const long part_len = (maxlen + stage_len - 1) / stage_len * 3 * kDotBlocks;
-  // persistent: d, m, rdm1, lag, stage, part, res; transient: I2, D, dg, CT, CnT, T, Wt
+  // split-K of the m GEMM (K = nbf5^2, an naux x naux output: too few tiles alone)
+  const int msplit = nn % 4 == 0 && nn / 4 >= 64 ? 4 : (nn % 2 == 0 && nn >= 128 ? 2 : 1);
+  // persistent: d, m, rdm1, lag, stage, part, res; transient: I2, D, dg, CT, CnT, T and
+  // the m partials
   const size_t keep = r256(len3) + r256(naux * naux) + 2 * r256(nbf * nbf) +
                       r256(3 * stage_len) + r256(part_len) + r256(16);
   const size_t tmp = 2 * r256(nn * naux) + r256(nbf5 * naux) + 2 * r256(nbf * nbf5) +
-                     r256(nbf * nbf) + r256(naux * naux);
+                     r256(nbf * nbf) + (msplit > 1 ? r256(msplit * naux * naux) : 0);
   auto* g = new dn_grad;
   g->s = s;
   g->nbf = dd.nbf; g->naux = dd.nbfaux; g->nbf5 = dd.nbf5;
@@ -244,27 +287,35 @@ int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj
     double* CT = dg + r256(nbf5 * naux);
     double* CnT = CT + r256(nbf * nbf5);
     double* T = CnT + r256(nbf * nbf5);
-    double* Wd = T + r256(nbf * nbf);
+    double* mpart = T + r256(nbf * nbf);
     hipStream_t st = s->stream;
     DN_TRY(h2d(s, s->C, C, nbf * nbf));
     DN_TRY(h2d(s, s->n, n, nbf5));
     DN_TRY(h2d(s, s->cj12, cj12, nn));
     DN_TRY(h2d(s, s->ck12, ck12, nn));
     DN_TRY(h2d(s, s->elag, elag, nbf * nbf));
-    if (Wt) DN_TRY(h2d(s, Wd, Wt, naux * naux));
-    // b_pq = C5^T b_Q C5 (the J/K build's general transform): bt[q + nbf5 p + nbf5^2 Q]
+    // b_pq[p,q,Q] = (C5^T b_Q C5)[p,q] at q + nbf5 p + nbf5^2 Q: the J/K build's transform
+    // (for a symmetric b its half contraction W_Q, then b_pq = W + W^T into D's buffer)
     DN_TRY(pad_c(s));
-    DN_TRY(gemm_f64(st, t1_gemm(s, false)));
-    DN_TRY(gemm_f64(st, t2_flat_gemm(s)));
+    const double* bpq = s->bt;
+    if (s->b_sym && s->use_sym) {
+      DN_TRY(gemm_f64(st, t1_gemm(s, true)));
+      DN_TRY(gemm_f64(st, t2_flat_gemm(s)));
+      const long nB = (long)(nn * naux);
+      hipLaunchKernelGGL(grad_sym_kernel, dim3((nB + 255) / 256), dim3(256), 0, st, s->bt,
+                         (int)nbf5, (int)naux, D);
+      DN_HIP(hipGetLastError());
+      bpq = D;
+    } else {
+      DN_TRY(gemm_f64(st, t1_gemm(s, false)));
+      DN_TRY(gemm_f64(st, t2_flat_gemm(s)));
+    }
     // I2[p,q,R] = sum_P b_pq[p,q,P] Wt[P,R], same layout (grads.jl:61,66)
     Gemm i2;
     i2.M = (int)nn; i2.N = (int)naux; i2.K = (int)naux;
-    i2.A = s->bt; i2.lda = (long)nn; i2.a_kcontig = false;
-    if (Wt) {
-      i2.B = Wd; i2.ldb = (long)naux; i2.b_kcontig = true;    // Wt[P + naux R]
-    } else {
-      i2.B = s->riw; i2.ldb = (long)naux; i2.b_kcontig = false;  // L^{-1}[P,R] = riw[R + naux P]
-    }
+    i2.A = bpq; i2.lda = (long)nn; i2.a_kcontig = false;
+    i2.B = s->riw; i2.ldb = (long)naux;
+    i2.b_kcontig = !s->riw_transposed;  // Wt[P + naux R], or riw = Wt^T: Wt[P,R] at R + naux P
     i2.C = I2; i2.ldc = (long)nn;
     DN_TRY(gemm_f64(st, i2));
     // D (grads.jl:68-84)
@@ -282,7 +333,13 @@ int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj
     gm.B = I2; gm.ldb = (long)nn; gm.b_kcontig = true;
     gm.C = g->m; gm.ldc = (long)naux;
     gm.lds = true;
-    DN_TRY(gemm_f64(st, gm));
+    if (msplit > 1) {
+      DN_TRY(gemm_f64(st, splitk_partials(gm, msplit, mpart)));
+      DN_TRY(reduce_splits(st, mpart, (long)(naux * naux), msplit, (int)naux, (int)naux,
+                           (long)naux, g->m, (long)naux, 0.0, 1.0));
+    } else {
+      DN_TRY(gemm_f64(st, gm));
+    }
     // d_Q = C5 D_Q C5^T (grads.jl:69,74,79,84): Zt[p,s,Q] = sum_q D[p,q,Q] C5[s,q] into
     // bpn (free after the transform), then d[l,(s,Q)] = sum_p C5[l,p] Zt[p,(s,Q)]
     const long nC = (long)(nbf * nbf5);

@@ -439,10 +439,12 @@ struct dn_sys {
   void* lockstep = nullptr;
   int build_groups = 0;
   int grad_objs = 0;     // live dn_grad objects (they run on the system's stream)
-  // the auxiliary-index factor of the RI metric kept by dn_sys_ri_metric: riw = L^{-T}
-  // with G = L L^T, so that I3c G^{-1} = b L^{-1} = b riw^T (nuclear gradients)
+  // the auxiliary-index factor of b for the nuclear gradients, I3c G^{-1} = b Wt:
+  // dn_sys_ri_metric keeps riw = L^{-T} (G = L L^T, b = I3c L^{-T}, Wt = riw^T);
+  // dn_sys_set_ri_factor stores a caller's Wt as is (column-major)
   double* riw = nullptr;  // naux*naux
   bool riw_set = false;
+  bool riw_transposed = false;  // riw holds Wt^T
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
   dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
                                      // the split-precision one at scal + 32)

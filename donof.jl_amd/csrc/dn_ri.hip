@@ -163,6 +163,18 @@ int trsm_right_lt(hipStream_t s, double* X, double* Y, long rows, int naux, cons
 
 using namespace dn;
 
+extern "C" int dn_sys_set_ri_factor(dn_sys* s, const double* Wt) {
+  DN_TRY(check_sys(s));
+  if (!Wt) return fail(DN_ERR_ARG, "null host pointer");
+  if (!s->b_ready) return fail(DN_ERR_STATE, "set b before its RI factor");
+  const size_t naux = s->d.nbfaux;
+  DN_TRY(h2d(s, s->riw, Wt, naux * naux));
+  DN_HIP(hipStreamSynchronize(s->stream));
+  s->riw_set = true;
+  s->riw_transposed = false;
+  return DN_OK;
+}
+
 extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   DN_TRY(check_sys(s));
   if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
@@ -209,6 +221,6 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   rc = run();
   dev_free(s->ctx, s->stream, w);
   (void)hipStreamSynchronize(s->stream);
-  if (rc == DN_OK) s->b_ready = s->riw_set = true;
+  if (rc == DN_OK) s->b_ready = s->riw_set = s->riw_transposed = true;
   return rc;
 }

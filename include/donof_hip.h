@@ -100,11 +100,12 @@ int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
  * grads.jl:24-25 zero them; the caller's arrays are not modified) and the Lagrangian
  * elag (nbf x nbf): RDM1 = 2 C5 diag(n) C5^T (grads.jl:5), lag = 2 C elag C^T (6), the
  * three-index density d[l,s,Q] (66-84) and the metric density m[Q,R] (89).  IG =
- * I3c G^{-1} is taken as b Wt: Wt = NULL uses the factor kept by dn_sys_ri_metric
- * (b = I3c L^{-T}, so Wt = L^{-1}); for any other b = I3c W pass Wt = W^{-1} G^{-1}
- * (naux x naux, column-major; G^{-1/2} for the reference's eigen-route b of
- * integrals.jl:68-91).  The object holds nbf^2 naux + naux^2 + 2 nbf^2 doubles of HBM
- * (charged to the context's budget) and keeps the system from being destroyed.
+ * I3c G^{-1} is taken as b Wt with the RI factor Wt of the resident b: dn_sys_ri_metric
+ * keeps it (b = I3c L^{-T}, Wt = L^{-1}); for an uploaded b = I3c W give it with
+ * dn_sys_set_ri_factor (Wt = W^{-1} G^{-1}, naux x naux, column-major: G^{-1/2} for the
+ * reference's eigen-route b of integrals.jl:68-91; setting b again drops it).  The
+ * object holds nbf^2 naux + naux^2 + 2 nbf^2 doubles of HBM (charged to the context's
+ * budget) and keeps the system from being destroyed.
  * dn_grad_atom returns one atom's electronic gradient (grads.jl:13-21, 91-96)
  *   grad3[k] = -<lag, dS_k> + <RDM1, dT_k + dV_k> + 2 <d, geri3c_k> - <m, geri2c_k>
  * from the derivative integrals of that atom (dS, dT, dV: nbf x nbf x 3; geri3c:
@@ -114,8 +115,9 @@ int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
  * (utils.jl:315-336) is the caller's.  dn_grad_get_densities copies out any of RDM1,
  * lag, m, d (NULL skips). */
 typedef struct dn_grad dn_grad;
+int dn_sys_set_ri_factor(dn_sys* sys, const double* Wt);
 int dn_grad_create(dn_sys* sys, const double* C, const double* n, const double* cj12,
-                   const double* ck12, const double* elag, const double* Wt, dn_grad** out);
+                   const double* ck12, const double* elag, dn_grad** out);
 int dn_grad_atom(dn_grad* grad, const double* dS, const double* dT, const double* dV,
                  const double* geri3c, const double* geri2c, double* grad3);
 int dn_grad_get_densities(dn_grad* grad, double* RDM1, double* lag, double* m, double* d);
