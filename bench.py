@@ -134,6 +134,19 @@ def time_builds(dev, steps, warmup):
     return (t1 - t0) * 1e3 / steps, k_ms / max(k_n, 1), k_n
 
 
+def time_lagrangian(api, sysd, dev, reps):
+    """(elag, H_mat, E) of compute_Lagrange2 at the system's orbitals and its ms per call
+    (host arrays in and out)."""
+    p = sysd.p
+    cj, ck = api.PNOFi_selector(sysd.n, p, dev)
+    args = (sysd.C, sysd.n, sysd.H, dev, cj, ck, p.nalpha, p.nbeta)
+    res = api.compute_Lagrange2(*args, return_energy=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        api.compute_Lagrange2(*args, return_energy=True)
+    return res, (time.perf_counter() - t0) * 1e3 / reps
+
+
 def make_system(api, synthetic, nbf, naux, nbf5, seed, precision):
     sysd = synthetic.SyntheticSystem(nbf, naux, nbf5, seed)
     dev = api.System(sysd.p, precision=precision)
@@ -151,6 +164,7 @@ def extra_legs(api, synthetic, steps, warmup, seed):
     sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "fp64")
     ms, kms, _ = time_builds(dev, steps, warmup)
     J64, K64, H64 = dev.get_jkh()
+    lag64, lag_ms64 = time_lagrangian(api, sysd, dev, max(3, steps // 2))
     dev.close()
     out["fp64"] = {"builds_per_s": 1e3 / ms, "ms_per_build": ms, "tflops": F / (ms * 1e-3) / 1e12,
                    "transform_kernel_ms": kms,
@@ -161,7 +175,11 @@ def extra_legs(api, synthetic, steps, warmup, seed):
     sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "split")
     ms16, kms16, _ = time_builds(dev, steps, warmup)
     J16, K16, H16 = dev.get_jkh()
+    lag16, lag_ms16 = time_lagrangian(api, sysd, dev, max(3, steps // 2))
     dev.close()
+    el64, _, El64 = lag64
+    el16, _, El16 = lag16
+    scale = float(np.abs(el64).max())
     k1_useful = 2.0 * nbf5 * nbf * nbf * naux
     x16 = executed_flops_sym_k1(nbf, naux, nbf5, "split")
     # GNOF energy (calce, pnof.jl:1273-1300) of both builds, on the device
@@ -182,6 +200,20 @@ def extra_legs(api, synthetic, steps, warmup, seed):
         "K_max_rel_err": float(np.abs(K16 - K64).max() / np.abs(K64).max()),
         "gnof_energy_abs_err_Ha": float(abs(E16 - E64)),
     }
+    # the Lagrangian (utils.jl:1-59) at the same orbitals: FP64, and split precision (both
+    # transforms on FP16 planes; C^J/C^K terms and the contraction FP64), host boundary
+    # included (C, H, n, cj12, ck12 up; elag, H_mat down)
+    out["lagrangian"] = {
+        "fp64_ms": lag_ms64, "split_ms": lag_ms16, "split_speedup": lag_ms64 / lag_ms16,
+        "elag_max_rel_err": float(np.abs(el16 - el64).max()) / scale,
+        "orbital_gradient_max_abs_err": float(np.abs(4 * (el16 - el16.T)
+                                                     - 4 * (el64 - el64.T)).max()),
+        "orbital_gradient_err_rel_to_max_elag":
+            float(np.abs(4 * (el16 - el16.T) - 4 * (el64 - el64.T)).max()) / scale,
+        "energy_abs_err_Ha": float(abs(El16 - El64)),
+    }
+    log("lagrangian 800/3200/240: fp64 %.2f ms, split %.2f ms, |d grad| %.2e"
+        % (lag_ms64, lag_ms16, out["lagrangian"]["orbital_gradient_max_abs_err"]))
     out["split_speedup_vs_fp64"] = ms / ms16
     log("split 800/3200/240: %.3f ms/build (x%.2f), |dE|=%.2e"
         % (ms16, ms / ms16, abs(E16 - E64)))
@@ -484,7 +516,9 @@ def main():
     b = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        b = dev.download_b()
+        # a split system keeps b only as FP16 planes: the host generator makes the same b
+        b = (dev.download_b() if args.precision == "fp64"
+             else synthetic.make_b(nbf, naux, args.seed))
     dev.close()
     if b is not None:
         out["cpu_baseline"] = cpu_baseline(sysd.C, sysd.H, b, nbf5, args.cpu_seconds)

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <functional>
+
 #include "dn_internal.h"
 
 namespace dn {
@@ -105,14 +107,14 @@ size_t lag_splits(const dn_dims& d) {
 Layout layout_of(const dn_dims& d, int precision) {
   Layout L{};
   const size_t nbf = d.nbf, nbf5 = d.nbf5, naux = d.nbfaux, ld = padded_ld(d.nbf);
-  L.b = ld * nbf * naux;
+  L.b = precision == DN_SPLIT_F16 ? 0 : ld * nbf * naux;  // split: FP16 planes only
   L.C = nbf * nbf;
   L.H = nbf * nbf;
   L.n = nbf5;
   L.nn = nbf5 * nbf5;
   L.nbf2 = nbf * nbf;
   L.nbf_nbf5 = nbf * nbf5;
-  L.big = nbf5 * nbf * naux;
+  L.big = nbf5 * nbf5 * naux;
   L.bpn = ld * nbf5 * naux;
   L.cp = ld != nbf ? ld * nbf : 0;
   L.bmat = nbf5 * nbf5 * naux;
@@ -296,16 +298,49 @@ int check_sys(dn_sys* s) {
   return DN_OK;
 }
 
-int prepare_b(dn_sys* s) {
+int prepare_b(dn_sys* s, const double* b) {
   // b[m,n,Q] = b[n,m,Q] (to 2^-50 of max|b|) lets the J/K build read half of b
   double dmax = 0.0, bmax = 0.0;
-  DN_TRY(b_symmetry(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux,
+  DN_TRY(b_symmetry(s->stream, b, s->d.nbf, s->ldb, s->d.nbfaux,
                     reinterpret_cast<unsigned long long*>(s->scal + 62), &dmax, &bmax));
   s->b_sym = dmax <= std::ldexp(bmax, -50);
   if (s->precision != DN_SPLIT_F16) return DN_OK;
-  DN_TRY(split_lhnorm(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->lhmax));
-  return split_prepare_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, s->Kpad, s->bscale, s->binv,
+  DN_TRY(split_lhnorm(s->stream, b, s->d.nbf, s->ldb, s->d.nbfaux, s->lhmax));
+  return split_prepare_b(s->stream, b, s->d.nbf, s->ldb, s->d.nbfaux, s->Kpad, s->bscale, s->binv,
                          s->bhi, s->blo);
+}
+
+int set_b(dn_sys* s, const std::function<int(double*)>& fill) {
+  s->b_ready = false;
+  s->riw_set = false;  // a new b: its RI factor is set with it, if at all
+  if (s->b) {
+    DN_TRY(fill(s->b));
+    DN_TRY(prepare_b(s, s->b));
+    DN_HIP(hipStreamSynchronize(s->stream));
+    s->b_ready = true;
+    return DN_OK;
+  }
+  // split precision: b exists in FP64 only while its planes are made
+  const size_t elems = (size_t)s->ldb * s->d.nbf * s->d.nbfaux;
+  const uint64_t bytes = elems * sizeof(double);
+  DN_TRY(ctx_reserve(s->ctx, bytes));
+  double* w = nullptr;
+  int rc = dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&w));
+  if (rc == DN_OK) {
+    auto run = [&]() -> int {
+      if (s->ldb != s->d.nbf) DN_HIP(hipMemsetAsync(w, 0, bytes, s->stream));  // pad rows
+      DN_TRY(fill(w));
+      DN_TRY(prepare_b(s, w));
+      DN_HIP(hipStreamSynchronize(s->stream));
+      return DN_OK;
+    };
+    rc = run();
+    dev_free(s->ctx, s->stream, w);
+    (void)hipStreamSynchronize(s->stream);
+  }
+  ctx_unreserve(s->ctx, bytes);
+  if (rc == DN_OK) s->b_ready = true;
+  return rc;
 }
 
 int begin_timing(dn_sys* s) {
@@ -400,15 +435,39 @@ int jkh_resident(dn_sys* s) {
   return DN_OK;
 }
 
+// The Lagrangian's two transforms in split precision (FP16 hi+lo planes of b and C5, FP32
+// accumulation): T[p,n,Q] = sum_m C5[m,p] b[m,n,Q] (the general first transform) and
+// b_pq[p,q,Q] = sum_n T[p,n,Q] C5[n,q], then both transposed into the FP64 layouts of
+// the FP64 path (bpn[n,p,Q], b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)) for its C^J/C^K terms
+// and contraction.
+int lagrange_split_transforms(dn_sys* s) {
+  const dn_dims& d = s->d;
+  DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
+                         s->clo, nullptr, d.nbfaux, s->qs, s->qinv));
+  DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
+                  s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
+  DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
+                  s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
+  const long nn = (long)d.nbf5 * d.nbf5;
+  DN_TRY(transpose_f32_f64(s->stream, s->bpn32, d.nbf5, d.nbf, d.nbf5, (long)d.nbf5 * d.nbf,
+                           s->bpn, s->ldb, (long)s->ldb * d.nbf5, d.nbfaux));
+  return transpose_f32_f64(s->stream, s->bpq32, d.nbf5, d.nbf5, d.nbf5, nn, s->bt, d.nbf5, nn,
+                           d.nbfaux);
+}
+
 int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   const dn_dims& d = s->d;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been uploaded or generated");
   if (nbeta < 0 || nalpha < 0 || nbeta > d.nbf5 || nalpha > d.nbf5)
     return fail(DN_ERR_ARG, "nalpha/nbeta out of range");
   const long slice = (long)d.nbf5 * d.nbf5;
-  DN_TRY(pad_c(s));
-  DN_TRY(transform1(s, false));
-  DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));  // b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)
+  if (s->precision == DN_SPLIT_F16) {
+    DN_TRY(lagrange_split_transforms(s));
+  } else {
+    DN_TRY(pad_c(s));
+    DN_TRY(transform1(s, false));
+    DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));  // b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)
+  }
   DN_TRY(hc_products(s, true, false));
   DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
                s->X, s->w));
@@ -667,7 +726,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     if (rc == DN_OK) rc = alloc(s, n, p);
   };
   s->ldb = padded_ld(dims->nbf);
-  A(L.b, &s->b);
+  if (L.b) A(L.b, &s->b);
   A(L.C, &s->C);
   if (L.cp) A(L.cp, &s->Cp);
   else s->Cp = s->C;
@@ -742,7 +801,8 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   }
   s->part_elems = L.part;
   if (rc == DN_OK && s->ldb != dims->nbf) {  // the pad rows of b, bpn and Cp read as zero
-    hipError_t e1 = hipMemsetAsync(s->b, 0, L.b * sizeof(double), s->stream);
+    hipError_t e1 = s->b ? hipMemsetAsync(s->b, 0, L.b * sizeof(double), s->stream)
+                         : hipSuccess;
     hipError_t e2 = hipMemsetAsync(s->bpn, 0, L.bpn * sizeof(double), s->stream);
     hipError_t e3 = hipMemsetAsync(s->Cp, 0, L.cp * sizeof(double), s->stream);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
@@ -816,19 +876,19 @@ int dn_sys_upload_b(dn_sys* s, const double* b) {
   if (!b) return fail(DN_ERR_ARG, "null host pointer");
   const size_t nbf = s->d.nbf;
   // into rows of ldb doubles (the pad rows keep their zeros)
-  DN_HIP(hipMemcpy2DAsync(s->b, s->ldb * sizeof(double), b, nbf * sizeof(double),
-                          nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyHostToDevice,
-                          s->stream));
-  DN_TRY(prepare_b(s));
-  DN_HIP(hipStreamSynchronize(s->stream));
-  s->b_ready = true;
-  s->riw_set = false;  // b no longer comes from dn_sys_ri_metric
-  return DN_OK;
+  return set_b(s, [&](double* dst) -> int {
+    DN_HIP(hipMemcpy2DAsync(dst, s->ldb * sizeof(double), b, nbf * sizeof(double),
+                            nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyHostToDevice,
+                            s->stream));
+    return DN_OK;
+  });
 }
 
 int dn_sys_download_b(dn_sys* s, double* b) {
   DN_TRY(check_sys(s));
   if (!b) return fail(DN_ERR_ARG, "null host pointer");
+  if (!s->b)
+    return fail(DN_ERR_STATE, "a split-precision system keeps b only as FP16 hi/lo planes");
   const size_t nbf = s->d.nbf;
   DN_HIP(hipMemcpy2DAsync(b, nbf * sizeof(double), s->b, s->ldb * sizeof(double),
                           nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyDeviceToHost,
@@ -839,12 +899,9 @@ int dn_sys_download_b(dn_sys* s, double* b) {
 
 int dn_sys_generate_b(dn_sys* s, uint64_t seed) {
   DN_TRY(check_sys(s));
-  DN_TRY(synth_b(s->stream, s->b, s->d.nbf, s->ldb, s->d.nbfaux, seed));
-  DN_TRY(prepare_b(s));
-  DN_HIP(hipStreamSynchronize(s->stream));
-  s->b_ready = true;
-  s->riw_set = false;  // b no longer comes from dn_sys_ri_metric
-  return DN_OK;
+  return set_b(s, [&](double* dst) {
+    return synth_b(s->stream, dst, s->d.nbf, s->ldb, s->d.nbfaux, seed);
+  });
 }
 
 int dn_sys_set_orbitals(dn_sys* s, const double* C) {

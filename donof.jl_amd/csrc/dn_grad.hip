@@ -238,6 +238,9 @@ int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj
   if (!C || !n || !cj12 || !ck12 || !elag || !out) return fail(DN_ERR_ARG, "null argument");
   *out = nullptr;
   if (!s->b_ready) return fail(DN_ERR_STATE, "b has not been set");
+  if (!s->b)
+    return fail(DN_ERR_STATE, "nuclear gradients need an FP64 system (a split-precision "
+                              "system keeps b only as FP16 planes)");
   if (!s->riw_set)
     return fail(DN_ERR_STATE, "no RI factor for b: set b with dn_sys_ri_metric, or give "
                               "the factor of an uploaded b with dn_sys_set_ri_factor");

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -152,6 +153,9 @@ int split_group_prep(hipStream_t s, const SplitPrepDesc* dev, int n, long plane_
 int split_group_scales(hipStream_t s, const SplitPrepDesc* dev, int n, long q_blocks);
 int split_k1_begin(hipStream_t s, float* qmax, int naux);
 int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float* qinv);
+// dst[c + ldd r] = src[r + lds c] (FP32 -> FP64) for r < R, c < Cc, per slab
+int transpose_f32_f64(hipStream_t s, const float* src, int R, int Cc, long lds, long sbs,
+                      double* dst, long ldd, long sbd, int nbatch);
 long split_grouped_plan(GroupedSplit* desc, int n, int config);
 int split_grouped(hipStream_t s, const GroupedSplit* desc_host, const GroupedSplit* desc_dev,
                   int n, long total, int config, int kind, int grid);
@@ -271,7 +275,10 @@ int d2h(dn_sys* s, double* dst, const double* src, size_t elems);
 int jkh_resident(dn_sys* s);
 int lagrange_resident(dn_sys* s, int nalpha, int nbeta);
 bool all_finite(const double* x, size_t n);
-int prepare_b(dn_sys* s);
+int prepare_b(dn_sys* s, const double* b);  // symmetry flag (and split planes) of b
+// b written by fill into the system's FP64 b, or for a split system into a transient FP64
+// buffer that its planes are made from; clears the RI factor
+int set_b(dn_sys* s, const std::function<int(double*)>& fill);
 int begin_timing(dn_sys* s);  // HIP events around the dominant kernel
 int end_timing(dn_sys* s);
 }  // namespace dn
@@ -360,9 +367,10 @@ struct dn_sys {
   // resident state (column-major)
   // b[m,n,Q] at m + ldb (n + nbf Q); ldb = nbf rounded up to even, the pad row zero, so
   // that every row of b (and of bpn, Cp) starts 16-byte aligned and the transforms take
-  // the LDS-direct tiles for odd nbf too (contracting over ldb: the pads add zeros)
+  // the LDS-direct tiles for odd nbf too (contracting over ldb: the pads add zeros).
+  // Split-precision systems keep b only as its FP16 planes (b is null there).
   int ldb = 0;
-  double* b = nullptr;      // ldb*nbf*naux
+  double* b = nullptr;      // ldb*nbf*naux (FP64 systems)
   double* C = nullptr;      // nbf*nbf
   double* Cp = nullptr;     // ldb*nbf: C with the zero pad row (C itself when ldb == nbf),
                             // refreshed from C before the transforms (pad_c)
@@ -378,8 +386,8 @@ struct dn_sys {
   // workspaces
   double* bpn = nullptr;    // ldb*nbf5*naux : [n,p,Q] = (b_Q^T C5)[n,p] (Lh^T C5 symmetric);
                             //                 pad row zero
-  double* bt = nullptr;     // nbf5*nbf*naux : Lagrangian b_MO^T[q,a,Q] at a + nbf (q + nbf5 Q);
-                            //   J/K build: b_pq[q,p,Q] (general b) or W^T (symmetric b)
+  double* bt = nullptr;     // nbf5*nbf5*naux: b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q) (general
+                            //   transform, the Lagrangian) or W^T (symmetric J/K build)
   double* bmat = nullptr;   // nbf5*nbf5*naux: [q,b,Q] at b + nbf5 (q + nbf5 Q), the C^K/C^J
                             //   right operand of the Lagrangian GEMM
   double* X = nullptr;      // nbf5*naux     : b_pp diagonal per Q

@@ -185,10 +185,15 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   const size_t nG = (size_t)naux * naux, nP = (size_t)naux * NB;
   const size_t nI = std::max((size_t)rows * naux, nG);  // I3c, then the identity
   const size_t bytes = (nI + nG + 2 * nP) * sizeof(double) + 256;
-  if (s->ctx->budget && s->ctx->used + bytes > s->ctx->budget)
-    return fail(DN_ERR_BUDGET, "RI metric workspace exceeds the HBM budget");
+  DN_TRY(ctx_reserve(s->ctx, bytes));
   char* w = nullptr;
-  DN_TRY(dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&w)));
+  {
+    const int rc0 = dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&w));
+    if (rc0 != DN_OK) {
+      ctx_unreserve(s->ctx, bytes);
+      return rc0;
+    }
+  }
   double* dI = reinterpret_cast<double*>(w);
   double* dG = dI + nI;
   double* Dinv = dG + nG;
@@ -207,20 +212,21 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
     DN_HIP(hipStreamSynchronize(s->stream));
     if (host_flag)
       return fail(DN_ERR_ARG, "RI metric is not positive definite (Cholesky pivot <= 0)");
-    DN_TRY(trsm_right_lt(s->stream, dI, s->b, rows, naux, dG, Dinv));
+    DN_TRY(set_b(s, [&](double* dst) {
+      return trsm_right_lt(s->stream, dI, dst, rows, naux, dG, Dinv);
+    }));
     // riw = 1 L^{-T}: I3c G^{-1} = b L^{-1} = b riw^T for the nuclear gradients
     hipLaunchKernelGGL(identity_kernel, dim3((nG + 255) / 256), dim3(256), 0, s->stream, dI,
                        naux);
     DN_HIP(hipGetLastError());
     DN_TRY(trsm_right_lt(s->stream, dI, s->riw, naux, naux, dG, Dinv));
-    DN_TRY(prepare_b(s));
     DN_HIP(hipStreamSynchronize(s->stream));
     return DN_OK;
   };
-  s->riw_set = false;
   rc = run();
   dev_free(s->ctx, s->stream, w);
   (void)hipStreamSynchronize(s->stream);
-  if (rc == DN_OK) s->b_ready = s->riw_set = s->riw_transposed = true;
+  ctx_unreserve(s->ctx, bytes);
+  if (rc == DN_OK) s->riw_set = s->riw_transposed = true;
   return rc;
 }

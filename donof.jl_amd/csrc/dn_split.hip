@@ -890,6 +890,45 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
   return DN_OK;
 }
 
+// dst[c + ldd r] = (double) src[r + lds c] for r < R, c < Cc in each of nbatch slabs: the
+// FP32 products of the split transforms, transposed into the FP64 layouts the Lagrangian
+// reads.  32 x 32 tiles through LDS (coalesced reads and writes); one block per tile.
+__global__ __launch_bounds__(256) void transpose_f32_f64_kernel(const float* __restrict__ src,
+                                                                int R, int Cc, long lds, long sbs,
+                                                                double* __restrict__ dst,
+                                                                long ldd, long sbd, int tiles_r,
+                                                                int tiles_c) {
+  __shared__ float t[32][33];
+  const long per = (long)tiles_r * tiles_c;
+  const long z = blockIdx.x / per;
+  const int rem = (int)(blockIdx.x - z * per);
+  const int r0 = 32 * (rem % tiles_r), c0 = 32 * (rem / tiles_r);
+  const float* S = src + z * sbs;
+  double* D = dst + z * sbd;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + tx, c = c0 + j;
+    if (r < R && c < Cc) t[j][tx] = S[r + lds * c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + tx, r = r0 + j;
+    if (r < R && c < Cc) D[c + ldd * r] = (double)t[tx][j];
+  }
+}
+
+int transpose_f32_f64(hipStream_t s, const float* src, int R, int Cc, long lds, long sbs,
+                      double* dst, long ldd, long sbd, int nbatch) {
+  const int tr = (R + 31) / 32, tc = (Cc + 31) / 32;
+  const long blocks = (long)tr * tc * nbatch;
+  if (blocks <= 0) return DN_OK;
+  if (blocks > 0x7fffffffL) return fail(DN_ERR_ARG, "transpose grid too large");
+  hipLaunchKernelGGL(transpose_f32_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, R,
+                     Cc, lds, sbs, dst, ldd, sbd, tr, tc);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
 int split_lhnorm(hipStream_t s, const double* b, int nbf, int ld, int naux, float* lhmax) {
   hipLaunchKernelGGL(b_lhnorm_kernel, dim3(naux), dim3(256), 0, s, b, nbf, ld, lhmax);
   DN_HIP(hipGetLastError());

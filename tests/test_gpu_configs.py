@@ -39,7 +39,8 @@ def test_split_800_3200_240_full_size(oracle):
     print("case split 800/3200/240", flush=True)
     sd = synthetic.SyntheticSystem(nbf, naux, nbf5, seed)
     p = sd.p
-    out = {}
+    out, lag = {}, {}
+    cj, ck = oracle.pnof_selector(sd.n, p)
     for prec in ("fp64", "split"):
         dev = api.System(p, precision=prec)
         dev.generate_b(seed)
@@ -49,11 +50,11 @@ def test_split_800_3200_240_full_size(oracle):
         out[prec] = dev.get_jkh()
         if prec == "fp64":
             b = dev.download_b()
-            # the Lagrangian (general transform, utils.jl:1-59) at the same orbitals
-            cj, ck = oracle.pnof_selector(sd.n, p)
-            _, _, E_lag = api.compute_Lagrange2(sd.C, sd.n, sd.H, dev, cj, ck, p.nalpha,
-                                                p.nbeta, return_energy=True)
+        # the Lagrangian (general transform, utils.jl:1-59) at the same orbitals
+        lag[prec] = api.compute_Lagrange2(sd.C, sd.n, sd.H, dev, cj, ck, p.nalpha, p.nbeta,
+                                          return_energy=True)
         dev.close()
+    E_lag = lag["fp64"][2]
     J64, K64, H64 = out["fp64"]
     J16, K16, H16 = out["split"]
     # full-size properties of the FP64 build
@@ -82,6 +83,17 @@ def test_split_800_3200_240_full_size(oracle):
     print("split 800: |dJ|/|J| %.2e |dK|/|K| %.2e |dE| %.2e Ha"
           % (_rel(J16, J64), _rel(K16, K64), abs(E16 - E64)), flush=True)
     assert abs(E16 - E64) <= 1e-6
+    # the split Lagrangian against the FP64 one: elag to 5e-6 of its scale, the orbital
+    # gradient 4 (elag - elag^T) stated against the same scale, E_elec to 1e-6 Ha
+    el64, Hm64, El64 = lag["fp64"]
+    el16, Hm16, El16 = lag["split"]
+    scale = np.abs(el64).max()
+    e_grad = np.abs(4 * (el16 - el16.T) - 4 * (el64 - el64.T)).max() / scale
+    print("split 800 lagrangian: |d elag|/|elag| %.2e, |d grad|/|elag| %.2e, |dE| %.2e Ha"
+          % (_rel(el16, el64), e_grad, abs(El16 - El64)), flush=True)
+    assert _rel(el16, el64) <= 5e-6 and e_grad <= 4e-5
+    assert abs(El16 - El64) <= 1e-6
+    assert np.abs(Hm16 - Hm64).max() <= 1e-12 * max(1.0, np.abs(Hm64).max())
 
 
 def test_grouped_64_systems_full_size(oracle):

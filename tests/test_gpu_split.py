@@ -1,6 +1,7 @@
-"""Split-precision J/K-MO build (FP16 hi+lo operands, FP32 accumulation, FP64 J/K
-reduction) against the FP64 oracle.  Stated error bounds (relative to max|J|, max|K|):
-5e-6 for J and K; occupation energy within 1e-6 Ha of FP64 on the same orbitals."""
+"""Split-precision J/K-MO build and Lagrangian (FP16 hi+lo operands, FP32 accumulation,
+FP64 reductions) against the FP64 oracle.  Stated error bounds (relative to max|J|, max|K|,
+max|elag|): 5e-6 for J, K and the Lagrangian (and so for the orbital gradient
+4 (elag - elag^T)); energies within 1e-6 Ha of FP64 on the same orbitals."""
 import numpy as np
 import pytest
 
@@ -47,9 +48,40 @@ def test_split_generated_matches_fp64_device():
     J16, K16, H16 = api.computeJKH_MO(s.C, s.H, d16, p)
     assert np.abs(J16 - J64).max() / np.abs(J64).max() < 5e-6
     assert np.abs(K16 - K64).max() / np.abs(K64).max() < 5e-6
-    # the Lagrangian of a split system runs in FP64
-    from oracle import donof_oracle as orc
-    cj, ck = orc.pnof_selector(s.n, p)
-    el, _ = api.compute_Lagrange2(s.C, s.n, s.H, d16, cj, ck, p.nalpha, p.nbeta)
-    assert np.all(np.isfinite(el))
     d16.close()
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(64, 128, 20), (130, 300, 45), (37, 70, 10),
+                                           (200, 800, 60)])
+def test_split_lagrangian_against_oracle(oracle, nbf, naux, nbf5):
+    """The Lagrangian of a split system (utils.jl:1-59): both transforms on FP16 planes,
+    the C^J/C^K terms and the contraction in FP64; b is not resident in FP64."""
+    from donof_amd._lib import DN_ERR_STATE
+    print("case split lagrangian", nbf, naux, nbf5, flush=True)
+    s = synthetic.SyntheticSystem(nbf, naux, nbf5, 5 + nbf, ndoc=5)
+    p = s.p
+    b = synthetic.make_b(nbf, naux, s.seed)
+    dev = api.System(p, precision=DN_SPLIT_F16).upload_b(b)
+    with pytest.raises(api.DnError) as e:
+        dev.download_b()
+    assert e.value.code == DN_ERR_STATE
+    cj, ck = oracle.pnof_selector(s.n, p)
+    el, Hm, E = api.compute_Lagrange2(s.C, s.n, s.H, dev, cj, ck, p.nalpha, p.nbeta,
+                                      return_energy=True)
+    elr, Hmr = oracle.compute_lagrange2(s.C, s.n, s.H, b, cj, ck, p.nalpha, p.nbeta)
+    Er = oracle.computeE_elec_hmat(Hmr, s.n, elr, nbf5)
+    assert np.abs(Hm - Hmr).max() <= 1e-10 * max(1.0, np.abs(Hmr).max())   # FP64
+    scale = np.abs(elr).max()
+    e_lag = np.abs(el - elr).max() / scale
+    g, gr = 4 * (el - el.T), 4 * (elr - elr.T)
+    e_grad = np.abs(g - gr).max() / scale
+    print("split lagrangian rel err %.2e, gradient %.2e of max|elag|, |dE| %.2e"
+          % (e_lag, e_grad, abs(E - Er)), flush=True)
+    assert e_lag <= 5e-6 and e_grad <= 4e-5
+    assert abs(E - Er) <= 1e-6
+    # the J/K build afterwards (its own scales) is unaffected
+    J, K, Hc = api.computeJKH_MO(s.C, s.H, dev, p)
+    Jr, Kr, _ = oracle.jkh_mo(s.C, s.H, b, nbf5)
+    assert np.abs(J - Jr).max() <= 5e-6 * np.abs(Jr).max()
+    assert np.abs(K - Kr).max() <= 5e-6 * np.abs(Kr).max()
+    dev.close()

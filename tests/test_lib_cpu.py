@@ -27,3 +27,15 @@ def test_bindings_cover_header():
 def test_last_error_without_gpu():
     lib = _lib.load_library()
     assert isinstance(lib.dn_last_error(), bytes)
+
+
+def test_split_footprint_below_fp64():
+    """A split-precision system keeps b only as its FP16 hi/lo planes (half the bytes of
+    the FP64 b), so it needs less HBM than the FP64 system of the same sizes (at the
+    benchmark sizes; tiny systems are dominated by the 128-row padding of the planes)."""
+    from donof_amd import api, synthetic
+    for nbf, naux, nbf5 in ((400, 1600, 120), (800, 3200, 240), (200, 800, 60)):
+        p = synthetic.synthetic_param(nbf, naux, nbf5)
+        f64 = api.footprint(p, _lib.DN_FP64)
+        f16 = api.footprint(p, _lib.DN_SPLIT_F16)
+        assert f16 < f64, (nbf, naux, nbf5, f16, f64)
