@@ -116,7 +116,9 @@ class System:
         return self
 
     def ri_metric(self, G, I3c):
-        """b = I3c G^{-1/2} on the device (FP64 Cholesky route; integrals.jl:68-91)."""
+        """b = I3c G^{-1/2} on the device (integrals.jl:68-91): the FP64 Cholesky route
+        for a positive-definite G, else the reference's eigen route (eigenvalues below
+        zero contribute zero)."""
         G = _colmajor(G, (self.nbfaux, self.nbfaux), "G")
         I3c = _colmajor(I3c, (self.nbf, self.nbf, self.nbfaux), "I3c")
         check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))

@@ -313,6 +313,7 @@ int prepare_b(dn_sys* s, const double* b) {
 int set_b(dn_sys* s, const std::function<int(double*)>& fill) {
   s->b_ready = false;
   s->riw_set = false;  // a new b: its RI factor is set with it, if at all
+  s->eig_clamped = false;
   if (s->b) {
     DN_TRY(fill(s->b));
     DN_TRY(prepare_b(s, s->b));

@@ -241,6 +241,9 @@ int dn_grad_create(dn_sys* s, const double* C, const double* n, const double* cj
   if (!s->b)
     return fail(DN_ERR_STATE, "nuclear gradients need an FP64 system (a split-precision "
                               "system keeps b only as FP16 planes)");
+  if (!s->riw_set && s->eig_clamped)
+    return fail(DN_ERR_STATE, "the RI metric had eigenvalues below zero (set to zero in "
+                              "b): I3c G^{-1} cannot be formed from b");
   if (!s->riw_set)
     return fail(DN_ERR_STATE, "no RI factor for b: set b with dn_sys_ri_metric, or give "
                               "the factor of an uploaded b with dn_sys_set_ri_factor");

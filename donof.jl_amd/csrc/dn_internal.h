@@ -453,6 +453,9 @@ struct dn_sys {
   double* riw = nullptr;  // naux*naux
   bool riw_set = false;
   bool riw_transposed = false;  // riw holds Wt^T
+  // dn_sys_ri_metric took the eigen route (G not positive definite), with an eigenvalue
+  // below zero set to zero
+  bool eig_route = false, eig_clamped = false;
   bool b_sym = false;    // b[m,n,Q] == b[n,m,Q] (checked whenever b is set)
   dn::SymKDesc* symk_dev = nullptr;  // K/X pass of the symmetric build (in scal + 16;
                                      // the split-precision one at scal + 32)

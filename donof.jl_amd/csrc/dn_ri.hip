@@ -8,8 +8,17 @@
 // positive definite (a Coulomb metric is); the eigen route's clamp of negative
 // eigenvalues has no Cholesky counterpart, so such a G is rejected.
 //
-// Layout: G column-major naux x naux (lower triangle read), I3c column-major
-// (nbf, nbf, naux), i.e. an (nbf^2 x naux) matrix with the auxiliary index as columns.
+// A G that is not positive definite takes the reference's own route instead: G = V L V^T
+// (rocSOLVER syevd on the device, upper triangle as Symmetric(G) reads it), 1/sqrt(l)
+// for l >= 0 and 0 for l < 0, G^{-1/2} = V diag V^T and b = I3c G^{-1/2}: the
+// reference's b itself.
+//
+// Layout: G column-major naux x naux (lower triangle read by the Cholesky route),
+// I3c column-major (nbf, nbf, naux), i.e. an (nbf^2 x naux) matrix with the auxiliary
+// index as columns.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "dn_internal.h"
 
 namespace dn {
@@ -89,6 +98,15 @@ int copy_block(hipStream_t s, double* dst, long ldd, const double* src, long lds
                      lds, rows, cols);
   DN_HIP(hipGetLastError());
   return DN_OK;
+}
+
+// VS[:, i] = V[:, i] * (l_i < 0 ? 0 : 1 / sqrt(l_i)) (integrals.jl:76-82)
+__global__ void scale_evec_kernel(const double* __restrict__ V, const double* __restrict__ lam,
+                                  int n, double* __restrict__ VS) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)n * n) return;
+  const double l = lam[i / n];
+  VS[i] = l < 0.0 ? 0.0 : V[i] / sqrt(l);
 }
 
 // the n x n identity, column-major
@@ -175,6 +193,33 @@ extern "C" int dn_sys_set_ri_factor(dn_sys* s, const double* Wt) {
   return DN_OK;
 }
 
+namespace dn {
+namespace {
+
+// G (in dG, overwritten by V) = V diag(lam) V^T on the device; *bad = syevd's info
+int syevd(hipStream_t st, double* dG, int naux, double* lam, double* E, int* info, int* bad) {
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success)
+    return fail(DN_ERR_HIP, "rocblas_create_handle failed");
+  rocblas_status r = rocblas_set_stream(h, st);
+  if (r == rocblas_status_success)
+    r = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, naux, dG, naux, lam, E,
+                         info);
+  if (r == rocblas_status_success) {
+    const hipError_t e = hipMemcpyAsync(bad, info, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) (void)hipStreamSynchronize(st);
+    else r = rocblas_status_internal_error;
+  }
+  rocblas_destroy_handle(h);
+  if (r != rocblas_status_success)
+    return fail(DN_ERR_HIP, std::string("rocsolver_dsyevd: ") + rocblas_status_to_string(r));
+  if (*bad) return fail(DN_ERR_ARG, "RI metric: the eigensolver did not converge");
+  return DN_OK;
+}
+
+}  // namespace
+}  // namespace dn
+
 extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   DN_TRY(check_sys(s));
   if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
@@ -184,7 +229,8 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   const long rows = (long)s->ldb * s->d.nbf;
   const size_t nG = (size_t)naux * naux, nP = (size_t)naux * NB;
   const size_t nI = std::max((size_t)rows * naux, nG);  // I3c, then the identity
-  const size_t bytes = (nI + nG + 2 * nP) * sizeof(double) + 256;
+  // + the eigen route: V diag(s) and G^{-1/2} (naux^2 each), eigenvalues, syevd's E, info
+  const size_t bytes = (nI + 3 * nG + 2 * nP + 2 * (size_t)naux) * sizeof(double) + 512;
   DN_TRY(ctx_reserve(s->ctx, bytes));
   char* w = nullptr;
   {
@@ -198,8 +244,13 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   double* dG = dI + nI;
   double* Dinv = dG + nG;
   double* T = Dinv + nP;
-  int* flag = reinterpret_cast<int*>(T + nP);
-  int rc = DN_OK, host_flag = 0;
+  double* VS = T + nP;
+  double* Gmh = VS + nG;
+  double* lam = Gmh + nG;
+  double* E = lam + naux;
+  int* flag = reinterpret_cast<int*>(E + naux);
+  int* info = flag + 2;
+  int rc = DN_OK, host_flag = 0, host_info = 0;
   auto run = [&]() -> int {
     if (s->ldb != s->d.nbf) DN_HIP(hipMemsetAsync(dI, 0, nI * sizeof(double), s->stream));
     const size_t nbf = s->d.nbf;
@@ -210,8 +261,45 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
     DN_TRY(cholesky_f64(s->stream, dG, naux, Dinv, T, flag));
     DN_HIP(hipMemcpyAsync(&host_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     DN_HIP(hipStreamSynchronize(s->stream));
-    if (host_flag)
-      return fail(DN_ERR_ARG, "RI metric is not positive definite (Cholesky pivot <= 0)");
+    if (host_flag) {
+      // not positive definite: the reference's eigen route with its clamp
+      DN_TRY(h2d(s, dG, G, nG));
+      DN_TRY(syevd(s->stream, dG, naux, lam, E, info, &host_info));
+      std::vector<double> l(naux);
+      DN_TRY(d2h(s, l.data(), lam, naux));
+      DN_HIP(hipStreamSynchronize(s->stream));
+      bool clamped = false;
+      for (double x : l) clamped = clamped || x < 0.0;
+      hipLaunchKernelGGL(scale_evec_kernel, dim3((nG + 255) / 256), dim3(256), 0, s->stream, dG,
+                         lam, naux, VS);
+      DN_HIP(hipGetLastError());
+      Gemm gm;  // G^{-1/2} = (V diag(s)) V^T
+      gm.M = naux; gm.N = naux; gm.K = naux;
+      gm.A = VS; gm.lda = naux; gm.a_kcontig = false;
+      gm.B = dG; gm.ldb = naux; gm.b_kcontig = false;  // V^T(k, c) = V[c + naux k]
+      gm.C = Gmh; gm.ldc = naux;
+      DN_TRY(gemm_f64(s->stream, gm));
+      DN_TRY(set_b(s, [&](double* dst) {
+        Gemm gb;  // b = I3c G^{-1/2} (integrals.jl:86-88)
+        gb.M = (int)rows; gb.N = naux; gb.K = naux;
+        gb.A = dI; gb.lda = rows; gb.a_kcontig = false;
+        gb.B = Gmh; gb.ldb = naux; gb.b_kcontig = true;
+        gb.C = dst; gb.ldc = rows;
+        return gemm_f64(s->stream, gb);
+      }));
+      if (!clamped) {
+        // I3c G^{-1} = b G^{-1/2}: the factor of the nuclear gradients; with a clamped
+        // eigenvalue I3c is not recoverable from b and the gradients are refused
+        DN_HIP(hipMemcpyAsync(s->riw, Gmh, nG * sizeof(double), hipMemcpyDeviceToDevice,
+                              s->stream));
+      }
+      DN_HIP(hipStreamSynchronize(s->stream));
+      s->eig_route = true;
+      s->eig_clamped = clamped;
+      return DN_OK;
+    }
+    s->eig_route = false;
+    s->eig_clamped = false;
     DN_TRY(set_b(s, [&](double* dst) {
       return trsm_right_lt(s->stream, dI, dst, rows, naux, dG, Dinv);
     }));
@@ -227,6 +315,9 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   dev_free(s->ctx, s->stream, w);
   (void)hipStreamSynchronize(s->stream);
   ctx_unreserve(s->ctx, bytes);
-  if (rc == DN_OK) s->riw_set = s->riw_transposed = true;
+  if (rc == DN_OK) {
+    s->riw_set = !s->eig_clamped;
+    s->riw_transposed = !s->eig_route;  // L^{-T} (Cholesky) or G^{-1/2} itself
+  }
   return rc;
 }

@@ -91,8 +91,11 @@ int dn_sys_generate_b(dn_sys* sys, uint64_t seed);
  * (ext/DoNOFGPUExt.jl:9-33).  FP64 blocked Cholesky G = L L^T and b = I3c L^{-T}: b
  * equals the reference's eigen-route b up to an orthogonal rotation of the auxiliary
  * index, which leaves J, K, the Lagrangian and every energy unchanged.  A G that is
- * not positive definite returns DN_ERR_ARG.  Host arrays are not modified; a transient
- * device workspace of the size of b is taken for the call. */
+ * not positive definite takes the reference's eigen route (rocSOLVER syevd of the upper
+ * triangle, integrals.jl:74-83: 1/sqrt(l) for eigenvalues l >= 0, 0 below): b is then
+ * the reference's b itself; with an eigenvalue below zero no RI factor is kept for the
+ * nuclear gradients.  Host arrays are not modified; a transient device workspace of the
+ * size of b is taken for the call. */
 int dn_sys_ri_metric(dn_sys* sys, const double* G, const double* I3c);
 /* RI nuclear gradients: replaces compute_geom_gradients' RI branch (grads.jl:1-100).
  * dn_grad_create forms on the device, from the resident b at orbitals C (nbf x nbf),

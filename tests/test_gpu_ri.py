@@ -1,7 +1,8 @@
 """RI setup on the device (dn_sys_ri_metric; integrals.jl:68-91) against the oracle.
-b itself follows the Cholesky route (an orthogonal rotation of the reference's b in the
-auxiliary index), so parity is checked on what consumers see: sum_Q b b and J/K, to
-the FP64 bar relative to their scale, and b against the oracle's Cholesky route."""
+For a positive-definite metric b follows the Cholesky route (an orthogonal rotation of the
+reference's b in the auxiliary index), so parity is checked on what consumers see:
+sum_Q b b and J/K, to the FP64 bar relative to their scale, and b against the oracle's
+Cholesky route.  Otherwise the eigen route gives the reference's b itself."""
 import numpy as np
 import pytest
 
@@ -37,14 +38,35 @@ def test_ri_metric_parity(oracle, nbf, naux, nbf5):
     dev.close()
 
 
-def test_ri_metric_not_positive_definite():
-    nbf, naux = 6, 40
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_ri_metric_not_positive_definite_takes_the_eigen_route(oracle, precision):
+    """A metric with a negative eigenvalue: the reference's own route (integrals.jl:74-88,
+    eigenvalues below zero contribute zero), so b is the reference's b itself."""
+    from donof_amd._lib import DN_ERR_STATE
+    nbf, naux, nbf5 = 12, 70, 6
     G, I3c, _ = synthetic.make_metric(nbf, naux, 1)
     w, V = np.linalg.eigh(G)
     G = G - 2.0 * w[-1] * np.outer(V[:, -1], V[:, -1])     # flip the largest eigenvalue
-    dev = api.System(synthetic.synthetic_param(nbf, naux, 3))
-    with pytest.raises(DnError, match="positive definite"):
-        dev.ri_metric(G, I3c)
+    G = 0.5 * (G + G.T)
+    p = synthetic.synthetic_param(nbf, naux, nbf5)
+    dev = api.System(p, precision=precision).ri_metric(G, I3c)
+    br = oracle.iaux(G, I3c)
+    C = synthetic.make_C(nbf, 3)
+    H = synthetic.make_H(nbf, 4)
+    J, K, _ = api.JKH_MO(C, H, dev, nbf, nbf5, naux)
+    Jr, Kr, _ = oracle.jkh_mo(C, H, br, nbf5)
+    tol = 1e-10 if precision == "fp64" else 5e-6
+    assert np.abs(J - Jr).max() <= tol * np.abs(Jr).max()
+    assert np.abs(K - Kr).max() <= tol * np.abs(Kr).max()
+    if precision == "fp64":
+        bd = dev.download_b()
+        assert np.abs(bd - br).max() <= 1e-10 * np.abs(br).max()
+        # the clamped eigenvalue makes I3c unrecoverable from b: no nuclear gradients
+        n = np.full(nbf5, 0.5)
+        cj = ck = np.zeros((nbf5, nbf5))
+        with pytest.raises(DnError) as e:
+            api.Gradient(dev, n, C, cj, ck, np.zeros((nbf, nbf)))
+        assert e.value.code == DN_ERR_STATE
     dev.close()
 
 
