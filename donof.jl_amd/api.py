@@ -710,6 +710,23 @@ def calcocce(gamma, J_MO, K_MO, H_core, p, I):
     return E.value
 
 
+def occ_conjugate_gradient(gamma, J_MO, K_MO, H_core, p, I, g_abstol, maxiter=1000):
+    """optimize(calcocce, calcoccg, gamma, ConjugateGradient(), g_abstol) of occoptr
+    (minimization.jl:100-139) on the device: (gamma, minimum, iterations, converged).
+    J_MO, K_MO, H_core = None uses the resident ones."""
+    s = _require_system(I)
+    f = functional_of(p)
+    g = _vec(gamma, p.nv, "gamma")
+    J, K, Hc = _jkh_args(p, J_MO, K_MO, H_core)
+    out = np.empty(p.nv)
+    E = ct.c_double()
+    nit, conv = ct.c_int(), ct.c_int()
+    check(s.lib.dn_occoptr(s.handle, ct.byref(f), dptr(g), dptr(J), dptr(K), dptr(Hc),
+                           float(g_abstol), int(maxiter), dptr(out), ct.byref(E),
+                           ct.byref(nit), ct.byref(conv)))
+    return out, E.value, nit.value, bool(conv.value)
+
+
 def calcoccg(gamma, J_MO, K_MO, H_core, p, I):
     """pnof.jl:1324-1377."""
     s = _require_system(I)

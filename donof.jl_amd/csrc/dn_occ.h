@@ -28,6 +28,23 @@ struct OccWork {
   double* scal = nullptr;
 };
 
+// State of the occupation optimiser's nonlinear conjugate gradient on the device (the
+// iteration of minimization.jl:100-139's optimize(..., ConjugateGradient()) with the
+// Hager-Zhang direction and a Wolfe line search); every field is written by the
+// controller at the start of an evaluation slot.  Vectors (nv each, `vec`): x, g (at x),
+// d, the trial point xt, the gradient at xt, the gradient at the best zoom point.
+struct OccCG {
+  int state, done, converged, it, ls_i, zoom_i, maxiter, best_valid, nfev, pad;
+  double tol, f, f0, d0, a, a_prev, f_prev, d_prev, alo, flo, dlo, ahi, fhi, dhi;
+  double best_a, best_f, ft;
+};
+enum { kCgX = 0, kCgG = 1, kCgD = 2, kCgXT = 3, kCgGT = 4, kCgGB = 5, kCgVecs = 6 };
+// nslots evaluation slots (each: the controller's next trial point and the energy at it
+// in one block, then calcoccg's gradient at it); slots after convergence exit at once
+int occ_cg_slots(hipStream_t s, const OccDims& P, const OccWork& w, OccCG* c, double* vec,
+                 double* dn, const double* J, const double* K, const double* Hcore,
+                 int nslots);
+
 int occ_ocupacion(hipStream_t s, const OccDims& P, const double* gamma, double* n, double* dn);
 int occ_prepare(hipStream_t s, const OccDims& P, const OccWork& w);
 int occ_coefficients(hipStream_t s, const OccDims& P, const OccWork& w, double* cj12,

@@ -208,8 +208,8 @@ __device__ double block_sum(double v, double* red) {
 // ---- occupations ------------------------------------------------------------------------
 
 // pnof.jl:1122-1163 (Softmax) and 1047-1120 (Trigonometric); one block.
-__global__ void ocupacion_kernel(OccDims P, const double* __restrict__ gamma,
-                                 double* __restrict__ n, double* __restrict__ dn) {
+__device__ void ocupacion_block(const OccDims& P, const double* __restrict__ gamma,
+                                double* __restrict__ n, double* __restrict__ dn) {
   const int nbf5 = P.nbf5, nv = P.nv, ndoc = P.ndoc, ncwo = P.ncwo, no1 = P.no1;
   for (long e = threadIdx.x; e < (long)nbf5 * nv; e += blockDim.x) dn[e] = 0.0;
   for (int i = threadIdx.x; i < nbf5; i += blockDim.x) {
@@ -306,10 +306,15 @@ __global__ void ocupacion_kernel(OccDims P, const double* __restrict__ gamma,
   }
 }
 
+__global__ void ocupacion_kernel(OccDims P, const double* __restrict__ gamma,
+                                 double* __restrict__ n, double* __restrict__ dn) {
+  ocupacion_block(P, gamma, n, dn);
+}
+
 // ---- per-orbital quantities ---------------------------------------------------------------
 
 // one block: role, pair, fi, n_d, sqrt(n_d), dfi/dn, dn_d coefficients, S_F
-__global__ void orb_prep_kernel(OccDims P, OccWork w) {
+__device__ void orb_prep_block(const OccDims& P, const OccWork& w) {
   const int nbf5 = P.nbf5, no1 = P.no1, ndoc = P.ndoc;
   const double* n = w.n;
   const double c = (P.ipnof == 8 && P.ista == 5) ? 0.8 : 1.0;
@@ -390,6 +395,8 @@ __global__ void orb_prep_kernel(OccDims P, OccWork w) {
   }
 }
 
+__global__ void orb_prep_kernel(OccDims P, OccWork w) { orb_prep_block(P, w); }
+
 // cj12, ck12 with the diagonal zeroed (pnof.jl:49-50)
 __global__ void coef_kernel(OccDims P, OccWork w, double* __restrict__ cj12,
                             double* __restrict__ ck12) {
@@ -434,14 +441,13 @@ __global__ void der_coef_kernel(OccDims P, OccWork w, const double* __restrict__
 }
 
 // grad[k] of calcoccg (pnof.jl:1324-1377): one block per k.
-__global__ void calcoccg_kernel(OccDims P, OccWork w, const double* __restrict__ dn,
-                                const double* __restrict__ J, const double* __restrict__ K,
-                                const double* __restrict__ Hcore, double* __restrict__ grad) {
-  __shared__ double red[16];
+__device__ void calcoccg_block(const OccDims& P, const OccWork& w, const double* __restrict__ dn,
+                               const double* __restrict__ J, const double* __restrict__ K,
+                               const double* __restrict__ Hcore, double* __restrict__ grad,
+                               int k, double* red) {
   const Orb o = orb_view(w);
   const double S_F = fmax(w.scal[0], 1e-7);
   const int nbf5 = P.nbf5;
-  const int k = blockIdx.x;
   const double* dk = dn + (long)k * nbf5;
   double DSF = 0.0;
   if (P.ipnof == 4)
@@ -469,6 +475,13 @@ __global__ void calcoccg_kernel(OccDims P, OccWork w, const double* __restrict__
   if (threadIdx.x == 0) grad[k] = s;
 }
 
+__global__ void calcoccg_kernel(OccDims P, OccWork w, const double* __restrict__ dn,
+                                const double* __restrict__ J, const double* __restrict__ K,
+                                const double* __restrict__ Hcore, double* __restrict__ grad) {
+  __shared__ double red[16];
+  calcoccg_block(P, w, dn, J, K, Hcore, grad, blockIdx.x, red);
+}
+
 // calce (pnof.jl:1273-1300): one block.
 __global__ void calce_kernel(OccDims P, const double* __restrict__ n,
                              const double* __restrict__ cj12, const double* __restrict__ ck12,
@@ -494,7 +507,314 @@ __global__ void calce_kernel(OccDims P, const double* __restrict__ n,
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// ---- occupation optimiser: nonlinear CG on the device ------------------------------------
+//
+// The iteration of the host conjugate_gradient (Hager & Zhang, SIAM J. Optim. 16 (2005)
+// 170-192: eta-truncated beta; Wolfe line search with bracketing and a safeguarded cubic
+// zoom, Nocedal & Wright Alg. 3.5/3.6, accepting the approximate Wolfe conditions) as a
+// state machine advanced once per evaluation: every thread of the block holds the same
+// scalars (block reductions broadcast), so all take the same branches.
+
+namespace {
+
+constexpr int kCgThreads = 256;
+enum { kCgInit = 0, kCgAfterInit = 1, kCgLs = 2, kCgZoom = 3 };
+constexpr double kC1 = 0.1, kC2 = 0.9, kEpsWolfe = 1e-6, kAmax = 1e6, kEta = 0.4, kPsi0 = 0.01;
+constexpr int kLsMax = 40, kZoomMax = 40;
+
+__device__ double bsum_all(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < kCgThreads / 64; ++k) s += red[k];
+  return s;
+}
+
+__device__ double bmax_all(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int k = 1; k < kCgThreads / 64; ++k) s = fmax(s, red[k]);
+  return s;
+}
+
+struct CgVec {
+  double *x, *g, *d, *xt, *gt, *gb;
+  int nv;
+};
+
+__device__ double dot(const double* a, const double* b, int nv, double* red) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nv; i += kCgThreads) v += a[i] * b[i];
+  return bsum_all(v, red);
+}
+
+__device__ double amax(const double* a, int nv, double* red) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nv; i += kCgThreads) v = fmax(v, fabs(a[i]));
+  return bmax_all(v, red);
+}
+
+__device__ void set_trial(const CgVec& v, double a) {  // xt = x + a d
+  for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.xt[i] = v.x[i] + a * v.d[i];
+  __syncthreads();
+}
+
+__device__ bool approx_wolfe(const OccCG& c, double f, double d) {
+  return (2 * kC1 - 1) * c.d0 >= d && d >= kC2 * c.d0 && f <= c.f0 + kEpsWolfe * fabs(c.f0);
+}
+
+// a trial step inside the bracket: cubic interpolation safeguarded to its interior
+__device__ double zoom_trial(const OccCG& c) {
+  double a = 0.0;
+  bool have = false;
+  if (isfinite(c.fhi) && isfinite(c.dhi) && c.ahi != c.alo) {
+    const double d1 = c.dlo + c.dhi - 3 * (c.flo - c.fhi) / (c.alo - c.ahi);
+    const double rad = d1 * d1 - c.dlo * c.dhi;
+    if (rad >= 0) {
+      const double d2 = copysign(sqrt(rad), c.ahi - c.alo);
+      const double den = c.dhi - c.dlo + 2 * d2;
+      if (den != 0) {
+        a = c.ahi - (c.ahi - c.alo) * (c.dhi + d2 - d1) / den;
+        have = true;
+      }
+    }
+  }
+  const double lo = fmin(c.alo, c.ahi), hi = fmax(c.alo, c.ahi);
+  if (!have || !(lo + 0.1 * (hi - lo) <= a && a <= hi - 0.1 * (hi - lo))) a = 0.5 * (c.alo + c.ahi);
+  return a;
+}
+
+// start the line search of an iteration from step a (xt set)
+__device__ void begin_iteration(OccCG& c, const CgVec& v, double* red) {
+  c.d0 = dot(v.g, v.d, v.nv, red);
+  if (c.d0 >= 0) {
+    for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.d[i] = -v.g[i];
+    __syncthreads();
+    c.d0 = dot(v.g, v.d, v.nv, red);
+  }
+  c.f0 = c.f;
+  c.a_prev = 0.0;
+  c.f_prev = c.f0;
+  c.d_prev = c.d0;
+  c.ls_i = 0;
+  c.state = kCgLs;
+  set_trial(v, c.a);
+}
+
+__device__ void start_zoom(OccCG& c, const CgVec& v, double alo, double flo, double dlo,
+                           double ahi, double fhi, double dhi) {
+  c.alo = alo; c.flo = flo; c.dlo = dlo;
+  c.ahi = ahi; c.fhi = fhi; c.dhi = dhi;
+  c.zoom_i = 0;
+  c.best_valid = 0;
+  c.a = zoom_trial(c);
+  c.state = kCgZoom;
+  set_trial(v, c.a);
+}
+
+// the line search returned (a, fn, gn): the CG step (x += a d), convergence, the
+// Hager-Zhang direction, the next iteration
+__device__ void accept(OccCG& c, const CgVec& v, double a, double fn, const double* gn,
+                       double* red) {
+  if (!(isfinite(fn) && fn <= c.f + 1e-6 * fabs(c.f)) || a == 0.0) {
+    c.done = 1;  // no acceptable step: stagnation at the round-off floor
+    return;
+  }
+  for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.x[i] += a * v.d[i];
+  __syncthreads();
+  if (amax(gn, v.nv, red) <= c.tol) {
+    c.f = fn;
+    for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.g[i] = gn[i];
+    __syncthreads();
+    c.converged = 1;
+    c.done = 1;
+    return;
+  }
+  // y = gn - g (in xt's buffer: the trial point is not needed again)
+  double* y = v.xt;
+  for (int i = threadIdx.x; i < v.nv; i += kCgThreads) y[i] = gn[i] - v.g[i];
+  __syncthreads();
+  const double dy = dot(v.d, y, v.nv, red);
+  double beta = 0.0;
+  if (dy != 0.0 && isfinite(dy)) {
+    const double yy = dot(y, y, v.nv, red);
+    double t = 0.0;
+    for (int i = threadIdx.x; i < v.nv; i += kCgThreads)
+      t += (y[i] - 2.0 * v.d[i] * yy / dy) * gn[i];
+    beta = bsum_all(t, red) / dy;
+    const double dn = sqrt(dot(v.d, v.d, v.nv, red));
+    const double gnorm = sqrt(dot(v.g, v.g, v.nv, red));
+    beta = fmax(beta, -1.0 / (dn * fmin(kEta, gnorm)));
+  }
+  for (int i = threadIdx.x; i < v.nv; i += kCgThreads) {
+    v.d[i] = -gn[i] + beta * v.d[i];
+    v.g[i] = gn[i];
+  }
+  __syncthreads();
+  c.f = fn;
+  if (c.it >= c.maxiter) {
+    c.done = 1;
+    return;
+  }
+  c.it += 1;
+  c.a = a;  // the next search starts from the accepted step
+  begin_iteration(c, v, red);
+}
+
+// one controller step: consume the evaluation at xt (c.ft, gt), set the next xt
+__device__ void cg_control(OccCG& c, const CgVec& v, double* red) {
+  switch (c.state) {
+    case kCgInit:
+      for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.xt[i] = v.x[i];
+      __syncthreads();
+      c.state = kCgAfterInit;
+      return;
+    case kCgAfterInit: {
+      c.f = c.ft;
+      for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.g[i] = v.gt[i];
+      __syncthreads();
+      const double gmax = amax(v.g, v.nv, red);
+      c.it = 0;
+      if (gmax <= c.tol) {
+        c.converged = 1;
+        c.done = 1;
+        return;
+      }
+      for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.d[i] = -v.g[i];
+      __syncthreads();
+      const double xn = amax(v.x, v.nv, red);
+      c.a = xn > 0 ? kPsi0 * xn / gmax
+                   : kPsi0 * fmax(fabs(c.f), 1.0) / dot(v.g, v.g, v.nv, red);
+      c.it = 1;
+      begin_iteration(c, v, red);
+      return;
+    }
+    case kCgLs: {
+      const double f = c.ft, d = dot(v.gt, v.d, v.nv, red);
+      if (isfinite(f) && approx_wolfe(c, f, d)) return accept(c, v, c.a, f, v.gt, red);
+      if (!isfinite(f) || f > c.f0 + kC1 * c.a * c.d0 || (c.ls_i > 0 && f >= c.f_prev))
+        return start_zoom(c, v, c.a_prev, c.f_prev, c.d_prev, c.a, f, d);
+      if (fabs(d) <= -kC2 * c.d0) return accept(c, v, c.a, f, v.gt, red);
+      if (d >= 0) return start_zoom(c, v, c.a, f, d, c.a_prev, c.f_prev, c.d_prev);
+      c.a_prev = c.a;
+      c.f_prev = f;
+      c.d_prev = d;
+      c.ls_i += 1;
+      if (c.ls_i >= kLsMax) return accept(c, v, c.a_prev, f, v.gt, red);
+      c.a = fmin(2.0 * c.a, kAmax);
+      set_trial(v, c.a);
+      return;
+    }
+    case kCgZoom: {
+      const double f = c.ft, d = dot(v.gt, v.d, v.nv, red);
+      if (!c.best_valid || f < c.best_f) {
+        c.best_valid = 1;
+        c.best_a = c.a;
+        c.best_f = f;
+        for (int i = threadIdx.x; i < v.nv; i += kCgThreads) v.gb[i] = v.gt[i];
+        __syncthreads();
+      }
+      if (isfinite(f) && approx_wolfe(c, f, d)) return accept(c, v, c.a, f, v.gt, red);
+      if (!isfinite(f) || f > c.f0 + kC1 * c.a * c.d0 || f >= c.flo) {
+        c.ahi = c.a; c.fhi = f; c.dhi = d;
+      } else {
+        if (fabs(d) <= -kC2 * c.d0) return accept(c, v, c.a, f, v.gt, red);
+        if (d * (c.ahi - c.alo) >= 0) {
+          c.ahi = c.alo; c.fhi = c.flo; c.dhi = c.dlo;
+        }
+        c.alo = c.a; c.flo = f; c.dlo = d;
+      }
+      c.zoom_i += 1;
+      if (fabs(c.ahi - c.alo) < 1e-16 * fmax(1.0, fabs(c.alo)) || c.zoom_i >= kZoomMax)
+        return accept(c, v, c.best_a, c.best_f, v.gb, red);
+      c.a = zoom_trial(c);
+      set_trial(v, c.a);
+      return;
+    }
+    default:
+      c.done = 1;
+  }
+}
+
+}  // namespace
+
+// slot part 1 (one block): the controller, then n, dn, the per-orbital quantities and the
+// occupation energy calce at xt with cj12/ck12 built in place (pnof.jl:1273-1300)
+__global__ __launch_bounds__(kCgThreads) void occ_cg_eval_kernel(
+    OccDims P, OccWork w, OccCG* __restrict__ gc, double* __restrict__ vec,
+    double* __restrict__ dn, const double* __restrict__ J, const double* __restrict__ K,
+    const double* __restrict__ Hcore) {
+  __shared__ double red[kCgThreads / 64];
+  OccCG c = *gc;  // every thread: the same copy
+  if (c.done) return;
+  CgVec v{vec + kCgX * P.nv, vec + kCgG * P.nv, vec + kCgD * P.nv,
+          vec + kCgXT * P.nv, vec + kCgGT * P.nv, vec + kCgGB * P.nv, P.nv};
+  cg_control(c, v, red);
+  if (!c.done) {
+    ocupacion_block(P, v.xt, w.n, dn);
+    __syncthreads();
+    orb_prep_block(P, w);
+    __syncthreads();
+    const Orb o = orb_view(w);
+    const double S_F = w.scal[0];
+    const int nbf5 = P.nbf5;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nbf5; i += kCgThreads) {
+      const double jii = J[i + (long)i * nbf5];
+      if (i < P.nbeta) acc += w.n[i] * (2 * Hcore[i] + jii);
+      else if (i < P.nalpha) {
+        if (P.nsoc > 0) acc += w.n[i] * 2 * Hcore[i];
+      } else acc += w.n[i] * (2 * Hcore[i] + jii);
+    }
+    const long nn = (long)nbf5 * nbf5;
+    for (long e = threadIdx.x; e < nn; e += kCgThreads) {
+      const int i = e % nbf5, j = e / nbf5;
+      if (i == j) continue;
+      double cj, ck;
+      coef_elem(P, o, S_F, i, j, &cj, &ck);
+      const long ji = j + (long)i * nbf5;
+      acc += cj * J[ji] - ck * K[ji];
+    }
+    c.ft = bsum_all(acc, red);
+    c.nfev += 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *gc = c;
+}
+
+// slot part 2 (one block per gamma index): calcoccg at xt (n, dn and the per-orbital
+// quantities of part 1)
+__global__ void occ_cg_grad_kernel(OccDims P, OccWork w, const OccCG* __restrict__ gc,
+                                   double* __restrict__ vec, const double* __restrict__ dn,
+                                   const double* __restrict__ J, const double* __restrict__ K,
+                                   const double* __restrict__ Hcore) {
+  __shared__ double red[16];
+  if (gc->done) return;
+  calcoccg_block(P, w, dn, J, K, Hcore, vec + kCgGT * P.nv, blockIdx.x, red);
+}
+
 // ---- launchers ------------------------------------------------------------------------
+
+int occ_cg_slots(hipStream_t s, const OccDims& P, const OccWork& w, OccCG* c, double* vec,
+                 double* dn, const double* J, const double* K, const double* Hcore,
+                 int nslots) {
+  if (P.nv <= 0) return fail(DN_ERR_ARG, "occupation optimiser: no gamma variables");
+  for (int i = 0; i < nslots; ++i) {
+    hipLaunchKernelGGL(occ_cg_eval_kernel, dim3(1), dim3(kCgThreads), 0, s, P, w, c, vec, dn,
+                       J, K, Hcore);
+    hipLaunchKernelGGL(occ_cg_grad_kernel, dim3(P.nv), dim3(256), 0, s, P, w, c, vec, dn, J,
+                       K, Hcore);
+  }
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
 
 int occ_ocupacion(hipStream_t s, const OccDims& P, const double* gamma, double* n, double* dn) {
   hipLaunchKernelGGL(ocupacion_kernel, dim3(1), dim3(256), 0, s, P, gamma, n, dn);

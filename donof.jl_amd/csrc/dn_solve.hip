@@ -322,6 +322,57 @@ int dn_calcocce(dn_sys* s, const dn_functional* f, const double* gamma, const do
   return DN_OK;
 }
 
+int dn_occoptr(dn_sys* s, const dn_functional* f, const double* gamma, const double* J,
+               const double* K, const double* Hcore, double threshen, int maxiter,
+               double* gamma_out, double* E, int* nit, int* converged) {
+  DN_TRY(check_sys(s));
+  OccDims P;
+  DN_TRY(functional_of(s, f, &P));
+  if (!gamma || !gamma_out || !E || !nit || !converged) return fail(DN_ERR_ARG, "null argument");
+  if (!(threshen > 0.0) || maxiter < 1) return fail(DN_ERR_ARG, "need threshen > 0, maxiter >= 1");
+  const int nv = s->d.nv;
+  if (nv <= 0) return fail(DN_ERR_ARG, "no occupation variables (ndoc = 0)");
+  DN_TRY(upload_jkh(s, J, K, Hcore));
+  const size_t bytes = sizeof(OccCG) + kCgVecs * (size_t)nv * sizeof(double) + 256;
+  char* mem = nullptr;
+  DN_TRY(dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&mem)));
+  OccCG* c = reinterpret_cast<OccCG*>(mem);
+  double* vec = reinterpret_cast<double*>(mem + (sizeof(OccCG) + 255) / 256 * 256);
+  OccCG h{};
+  h.state = 0;  // first slot: evaluate at gamma
+  h.tol = threshen;
+  h.maxiter = maxiter;
+  auto run = [&]() -> int {
+    DN_HIP(hipMemcpyAsync(c, &h, sizeof(h), hipMemcpyHostToDevice, s->stream));
+    DN_TRY(h2d(s, vec + kCgX * (size_t)nv, gamma, nv));
+    const OccWork w = work_of(s);
+    // evaluation slots in batches; the done flag is read once per batch (a solve of
+    // ordinary length fits in the first)
+    const int batch = 64;
+    for (long evals = 0;; evals += batch) {
+      DN_TRY(occ_cg_slots(s->stream, P, w, c, vec, s->dn, s->J, s->K, s->Hcore, batch));
+      DN_HIP(hipMemcpyAsync(&h, c, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+      DN_HIP(hipStreamSynchronize(s->stream));
+      if (h.done) break;
+      if (evals > 64L * maxiter + 4096)
+        return fail(DN_ERR_STATE, "occupation optimiser: evaluation limit reached");
+    }
+    DN_TRY(d2h(s, gamma_out, vec + kCgX * (size_t)nv, nv));
+    DN_HIP(hipStreamSynchronize(s->stream));
+    return DN_OK;
+  };
+  const int rc = run();
+  dev_free(s->ctx, s->stream, mem);
+  (void)hipStreamSynchronize(s->stream);
+  if (rc != DN_OK) return rc;
+  *E = h.f;
+  *nit = h.it;
+  *converged = h.converged;
+  if (!std::isfinite(h.f) || !all_finite(gamma_out, nv))
+    return fail(DN_ERR_NONFINITE, "non-finite occupation optimiser result");
+  return DN_OK;
+}
+
 int dn_calcoccg(dn_sys* s, const dn_functional* f, const double* gamma, const double* J,
                 const double* K, const double* Hcore, double* grad) {
   DN_TRY(check_sys(s));

@@ -107,7 +107,7 @@ def _line_search(phi, f0, d0, a1, c1=0.1, c2=0.9, amax=1e6, maxiter=40):
     """Strong-Wolfe line search with bracketing and cubic/bisection zoom (Nocedal &
     Wright, Numerical Optimization, Alg. 3.5/3.6), also accepting the approximate Wolfe
     conditions of Hager & Zhang.  phi(a) -> (f, g, slope)."""
-    a_prev, f_prev, d_prev = 0.0, f0, d0
+    a_prev, f_prev, d_prev, g_prev = 0.0, f0, d0, None
     a = a1
     for i in range(maxiter):
         f, g, d = phi(a)
@@ -119,9 +119,9 @@ def _line_search(phi, f0, d0, a1, c1=0.1, c2=0.9, amax=1e6, maxiter=40):
             return a, f, g
         if d >= 0:
             return _zoom(phi, f0, d0, a, f, d, a_prev, f_prev, d_prev, c1, c2)
-        a_prev, f_prev, d_prev = a, f, d
+        a_prev, f_prev, d_prev, g_prev = a, f, d, g
         a = min(2.0 * a, amax)
-    return a, f, g
+    return a_prev, f_prev, g_prev  # the last point evaluated
 
 
 def _zoom(phi, f0, d0, alo, flo, dlo, ahi, fhi, dhi, c1, c2, maxiter=40):
@@ -208,16 +208,43 @@ def conjugate_gradient(fun, grad, x0, g_abstol=1e-8, maxiter=1000, eta=0.4, psi0
 # ---- minimization.jl:100-139 -----------------------------------------------------------
 
 
+def occoptr_host_cg(gamma, p, I, g_abstol=None):
+    """The same CG driven from the host (one device call per energy or gradient): the
+    reference iteration the device optimiser is checked against.  A trial point whose
+    energy or gradient is not finite reads as inf / nan, as the device CG treats it."""
+    from ._lib import DN_ERR_NONFINITE, DnError
+
+    def fun(g):
+        try:
+            return api.calcocce(g, None, None, None, p, I)
+        except DnError as e:
+            if e.code != DN_ERR_NONFINITE:
+                raise
+            return math.inf
+
+    def grad(g):
+        try:
+            return api.calcoccg(g, None, None, None, p, I)
+        except DnError as e:
+            if e.code != DN_ERR_NONFINITE:
+                raise
+            return np.full(p.nv, math.nan)
+
+    return conjugate_gradient(fun, grad, gamma,
+                              g_abstol=p.threshen if g_abstol is None else g_abstol)
+
+
 def occoptr(gamma, C, H, I, freeze_occ, p):
     """Occupation optimisation with J/K/H_core frozen at C (minimization.jl:100-139).
-    One J/K build on the device, then nonlinear CG over gamma whose energy and gradient
-    (calcocce/calcoccg) run on the device against the resident J/K/H_core."""
+    One J/K build on the device, then nonlinear CG over gamma on the device (dn_occoptr:
+    iterate, line search and every calcocce/calcoccg evaluation stay in HBM) against the
+    resident J/K/H_core."""
     res = None
     if p.ndoc > 0 and not freeze_occ:
         api.computeJKH_MO(C, H, I, p)
-        res = conjugate_gradient(lambda g: api.calcocce(g, None, None, None, p, I),
-                                 lambda g: api.calcoccg(g, None, None, None, p, I),
-                                 gamma, g_abstol=p.threshen)
+        x, fmin, nit, conv = api.occ_conjugate_gradient(gamma, None, None, None, p, I,
+                                                         p.threshen)
+        res = CGResult(x, fmin, nit, conv)
         gamma = res.x
         if res.iterations < 1:
             p.maxloop = p.maxloop + 10

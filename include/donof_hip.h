@@ -200,6 +200,16 @@ int dn_calcocce(dn_sys* sys, const dn_functional* f, const double* gamma, const 
                 const double* K, const double* Hcore, double* E);
 int dn_calcoccg(dn_sys* sys, const dn_functional* f, const double* gamma, const double* J,
                 const double* K, const double* Hcore, double* grad);
+/* The occupation optimiser of occoptr (minimization.jl:100-139): optimize(calcocce,
+ * calcoccg, gamma, ConjugateGradient(), Optim.Options(g_abstol = threshen)) run on the
+ * device with J, K, H_core (NULL = resident, e.g. from dn_jkh_mo_resident): nonlinear CG
+ * with the Hager-Zhang direction and a Wolfe line search whose state stays in HBM; each
+ * evaluation is calcocce + calcoccg at a trial gamma, the host reads one flag per 64
+ * evaluations.  gamma_out (nv), E (the minimum), nit (CG iterations, res.iterations) and
+ * converged (res.stopped_by.g_converged) are returned. */
+int dn_occoptr(dn_sys* sys, const dn_functional* f, const double* gamma, const double* J,
+               const double* K, const double* Hcore, double threshen, int maxiter,
+               double* gamma_out, double* E, int* nit, int* converged);
 
 /* ---- orbital rotation and optimiser ------------------------------------------------ */
 

@@ -1,0 +1,59 @@
+"""The occupation optimiser on the device (dn_occoptr; minimization.jl:100-139): the
+nonlinear CG with every evaluation, the line search and the iterate in HBM, against the
+same CG driven from the host (one device calcocce/calcoccg call per evaluation) and the
+oracle's occoptr: the same stationary point (max|calcoccg| <= threshen) and energy to
+1e-9 Ha.  The iterates need not agree bitwise: the host sums its dot products in another
+order, which can move a line-search decision."""
+import numpy as np
+import pytest
+
+from donof_amd import api, minimization as mini, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(nbf, naux, nbf5, ipnof, occ, seed):
+    s = synthetic.SyntheticSystem(nbf, naux, nbf5, seed, ndoc=5 if nbf5 % 5 == 0 else 3)
+    p = s.p
+    p.ipnof, p.occ_method = ipnof, occ
+    dev = api.System(p).generate_b(seed)
+    api.computeJKH_MO(s.C, s.H, dev, p)          # J, K, H_core resident
+    g0 = (mini.guess_gamma_softmax(p.ndoc, p.ncwo) if occ == "Softmax"
+          else mini.guess_gamma_trigonometric(p.ndoc, p.ncwo))
+    return s, p, dev, g0
+
+
+@pytest.mark.parametrize("ipnof", [7, 8])
+@pytest.mark.parametrize("occ", ["Softmax", "Trigonometric"])
+@pytest.mark.parametrize("nbf,naux,nbf5", [(40, 160, 15), (120, 480, 40)])
+def test_device_cg_matches_host_cg(oracle, nbf, naux, nbf5, ipnof, occ):
+    print("case occoptr", nbf, naux, nbf5, ipnof, occ, flush=True)
+    s, p, dev, g0 = _setup(nbf, naux, nbf5, ipnof, occ, 11 + nbf)
+    tol = 1e-7
+    x, E, nit, conv = api.occ_conjugate_gradient(g0, None, None, None, p, dev, tol)
+    host = mini.occoptr_host_cg(g0, p, dev, g_abstol=tol)
+    assert conv and host.g_converged and nit >= 1
+    g = api.calcoccg(x, None, None, None, p, dev)
+    assert np.max(np.abs(g)) <= tol
+    assert E == api.calcocce(x, None, None, None, p, dev)     # the minimum is E(gamma)
+    assert abs(E - host.minimum) <= 1e-9
+    # and the oracle's energy at the device's gamma
+    J, K, Hc = oracle.jkh_mo(s.C, s.H, dev.download_b(), nbf5)
+    assert abs(oracle.calcocce(x, J, K, Hc, p) - E) <= 1e-9
+    dev.close()
+
+
+def test_occoptr_uses_the_device_cg(oracle):
+    """occoptr at 400/1600/120 (GNOF, Softmax, threshen = 1e-4): the returned gamma is a
+    stationary point and cj12/ck12 are those of its occupations."""
+    s, p, dev, g0 = _setup(400, 1600, 120, 8, "Softmax", 2024)
+    E, nit, conv, gamma, n, cj12, ck12 = mini.occoptr(g0, s.C, s.H, dev, False, p)
+    assert conv and nit >= 1
+    assert np.max(np.abs(api.calcoccg(gamma, None, None, None, p, dev))) <= p.threshen
+    nr, _ = oracle.ocupacion(gamma, p)
+    assert np.abs(n - nr).max() <= 1e-14
+    cjr, ckr = oracle.pnof_selector(nr, p)
+    assert np.abs(cj12 - cjr).max() <= 1e-13 and np.abs(ck12 - ckr).max() <= 1e-13
+    host = mini.occoptr_host_cg(g0, p, dev)
+    assert abs(E - host.minimum) <= 1e-9
+    dev.close()
