@@ -85,6 +85,7 @@ _SIGS = {
     "dn_calcocce": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, _D]),
     "dn_occoptr": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, ct.c_double, _I, _D,
                         _D, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+    "dn_order_occupations_softmax": (_I, [_P, _D, _D, _D, _D]),
     "dn_calcoccg": (_I, [_P, ct.POINTER(dn_functional), _D, _D, _D, _D, _D]),
     "dn_rotate_orbital": (_I, [_P, _D, _D, _D]),
     "dn_orbopt": (_I, [_P, ct.POINTER(dn_functional), ct.POINTER(dn_orbopt_state), _D, _D, _D]),

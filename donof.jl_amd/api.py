@@ -727,6 +727,17 @@ def occ_conjugate_gradient(gamma, J_MO, K_MO, H_core, p, I, g_abstol, maxiter=10
     return out, E.value, nit.value, bool(conv.value)
 
 
+def order_occupations_softmax(old_C, old_gamma, p, I):
+    """utils.jl:586-616 on the device: (C, gamma) with each pair sorted by gamma."""
+    s = _require_system(I)
+    C = _colmajor(old_C, (s.nbf, s.nbf), "C")
+    g = _vec(old_gamma, p.nv, "gamma")
+    Cout = np.empty((s.nbf, s.nbf), order="F")
+    gout = np.empty(p.nv)
+    check(s.lib.dn_order_occupations_softmax(s.handle, dptr(C), dptr(g), dptr(Cout), dptr(gout)))
+    return Cout, gout
+
+
 def calcoccg(gamma, J_MO, K_MO, H_core, p, I):
     """pnof.jl:1324-1377."""
     s = _require_system(I)

@@ -373,6 +373,105 @@ int dn_occoptr(dn_sys* s, const dn_functional* f, const double* gamma, const dou
   return DN_OK;
 }
 
+namespace {
+
+// order_occupations_softmax (utils.jl:586-616), one block: when some gamma > 0, each
+// pair's strong orbital and its ncwo weak orbitals are sorted by gamma (0 for the strong
+// one) in the reverse of a stable ascending order, the gammas shifted by their maximum;
+// perm[c] = the old column that becomes column c (identity elsewhere).  gt / idx: ndoc
+// rows of ncwo + 1 (scratch).
+__global__ void order_softmax_kernel(int nbf5, int no1, int ndoc, int ndns, int ncwo, int nv,
+                                     const double* __restrict__ gamma,
+                                     double* __restrict__ gamma_out, int* __restrict__ perm,
+                                     double* __restrict__ gt, int* __restrict__ idx) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+    gamma_out[k] = gamma[k];
+    if (gamma[k] > 0.0) atomicOr(&any, 1);
+  }
+  for (int c = threadIdx.x; c < nbf5; c += blockDim.x) perm[c] = c;
+  __syncthreads();
+  if (!any) return;
+  const int m = ncwo + 1;
+  for (int i = 1 + threadIdx.x; i <= ndoc; i += blockDim.x) {
+    double* g = gt + (long)(i - 1) * m;
+    int* id = idx + (long)(i - 1) * m;
+    const int ll = no1 + ndns + (ndoc - i);  // first weak column (0-based)
+    const int gx = ndoc - i;                 // first gamma of the pair
+    g[0] = 0.0;
+    for (int j = 0; j < ncwo; ++j) g[1 + j] = gamma[gx + ndoc * j];
+    // stable ascending argsort (insertion: equal keys keep their order)
+    for (int t = 0; t < m; ++t) id[t] = t;
+    for (int t = 1; t < m; ++t) {
+      const int v = id[t];
+      int u = t;
+      while (u > 0 && g[id[u - 1]] > g[v]) {
+        id[u] = id[u - 1];
+        --u;
+      }
+      id[u] = v;
+    }
+    double gmax = g[0];
+    for (int t = 1; t < m; ++t) gmax = fmax(gmax, g[t]);
+    // sort_idx = reverse; column t of the pair takes old column sort_idx[t]
+    for (int t = 0; t < m; ++t) {
+      const int src = id[m - 1 - t];
+      const int dst_col = t == 0 ? no1 + i - 1 : ll + ndoc * (t - 1);
+      const int src_col = src == 0 ? no1 + i - 1 : ll + ndoc * (src - 1);
+      perm[dst_col] = src_col;
+      if (t > 0) gamma_out[gx + ndoc * (t - 1)] = g[src] - gmax;
+    }
+  }
+}
+
+// Cout[:, c] = C[:, perm[c]] for c < nbf5, Cout[:, c] = C[:, c] beyond
+__global__ void permute_cols_kernel(const double* __restrict__ C, const int* __restrict__ perm,
+                                    int nbf, int nbf5, double* __restrict__ Cout) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)nbf * nbf) return;
+  const int r = (int)(e % nbf), c = (int)(e / nbf);
+  const int src = c < nbf5 ? perm[c] : c;
+  Cout[e] = C[r + (long)nbf * src];
+}
+
+}  // namespace
+
+int dn_order_occupations_softmax(dn_sys* s, const double* C, const double* gamma,
+                                 double* C_out, double* gamma_out) {
+  DN_TRY(check_sys(s));
+  if (!C || !gamma || !C_out || !gamma_out) return fail(DN_ERR_ARG, "null argument");
+  const dn_dims& d = s->d;
+  if (d.ndoc <= 0 || d.nv <= 0) return fail(DN_ERR_ARG, "no pairs to order (ndoc = 0)");
+  const size_t nn = (size_t)d.nbf * d.nbf, m = (size_t)d.ncwo + 1;
+  const size_t bytes = (d.ndoc * m + d.nv) * sizeof(double) + (d.ndoc * m + d.nbf5) * sizeof(int) + 64;
+  char* w = nullptr;
+  DN_TRY(dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&w)));
+  double* gt = reinterpret_cast<double*>(w);
+  double* gout = gt + d.ndoc * m;
+  int* idx = reinterpret_cast<int*>(gout + d.nv);
+  int* perm = idx + d.ndoc * m;
+  auto run = [&]() -> int {
+    DN_TRY(h2d(s, s->Ca, C, nn));
+    DN_TRY(h2d(s, s->gamma, gamma, d.nv));
+    hipLaunchKernelGGL(order_softmax_kernel, dim3(1), dim3(256), 0, s->stream, d.nbf5, d.no1,
+                       d.ndoc, d.ndoc + d.nsoc, d.ncwo, d.nv, s->gamma, gout, perm, gt, idx);
+    DN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(permute_cols_kernel, dim3((nn + 255) / 256), dim3(256), 0, s->stream,
+                       s->Ca, perm, d.nbf, d.nbf5, s->bestC);
+    DN_HIP(hipGetLastError());
+    DN_TRY(d2h(s, C_out, s->bestC, nn));
+    DN_TRY(d2h(s, gamma_out, gout, d.nv));
+    DN_HIP(hipStreamSynchronize(s->stream));
+    return DN_OK;
+  };
+  const int rc = run();
+  dev_free(s->ctx, s->stream, w);
+  (void)hipStreamSynchronize(s->stream);
+  return rc;
+}
+
 int dn_calcoccg(dn_sys* s, const dn_functional* f, const double* gamma, const double* J,
                 const double* K, const double* Hcore, double* grad) {
   DN_TRY(check_sys(s));

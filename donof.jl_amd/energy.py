@@ -151,7 +151,7 @@ def energy(I, H, C, p, E_nuc=0.0, n=None, freeze_occ=False, printmode=True, chec
         t2 = time.time()
         _, nit_occ, _, gamma, n, cj12, ck12 = mini.occoptr(gamma, C, H, I, freeze_occ, p)
         if p.occ_method == "Softmax":
-            C, gamma = mini.order_occupations_softmax(C, gamma, p)
+            C, gamma = mini.order_occupations_softmax(C, gamma, p, I)
         t3 = time.time()
         E = E_orb
         E_diff = E - E_old

@@ -2,9 +2,10 @@
 242-630), plus the host bookkeeping the reference runs between them (gamma guesses,
 utils.jl:539-616 / guess.jl:184-213).
 
-Every energy, gradient, Lagrangian and rotation is evaluated by libdonofhip on the
-device; the host keeps the O(nv) optimiser state of the conjugate-gradient loop and the
-column bookkeeping of order_occupations_softmax.
+Every energy, gradient, Lagrangian, rotation, the occupation optimiser's CG iteration and
+the re-ordering of the pairs run in libdonofhip on the device; the host-driven CG
+(conjugate_gradient over device calcocce/calcoccg calls) is kept as the iteration the
+device optimiser is checked against.
 """
 from __future__ import annotations
 
@@ -67,22 +68,10 @@ def n_to_gammas_trigonometric(n, p):
     return g
 
 
-def order_occupations_softmax(old_C, old_gamma, p):
-    """Sort each pair's orbitals by gamma, strongest first (utils.jl:586-616)."""
-    C = np.array(old_C, order="F", copy=True)
-    gamma = np.array(old_gamma, copy=True)
-    if np.any(gamma > 0):
-        for i in range(1, p.ndoc + 1):
-            w = _weak(p, i)
-            cols = np.r_[p.no1 + i - 1, w]
-            gi = (p.ndoc - i) + p.ndoc * np.arange(p.ncwo)
-            gt = np.r_[0.0, gamma[gi]]
-            order = np.argsort(gt, kind="stable")[::-1]
-            gt = gt[order] - gt.max()
-            Ct = np.asarray(old_C)[:, cols][:, order]
-            gamma[gi] = gt[1:]
-            C[:, cols] = Ct
-    return C, gamma
+def order_occupations_softmax(old_C, old_gamma, p, I):
+    """Sort each pair's orbitals by gamma, strongest first, gammas shifted by their
+    maximum (utils.jl:586-616), on the device of the System I."""
+    return api.order_occupations_softmax(old_C, old_gamma, p, I)
 
 
 # ---- nonlinear conjugate gradient ------------------------------------------------------

@@ -210,6 +210,12 @@ int dn_calcoccg(dn_sys* sys, const dn_functional* f, const double* gamma, const 
 int dn_occoptr(dn_sys* sys, const dn_functional* f, const double* gamma, const double* J,
                const double* K, const double* Hcore, double threshen, int maxiter,
                double* gamma_out, double* E, int* nit, int* converged);
+/* order_occupations_softmax (utils.jl:586-616): when some gamma > 0, each pair's
+ * orbitals (its strong one and ncwo weak ones) sorted by gamma, strongest first, and the
+ * gammas shifted by their maximum; C (nbf x nbf) and gamma (nv) in, the re-ordered C
+ * and gamma out (host arrays; the permutation runs on the device). */
+int dn_order_occupations_softmax(dn_sys* sys, const double* C, const double* gamma,
+                                 double* C_out, double* gamma_out);
 
 /* ---- orbital rotation and optimiser ------------------------------------------------ */
 

@@ -57,3 +57,30 @@ def test_occoptr_uses_the_device_cg(oracle):
     host = mini.occoptr_host_cg(g0, p, dev)
     assert abs(E - host.minimum) <= 1e-9
     dev.close()
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_order_occupations_softmax_on_device(oracle, ties):
+    """utils.jl:586-616 on the device against the oracle, bitwise: pairs re-ordered when
+    some gamma > 0 (with ties: equal gammas, and gammas equal to the strong orbital's 0,
+    keep the reverse of their stable order), unchanged otherwise."""
+    s = synthetic.SyntheticSystem(40, 80, 15, 3, ndoc=3)
+    p = s.p
+    dev = api.System(p).generate_b(3)
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal(p.nv)
+    if ties:
+        g[::3] = 0.0
+        g[1::4] = g[1]
+    C2, g2 = mini.order_occupations_softmax(s.C, g, p, dev)
+    Cr, gr = oracle.order_occupations_softmax(s.C, g, p)
+    assert np.array_equal(C2, Cr) and np.array_equal(g2, gr)
+    n1, _ = oracle.ocupacion_softmax(g, p)
+    n2, _ = oracle.ocupacion_softmax(g2, p)
+    for l in range(1, p.ndoc + 1):
+        cols = np.r_[p.no1 + l - 1, oracle.pair_weak(p, l)]
+        assert np.allclose(np.sort(n1[cols]), np.sort(n2[cols]))
+        assert n2[p.no1 + l - 1] == n2[cols].max()
+    C3, g3 = mini.order_occupations_softmax(s.C, -np.abs(g) - 0.1, p, dev)
+    assert np.array_equal(C3, s.C) and np.array_equal(g3, -np.abs(g) - 0.1)
+    dev.close()

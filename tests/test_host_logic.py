@@ -31,26 +31,6 @@ def test_gamma_guesses_and_conversions(oracle):
     assert np.abs(oracle.ocupacion_trigonometric(g2, pt)[0] - nt).max() < 1e-12
 
 
-def test_order_occupations_softmax(oracle):
-    s = system()
-    p = s.p
-    rng = np.random.default_rng(1)
-    g = rng.standard_normal(p.nv)          # some gammas > 0: pairs get re-ordered
-    C2, g2 = mini.order_occupations_softmax(s.C, g, p)
-    Cr, gr = oracle.order_occupations_softmax(s.C, g, p)
-    assert np.array_equal(C2, Cr) and np.array_equal(g2, gr)
-    # occupations are permuted within each pair, the strongest becomes the leader
-    n1, _ = oracle.ocupacion_softmax(g, p)
-    n2, _ = oracle.ocupacion_softmax(g2, p)
-    for l in range(1, p.ndoc + 1):
-        cols = np.r_[p.no1 + l - 1, oracle.pair_weak(p, l)]
-        assert np.allclose(np.sort(n1[cols]), np.sort(n2[cols]))
-        assert n2[p.no1 + l - 1] == n2[cols].max()
-    # no gamma > 0: unchanged
-    C3, g3 = mini.order_occupations_softmax(s.C, -np.abs(g), p)
-    assert np.array_equal(C3, s.C) and np.array_equal(g3, -np.abs(g))
-
-
 @pytest.mark.parametrize("ipnof", [7, 8, 9])
 @pytest.mark.parametrize("occ", ["Softmax", "Trigonometric"])
 def test_conjugate_gradient_reaches_stationary_point(oracle, ipnof, occ):
