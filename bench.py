@@ -22,10 +22,11 @@ Extra legs (--legs, default all; --no-extra for none):
           one captured HIP graph, against per-system builds on their own streams;
   served  Poisson arrivals of orbital-optimisation solves, p50/p99 latency and solves/s;
   grad    RI nuclear gradients: the densities at 400/1600/120 and 800/3200/240 and one
-          atom's contractions with derivative integrals in HBM (and from host arrays).
+          atom's contractions with derivative integrals in HBM (and from host arrays);
+  occ     occoptr's conjugate gradient on the device against the host-driven iteration.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
-                    [--legs 800,grouped,served,grad] [--no-cpu-baseline] [--no-extra]
+                    [--legs 800,grouped,served,grad,occ] [--no-cpu-baseline] [--no-extra]
 """
 from __future__ import annotations
 
@@ -283,6 +284,37 @@ def grad_leg(api, synthetic, seed, shapes=((400, 1600, 120), (800, 3200, 240)), 
     return out
 
 
+def occ_leg(api, synthetic, seed, nbf=400, naux=1600, nbf5=120, reps=3):
+    """occoptr's CG (minimization.jl:100-139) at 400/1600/120, GNOF, Softmax: on the
+    device (dn_occoptr, one host read per 64 evaluations) against the same iteration
+    driven from the host (one device call and read per energy or gradient)."""
+    from donof_amd import minimization as mini
+    sysd, dev = make_system(api, synthetic, nbf, naux, nbf5, seed, "fp64")
+    p = sysd.p
+    dev.jkh_mo_resident()
+    dev.sync()
+    g0 = mini.guess_gamma_softmax(p.ndoc, p.ncwo)
+    tol = p.threshen
+    api.occ_conjugate_gradient(g0, None, None, None, p, dev, tol)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x, E, nit, conv = api.occ_conjugate_gradient(g0, None, None, None, p, dev, tol)
+    dev_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h = mini.occoptr_host_cg(g0, p, dev)
+    host_ms = (time.perf_counter() - t0) * 1e3 / reps
+    dev.close()
+    out = {"config": {"nbf": nbf, "nbfaux": naux, "nbf5": nbf5, "nv": p.nv,
+                      "threshen": tol},
+           "device_ms": dev_ms, "host_driven_ms": host_ms, "speedup": host_ms / dev_ms,
+           "iterations": nit, "host_iterations": h.iterations, "converged": bool(conv),
+           "energy_diff_Ha": abs(E - h.minimum)}
+    log("occoptr CG 400/1600/120: device %.2f ms, host-driven %.2f ms (%d / %d iterations)"
+        % (dev_ms, host_ms, nit, h.iterations))
+    return out
+
+
 def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64, nbf_range=(48, 400)):
     """64 systems, nbf ~ U[48, 400], naux = 4 nbf, nbf5 = ceil(nbf/3): one grouped build
     of all of them per step (graph replay), against per-system builds on their own
@@ -421,8 +453,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--precision", choices=("fp64", "split"), default="fp64",
                     help="arithmetic of the headline build (fp64 is the metric's)")
-    ap.add_argument("--legs", default="800,grouped,served,grad",
-                    help="extra legs to run: any of 800, grouped, served, grad (or none)")
+    ap.add_argument("--legs", default="800,grouped,served,grad,occ",
+                    help="extra legs to run: any of 800, grouped, served, grad, occ (or none)")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served-solves leg and print its JSON (the main run "
                          "starts this in a child process under a time limit)")
@@ -539,6 +571,8 @@ def main():
                                                     nbf_range=(16, 64))
     if "grad" in legs:
         out["extra_gradients"] = grad_leg(api, synthetic, args.seed)
+    if "occ" in legs:
+        out["extra_occoptr"] = occ_leg(api, synthetic, args.seed)
     if "served" in legs:
         out["extra_served"] = served_in_child(args.served_timeout)
     print(json.dumps(out))

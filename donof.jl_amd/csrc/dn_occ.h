@@ -39,8 +39,11 @@ struct OccCG {
   double best_a, best_f, ft;
 };
 enum { kCgX = 0, kCgG = 1, kCgD = 2, kCgXT = 3, kCgGT = 4, kCgGB = 5, kCgVecs = 6 };
-// nslots evaluation slots (each: the controller's next trial point and the energy at it
-// in one block, then calcoccg's gradient at it); slots after convergence exit at once
+// nslots evaluation slots (each: the controller's next trial point and its occupations
+// in one block, then calcoccg's gradient and slices of the energy at it); slots after
+// convergence exit at once.  vec holds kCgVecs * nv doubles and then the energy slices
+// (occ_cg_energy_blocks of them).
+int occ_cg_energy_blocks(const OccDims& P);
 int occ_cg_slots(hipStream_t s, const OccDims& P, const OccWork& w, OccCG* c, double* vec,
                  double* dn, const double* J, const double* K, const double* Hcore,
                  int nslots);

@@ -745,72 +745,96 @@ __device__ void cg_control(OccCG& c, const CgVec& v, double* red) {
 
 }  // namespace
 
-// slot part 1 (one block): the controller, then n, dn, the per-orbital quantities and the
-// occupation energy calce at xt with cj12/ck12 built in place (pnof.jl:1273-1300)
-__global__ __launch_bounds__(kCgThreads) void occ_cg_eval_kernel(
-    OccDims P, OccWork w, OccCG* __restrict__ gc, double* __restrict__ vec,
-    double* __restrict__ dn, const double* __restrict__ J, const double* __restrict__ K,
-    const double* __restrict__ Hcore) {
+// slot part 1 (one block): the occupation energy at the previous trial point from part
+// 2's partials (fixed order), the controller, then n, dn and the per-orbital quantities
+// at the new trial point xt
+__global__ __launch_bounds__(kCgThreads) void occ_cg_step_kernel(OccDims P, OccWork w,
+                                                                 OccCG* __restrict__ gc,
+                                                                 double* __restrict__ vec,
+                                                                 double* __restrict__ dn,
+                                                                 int nE) {
   __shared__ double red[kCgThreads / 64];
   OccCG c = *gc;  // every thread: the same copy
   if (c.done) return;
   CgVec v{vec + kCgX * P.nv, vec + kCgG * P.nv, vec + kCgD * P.nv,
           vec + kCgXT * P.nv, vec + kCgGT * P.nv, vec + kCgGB * P.nv, P.nv};
+  if (c.state != kCgInit) {
+    const double* part = vec + kCgVecs * P.nv;
+    double f = 0.0;
+    for (int b = 0; b < nE; ++b) f += part[b];
+    c.ft = f;
+    c.nfev += 1;
+  }
   cg_control(c, v, red);
   if (!c.done) {
     ocupacion_block(P, v.xt, w.n, dn);
     __syncthreads();
     orb_prep_block(P, w);
-    __syncthreads();
-    const Orb o = orb_view(w);
-    const double S_F = w.scal[0];
-    const int nbf5 = P.nbf5;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nbf5; i += kCgThreads) {
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *gc = c;
+}
+
+// slot part 2: blocks k < nv the gradient calcoccg[k] at xt (pnof.jl:1324-1377), blocks
+// nv .. nv + nE - 1 slices of the occupation energy calce at xt (pnof.jl:1273-1300,
+// cj12/ck12 built in place), J and K read along their columns
+__global__ __launch_bounds__(512) void occ_cg_eval_kernel(OccDims P, OccWork w,
+                                                          const OccCG* __restrict__ gc,
+                                                          double* __restrict__ vec,
+                                                          const double* __restrict__ dn,
+                                                          const double* __restrict__ J,
+                                                          const double* __restrict__ K,
+                                                          const double* __restrict__ Hcore,
+                                                          int nE) {
+  __shared__ double red[16];
+  if (gc->done) return;
+  if ((int)blockIdx.x < P.nv) {
+    calcoccg_block(P, w, dn, J, K, Hcore, vec + kCgGT * P.nv, blockIdx.x, red);
+    return;
+  }
+  const int b = blockIdx.x - P.nv;
+  const Orb o = orb_view(w);
+  const double S_F = w.scal[0];
+  const int nbf5 = P.nbf5;
+  double acc = 0.0;
+  if (b == 0) {
+    for (int i = threadIdx.x; i < nbf5; i += blockDim.x) {
       const double jii = J[i + (long)i * nbf5];
       if (i < P.nbeta) acc += w.n[i] * (2 * Hcore[i] + jii);
       else if (i < P.nalpha) {
         if (P.nsoc > 0) acc += w.n[i] * 2 * Hcore[i];
       } else acc += w.n[i] * (2 * Hcore[i] + jii);
     }
-    const long nn = (long)nbf5 * nbf5;
-    for (long e = threadIdx.x; e < nn; e += kCgThreads) {
-      const int i = e % nbf5, j = e / nbf5;
-      if (i == j) continue;
-      double cj, ck;
-      coef_elem(P, o, S_F, i, j, &cj, &ck);
-      const long ji = j + (long)i * nbf5;
-      acc += cj * J[ji] - ck * K[ji];
-    }
-    c.ft = bsum_all(acc, red);
-    c.nfev += 1;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) *gc = c;
-}
-
-// slot part 2 (one block per gamma index): calcoccg at xt (n, dn and the per-orbital
-// quantities of part 1)
-__global__ void occ_cg_grad_kernel(OccDims P, OccWork w, const OccCG* __restrict__ gc,
-                                   double* __restrict__ vec, const double* __restrict__ dn,
-                                   const double* __restrict__ J, const double* __restrict__ K,
-                                   const double* __restrict__ Hcore) {
-  __shared__ double red[16];
-  if (gc->done) return;
-  calcoccg_block(P, w, dn, J, K, Hcore, vec + kCgGT * P.nv, blockIdx.x, red);
+  const long nn = (long)nbf5 * nbf5;
+  for (long e = b * (long)blockDim.x + threadIdx.x; e < nn; e += (long)nE * blockDim.x) {
+    const int j = e % nbf5, i = e / nbf5;  // J[j,i] at e: coalesced
+    if (i == j) continue;
+    double cj, ck;
+    coef_elem(P, o, S_F, i, j, &cj, &ck);
+    acc += cj * J[e] - ck * K[e];
+  }
+  const double sum = block_sum(acc, red);
+  if (threadIdx.x == 0) vec[kCgVecs * P.nv + b] = sum;
 }
 
 // ---- launchers ------------------------------------------------------------------------
+
+int occ_cg_energy_blocks(const OccDims& P) {
+  const long nn = (long)P.nbf5 * P.nbf5;
+  return (int)std::min<long>(64, std::max<long>(1, (nn + 4095) / 4096));
+}
 
 int occ_cg_slots(hipStream_t s, const OccDims& P, const OccWork& w, OccCG* c, double* vec,
                  double* dn, const double* J, const double* K, const double* Hcore,
                  int nslots) {
   if (P.nv <= 0) return fail(DN_ERR_ARG, "occupation optimiser: no gamma variables");
+  const int nE = occ_cg_energy_blocks(P);
   for (int i = 0; i < nslots; ++i) {
-    hipLaunchKernelGGL(occ_cg_eval_kernel, dim3(1), dim3(kCgThreads), 0, s, P, w, c, vec, dn,
-                       J, K, Hcore);
-    hipLaunchKernelGGL(occ_cg_grad_kernel, dim3(P.nv), dim3(256), 0, s, P, w, c, vec, dn, J,
-                       K, Hcore);
+    hipLaunchKernelGGL(occ_cg_step_kernel, dim3(1), dim3(kCgThreads), 0, s, P, w, c, vec, dn,
+                       nE);
+    hipLaunchKernelGGL(occ_cg_eval_kernel, dim3(P.nv + nE), dim3(512), 0, s, P, w, c, vec, dn,
+                       J, K, Hcore, nE);
   }
   DN_HIP(hipGetLastError());
   return DN_OK;

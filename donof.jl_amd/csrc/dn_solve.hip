@@ -333,7 +333,8 @@ int dn_occoptr(dn_sys* s, const dn_functional* f, const double* gamma, const dou
   const int nv = s->d.nv;
   if (nv <= 0) return fail(DN_ERR_ARG, "no occupation variables (ndoc = 0)");
   DN_TRY(upload_jkh(s, J, K, Hcore));
-  const size_t bytes = sizeof(OccCG) + kCgVecs * (size_t)nv * sizeof(double) + 256;
+  const size_t bytes = sizeof(OccCG) + (kCgVecs * (size_t)nv + occ_cg_energy_blocks(P)) *
+                                          sizeof(double) + 256;
   char* mem = nullptr;
   DN_TRY(dev_alloc(s->ctx, s->stream, bytes, reinterpret_cast<void**>(&mem)));
   OccCG* c = reinterpret_cast<OccCG*>(mem);

@@ -35,7 +35,8 @@ def test_device_cg_matches_host_cg(oracle, nbf, naux, nbf5, ipnof, occ):
     assert conv and host.g_converged and nit >= 1
     g = api.calcoccg(x, None, None, None, p, dev)
     assert np.max(np.abs(g)) <= tol
-    assert E == api.calcocce(x, None, None, None, p, dev)     # the minimum is E(gamma)
+    # the minimum is E(gamma) (the device sums the energy in other slices)
+    assert abs(E - api.calcocce(x, None, None, None, p, dev)) <= 1e-12 * max(1.0, abs(E))
     assert abs(E - host.minimum) <= 1e-9
     # and the oracle's energy at the device's gamma
     J, K, Hc = oracle.jkh_mo(s.C, s.H, dev.download_b(), nbf5)
