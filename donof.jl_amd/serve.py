@@ -7,11 +7,11 @@ request gets its own resident system (b generated in HBM, C, H and the occupatio
 once) and runs the ADAM loop of minimization.jl:344-438 (maxloop = max_iter, stop when
 2 max|g| < gtol).
 
-mode "grouped" (default): lock-step engines (one per nbf size class, each on its own
-thread and stream) advance their live solves together (dn_ogroup_run, a native loop of
-dn_ogroup_step): per step every solve takes its ADAM step and rotation as grouped
+mode "grouped" (default): lock-step engines (per nbf size class, each on its own thread
+and stream, each holding at most the class's cap of live solves) advance their live
+solves together (dn_ogroup_run, a native loop of dn_ogroup_step): per step every solve takes its ADAM step and rotation as grouped
 launches, and the orbital Lagrangians of all of them run as grouped GEMMs in one captured
-HIP graph.  Admitted requests are set up by a preparer thread (on their own streams)
+HIP graph.  Admitted requests are set up by preparer threads (on their own streams)
 while the engines keep stepping; when a solve finishes it leaves, and prepared requests
 join at the next step (the group is rebuilt).  Size classes keep a small solve from
 advancing at the pace of a large one.  mode "streams": one host thread per live solve,
@@ -137,6 +137,7 @@ class _LockStepEngine(threading.Thread):
         self.ctx, self.t0, self.phases, self.lock = ctx, t0, phases, lock
         self.finished, self.log = finished, log
         self.inbox = queue.Queue()
+        self.load = 0        # solves submitted and not finished (under `lock`)
         # set when something waits in the inbox: the native loop ends at the next
         # iteration boundary (dn_ogroup_run), the newcomers join there
         self.stop = ct.c_int(0)
@@ -184,6 +185,8 @@ class _LockStepEngine(threading.Thread):
                                 pass
                             group = None
                     t = time.perf_counter() - self.t0
+                    with self.lock:
+                        self.load -= len(live)
                     for rid, (req, dev, p, need, admitted) in list(live.items()):
                         try:
                             with _STRUCT_LOCK:
@@ -230,6 +233,8 @@ class _LockStepEngine(threading.Thread):
             with _STRUCT_LOCK:
                 dev.close()
             del live[rid]
+            with self.lock:
+                self.load -= 1
             if self.log:
                 self.log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"
                          % (rid, req.nbf, E, nit, t - req.arrival))
@@ -243,10 +248,21 @@ class Server:
     kernels (mode "grouped") or by one thread and stream each (mode "streams")."""
 
     def __init__(self, hbm_budget=0, max_live=8, device=0, mode="grouped",
-                 size_classes=(160, 320)):
+                 size_classes=(160, 320), group_caps=(8, 1, 1), preparers=4):
         if mode not in ("grouped", "streams"):
             raise ValueError("mode must be 'grouped' or 'streams'")
         self.size_classes = tuple(sorted(size_classes))   # nbf edges of the engines
+        # live solves one lock-step group of each class holds: small systems share
+        # grouped kernels (their steps are launch-bound); a larger one fills the GPU on
+        # its own and would only wait for its group's slowest member: with a cap of one
+        # it runs the single-system device loop on a stream of its own
+        if len(group_caps) != len(self.size_classes) + 1 or min(group_caps) < 1:
+            raise ValueError("group_caps needs one cap >= 1 per size class")
+        self.group_caps = tuple(int(c) for c in group_caps)
+        # threads setting admitted solves up (host generators, system creation, b in HBM,
+        # the first Lagrangian) while the engines step: one per request at a time would
+        # make admission the queue at a few requests per second
+        self.preparers = max(1, int(preparers))
         self.ctx = api.Context(device, hbm_budget)
         self.max_live = max_live
         self.mode = mode
@@ -283,22 +299,57 @@ class Server:
                 state["slots"] -= 1
             done_ev.set()
 
-        # one lock-step engine per size class, each on its own thread and stream: solves of
-        # similar cost share grouped kernels, and small ones do not wait for large ones
-        engines = [_LockStepEngine(self.ctx, t0, ph, lock, finished, log)
-                   for _ in range(len(self.size_classes) + 1)]
-        for e in engines:
-            e.start()
+        # lock-step engines per size class, each on its own thread and stream, holding at
+        # most group_caps[class] live solves: solves of similar cost share grouped kernels,
+        # small ones do not wait for large ones; an engine is started when every engine of
+        # the class is full
+        engines = [[] for _ in range(len(self.size_classes) + 1)]
 
-        def engine_for(nbf):
+        def class_of(nbf):
             k = 0
             while k < len(self.size_classes) and nbf >= self.size_classes[k]:
                 k += 1
-            return engines[k]
+            return k
+
+        def engine_for(nbf):
+            k = class_of(nbf)
+            with lock:
+                for e in engines[k]:
+                    if e.load < self.group_caps[k]:
+                        e.load += 1
+                        return e
+            e = _LockStepEngine(self.ctx, t0, ph, lock, finished, log)
+            e.load = 1
+            e.start()
+            with lock:
+                engines[k].append(e)
+            return e
+
+        def all_engines():
+            with lock:
+                return [e for cls in engines for e in cls]
 
         # admitted requests are set up (system created, b generated, run started) by a
-        # preparer thread on their own streams while the engines keep stepping
+        # few preparer threads on their own streams while the engines keep stepping
         prep_q = queue.Queue()
+
+        solos = []
+
+        def solo(req, need, admitted):
+            # a class whose groups hold one solve: the whole device loop on the solve's own
+            # stream (dn_orbopt_resident), as in mode "streams"
+            err = ""
+            E, nit, ok = float("nan"), 0, False
+            try:
+                E, nit, ok = solve_one(req, self.ctx)
+            except Exception as e:  # noqa: BLE001 - reported as the request's error
+                err = "%s: %s" % (type(e).__name__, e)
+            t = time.perf_counter() - t0
+            if log and not err:
+                log("done %d nbf=%d E=%.8f nit=%d latency %.3fs"
+                    % (req.rid, req.nbf, E, nit, t - req.arrival))
+            finished(SolveResult(req.rid, req.nbf, E, nit, ok, req.arrival, admitted, t, err),
+                     need)
 
         def preparer():
             while True:
@@ -306,6 +357,12 @@ class Server:
                 if item is None:
                     return
                 req, need, admitted = item
+                if self.group_caps[class_of(req.nbf)] == 1:
+                    th = threading.Thread(target=solo, args=(req, need, admitted), daemon=True)
+                    th.start()
+                    with lock:
+                        solos.append(th)
+                    continue
                 ta = time.perf_counter()
                 try:
                     dev, p = start_solve(req, self.ctx)
@@ -319,8 +376,10 @@ class Server:
                         ph["admit"] += time.perf_counter() - ta
                 engine_for(req.nbf).submit((req, dev, p, need, admitted))
 
-        worker = threading.Thread(target=preparer, daemon=True)
-        worker.start()
+        workers = [threading.Thread(target=preparer, daemon=True)
+                   for _ in range(getattr(self, "preparers", 1))]
+        for w in workers:
+            w.start()
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting = []
         try:
@@ -353,17 +412,22 @@ class Server:
                 with lock:
                     if len(results) == len(requests):
                         break
-                if not worker.is_alive() or not all(e.is_alive() for e in engines):
+                if not all(w.is_alive() for w in workers) or \
+                        not all(e.is_alive() for e in all_engines()):
                     raise RuntimeError("a serving thread stopped with requests outstanding")
                 done_ev.wait(min(0.002, max(0.0, pending[0].arrival - now)) if pending
                              else 0.002)
                 done_ev.clear()
         finally:
-            prep_q.put(None)
-            worker.join()
-            for e in engines:
+            for _ in workers:
+                prep_q.put(None)
+            for w in workers:
+                w.join()
+            for th in list(solos):
+                th.join()
+            for e in all_engines():
                 e.submit(None)
-            for e in engines:
+            for e in all_engines():
                 e.join()
         self.results = sorted(results, key=lambda r: r.rid)
         return self.results
@@ -434,10 +498,15 @@ class Server:
 
 
 def latency_stats(results, wall):
-    lat = np.array([r.latency for r in results if not r.error])
+    """Latency = time from arrival to exit (converged, or stopped at the iteration cap);
+    the converged solves' own percentiles beside it."""
+    ok = [r for r in results if not r.error]
+    lat = np.array([r.latency for r in ok])
+    conv = np.array([r.latency for r in ok if r.success])
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None
     return {"solves": len(lat), "solves_per_s": len(lat) / wall if wall > 0 else 0.0,
-            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
-            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
-            "mean_iterations": float(np.mean([r.nit for r in results if not r.error]))
-            if len(lat) else None,
-            "converged": int(sum(r.success for r in results))}
+            "latency": "time to exit (converged or at the iteration cap)",
+            "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
+            "mean_iterations": float(np.mean([r.nit for r in ok])) if ok else None,
+            "converged": int(sum(r.success for r in ok)),
+            "converged_p50_s": pct(conv, 50), "converged_p99_s": pct(conv, 99)}

@@ -65,18 +65,31 @@ class FakeCtx:
 @pytest.fixture
 def fake_device(monkeypatch):
     monkeypatch.setattr(serve, "start_solve", lambda req, ctx: (FakeDev(req), object()))
+    monkeypatch.setattr(serve, "solve_one", _fake_solo)
     monkeypatch.setattr(serve.api, "OGroup", FakeGroup)
     monkeypatch.setattr(serve.api, "footprint", lambda p: 10 ** 9)
     FakeGroup.fail_rids = set()
     FakeGroup.sizes = []
 
 
-def _server(max_live=4, classes=(160, 320)):
+SOLOS = []
+
+
+def _fake_solo(req, ctx):
+    """A solve of a class with cap one (the single-system loop): max_iter iterations."""
+    SOLOS.append(req.nbf)
+    if req.rid in FakeGroup.fail_rids:
+        raise DnError(6, "non-finite energy or orbital gradient")
+    return -1.0 * req.nbf, req.max_iter, False
+
+
+def _server(max_live=4, classes=(160, 320), caps=(8, 2, 2)):
     srv = serve.Server.__new__(serve.Server)
     srv.ctx = FakeCtx()
     srv.max_live = max_live
     srv.mode = "grouped"
     srv.size_classes = tuple(classes)
+    srv.group_caps = tuple(caps)
     srv.results = []
     srv.phases = {}
     srv._lock = threading.Lock()
@@ -108,7 +121,7 @@ def test_failed_step_is_reported_and_engine_goes_on(fake_device):
     res = srv.run(_requests(10))
     assert len(res) == 10
     assert "non-finite" in res[1].error
-    # the same engine (nbf >= 320) serves later requests: 4 (350) and 6 (600)
+    # the large class serves later requests: 4 (350) and 6 (600)
     assert not res[4].error and not res[6].error
     assert sum(1 for r in res if not r.error) >= 8
 
@@ -119,3 +132,27 @@ def test_admission_respects_slots(fake_device):
     assert all(not r.error for r in res)
     # one slot: every group holds exactly one solve
     assert all(len(g) == 1 for g in FakeGroup.sizes)
+
+
+def test_group_caps(fake_device):
+    """No lock-step group holds more live solves than its class's cap; with a cap of one a
+    solve runs in a group of its own."""
+    srv = _server(max_live=8, caps=(3, 2, 1))
+    sizes = [60, 70, 80, 90, 100, 200, 210, 220, 400, 500]
+    reqs = [serve.SolveRequest(rid=i, nbf=n, seed=i, arrival=0.0, max_iter=4)
+            for i, n in enumerate(sizes)]
+    res = srv.run(reqs)
+    assert all(not r.error for r in res)
+    cap = lambda n: 3 if n < 160 else (2 if n < 320 else 1)
+    assert FakeGroup.sizes and all(len(g) <= cap(g[0]) for g in FakeGroup.sizes)
+    assert any(len(g) == 3 for g in FakeGroup.sizes)   # small solves do share groups
+
+
+def test_cap_one_runs_the_single_system_loop(fake_device):
+    """With a cap of one a class's solves run the single-system device loop (no group)."""
+    SOLOS.clear()
+    srv = _server(max_live=4, caps=(8, 1, 1))
+    res = srv.run(_requests(10))
+    assert all(not r.error and r.nit == 5 for r in res)
+    assert sorted(SOLOS) == sorted(n for n in (r.nbf for r in res) if n >= 160)
+    assert all(max(g) < 160 for g in FakeGroup.sizes)
