@@ -391,7 +391,8 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64, nbf_range=(48, 400
     return out
 
 
-def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8):
+def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8,
+                modes=("grouped", "streams")):
     """Served solves: Poisson arrivals at `rate`/s of GNOF orbital optimisations (ADAM, at
     most 200 iterations, stop when 2 max|g| < 1e-4), nbf log-uniform in [50, 600],
     naux = 4 nbf, nbf5 = ceil(nbf/3); admission by the device's HBM budget.  The lock-step
@@ -400,7 +401,7 @@ def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8):
     count is reported (the random synthetic Hamiltonians rarely reach 1e-4 in 200)."""
     out = {"requests": n, "arrival_rate_per_s": rate, "nbf_range": [50, 600],
            "max_live": max_live, "max_iter": 200, "gtol": 1e-4}
-    for mode in ("grouped", "streams"):
+    for mode in modes:
         reqs = serve.synthetic_requests(n, rate, seed=seed, nbf_min=50, nbf_max=600,
                                         max_iter=200, gtol=1e-4)
         srv = serve.Server(hbm_budget=0, max_live=max_live, mode=mode)
@@ -463,12 +464,16 @@ def main():
                     help="run only the served-solves leg and print its JSON (the main run "
                          "starts this in a child process under a time limit)")
     ap.add_argument("--served-timeout", type=float, default=300.0)
+    ap.add_argument("--served-modes", default="grouped,streams",
+                    help="modes of the served-solves leg (--served-only)")
+    ap.add_argument("--served-requests", type=int, default=100)
     args = ap.parse_args()
     if args.served_only:
         import __graft_entry__
         __graft_entry__.load_package()
         from donof_amd import api, serve
-        print(json.dumps(serving_leg(api, serve)))
+        print(json.dumps(serving_leg(api, serve, n=args.served_requests,
+                                     modes=tuple(args.served_modes.split(",")))))
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X and this "
