@@ -139,6 +139,22 @@ static int sq_gemm(hipStream_t s, const double* A, const double* B, double* out,
   return gemm_f64_splitk(s, g, part, part_elems);
 }
 
+// out = A^T B (A read along its columns), split along K as sq_gemm splits
+static int tsq_gemm(hipStream_t s, const double* A, const double* B, double* out, int nbf,
+                    double* part, size_t part_elems) {
+  Gemm g;
+  g.M = nbf; g.N = nbf; g.K = nbf;
+  g.A = A; g.lda = nbf; g.a_kcontig = true;
+  g.B = B; g.ldb = nbf; g.b_kcontig = true;
+  g.C = out; g.ldc = nbf;
+  return gemm_f64_splitk(s, g, part, part_elems);
+}
+
+__global__ void ns_polish_kernel(double* __restrict__ U, const double* __restrict__ X, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) U[i] = ns_polish(U[i], X[i]);
+}
+
 int matmul_sq(hipStream_t s, const double* A, const double* B, double* out, int nbf,
               double* part, size_t part_elems) {
   return sq_gemm(s, A, B, out, nbf, part, part_elems);
@@ -195,6 +211,13 @@ int expm_taylor(hipStream_t s, const double* Y, double nrm, int nbf, double* wor
     std::swap(cur, oth);
   }
   if (cur != U) DN_HIP(hipMemcpyAsync(U, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // re-orthonormalisation: one Newton-Schulz step, U <- U (3 - U^T U) / 2, so that the
+  // rounding of the products (and of the squarings) does not accumulate in C over the
+  // iterations of the optimiser
+  DN_TRY(tsq_gemm(s, U, U, X, nbf, part, part_elems));   // X = U^T U
+  DN_TRY(sq_gemm(s, U, X, X2, nbf, part, part_elems));   // X2 = U X
+  hipLaunchKernelGGL(ns_polish_kernel, dim3(grid_for(nn)), dim3(256), 0, s, U, X2, nn);
+  DN_HIP(hipGetLastError());
   return DN_OK;
 }
 

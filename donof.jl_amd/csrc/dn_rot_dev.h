@@ -75,4 +75,9 @@ __device__ __forceinline__ double poly_element(long e, int nbf, const double* __
   return v;
 }
 
+// One Newton-Schulz step towards the orthogonal factor of U: U' = U (3 - U^T U) / 2 =
+// 1.5 U - 0.5 U (U^T U), element-wise from U and x = (U (U^T U))[e]; the error of
+// U^T U = 1 + E becomes O(E^2).  An explicit fma, so that every kernel rounds alike.
+__device__ __forceinline__ double ns_polish(double u, double x) { return fma(-0.5, x, 1.5 * u); }
+
 }  // namespace dn

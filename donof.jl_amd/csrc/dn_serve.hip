@@ -109,7 +109,7 @@ __global__ void rot_gather_kernel(const RotSys* __restrict__ d, int n, double* o
 }
 
 // element-wise steps of exp(Y) and of the orbital update over every system's nbf^2 block
-enum RotElem { kScale, kPolyB2, kPolyB1, kPolyB0, kAddTU, kAddUT, kCopyUT, kCopyCCa };
+enum RotElem { kScale, kPolyB2, kPolyB1, kPolyB0, kAddTU, kAddUT, kCopyUT, kPolish, kCopyCCa };
 
 __global__ __launch_bounds__(256) void rot_elem_kernel(const RotSys* __restrict__ d, int n,
                                                        int op, double c0, double c1, double c2,
@@ -129,6 +129,7 @@ __global__ __launch_bounds__(256) void rot_elem_kernel(const RotSys* __restrict_
     case kCopyUT:
       if (r.copy_u) r.U[e] = r.T[e];
       break;
+    case kPolish: r.U[e] = ns_polish(r.U[e], r.X2[e]); break;
     default: r.C[e] = r.Ca[e]; break;
   }
 }
@@ -238,7 +239,7 @@ namespace {
 
 // square-GEMM stages of a step: exp(Y)'s Taylor products, the two squaring directions and
 // C exp(Y)
-enum RotGemm { kXX, kX2X, kX2X2, kX4T, kSqUT, kSqTU, kCU, kNumRotGemm };
+enum RotGemm { kXX, kX2X, kX2X2, kX4T, kSqUT, kSqTU, kUtU, kUX, kCU, kNumRotGemm };
 
 struct RotLayout {  // offsets into the staging area for n systems
   size_t sys, gemm, red, bytes;
@@ -269,11 +270,13 @@ struct RotStage {
   std::vector<int> red_sq;   // of each reduction
 };
 
-// The square GEMM out = A B of sq_gemm (dn_rot.hip) for nbf x nbf operands.
-Gemm rot_square_gemm(const double* A, const double* B, double* out, int nbf) {
+// The square GEMM out = A B of sq_gemm (dn_rot.hip) for nbf x nbf operands (out = A^T B,
+// tsq_gemm's, with transpose_a).
+Gemm rot_square_gemm(const double* A, const double* B, double* out, int nbf,
+                     bool transpose_a = false) {
   Gemm g;
   g.M = nbf; g.N = nbf; g.K = nbf;
-  g.A = A; g.lda = nbf; g.a_kcontig = false;
+  g.A = A; g.lda = nbf; g.a_kcontig = transpose_a;
   g.B = B; g.ldb = nbf; g.b_kcontig = true;
   g.C = out; g.ldc = nbf;
   return g;
@@ -287,6 +290,8 @@ void rot_operands(const RotSys& r, int kind, const double** A, const double** B,
     case kX4T: *A = r.X4; *B = r.T; *out = r.U; break;
     case kSqUT: *A = r.U; *B = r.U; *out = r.T; break;
     case kSqTU: *A = r.T; *B = r.T; *out = r.U; break;
+    case kUtU: *A = r.U; *B = r.U; *out = r.X; break;    // U^T U (A transposed)
+    case kUX: *A = r.U; *B = r.X; *out = r.X2; break;
     default: *A = r.C; *B = r.U; *out = r.Ca; break;
   }
 }
@@ -310,7 +315,7 @@ void rot_build_stage(dn_ogroup* G, const RotLayout& L, int kind, const RotSys* r
     const double *A, *B;
     double* out;
     rot_operands(rs[i], kind, &A, &B, &out);
-    const Gemm g = rot_square_gemm(A, B, out, rs[i].nbf);
+    const Gemm g = rot_square_gemm(A, B, out, rs[i].nbf, kind == kUtU);
     Item& it = items[i];
     it.nsplit = splitk_count(g, sys[i]->part2_elems);
     it.gg.g = it.nsplit > 1 ? splitk_partials(g, it.nsplit, sys[i]->part2) : g;
@@ -784,6 +789,10 @@ int dn_ogroup_step(dn_ogroup* G, int* active) {
   bool odd = false;
   for (int t = 0; t < live; ++t) odd = odd || (sq[t] & 1);
   if (odd) DN_TRY(rot_elem(G, rs_d, live, elems, kCopyUT));  // results that ended in T
+  // re-orthonormalisation (expm_taylor's): U <- U (3 - U^T U) / 2
+  DN_TRY(rot_run_stage(G, &st[kUtU], -1));
+  DN_TRY(rot_run_stage(G, &st[kUX], -1));
+  DN_TRY(rot_elem(G, rs_d, live, elems, kPolish));
   DN_TRY(rot_run_stage(G, &st[kCU], -1));
   DN_TRY(rot_elem(G, rs_d, live, elems, kCopyCCa));
   // every system's Lagrangian, energy and gradient in one graph (systems that stopped

@@ -132,6 +132,28 @@ def test_rotate_orbital(oracle, nbf, nbf5, scale):
     dev.close()
 
 
+def test_rotations_keep_orbitals_orthonormal(oracle):
+    """exp(Y) is re-orthonormalised (a Newton-Schulz step) before C <- C exp(Y): over 300
+    rotations with many squarings each, C^T C stays at the identity to rounding, below the
+    drift of the same rotations through scipy's expm."""
+    nbf, nbf5 = 160, 50
+    p = synthetic.synthetic_param(nbf, 2 * nbf, nbf5, ndoc=5)
+    dev = api.System(p)
+    rng = np.random.default_rng(7)
+    C, _ = np.linalg.qr(rng.standard_normal((nbf, nbf)))
+    Cr = C.copy()
+    for _ in range(300):
+        y = 8.0 * rng.standard_normal(p.nvar) / np.sqrt(nbf)
+        C = api.rotate_orbital(y, C, p, dev)
+        Cr = oracle.rotate_orbital(y, Cr, nbf5)
+    dev.close()
+    e_dev = np.abs(C.T @ C - np.eye(nbf)).max()
+    e_ref = np.abs(Cr.T @ Cr - np.eye(nbf)).max()
+    print("orthonormality after 300 rotations: device %.2e, expm %.2e" % (e_dev, e_ref))
+    assert e_dev <= 2e-14
+    assert e_dev <= max(e_ref, 1e-14)
+
+
 @pytest.mark.parametrize("method", ["ADAM", "ADABelief", "YOGI", "Demon"])
 def test_orbopt_matches_oracle(oracle, method):
     s, p = closed(8, 0, nbf=36, naux=72, nbf5=15, ndoc=5, seed=9)
