@@ -959,8 +959,11 @@ int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int K
     const int tm = (g1.M + 127) / 128, tn = (g1.N + 127) / 128;
     const long total = (long)tm * tn * naux;
     if (total > 0x7fffffffL || tm * 128 > Mpad) return fail(DN_ERR_ARG, "split K1 grid");
-    hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 2, 2, true, true>), dim3((unsigned)total),
-                       dim3(256), 0, s, g1, tm, tn);
+    // eight wavefronts (2 x 4, 64 x 32 each): four per SIMD at two workgroups per CU
+    // hide more of the load latency than four (2.51-2.59 against 2.61-2.76 ms at
+    // 800/3200/240; the same MFMAs in the same order, so bitwise the same T)
+    hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 4, 2, true, true>), dim3((unsigned)total),
+                       dim3(512), 0, s, g1, tm, tn);
     DN_HIP(hipGetLastError());
     return DN_OK;
   }
@@ -970,6 +973,16 @@ int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int K
   g2.bhi = tp; g2.blo = tp + pl; g2.ldb = (long)Mpad * 32; g2.sB = (long)Mpad * Kpad;
   g2.C = wt; g2.ldc = nbf5; g2.sC = (long)nbf5 * nbf5;
   g2.cs = cscale_dev; g2.batch_inv_scale = qinv; g2.nbatch = naux;
+  if (nbf5 > 128 && Mpad == 256) {
+    // one 256 x 256 tile per slab (nbf5 in (128, 256]): T_Q and the C5 planes pass through
+    // LDS once per slab instead of twice (1.13-1.21 against 1.23-1.30 ms at 800/3200/240,
+    // bitwise the same W); the A rows read stay below Mpad
+    if ((long)naux > 0x7fffffffL) return fail(DN_ERR_ARG, "split K2 grid");
+    hipLaunchKernelGGL((split_lds_kernel<256, 256, 4, 2, 2, false>), dim3((unsigned)naux),
+                       dim3(512), 0, s, g2, 1, 1);
+    DN_HIP(hipGetLastError());
+    return DN_OK;
+  }
   const int tm = (g2.M + 127) / 128, tn = (g2.N + 127) / 128;
   const long total = (long)tm * tn * naux;
   if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split K2 grid");
