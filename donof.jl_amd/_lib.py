@@ -109,6 +109,14 @@ _SIGS = {
     "dn_group_set_timing": (_I, [_P, _I]),
     "dn_group_get_timing": (_I, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]),
     "dn_jkh_mo_noalloc": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "dn_basis_create": (_I, [_I, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                             ct.POINTER(ct.c_int), _D, _D, _D, ct.POINTER(_P)]),
+    "dn_basis_destroy": (_I, [_P]),
+    "dn_basis_nbf": (_I, [_P, ct.POINTER(ct.c_int)]),
+    "dn_int1e": (_I, [_P, _I, _D, _D, _D, _D, _D]),
+    "dn_int2c2e": (_I, [_P, _D]),
+    "dn_int3c2e": (_I, [_P, _P, _D]),
+    "dn_int4c2e": (_I, [_P, _D]),
 }
 
 _lib = None
@@ -153,3 +161,50 @@ def dptr(a):
 def as_f64(a, order="F"):
     """Column-major float64 view/copy of a host array, as the ABI expects."""
     return np.require(np.asarray(a, dtype=np.float64), requirements=[order])
+
+
+class Basis:
+    """A dn_basis (contracted Gaussian shells) and the host integrals over it."""
+
+    def __init__(self, l, nprim, pure, centers, exps, coefs):
+        lib = load_library()
+        ints = lambda v: (ct.c_int * len(v))(*[int(x) for x in v])
+        self._keep = [as_f64(np.asarray(centers).reshape(-1), "C"), as_f64(exps, "C"),
+                      as_f64(coefs, "C")]
+        h = ct.c_void_p()
+        check(lib.dn_basis_create(len(l), ints(l), ints(nprim), ints(pure), dptr(self._keep[0]),
+                                  dptr(self._keep[1]), dptr(self._keep[2]), ct.byref(h)))
+        self.h = h
+        n = ct.c_int()
+        check(lib.dn_basis_nbf(self.h, ct.byref(n)))
+        self.nbf = n.value
+
+    def int1e(self, Z, xyz):
+        """(S, T, V) for charges Z at xyz (natom x 3, bohr)."""
+        n = self.nbf
+        S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+        Z = as_f64(Z, "C")
+        xyz = as_f64(np.asarray(xyz).reshape(-1), "C")
+        check(_lib.dn_int1e(self.h, len(Z), dptr(Z), dptr(xyz), dptr(S), dptr(T), dptr(V)))
+        return S, T, V
+
+    def int2c2e(self):
+        G = np.zeros((self.nbf, self.nbf), order="F")
+        check(_lib.dn_int2c2e(self.h, dptr(G)))
+        return G
+
+    def int3c2e(self, aux):
+        I3 = np.zeros((self.nbf, self.nbf, aux.nbf), order="F")
+        check(_lib.dn_int3c2e(self.h, aux.h, dptr(I3)))
+        return I3
+
+    def int4c2e(self):
+        n = self.nbf
+        eri = np.zeros((n, n, n, n), order="F")
+        check(_lib.dn_int4c2e(self.h, dptr(eri)))
+        return eri
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.dn_basis_destroy(self.h)
+            self.h = None

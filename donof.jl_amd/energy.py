@@ -1,9 +1,10 @@
 """The outer macro-iteration driver of DoNOF.energy (energy.jl:3-320) over a
 device-resident system, and ENERGY1r (utils.jl:117-136).
 
-The integral setup (GaussianBasis, energy.jl:29) is outside this package: the caller
-supplies the RI tensor (a `System` with b uploaded or generated), H_core, starting
-orbitals and the nuclear repulsion.  Each outer iteration runs, as the reference does:
+energy(bset, p) sets up the integrals as energy.jl:29-80 does (integrals.py: the host
+integral engine, the RI metric on the device, the core guess and check_ortho) and writes
+the FCHK file at the end; energy(I, H, C, p) takes a device-resident system the caller set
+up (b uploaded or generated).  Each outer iteration runs, as the reference does:
 the orbital optimiser (p.orb_method) on the device, the occupation optimiser, the
 Softmax re-ordering, and the diagnostics (ENERGY1r, calcorbg, J/K and calcoccg), then
 the convergence test max|grad_orb| < threshgorb and ||grad_occ|| < threshgocc.  After
@@ -122,8 +123,51 @@ def read_all(title="donof"):
     return read_C(title), read_n(title)
 
 
-def energy(I, H, C, p, E_nuc=0.0, n=None, freeze_occ=False, printmode=True, checkpoint=False,
-           log=print):
+def energy(*args, **kwargs):
+    """DoNOF.energy.  energy(bset, p, ...) is the reference's entry point (energy.jl:3-320):
+    integrals, guess and the outer loop for a `basis.BasisSet` (see energy_bset).
+    energy(I, H, C, p, ...) runs the outer loop on a device-resident system the caller
+    set up (see energy_resident)."""
+    from .basis import BasisSet
+    if args and isinstance(args[0], BasisSet):
+        return energy_bset(*args, **kwargs)
+    return energy_resident(*args, **kwargs)
+
+
+def energy_bset(bset, p, C=None, n=None, aux=None, freeze_occ=False, printmode=True,
+                do_fchk=True, checkpoint=True, ctx=None, precision="fp64", log=print,
+                return_history=False):
+    """energy.jl:3-320 for the RI path: compute_integrals (the integral engine and the
+    device RI metric), E_nuc, the guess C = eigen(H, S) unless C is given, check_ortho,
+    the outer loop (energy_resident), and the FCHK file `<p.title>.fchk`
+    (energy.jl:274).  Returns (E_nuc + E, C, n, None) as the reference returns
+    (E_t, C, n, fmiug0) (fmiug0 belongs to the ID orbital method, not supported), plus
+    the per-iteration history with return_history."""
+    from . import integrals as ints
+    from . import output
+    S, T, V, H, I = ints.compute_integrals(bset, p, aux=aux, ctx=ctx, precision=precision,
+                                           log=log if printmode else None)
+    try:
+        E_nuc = ints.compute_E_nuc(bset, p)
+        if C is None:
+            _, C = ints.core_guess(H, S)
+        C = ints.check_ortho(C, S, p, log=log if printmode else None)
+        E_t, C, n, elag, hist = energy_resident(I, H, C, p, E_nuc=E_nuc, n=n,
+                                                freeze_occ=freeze_occ, printmode=printmode,
+                                                checkpoint=checkpoint, log=log)
+        if printmode:
+            log("Final NOF Total Energy = %15.7f" % E_t)
+        if do_fchk:
+            output.fchk(p.title, p, bset, "Energy", E_t, elag, n, C)
+    finally:
+        I.close()
+    if return_history:
+        return E_t, C, n, None, hist
+    return E_t, C, n, None
+
+
+def energy_resident(I, H, C, p, E_nuc=0.0, n=None, freeze_occ=False, printmode=True,
+                    checkpoint=False, log=print):
     """Outer loop of energy.jl:140-224.  Returns (E_nuc + E, C, n, elag, history); C, n
     and elag are sorted by order_subspaces for ipnof > 4.  checkpoint writes
     `<p.title>.npz` every iteration and at the end; a later call resumes with

@@ -30,6 +30,10 @@ class Param:
     occ_method: str = "Softmax"
     orb_method: str = "ADABelief"
     h_cut: float = field(default_factory=lambda: 0.02 * math.sqrt(2.0))
+    # the molecule (param.jl:4-5, 54): atom lines, atom count, spherical functions
+    mol: str = ""
+    natoms: int = 0
+    spherical: bool = False
     # derived sizes (param.jl:71-125)
     nalpha: int = 0
     nbeta: int = 0

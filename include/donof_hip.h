@@ -319,6 +319,34 @@ int dn_group_set_timing(dn_group* group, int on);
 /* Summed HIP-event time of the builds since the last call, and their count. */
 int dn_group_get_timing(dn_group* group, double* total_ms, long* builds);
 
+/* ---- Integral source of an RI system (compute_integrals, integrals.jl:1-65) ----------
+ * The reference evaluates its integrals with GaussianBasis (overlap, kinetic, nuclear,
+ * ERI_2e2c, ERI_2e3c, ERI_2e4c; integrals.jl:4-10,50-53) and hands G and I3c to Iaux
+ * (dn_sys_ri_metric here).  A dn_basis holds contracted Gaussian shells: shell s has
+ * angular momentum l[s] (0..6), nprim[s] primitives, pure[s] != 0 for real solid
+ * harmonics (m = -l..l; p as x, y, z) or 0 for Cartesian components (x^l, x^(l-1)y,
+ * x^(l-1)z, ...), centre centers[3s..3s+2] in bohr, and its exponents and contraction
+ * coefficients (for normalised primitives, as basis-set files list them) consecutively
+ * in exps/coefs.  Every function is normalised to one.  Matrices are column-major over
+ * the functions in shell order; the integrals are computed on the host's threads
+ * (McMurchie-Davidson). */
+typedef struct dn_basis dn_basis;
+int dn_basis_create(int nshell, const int* l, const int* nprim, const int* pure,
+                    const double* centers, const double* exps, const double* coefs,
+                    dn_basis** out);
+int dn_basis_destroy(dn_basis* basis);
+int dn_basis_nbf(const dn_basis* basis, int* nbf);
+/* overlap, kinetic and nuclear attraction (charges Z[natom] at xyz[3 natom], bohr),
+ * nbf x nbf each; a NULL output is skipped (integrals.jl:4-9) */
+int dn_int1e(const dn_basis* basis, int natom, const double* Z, const double* xyz, double* S,
+             double* T, double* V);
+/* (P|Q), naux x naux (ERI_2e2c, integrals.jl:50) */
+int dn_int2c2e(const dn_basis* aux, double* G);
+/* (mn|P), nbf x nbf x naux (ERI_2e3c, integrals.jl:52) */
+int dn_int3c2e(const dn_basis* basis, const dn_basis* aux, double* I3c);
+/* (mn|ls), nbf^4 (ERI_2e4c, integrals.jl:16; the non-RI integrals) */
+int dn_int4c2e(const dn_basis* basis, double* eri);
+
 #ifdef __cplusplus
 }
 #endif
