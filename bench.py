@@ -23,10 +23,13 @@ Extra legs (--legs, default all; --no-extra for none):
   served  Poisson arrivals of orbital-optimisation solves, p50/p99 latency and solves/s;
   grad    RI nuclear gradients: the densities at 400/1600/120 and 800/3200/240 and one
           atom's contractions with derivative integrals in HBM (and from host arrays);
-  occ     occoptr's conjugate gradient on the device against the host-driven iteration.
+  occ     occoptr's conjugate gradient on the device against the host-driven iteration;
+  h2o     the full energy() of H2O / cc-pVDZ (GNOF, ADAM, Softmax, default thresholds):
+          integrals from the package's engine, RI metric and the whole outer loop on the
+          device (the CPU baseline times the oracle's loop on the same integrals).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|split]
-                    [--legs 800,grouped,served,grad,occ] [--no-cpu-baseline] [--no-extra]
+                    [--legs 800,grouped,served,grad,occ,h2o] [--no-cpu-baseline] [--no-extra]
 """
 from __future__ import annotations
 
@@ -444,6 +447,55 @@ def served_in_child(timeout_s):
     return json.loads(lines[-1])
 
 
+H2O_MOL = "0 1\n O  0.0000   0.000   0.121\n H  0.0000   0.751  -0.485\n H  0.0000  -0.751  -0.485\n"
+
+
+def h2o_leg():
+    """energy(bset, p) of H2O / cc-pVDZ on the device with the reference's default
+    thresholds: wall time of the whole call (integrals, RI metric, outer loop, FCHK)."""
+    import tempfile
+    from donof_amd import basis, energy
+    out = {"molecule": "H2O O(0,0,0.121) H(0,+-0.751,-0.485) A", "basis": "cc-pvdz",
+           "functional": "GNOF", "orb_method": "ADAM", "occ_method": "Softmax"}
+    with tempfile.TemporaryDirectory() as tmp:
+        for rep in range(2):          # the first call also loads the integral code paths
+            bset, p = basis.molecule(H2O_MOL, "cc-pvdz", spherical=True)
+            p.ipnof, p.orb_method = 8, "ADAM"
+            p.title = os.path.join(tmp, "h2o")
+            t0 = time.perf_counter()
+            E_t, C, n, _, hist = energy.energy(bset, p, printmode=False, return_history=True)
+            dt = time.perf_counter() - t0
+        out.update({"nbf": p.nbf, "nbfaux": p.nbfaux, "nbf5": p.nbf5, "E_total": E_t,
+                    "outer_iterations": len(hist), "seconds": dt,
+                    "final_grad_orb": hist[-1]["grad_orb"],
+                    "final_grad_occ": hist[-1]["grad_occ"]})
+    return out
+
+
+def h2o_cpu_baseline():
+    """The oracle's outer loop for the same H2O / cc-pVDZ energy on the host."""
+    import __graft_entry__
+    orc = __graft_entry__.load_oracle()
+    from donof_amd import basis, integrals
+    bset, p = basis.molecule(H2O_MOL, "cc-pvdz", spherical=True)
+    h = bset.handle()
+    Z = np.array([a.Z for a in bset.atoms], dtype=np.float64)
+    S, T, V = h.int1e(Z, bset.centers_bohr())
+    S, H = (S + S.T) / 2, (T + T.T + V + V.T) / 2
+    G, I3c = integrals.integrals_for_gradients(bset)
+    t0 = time.perf_counter()
+    b = orc.iaux(G, I3c)
+    _, C = integrals.core_guess(H, S)
+    C = integrals.check_ortho(C, S, p, log=None)
+    pr = orc.make_param(p.nbf, G.shape[0], p.ne, ipnof=8, orb_method="ADAM")
+    E, _, _, hist, _ = orc.energy_loop(b, H, C, pr)
+    dt = time.perf_counter() - t0
+    return {"seconds": dt, "E_total": E + integrals.compute_E_nuc(bset, p),
+            "outer_iterations": len(hist),
+            "sample": "the oracle's whole outer loop (NumPy, scipy CG) for H2O / cc-pVDZ "
+                      "from the same integrals and core guess, default thresholds"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -458,8 +510,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--precision", choices=("fp64", "split"), default="fp64",
                     help="arithmetic of the headline build (fp64 is the metric's)")
-    ap.add_argument("--legs", default="800,grouped,served,grad,occ",
-                    help="extra legs to run: any of 800, grouped, served, grad, occ (or none)")
+    ap.add_argument("--legs", default="800,grouped,served,grad,occ,h2o",
+                    help="extra legs to run: any of 800, grouped, served, grad, occ, h2o "
+                         "(or none)")
     ap.add_argument("--served-only", action="store_true",
                     help="run only the served-solves leg and print its JSON (the main run "
                          "starts this in a child process under a time limit)")
@@ -582,6 +635,10 @@ def main():
         out["extra_gradients"] = grad_leg(api, synthetic, args.seed)
     if "occ" in legs:
         out["extra_occoptr"] = occ_leg(api, synthetic, args.seed)
+    if "h2o" in legs:
+        out["extra_h2o_energy"] = h2o_leg()
+        if out["cpu_baseline"] is not None:
+            out["cpu_baseline"]["h2o_energy"] = h2o_cpu_baseline()
     if "served" in legs:
         out["extra_served"] = served_in_child(args.served_timeout)
     print(json.dumps(out))

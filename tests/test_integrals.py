@@ -90,6 +90,38 @@ def test_single_centre_shells_orthonormal():
         assert np.abs(np.diag(Sc) - 1).max() < 1e-14
 
 
+def test_solid_harmonic_convention():
+    # which function is which: overlaps of a pure shell with the Cartesian components of the
+    # same shell at the same centre (x^l, x^(l-1)y, x^(l-1)z, ...) show each real solid
+    # harmonic's polynomial (m = -l..l; p as x, y, z): d_-2 ~ xy, d_-1 ~ yz, d_0 ~ 2zz-xx-yy,
+    # d_1 ~ xz, d_2 ~ xx-yy; f_-3 ~ y(3xx-yy), f_3 ~ x(xx-3yy), f_0 ~ z(2zz-3xx-3yy)
+    def ov(l):
+        h = _lib.Basis([l, l], [1, 1], [1, 0], np.zeros(6), np.array([0.8, 0.8]),
+                       np.array([1.0, 1.0]))
+        S, _, _ = h.int1e(np.zeros(0), np.zeros(0))
+        return S[:2 * l + 1, 2 * l + 1:]
+
+    def comp(l, lx, ly):
+        return [(a, b) for a in range(l, -1, -1) for b in range(l - a, -1, -1)].index((lx, ly))
+
+    Sp = ov(1)
+    assert np.allclose(Sp, np.eye(3), atol=1e-14)
+    Sd = ov(2)
+    xx, xy, xz, yy, yz, zz = (comp(2, a, b) for a, b in ((2, 0), (1, 1), (1, 0), (0, 2), (0, 1), (0, 0)))
+    # the pure functions are orthogonal to the spherical part of the Cartesian d set, so
+    # each row of Sd holds that function's polynomial in the (non-orthogonal) components
+    assert Sd[0, xy] > 0.5 and Sd[1, yz] > 0.5 and Sd[3, xz] > 0.5
+    assert Sd[4, xx] > 0 and Sd[4, yy] < 0 and abs(Sd[4, zz]) < 1e-14
+    assert Sd[2, zz] > 0 and Sd[2, xx] < 0 and Sd[2, yy] < 0
+    for row, keep in ((0, xy), (1, yz), (3, xz)):
+        assert np.abs(np.delete(Sd[row], keep)).max() < 1e-14
+    Sf = ov(3)
+    c = lambda a, b: comp(3, a, b)
+    assert Sf[0, c(2, 1)] > 0 and Sf[0, c(0, 3)] < 0          # y(3xx - yy)
+    assert Sf[6, c(3, 0)] > 0 and Sf[6, c(1, 2)] < 0          # x(xx - 3yy)
+    assert Sf[3, c(0, 0)] > 0 and Sf[3, c(2, 0)] < 0          # z(2zz - 3xx - 3yy)
+
+
 def test_overlap_kinetic_quadrature():
     # Cartesian d on A against Cartesian f (two primitives) on B: every integrand is a
     # product of one-dimensional factors per primitive pair, integrated here on a fine 1D
