@@ -103,6 +103,7 @@ _SIGS = {
     "dn_group_create": (_I, [_P, ct.POINTER(_P), _I, ct.POINTER(_P)]),
     "dn_group_destroy": (_I, [_P]),
     "dn_group_tune": (_I, [_P, ct.POINTER(ct.c_int)]),
+    "dn_group_place": (_I, [_P, _I]),
     "dn_group_jkh": (_I, [_P]),
     "dn_group_plan_text": (_I, [_P, ct.c_char_p, _I]),
     "dn_group_sync": (_I, [_P]),

@@ -252,17 +252,24 @@ class Group:
         self.handle = h
         self.configs = None
 
+    def place(self, max_grouped_nbf5):
+        """Fix the placement: systems with nbf5 <= max_grouped_nbf5 grouped, the others
+        built by the single-system build on their own streams (dn_group_place)."""
+        check(self.lib.dn_group_place(self.handle, int(max_grouped_nbf5)))
+        return self
+
     def tune(self):
-        """Autotune the tile configuration of every grouped GEMM stage and capture the
-        build graph; returns {stage: tile shape}."""
+        """Autotune the tile configuration of every grouped GEMM stage, capture the build
+        graph and (unless place() fixed it) choose the placement; returns
+        {stage: tile shape} (None for a stage when no system is grouped)."""
         cfg = (ct.c_int * 4)()
         check(self.lib.dn_group_tune(self.handle, cfg))
-        self.configs = {st: self.CONFIGS[c] for st, c in zip(self.STAGES, cfg)}
+        self.configs = {st: self.CONFIGS.get(c) for st, c in zip(self.STAGES, cfg)}
         return self.configs
 
     def plan(self):
-        """Per stage: each class's system count, tile shape and GFLOP."""
-        buf = ct.create_string_buffer(4096)
+        """Placement, then per stage each class's system count, tile shape and GFLOP."""
+        buf = ct.create_string_buffer(16384)
         check(self.lib.dn_group_plan_text(self.handle, buf, len(buf)))
         return buf.value.decode()
 

@@ -140,6 +140,9 @@ struct dn_group {
   // another's second transform and K/X as independent streams would, while every chain
   // still builds its systems with grouped launches.
   std::vector<dn_group*> chains;
+  // a leaf that is one chain of a parent: its stages' classes run one after another on
+  // its own stream (see launch_stage)
+  bool chain_member = false;
   // FP64 systems with odd nbf: C copied into its padded form before the transforms
   PadDesc* pad = nullptr;
   int npad = 0;
@@ -178,6 +181,16 @@ struct dn_group {
   hipGraphExec_t exec = nullptr;
   bool tuned = false;
   bool timing = false;
+  // Placement of the top-level group's systems (dn_group_place / dn_group_tune): `sub`
+  // builds the systems with nbf5 <= place_nbf5 by grouped launches in its captured graph,
+  // the `indep` systems run the single-system build on their own streams beside it.
+  // Without a placement the group itself is the grouped build of every system.
+  bool placed = false;
+  bool place_fixed = false;  // set by dn_group_place: dn_group_tune does not search
+  int place_nbf5 = 0;
+  dn_group* sub = nullptr;
+  std::vector<dn_sys*> indep;
+  std::vector<hipEvent_t> indep_ev;
   std::vector<hipEvent_t> ev_start, ev_stop;
   size_t ev_used = 0;
   double build_ms = 0.0;
@@ -362,7 +375,10 @@ int join(dn_group* G, int k) {
 
 int launch_stage(dn_group* G, int stage) {
   const int L = (int)G->launches[stage].size();
-  if (L <= 1) {
+  // A chain's classes stay on the chain's stream: when each chain forked its classes over
+  // the four branch streams, the graph ran the chains one after another (each chain's
+  // classes side by side on the four hardware queues); linear chains run side by side.
+  if (L <= 1 || G->chain_member) {
     for (int li = 0; li < L; ++li) DN_TRY(launch_one(G, stage, li));
     return DN_OK;
   }
@@ -477,8 +493,9 @@ extern "C" {
 
 static int group_create_leaf(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out);
 
-int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out) {
-  if (!ctx || !systems || nsys <= 0 || !out) return fail(DN_ERR_ARG, "bad dn_group_create arguments");
+// A leaf group, or a parent over chains of leaves, for these systems (build-group counts
+// untouched).
+static int group_create_any(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out) {
   *out = nullptr;
   // build work of each system (FLOP of its two transforms, up to a factor)
   std::vector<int> order(nsys);
@@ -494,12 +511,7 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
   // chains pay when their persistent launches are long enough to overlap; for small
   // systems the extra graph nodes cost more than the overlap gains (measured on MI355X:
   // 64 systems of nbf 16-64 build in 1.0 ms as one chain, 1.2 ms as four)
-  if (nsys < kChainMin || total < kChainWork) {
-    DN_TRY(group_create_leaf(ctx, systems, nsys, out));
-    count_build_group(systems, nsys, +1);  // the systems may not be destroyed meanwhile
-    (*out)->counted = true;
-    return DN_OK;
-  }
+  if (nsys < kChainMin || total < kChainWork) return group_create_leaf(ctx, systems, nsys, out);
   // a parent over chains: the systems dealt by build work, largest first, to the
   // least-loaded chain
   const int nch = std::min(kAux, nsys / 2);
@@ -520,11 +532,18 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
       dn_group_destroy(G);
       return rc;
     }
+    leaf->chain_member = true;
     G->chains.push_back(leaf);
   }
-  count_build_group(systems, nsys, +1);
-  G->counted = true;
   *out = G;
+  return DN_OK;
+}
+
+int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** out) {
+  if (!ctx || !systems || nsys <= 0 || !out) return fail(DN_ERR_ARG, "bad dn_group_create arguments");
+  DN_TRY(group_create_any(ctx, systems, nsys, out));
+  count_build_group(systems, nsys, +1);  // the systems may not be destroyed meanwhile
+  (*out)->counted = true;
   return DN_OK;
 }
 
@@ -675,6 +694,10 @@ int dn_group_destroy(dn_group* G) {
   drop_graph(G);
   for (dn_group* c : G->chains) dn_group_destroy(c);
   G->chains.clear();
+  if (G->sub) dn_group_destroy(G->sub);
+  G->sub = nullptr;
+  for (dn_sys* s : G->indep) (void)hipStreamSynchronize(s->stream);
+  for (auto e : G->indep_ev) (void)hipEventDestroy(e);
   for (int st = 0; st < kNumStages; ++st)
     if (G->dev[st]) (void)hipFree(G->dev[st]);
   if (G->kx) (void)hipFree(G->kx);
@@ -737,17 +760,30 @@ int tune_leaf(dn_group* G) {
   return DN_OK;
 }
 
-}  // namespace
-}  // namespace dn
+// The placement with the systems of nbf5 <= max_nbf5 grouped and the others independent
+// (every system grouped: no placement).  Replaces an earlier placement.
+int place(dn_group* G, int max_nbf5) {
+  if (G->sub) DN_TRY(dn_group_destroy(G->sub));
+  G->sub = nullptr;
+  for (dn_sys* s : G->indep) DN_HIP(hipStreamSynchronize(s->stream));
+  G->indep.clear();
+  std::vector<dn_sys*> grouped;
+  for (dn_sys* s : G->sys) (s->d.nbf5 <= max_nbf5 ? grouped : G->indep).push_back(s);
+  G->place_nbf5 = max_nbf5;
+  G->placed = !G->indep.empty();
+  if (!G->placed) return DN_OK;
+  while (G->indep_ev.size() < G->indep.size()) {
+    hipEvent_t e = nullptr;
+    DN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    G->indep_ev.push_back(e);
+  }
+  if (!grouped.empty()) DN_TRY(group_create_any(G->ctx, grouped.data(), (int)grouped.size(), &G->sub));
+  return DN_OK;
+}
 
-extern "C" {
-
-int dn_group_tune(dn_group* G, int* configs) {
-  if (!G) return fail(DN_ERR_ARG, "null dn_group");
-  DN_HIP(hipSetDevice(G->ctx->device));
+// Tunes a group's launches on its current inputs (one eager build first) and captures it.
+int tune_group(dn_group* G) {
   DN_TRY(join_systems(G));
-  // one eager build first: every stage's inputs (C planes and per-Q scales of split
-  // groups included) then hold real values while candidate launches are timed
   DN_TRY(launch_build(G));
   DN_HIP(hipStreamSynchronize(G->stream));
   std::vector<dn_group*> leaves = G->chains;
@@ -756,8 +792,128 @@ int dn_group_tune(dn_group* G, int* configs) {
   DN_TRY(capture(G));
   DN_HIP(hipStreamSynchronize(G->stream));
   G->tuned = true;
-  if (configs)  // the choice for each stage's class with the most work
+  return DN_OK;
+}
+
+// One build of every system under the placement: the grouped part's graph on its stream
+// and each independent system's single-system build on its own stream, all after the
+// work issued on G->stream so far, which then waits for every part.
+int launch_placed(dn_group* G) {
+  DN_HIP(hipEventRecord(G->fork_ev, G->stream));
+  if (G->sub) {
+    DN_HIP(hipStreamWaitEvent(G->sub->stream, G->fork_ev, 0));
+    DN_TRY(dn_group_jkh(G->sub));
+    DN_HIP(hipEventRecord(G->sub->done, G->sub->stream));
+    DN_HIP(hipStreamWaitEvent(G->stream, G->sub->done, 0));
+  }
+  for (size_t i = 0; i < G->indep.size(); ++i) {
+    dn_sys* s = G->indep[i];
+    DN_HIP(hipStreamWaitEvent(s->stream, G->fork_ev, 0));
+    DN_TRY(jkh_resident(s));
+    DN_HIP(hipEventRecord(G->indep_ev[i], s->stream));
+    DN_HIP(hipStreamWaitEvent(G->stream, G->indep_ev[i], 0));
+  }
+  return DN_OK;
+}
+
+// Event time of one build of every system under the current placement (after a warm-up).
+int time_placed(dn_group* G, float* ms) {
+  hipEvent_t a = nullptr, b = nullptr;
+  DN_HIP(hipEventCreate(&a));
+  DN_HIP(hipEventCreate(&b));
+  auto one = [&]() { return G->placed ? launch_placed(G) : dn_group_jkh(G); };
+  int rc = one();
+  if (rc == DN_OK && hipStreamSynchronize(G->stream) != hipSuccess) rc = DN_ERR_HIP;
+  if (rc == DN_OK && hipEventRecord(a, G->stream) != hipSuccess) rc = DN_ERR_HIP;
+  for (int r = 0; r < 3 && rc == DN_OK; ++r) rc = one();
+  if (rc == DN_OK && hipEventRecord(b, G->stream) != hipSuccess) rc = DN_ERR_HIP;
+  if (rc == DN_OK && hipEventSynchronize(b) != hipSuccess) rc = DN_ERR_HIP;
+  if (rc == DN_OK && hipEventElapsedTime(ms, a, b) != hipSuccess) rc = DN_ERR_HIP;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc == DN_ERR_HIP) return fail(rc, "timing a placement of the group");
+  *ms /= 3.f;
+  return rc;
+}
+
+// Grouped launches win where a system's build is too small to fill the GPU (launch-bound
+// stages); a large system's single-system build on its own stream reaches the same
+// throughput, and independent streams overlap its stages with other systems' at a finer
+// grain than a stage-ordered grouped chain does.  The placement search times the whole
+// build of all systems with the smallest systems grouped up to a few nbf5 limits (their
+// share of the build work at most a half), with every system grouped and with none, and
+// keeps the fastest.
+int search_placement(dn_group* G) {
+  float best_ms = 0.f;
+  DN_TRY(time_placed(G, &best_ms));  // every system grouped (tuned above)
+  int best = INT32_MAX;
+  double total = 0.0;
+  for (dn_sys* s : G->sys) total += (double)s->d.nbf * s->d.nbf5 * s->d.nbfaux * (s->d.nbf + 2.0 * s->d.nbf5);
+  static const int limits[] = {-1, 32, 64, 96, 128, 160, 192};
+  int prev_count = -1;
+  for (int lim : limits) {
+    double w = 0.0;
+    int count = 0;
+    for (dn_sys* s : G->sys)
+      if (s->d.nbf5 <= lim) {
+        w += (double)s->d.nbf * s->d.nbf5 * s->d.nbfaux * (s->d.nbf + 2.0 * s->d.nbf5);
+        ++count;
+      }
+    if (count == prev_count || count == (int)G->sys.size() || w > 0.5 * total) continue;
+    prev_count = count;
+    DN_TRY(place(G, lim));
+    if (G->sub) DN_TRY(tune_group(G->sub));
+    float ms = 0.f;
+    DN_TRY(time_placed(G, &ms));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = lim;
+    }
+  }
+  if (best != G->place_nbf5 || !G->placed) {
+    DN_TRY(place(G, best));
+    if (G->sub) DN_TRY(tune_group(G->sub));
+  }
+  return DN_OK;
+}
+
+}  // namespace
+}  // namespace dn
+
+extern "C" {
+
+int dn_group_place(dn_group* G, int max_grouped_nbf5) {
+  if (!G) return fail(DN_ERR_ARG, "null dn_group");
+  if (!G->counted) return fail(DN_ERR_ARG, "dn_group_place: not a group made by dn_group_create");
+  DN_HIP(hipSetDevice(G->ctx->device));
+  DN_HIP(hipStreamSynchronize(G->stream));
+  DN_TRY(place(G, max_grouped_nbf5));
+  G->place_fixed = true;
+  G->tuned = false;
+  return DN_OK;
+}
+
+int dn_group_tune(dn_group* G, int* configs) {
+  if (!G) return fail(DN_ERR_ARG, "null dn_group");
+  DN_HIP(hipSetDevice(G->ctx->device));
+  // one eager build first: every stage's inputs (C planes and per-Q scales of split
+  // groups included) then hold real values while candidate launches are timed
+  if (G->place_fixed && G->placed) {
+    if (G->sub) DN_TRY(tune_group(G->sub));
+  } else {
+    DN_TRY(tune_group(G));
+    if (!G->place_fixed) DN_TRY(search_placement(G));
+  }
+  G->tuned = true;
+  dn_group* T = G->placed ? G->sub : G;  // the grouped part
+  std::vector<dn_group*> leaves;
+  if (T) {
+    leaves = T->chains;
+    if (leaves.empty()) leaves.push_back(T);
+  }
+  if (configs)  // the choice for each stage's class with the most work (-1: none grouped)
     for (int st = 0; st < kNumStages; ++st) {
+      configs[st] = -1;
       double fmax = -1.0;
       for (dn_group* c : leaves)
         for (const auto& l : c->launches[st])
@@ -779,6 +935,18 @@ int dn_group_plan_text(dn_group* G, char* buf, int len) {
   static_assert(sizeof(snames) / sizeof(snames[0]) == kNumSplitConfigs, "a name per config");
   static const char* stages[] = {"K1", "K2", "J", "HC"};
   std::string t;
+  if (G->placed) {
+    t = "placement: " + std::to_string(G->sys.size() - G->indep.size()) + " grouped (nbf5 <= " +
+        std::to_string(G->place_nbf5) + "), " + std::to_string(G->indep.size()) +
+        " single-system builds on their own streams";
+    if (G->sub) {
+      std::vector<char> one(16384);
+      DN_TRY(dn_group_plan_text(G->sub, one.data(), (int)one.size()));
+      t += " | " + std::string(one.data());
+    }
+    std::snprintf(buf, (size_t)len, "%s", t.c_str());
+    return DN_OK;
+  }
   if (!G->chains.empty()) {
     for (size_t c = 0; c < G->chains.size(); ++c) {
       std::vector<char> one(8192);
@@ -811,6 +979,21 @@ int dn_group_plan_text(dn_group* G, char* buf, int len) {
 int dn_group_jkh(dn_group* G) {
   if (!G) return fail(DN_ERR_ARG, "null dn_group");
   DN_HIP(hipSetDevice(G->ctx->device));
+  if (G->placed) {  // grouped part and independent systems (dn_group_place / dn_group_tune)
+    if (G->timing) {
+      if (G->ev_used == G->ev_start.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        DN_HIP(hipEventCreate(&a));
+        DN_HIP(hipEventCreate(&b));
+        G->ev_start.push_back(a);
+        G->ev_stop.push_back(b);
+      }
+      DN_HIP(hipEventRecord(G->ev_start[G->ev_used], G->stream));
+    }
+    DN_TRY(launch_placed(G));
+    if (G->timing) DN_HIP(hipEventRecord(G->ev_stop[G->ev_used++], G->stream));
+    return DN_OK;
+  }
   if (!G->exec) DN_TRY(capture(G));
   DN_TRY(join_systems(G));
   if (G->timing) {

@@ -306,10 +306,21 @@ int dn_group_create(dn_ctx* ctx, dn_sys* const* systems, int nsys, dn_group** ou
 int dn_group_destroy(dn_group* group);
 /* The systems of each grouped GEMM stage (K1, K2, J, H C5) are split into classes by
  * operand layout, alignment and size; this times every tile configuration of every
- * class on this group, keeps the fastest, and captures the build graph.  configs
- * (4 ints, may be NULL) receives each stage's choice for its class with the most work
- * (FP64 tiles 0-8; split-precision transforms 100 + their split configuration). */
+ * class on this group, keeps the fastest, and captures the build graph; then, unless
+ * dn_group_place fixed it, it chooses the placement (see dn_group_place).  configs
+ * (4 ints, may be NULL) receives each stage's choice for its grouped class with the most
+ * work (FP64 tiles 0-10; split-precision transforms 100 + their split configuration; -1
+ * when no system is grouped). */
 int dn_group_tune(dn_group* group, int* configs);
+/* Placement of the group's systems (no reference counterpart): the systems with nbf5 <=
+ * max_grouped_nbf5 are built by the grouped launches, the others by the single-system
+ * build (dn_jkh_mo_resident's kernels) on their own streams beside them; a limit at or
+ * above every system's nbf5 groups all of them, one below every nbf5 none.  Without this
+ * call dn_group_tune chooses the placement by timing whole builds (every system grouped,
+ * none, and the smallest systems grouped up to a few nbf5 limits); after it, dn_group_tune
+ * tunes the grouped part only.  Results do not depend on the placement beyond the last
+ * bits (each GEMM family sums in its own fixed order). */
+int dn_group_place(dn_group* group, int max_grouped_nbf5);
 /* The launch plan as text: per stage, each class's system count, tile shape and GFLOP. */
 int dn_group_plan_text(dn_group* group, char* buf, int len);
 /* One J/K/H_core build of every system (a graph replay; asynchronous). */

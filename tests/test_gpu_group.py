@@ -144,7 +144,7 @@ def test_chained_group_matches_per_system_builds(precision):
         s.sync()
         ref.append(s.get_jkh())
         systems.append(s)
-    g = api.Group(systems)
+    g = api.Group(systems).place(10 ** 6)      # every system grouped (no placement search)
     g.tune()
     plan = g.plan()
     print(plan[:300], flush=True)
@@ -169,3 +169,44 @@ def test_group_requires_ready_systems():
     with pytest.raises(DnError, match="not been uploaded"):
         api.Group([s])
     s.close()
+
+
+@pytest.mark.parametrize("limit", [15, -1])
+def test_group_placement_matches_oracle(oracle, limit):
+    # the systems with nbf5 <= limit grouped, the others built by the single-system build
+    # on their own streams beside them (dn_group_place); -1: none grouped
+    print("case placement", limit, flush=True)
+    systems, inputs = _group(SHAPES, seed=50)
+    g = api.Group(systems).place(limit)
+    cfg = g.tune()
+    plan = g.plan()
+    assert plan.startswith("placement: %d grouped" % sum(sh[2] <= limit for sh in SHAPES))
+    if limit < 0:
+        assert all(v is None for v in cfg.values())
+    g.set_timing(True)
+    for rep in range(3):
+        if rep == 1:
+            for i, s in enumerate(systems):
+                C = synthetic.make_C(s.nbf, 400 + i)
+                s.set_orbitals(C)
+                inputs[i] = (inputs[i][0], C, inputs[i][2], inputs[i][3])
+        g.jkh()
+        g.sync()
+        for s, (b, C, H, nbf5) in zip(systems, inputs):
+            J, K, Hc = s.get_jkh()
+            Jr, Kr, Hr = oracle.jkh_mo(C, H, b, nbf5)
+            assert np.abs(J - Jr).max() <= TOL, (rep, s.nbf)
+            assert np.abs(K - Kr).max() <= TOL, (rep, s.nbf)
+            assert np.abs(Hc - Hr).max() <= TOL, (rep, s.nbf)
+    ms, n = g.get_timing()
+    assert n == 3 and ms > 0
+    # an independent system's result is its single-system build's, bit for bit
+    s = systems[-1]
+    J, K, _ = s.get_jkh()
+    s.jkh_mo_resident()
+    s.sync()
+    J2, K2, _ = s.get_jkh()
+    assert np.array_equal(J, J2) and np.array_equal(K, K2)
+    g.close()
+    for s in systems:
+        s.close()
