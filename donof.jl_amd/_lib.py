@@ -118,6 +118,9 @@ _SIGS = {
     "dn_int2c2e": (_I, [_P, _D]),
     "dn_int3c2e": (_I, [_P, _P, _D]),
     "dn_int4c2e": (_I, [_P, _D]),
+    "dn_int1e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _I, _D, _D, _I, _D, _D, _D]),
+    "dn_int2c2e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _I, _D]),
+    "dn_int3c2e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _P, ct.POINTER(ct.c_int), _I, _D]),
 }
 
 _lib = None
@@ -204,6 +207,31 @@ class Basis:
         eri = np.zeros((n, n, n, n), order="F")
         check(_lib.dn_int4c2e(self.h, dptr(eri)))
         return eri
+
+    @staticmethod
+    def _ints(v):
+        return (ct.c_int * len(v))(*[int(x) for x in v])
+
+    def int1e_deriv(self, shell_atom, Z, xyz, atom):
+        """(dS, dT, dV), each nbf x nbf x 3: d/dR of S, T, V for atom `atom`'s position."""
+        n = self.nbf
+        dS, dT, dV = (np.zeros((n, n, 3), order="F") for _ in range(3))
+        Z = as_f64(Z, "C")
+        xyz = as_f64(np.asarray(xyz).reshape(-1), "C")
+        check(_lib.dn_int1e_deriv(self.h, self._ints(shell_atom), len(Z), dptr(Z), dptr(xyz),
+                                  int(atom), dptr(dS), dptr(dT), dptr(dV)))
+        return dS, dT, dV
+
+    def int2c2e_deriv(self, aux_atom, atom):
+        dG = np.zeros((self.nbf, self.nbf, 3), order="F")
+        check(_lib.dn_int2c2e_deriv(self.h, self._ints(aux_atom), int(atom), dptr(dG)))
+        return dG
+
+    def int3c2e_deriv(self, shell_atom, aux, aux_atom, atom):
+        dI = np.zeros((self.nbf, self.nbf, aux.nbf, 3), order="F")
+        check(_lib.dn_int3c2e_deriv(self.h, self._ints(shell_atom), aux.h, self._ints(aux_atom),
+                                    int(atom), dptr(dI)))
+        return dI
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:

@@ -203,6 +203,44 @@ class BasisSet:
     def centers_bohr(self):
         return np.array([a.xyz for a in self.atoms], dtype=np.float64) * ANGSTROM_TO_BOHR
 
+    # derivative integrals for atom i (0-based), as api.compute_geom_gradients reads them
+    # from the basis set (GaussianBasis' ∇overlap, ∇kinetic, ∇nuclear, ∇ERI_2e3c,
+    # ∇ERI_2e2c in grads.jl:13-19,88-89); bohr, the Cartesian component last
+
+    def _one_electron_derivs(self, i):
+        key = ("1e", i)
+        cache = self.__dict__.setdefault("_deriv_cache", {})
+        if key not in cache:
+            cache.clear()
+            Z = np.array([a.Z for a in self.atoms], dtype=np.float64)
+            cache[key] = self.handle().int1e_deriv([s.atom for s in self.basis], Z,
+                                                   self.centers_bohr(), i)
+        return cache[key]
+
+    def overlap_deriv(self, i):
+        return self._one_electron_derivs(i)[0]
+
+    def kinetic_deriv(self, i):
+        return self._one_electron_derivs(i)[1]
+
+    def nuclear_deriv(self, i):
+        return self._one_electron_derivs(i)[2]
+
+    def fitting_set(self):
+        """The RI fitting set the integrals of this basis use (`aux_basis`, made once)."""
+        if self.__dict__.get("_aux") is None:
+            self.__dict__["_aux"] = aux_basis(self)
+        return self.__dict__["_aux"]
+
+    def eri_2e3c_deriv(self, i):
+        aux = self.fitting_set()
+        return self.handle().int3c2e_deriv([s.atom for s in self.basis], aux.handle(),
+                                           [s.atom for s in aux.basis], i)
+
+    def eri_2e2c_deriv(self, i):
+        aux = self.fitting_set()
+        return aux.handle().int2c2e_deriv([s.atom for s in aux.basis], i)
+
     def handle(self):
         """The library's dn_basis for these shells (created once)."""
         if self._handle is None:

@@ -448,6 +448,54 @@ void parallel_for(long n, const std::function<void(long)>& f) {
   for (auto& t : th) t.join();
 }
 
+// ---- derivatives with respect to a shell's centre --------------------------------------
+//
+// d/dA_x of (x - A_x)^i exp(-a |r - A|^2) is 2a (x - A_x)^(i+1) exp(..) - i (x - A_x)^(i-1)
+// exp(..): the derivative of a Cartesian component of shell A is a combination of the
+// components of the same contraction at l + 1 (primitive coefficients times 2a) and l - 1.
+
+dn_basis::Shell shifted(const dn_basis::Shell& A, int dl) {
+  dn_basis::Shell s = A;
+  s.l = A.l + dl;
+  s.ncart = ncart_of(s.l);
+  s.nfunc = s.ncart;
+  if (dl > 0)
+    for (int i = 0; i < s.nprim; ++i) s.coef[i] = 2.0 * A.exps[i] * A.coef[i];
+  s.tr.clear();
+  return s;
+}
+
+// out[k][c][r] = d/dA_k of f's Cartesian integrals of shell A (rows c, nrest columns r),
+// where f(shell, buf) writes [ncart(shell)][nrest] for a shell on A's centre.
+template <class F>
+void deriv_center(const dn_basis::Shell& A, size_t nrest, F&& f, double* out) {
+  const int na = A.ncart;
+  const dn_basis::Shell Ap = shifted(A, +1);
+  std::vector<double> P((size_t)Ap.ncart * nrest), M;
+  f(Ap, P.data());
+  if (A.l > 0) {
+    const dn_basis::Shell Am = shifted(A, -1);
+    M.resize((size_t)Am.ncart * nrest);
+    f(Am, M.data());
+  }
+  int ca[ncart_of(kMaxL)][3];
+  cart_list(A.l, ca);
+  for (int k = 0; k < 3; ++k)
+    for (int c = 0; c < na; ++c) {
+      int up[3] = {ca[c][0], ca[c][1], ca[c][2]};
+      up[k] += 1;
+      const double* pp = &P[(size_t)cart_index(A.l + 1, up[0], up[1]) * nrest];
+      double* o = out + ((size_t)k * na + c) * nrest;
+      for (size_t r = 0; r < nrest; ++r) o[r] = pp[r];
+      if (ca[c][k] > 0) {
+        int dn_[3] = {ca[c][0], ca[c][1], ca[c][2]};
+        dn_[k] -= 1;
+        const double* mm = &M[(size_t)cart_index(A.l - 1, dn_[0], dn_[1]) * nrest];
+        for (size_t r = 0; r < nrest; ++r) o[r] -= ca[c][k] * mm[r];
+      }
+    }
+}
+
 }  // namespace
 }  // namespace dn
 
@@ -653,6 +701,170 @@ int dn_int4c2e(const dn_basis* b, double* eri) {
             const long idx[8][4] = {{m, n, r, q}, {n, m, r, q}, {m, n, q, r}, {n, m, q, r},
                                     {r, q, m, n}, {q, r, m, n}, {r, q, n, m}, {q, r, n, m}};
             for (auto& e : idx) eri[e[0] + nbf * (e[1] + nbf * (e[2] + nbf * e[3]))] = s;
+          }
+  });
+  return DN_OK;
+}
+
+// d/dR of the one-electron integrals for the nuclear position R of atom `atom` (bohr):
+// the basis functions on that atom move, and for dV its nucleus moves too
+// (translational invariance: d/dC = -(d/dA + d/dB) for each nucleus C); nbf x nbf x 3
+// each (the Cartesian component last); a NULL output is skipped.  shell_atom[s] is the
+// atom of shell s (0-based).
+int dn_int1e_deriv(const dn_basis* b, const int* shell_atom, int natom, const double* Z,
+                   const double* xyz, int atom, double* dS, double* dT, double* dV) {
+  if (!b || !shell_atom || natom <= 0 || !Z || !xyz || atom < 0 || atom >= natom)
+    return fail(DN_ERR_ARG, "dn_int1e_deriv: bad argument");
+  const int ns = (int)b->shells.size();
+  const long nbf = b->nbf, nn = nbf * nbf;
+  for (double* o : {dS, dT, dV})
+    if (o) std::fill(o, o + 3 * nn, 0.0);
+  parallel_for((long)ns * ns, [&](long k) {
+    const int i = (int)(k / ns), j = (int)(k % ns);
+    const auto& A = b->shells[i];
+    const auto& B = b->shells[j];
+    const bool ai = shell_atom[i] == atom, aj = shell_atom[j] == atom;
+    const size_t nb = B.ncart, na = A.ncart;
+    std::vector<double> d(3 * na * nb), cart(na * nb), w;
+    auto put = [&](double* M, double f) {  // M[.., .., k] += f * (Ta d_k Tb^T)
+      for (int kk = 0; kk < 3; ++kk) {
+        for (size_t c = 0; c < na * nb; ++c) cart[c] = f * d[kk * na * nb + c];
+        for (int fa = 0; fa < A.nfunc; ++fa)
+          for (int fb = 0; fb < B.nfunc; ++fb) {
+            double s = 0.0;
+            for (size_t x = 0; x < na; ++x) {
+              const double ta = A.tr[(size_t)fa * na + x];
+              if (ta == 0.0) continue;
+              for (size_t y = 0; y < nb; ++y) s += ta * cart[x * nb + y] * B.tr[(size_t)fb * nb + y];
+            }
+            M[(A.offset + fa) + nbf * (B.offset + fb) + nn * kk] += s;
+          }
+      }
+    };
+    // two-centre integrals: d/dB = -d/dA, so atom-pairs move only when exactly one is on atom
+    if ((dS || dT) && ai != aj) {
+      const double f = ai ? 1.0 : -1.0;
+      if (dS) {
+        deriv_center(A, nb, [&](const dn_basis::Shell& X, double* o) { cart_overlap(X, B, o); }, d.data());
+        put(dS, f);
+      }
+      if (dT) {
+        deriv_center(A, nb, [&](const dn_basis::Shell& X, double* o) { cart_kinetic(X, B, o); }, d.data());
+        put(dT, f);
+      }
+    }
+    if (dV) {
+      // per nucleus c: (a_i - [c == atom]) dA V^c + (a_j - [c == atom]) dB V^c, with
+      // dB V^c(A, B) the transpose of dA V^c(B, A)
+      std::vector<double> dA(3 * na * nb), dBt(3 * nb * na);
+      for (int c = 0; c < natom; ++c) {
+        const double fc = c == atom ? 1.0 : 0.0;
+        const double wa = (ai ? 1.0 : 0.0) - fc, wb = (aj ? 1.0 : 0.0) - fc;
+        if (wa == 0.0 && wb == 0.0) continue;
+        if (wa != 0.0)
+          deriv_center(A, nb, [&](const dn_basis::Shell& X, double* o) {
+            cart_nuclear(X, B, 1, Z + c, xyz + 3 * c, o, w);
+          }, dA.data());
+        if (wb != 0.0)
+          deriv_center(B, na, [&](const dn_basis::Shell& X, double* o) {
+            cart_nuclear(X, A, 1, Z + c, xyz + 3 * c, o, w);
+          }, dBt.data());
+        for (int kk = 0; kk < 3; ++kk)
+          for (size_t x = 0; x < na; ++x)
+            for (size_t y = 0; y < nb; ++y)
+              d[(kk * na + x) * nb + y] = (wa != 0.0 ? wa * dA[(kk * na + x) * nb + y] : 0.0) +
+                                          (wb != 0.0 ? wb * dBt[(kk * nb + y) * na + x] : 0.0);
+        put(dV, 1.0);
+      }
+    }
+  });
+  return DN_OK;
+}
+
+// d/dR of (P|Q) for atom `atom`: naux x naux x 3 (d/dQ = -d/dP)
+int dn_int2c2e_deriv(const dn_basis* aux, const int* aux_atom, int atom, double* dG) {
+  if (!aux || !aux_atom || !dG || atom < 0) return fail(DN_ERR_ARG, "dn_int2c2e_deriv: bad argument");
+  const int ns = (int)aux->shells.size();
+  const long n = aux->nbf, nn = n * n;
+  std::fill(dG, dG + 3 * nn, 0.0);
+  parallel_for((long)ns * ns, [&](long k) {
+    const int i = (int)(k / ns), j = (int)(k % ns);
+    const bool ai = aux_atom[i] == atom, aj = aux_atom[j] == atom;
+    if (ai == aj) return;
+    const auto& P = aux->shells[i];
+    const auto& Q = aux->shells[j];
+    const dn_basis::Shell up = unit_shell(P.ctr), uq = unit_shell(Q.ctr);
+    const size_t np = P.ncart, nq = Q.ncart;
+    std::vector<double> d(3 * np * nq), w;
+    deriv_center(P, nq, [&](const dn_basis::Shell& X, double* o) { cart_eri(X, up, Q, uq, o, w); },
+                 d.data());
+    const double f = ai ? 1.0 : -1.0;
+    for (int kk = 0; kk < 3; ++kk)
+      for (int fp = 0; fp < P.nfunc; ++fp)
+        for (int fq = 0; fq < Q.nfunc; ++fq) {
+          double s = 0.0;
+          for (size_t x = 0; x < np; ++x) {
+            const double tp = P.tr[(size_t)fp * np + x];
+            if (tp == 0.0) continue;
+            for (size_t y = 0; y < nq; ++y) s += tp * d[(kk * np + x) * nq + y] * Q.tr[(size_t)fq * nq + y];
+          }
+          dG[(P.offset + fp) + n * (Q.offset + fq) + nn * kk] = f * s;
+        }
+  });
+  return DN_OK;
+}
+
+// d/dR of (mn|P) for atom `atom`: nbf x nbf x naux x 3 (d/dP = -(d/dA + d/dB))
+int dn_int3c2e_deriv(const dn_basis* b, const int* shell_atom, const dn_basis* aux,
+                     const int* aux_atom, int atom, double* dI) {
+  if (!b || !aux || !shell_atom || !aux_atom || !dI || atom < 0)
+    return fail(DN_ERR_ARG, "dn_int3c2e_deriv: bad argument");
+  const int ns = (int)b->shells.size(), nas = (int)aux->shells.size();
+  const long nbf = b->nbf, naux = aux->nbf, slab = nbf * nbf, comp = slab * naux;
+  std::fill(dI, dI + 3 * comp, 0.0);
+  parallel_for((long)ns * ns * nas, [&](long k) {
+    const int q = (int)(k % nas);
+    const int i = (int)((k / nas) / ns), j = (int)((k / nas) % ns);
+    const auto& A = b->shells[i];
+    const auto& B = b->shells[j];
+    const auto& P = aux->shells[q];
+    const double fp_ = aux_atom[q] == atom ? 1.0 : 0.0;
+    const double wa = (shell_atom[i] == atom ? 1.0 : 0.0) - fp_;
+    const double wb = (shell_atom[j] == atom ? 1.0 : 0.0) - fp_;
+    if (wa == 0.0 && wb == 0.0) return;
+    const dn_basis::Shell u = unit_shell(P.ctr);
+    const size_t na = A.ncart, nb = B.ncart, np = P.ncart;
+    std::vector<double> dA, dB, w;
+    if (wa != 0.0) {
+      dA.resize(3 * na * nb * np);
+      deriv_center(A, nb * np, [&](const dn_basis::Shell& X, double* o) { cart_eri(X, B, P, u, o, w); },
+                   dA.data());
+    }
+    if (wb != 0.0) {
+      dB.resize(3 * nb * na * np);
+      deriv_center(B, na * np, [&](const dn_basis::Shell& X, double* o) { cart_eri(X, A, P, u, o, w); },
+                   dB.data());
+    }
+    for (int kk = 0; kk < 3; ++kk)
+      for (int fa = 0; fa < A.nfunc; ++fa)
+        for (int fb = 0; fb < B.nfunc; ++fb)
+          for (int fq = 0; fq < P.nfunc; ++fq) {
+            double s = 0.0;
+            for (size_t x = 0; x < na; ++x) {
+              const double ta = A.tr[(size_t)fa * na + x];
+              if (ta == 0.0) continue;
+              for (size_t y = 0; y < nb; ++y) {
+                const double tb = B.tr[(size_t)fb * nb + y];
+                if (tb == 0.0) continue;
+                for (size_t z = 0; z < np; ++z) {
+                  double v = 0.0;
+                  if (wa != 0.0) v += wa * dA[((kk * na + x) * nb + y) * np + z];
+                  if (wb != 0.0) v += wb * dB[((kk * nb + y) * na + x) * np + z];
+                  s += ta * tb * P.tr[(size_t)fq * np + z] * v;
+                }
+              }
+            }
+            dI[(A.offset + fa) + nbf * (B.offset + fb) + slab * (P.offset + fq) + comp * kk] = s;
           }
   });
   return DN_OK;

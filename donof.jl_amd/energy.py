@@ -136,13 +136,16 @@ def energy(*args, **kwargs):
 
 def energy_bset(bset, p, C=None, n=None, aux=None, freeze_occ=False, printmode=True,
                 do_fchk=True, checkpoint=True, ctx=None, precision="fp64", log=print,
-                return_history=False):
+                return_history=False, do_gradients=False):
     """energy.jl:3-320 for the RI path: compute_integrals (the integral engine and the
     device RI metric), E_nuc, the guess C = eigen(H, S) unless C is given, check_ortho,
     the outer loop (energy_resident), and the FCHK file `<p.title>.fchk`
     (energy.jl:274).  Returns (E_nuc + E, C, n, None) as the reference returns
     (E_t, C, n, fmiug0) (fmiug0 belongs to the ID orbital method, not supported), plus
-    the per-iteration history with return_history."""
+    the per-iteration history with return_history.  do_gradients returns
+    (grad, E_t, C, n) as energy.jl:309-314 does: the RI nuclear gradient (natoms x 3,
+    Hartree/bohr) of the final state on the device (api.compute_geom_gradients, with the
+    derivative integrals of the package's integral engine)."""
     from . import integrals as ints
     from . import output
     S, T, V, H, I = ints.compute_integrals(bset, p, aux=aux, ctx=ctx, precision=precision,
@@ -159,8 +162,14 @@ def energy_bset(bset, p, C=None, n=None, aux=None, freeze_occ=False, printmode=T
             log("Final NOF Total Energy = %15.7f" % E_t)
         if do_fchk:
             output.fchk(p.title, p, bset, "Energy", E_t, elag, n, C)
+        grad = None
+        if do_gradients:
+            cj12, ck12 = api.PNOFi_selector(n, p, I)
+            grad = api.compute_geom_gradients(bset, n, C, cj12, ck12, elag, p, I)
     finally:
         I.close()
+    if do_gradients:
+        return (grad, E_t, C, n, hist) if return_history else (grad, E_t, C, n)
     if return_history:
         return E_t, C, n, None, hist
     return E_t, C, n, None

@@ -33,7 +33,7 @@ def compute_integrals(bset, p, aux=None, ctx=None, precision="fp64", log=print):
     V = (V + V.T) / 2
     H = T + V
     if aux is None:
-        aux = basis_mod.aux_basis(bset)
+        aux = bset.fitting_set()
     if log:
         log("Basis Set                                            = %s" % bset.name)
         log("Auxiliary Basis Set                                  = %s" % aux.name)
@@ -51,7 +51,7 @@ def compute_integrals(bset, p, aux=None, ctx=None, precision="fp64", log=print):
 def integrals_for_gradients(bset, aux=None):
     """(G, I3c) of compute_integrals, for the RI nuclear gradients (grads.jl:1-100)."""
     if aux is None:
-        aux = basis_mod.aux_basis(bset)
+        aux = bset.fitting_set()
     ha = aux.handle()
     G = ha.int2c2e()
     return (G + G.T) / 2, bset.handle().int3c2e(ha)

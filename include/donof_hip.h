@@ -357,6 +357,17 @@ int dn_int2c2e(const dn_basis* aux, double* G);
 int dn_int3c2e(const dn_basis* basis, const dn_basis* aux, double* I3c);
 /* (mn|ls), nbf^4 (ERI_2e4c, integrals.jl:16; the non-RI integrals) */
 int dn_int4c2e(const dn_basis* basis, double* eri);
+/* Derivatives for the nuclear gradients (GaussianBasis' ∇overlap, ∇kinetic, ∇nuclear,
+ * ∇ERI_2e2c, ∇ERI_2e3c as grads.jl:13-19,88-89 calls them): d/dR of each integral for
+ * the position R (bohr) of atom `atom`, the Cartesian component last (nbf x nbf x 3,
+ * naux x naux x 3, nbf x nbf x naux x 3).  shell_atom / aux_atom give each shell's atom
+ * (0-based); for dV the nuclei Z, xyz are those of dn_int1e and the moving atom's nucleus
+ * moves with its functions.  A NULL output of dn_int1e_deriv is skipped. */
+int dn_int1e_deriv(const dn_basis* basis, const int* shell_atom, int natom, const double* Z,
+                   const double* xyz, int atom, double* dS, double* dT, double* dV);
+int dn_int2c2e_deriv(const dn_basis* aux, const int* aux_atom, int atom, double* dG);
+int dn_int3c2e_deriv(const dn_basis* basis, const int* shell_atom, const dn_basis* aux,
+                     const int* aux_atom, int atom, double* dI);
 
 #ifdef __cplusplus
 }

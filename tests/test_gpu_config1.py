@@ -90,3 +90,62 @@ def test_h2o_ccpvdz_energy_is_an_oracle_stationary_point(oracle, tmp_path):
     e_line = next(ln for ln in text if ln.startswith("Total Energy"))
     assert abs(float(e_line.split()[-1]) - E_t) < 1e-12
     assert np.abs(energy.read_C(p.title) - C).max() == 0.0
+
+
+def test_h2o_ccpvdz_nuclear_gradient(oracle, tmp_path):
+    """energy(bset, p, do_gradients=True) (energy.jl:309-314, grads.jl:1-100): the device
+    gradient from the package's derivative integrals equals the oracle's contraction of
+    the same integrals at the device's solution, and central differences of the device's
+    converged total energy (displacements of 2e-3 bohr, each solve restarted from the
+    undisplaced solution)."""
+    print("case config1 gradient", flush=True)
+    gold = _gold()
+    thresh = {"threshgorb": 1e-6, "threshgocc": 1e-7, "threshen": 1e-9}
+
+    def solve(xyz, C=None, n=None, grad=False):
+        mol = "0 1\n" + "\n".join("%s %.15f %.15f %.15f" % (s, *r) for s, r in zip("OHH", xyz))
+        bset, p = basis.molecule(mol, "cc-pvdz", spherical=True)
+        p.ipnof, p.orb_method = 8, "ADAM"
+        for k, v in thresh.items():
+            setattr(p, k, v)
+        p.title = str(tmp_path / "h2o_g")
+        out = energy.energy(bset, p, C=C, n=n, printmode=False, do_fchk=False,
+                            checkpoint=False, do_gradients=grad)
+        return bset, p, out
+
+    xyz0 = np.array([[0.0, 0.0, 0.121], [0.0, 0.751, -0.485], [0.0, -0.751, -0.485]])
+    bset, p, (grad, E0, C0, n0) = solve(xyz0, grad=True)
+    # the oracle's contraction of the same derivative integrals at (C0, n0)
+    G, I3c = integrals.integrals_for_gradients(bset)
+    pr = oracle.make_param(p.nbf, p.nbfaux, p.ne, ipnof=8, orb_method="ADAM")
+    cj12, ck12 = oracle.pnof_selector(n0, pr)
+    S, T, V, H, I = integrals.compute_integrals(bset, p, log=None)
+    I.close()
+    b = oracle.iaux(G, I3c)
+    _, elag, _, _ = oracle.energy1r(C0, n0, H, b, cj12, ck12, pr)
+    derivs = [(bset.overlap_deriv(i), bset.kinetic_deriv(i), bset.nuclear_deriv(i),
+               bset.eri_2e3c_deriv(i), bset.eri_2e2c_deriv(i)) for i in range(3)]
+    gr = oracle.compute_geom_gradients(n0, C0, cj12, ck12, elag, pr, G, I3c, derivs,
+                                       np.array([8.0, 1.0, 1.0]), xyz0)
+    assert np.abs(grad - gr).max() < 1e-8
+    # translational invariance of the total gradient
+    assert np.abs(grad.sum(axis=0)).max() < 1e-6
+    # central differences of the converged energy.  The reference converts Angstrom to
+    # bohr with 1.88973 in E_nuc and its gradient (utils.jl:306,320) and the integrals use
+    # the Bohr radius (1.8897261...): the energy's nuclear part therefore changes with a
+    # displacement of the integrals' bohr 1.0000021 times as fast as grad_nuc says, which
+    # the comparison puts back (2e-6 of a nuclear gradient of ~3 Ha/bohr)
+    gnuc = api.compute_grad_nuc(bset.atoms)
+    ratio = 1.88973 / basis.ANGSTROM_TO_BOHR
+    h = 2e-3
+    for atom, k in ((0, 2), (1, 1), (1, 2)):
+        E = []
+        for sgn in (+1, -1):
+            xyz = xyz0.copy()
+            xyz[atom, k] += sgn * h / basis.ANGSTROM_TO_BOHR
+            E.append(solve(xyz, C=C0, n=n0)[2][0])
+        fd = (E[0] - E[1]) / (2 * h)
+        expect = grad[atom, k] + gnuc[atom, k] * (ratio - 1.0)
+        print("atom %d comp %d analytic %.9f (%.9f with the factor) fd %.9f"
+              % (atom, k, grad[atom, k], expect, fd), flush=True)
+        assert abs(fd - expect) < 1e-6

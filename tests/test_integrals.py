@@ -307,3 +307,37 @@ def test_fchk_layout(tmp_path):
     assert np.abs(section("Natural orbital occupancies") - n).max() < 1e-8
     assert abs(float(text[next(i for i, ln in enumerate(text) if ln.startswith("Total Energy"))]
                      .split()[-1]) + 76.25) < 1e-12
+
+
+def test_derivative_integrals_finite_differences():
+    # d/dR of S, T, V, (P|Q) and (mn|P) for one atom against central differences of the
+    # integrals at displaced geometries (h = 1e-4 bohr)
+    xyz = np.array([[0.0, 0.0, 0.121], [0.0, 0.751, -0.485], [0.0, -0.751, -0.485]])
+
+    def ints(g):
+        mol = "\n".join("%s %.15f %.15f %.15f" % (s, *r) for s, r in zip("OHH", g))
+        bs = basis.basis_set("cc-pvdz", basis.parse_molecule(mol)[0], True)
+        h = bs.handle()
+        S, T, V = h.int1e(np.array([8.0, 1.0, 1.0]), bs.centers_bohr())
+        aux = bs.fitting_set()
+        return bs, S, T, V, aux.handle().int2c2e(), h.int3c2e(aux.handle())
+
+    bs, *_ = ints(xyz)
+    aux = bs.fitting_set()
+    step = 1e-4 / basis.ANGSTROM_TO_BOHR
+    for atom, k in ((1, 1), (0, 2), (2, 0)):
+        dS, dT, dV = bs.overlap_deriv(atom), bs.kinetic_deriv(atom), bs.nuclear_deriv(atom)
+        dG, dI = bs.eri_2e2c_deriv(atom), bs.eri_2e3c_deriv(atom)
+        gp, gm = xyz.copy(), xyz.copy()
+        gp[atom, k] += step
+        gm[atom, k] -= step
+        plus, minus = ints(gp)[1:], ints(gm)[1:]
+        for ana, a, b_ in zip((dS, dT, dV, dG, dI), plus, minus):
+            fd = (a - b_) / 2e-4
+            assert np.abs(ana[..., k] - fd).max() < 1e-6 * max(1.0, np.abs(fd).max()), (atom, k)
+        # translational invariance: the derivatives of all atoms sum to zero
+    tot = [sum(f(a) for a in range(3)) for f in (bs.overlap_deriv, bs.kinetic_deriv,
+                                                   bs.nuclear_deriv, bs.eri_2e2c_deriv,
+                                                   bs.eri_2e3c_deriv)]
+    for t in tot:
+        assert np.abs(t).max() < 1e-10
