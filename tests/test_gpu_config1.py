@@ -92,6 +92,36 @@ def test_h2o_ccpvdz_energy_is_an_oracle_stationary_point(oracle, tmp_path):
     assert np.abs(energy.read_C(p.title) - C).max() == 0.0
 
 
+@pytest.mark.parametrize("ipnof,orb_method,occ_method", [(5, "ADABelief", "Softmax"),
+                                                         (7, "YOGI", "Trigonometric"),
+                                                         (8, "Demon", "Softmax")])
+def test_h2o_other_functionals_are_oracle_stationary_points(oracle, tmp_path, ipnof,
+                                                           orb_method, occ_method):
+    # PNOF5 / PNOF7 / GNOF with the other optimisers and occupation parametrisations: the
+    # device's converged (C, n) is a stationary point of the oracle's functional and the
+    # oracle's ENERGY1r there gives the device's energy
+    print("case config1", ipnof, orb_method, occ_method, flush=True)
+    gold = _gold()
+    bset, p = basis.molecule(gold["molecule"], "cc-pvdz", spherical=True)
+    p.ipnof, p.orb_method, p.occ_method = ipnof, orb_method, occ_method
+    p.threshgorb, p.threshgocc, p.threshen = 1e-5, 1e-6, 1e-8
+    p.title = str(tmp_path / "h2o_f")
+    E_t, C, n, _, hist = energy.energy(bset, p, printmode=False, return_history=True,
+                                       do_fchk=False, checkpoint=False)
+    assert hist[-1]["grad_orb"] < p.threshgorb and hist[-1]["grad_occ"] < p.threshgocc
+    S, T, V, H, I = integrals.compute_integrals(bset, p, log=None)
+    I.close()
+    G, I3c = integrals.integrals_for_gradients(bset)
+    b = oracle.iaux(G, I3c)
+    pr = oracle.make_param(p.nbf, p.nbfaux, p.ne, ipnof=ipnof, orb_method=orb_method,
+                           occ_method=occ_method)
+    cj12, ck12 = oracle.pnof_selector(n, pr)
+    E1, _, _, _ = oracle.energy1r(C, n, H, b, cj12, ck12, pr)
+    assert abs(E1 + integrals.compute_E_nuc(bset, p) - E_t) < 1e-8
+    g_orb = oracle.calcorbg(np.zeros(pr.nvar), n, cj12, ck12, C, H, b, pr)
+    assert np.max(np.abs(g_orb)) < p.threshgorb
+
+
 def test_h2o_ccpvdz_nuclear_gradient(oracle, tmp_path):
     """energy(bset, p, do_gradients=True) (energy.jl:309-314, grads.jl:1-100): the device
     gradient from the package's derivative integrals equals the oracle's contraction of
