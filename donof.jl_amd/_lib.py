@@ -118,6 +118,8 @@ _SIGS = {
     "dn_int2c2e": (_I, [_P, _D]),
     "dn_int3c2e": (_I, [_P, _P, _D]),
     "dn_int4c2e": (_I, [_P, _D]),
+    "dn_int3c2e_device": (_I, [_P, _P, _D]),
+    "dn_sys_ri_from_basis": (_I, [_P, _P, _P]),
     "dn_int1e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _I, _D, _D, _I, _D, _D, _D]),
     "dn_int2c2e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _I, _D]),
     "dn_int3c2e_deriv": (_I, [_P, ct.POINTER(ct.c_int), _P, ct.POINTER(ct.c_int), _I, _D]),
@@ -200,6 +202,12 @@ class Basis:
     def int3c2e(self, aux):
         I3 = np.zeros((self.nbf, self.nbf, aux.nbf), order="F")
         check(_lib.dn_int3c2e(self.h, aux.h, dptr(I3)))
+        return I3
+
+    def int3c2e_device(self, aux):
+        """(mn|P) computed on the GPU (orbital l <= 2, fitting l <= 4)."""
+        I3 = np.zeros((self.nbf, self.nbf, aux.nbf), order="F")
+        check(_lib.dn_int3c2e_device(self.h, aux.h, dptr(I3)))
         return I3
 
     def int4c2e(self):

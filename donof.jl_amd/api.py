@@ -124,6 +124,13 @@ class System:
         check(self.lib.dn_sys_ri_metric(self.handle, dptr(G), dptr(I3c)))
         return self
 
+    def ri_from_basis(self, basis, aux):
+        """The RI setup from a basis and a fitting basis (`_lib.Basis` handles): (P|Q) on
+        the host, (mn|P) computed on the device into the metric solve (integrals.jl:47-53,
+        68-91)."""
+        check(self.lib.dn_sys_ri_from_basis(self.handle, basis.h, aux.h))
+        return self
+
     def set_ri_factor(self, Wt):
         """The RI factor of an uploaded b = I3c W for the nuclear gradients:
         Wt = W^{-1} G^{-1} (naux x naux), so that I3c G^{-1} = b Wt; G^{-1/2} for the

@@ -19,6 +19,11 @@ namespace dn {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 
+// (mn|P) of an orbital and a fitting basis on the device (dn_ints.hip): into dI with
+// leading dimension ld (>= nbf) and slab stride ld * nbf; pad rows are not written.
+int int3c2e_device(hipStream_t st, const dn_basis* basis, const dn_basis* aux, double* dI,
+                   long ld);
+
 #define DN_HIP(expr)                                                                  \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \

@@ -496,7 +496,368 @@ void deriv_center(const dn_basis::Shell& A, size_t nrest, F&& f, double* out) {
     }
 }
 
+// ---- three-centre integrals on the device ---------------------------------------------
+//
+// One workgroup per (orbital shell pair A >= B, auxiliary shell P); a launch per angular
+// momentum class (la, lb, lc), which fixes every size at compile time.  For each primitive
+// triple the workgroup builds the Hermite Coulomb integrals R_tuv (t + u + v <= L) in LDS,
+// level by level of the Boys-function recursion; thread (a, b) then contracts them with its
+// Cartesian pair's Hermite coefficients and every component c of P, accumulating the
+// contracted Cartesian block in registers.  The block goes through LDS to the shells'
+// transform matrices and is written to I3c[m + ld n + ld nbf Q] and its mirror.
+
+struct DevShell {
+  double x, y, z;
+  int l, nprim, prim, func, tr;  // prim: offset of exps/coefs, func: first function, tr: offset
+  int nfunc;
+};
+
+constexpr int kDevMaxLab = 2;   // orbital shells up to d
+constexpr int kDevMaxLc = 4;    // fitting shells up to g
+
+__host__ __device__ constexpr int nc_of(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Cartesian exponents of component k of shell l (the order of cart_list)
+__device__ __forceinline__ void comp_of(int l, int k, int* ex) {
+  int c = 0;
+  for (int lx = l; lx >= 0; --lx)
+    for (int ly = l - lx; ly >= 0; --ly) {
+      if (c == k) {
+        ex[0] = lx;
+        ex[1] = ly;
+        ex[2] = l - lx - ly;
+        return;
+      }
+      ++c;
+    }
+}
+
+__device__ void boys_dev(int nmax, double T, double* F) {
+  if (T < 1e-15) {
+    for (int n = 0; n <= nmax; ++n) F[n] = 1.0 / (2 * n + 1);
+    return;
+  }
+  const double eT = exp(-T);
+  if (T > 40.0) {
+    F[0] = 0.5 * sqrt(M_PI / T);
+    for (int n = 0; n < nmax; ++n) F[n + 1] = ((2 * n + 1) * F[n] - eT) / (2.0 * T);
+    return;
+  }
+  double term = 1.0 / (2 * nmax + 1), sum = term;
+  for (int k = 1; k < 400; ++k) {
+    term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+    sum += term;
+    if (term < 1e-17 * sum) break;
+  }
+  F[nmax] = eT * sum;
+  for (int n = nmax - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + eT) / (2 * n + 1);
+}
+
+// 1D Hermite coefficients of x_A^i x_B^j (i <= LI, j <= LJ) for one Cartesian exponent pair
+template <int LI, int LJ>
+__device__ __forceinline__ void herm1(int i, int j, double a, double b, double XAB,
+                                      double (&E)[LI + LJ + 1]) {
+  const double p = a + b, h = 0.5 / p;
+  const double XPA = -b * XAB / p, XPB = a * XAB / p;
+  double cur[LI + LJ + 1];
+  for (int t = 0; t <= LI + LJ; ++t) cur[t] = 0.0;
+  cur[0] = exp(-a * b / p * XAB * XAB);
+  for (int ii = 0; ii < i; ++ii) {  // raise i
+    double nx[LI + LJ + 1];
+    for (int t = 0; t <= LI + LJ; ++t) {
+      double v = XPA * cur[t];
+      if (t > 0) v += h * cur[t - 1];
+      if (t + 1 <= ii) v += (t + 1) * cur[t + 1];
+      nx[t] = t <= ii + 1 ? v : 0.0;
+    }
+    for (int t = 0; t <= LI + LJ; ++t) cur[t] = nx[t];
+  }
+  for (int jj = 0; jj < j; ++jj) {  // raise j
+    double nx[LI + LJ + 1];
+    for (int t = 0; t <= LI + LJ; ++t) {
+      double v = XPB * cur[t];
+      if (t > 0) v += h * cur[t - 1];
+      if (t + 1 <= i + jj) v += (t + 1) * cur[t + 1];
+      nx[t] = t <= i + jj + 1 ? v : 0.0;
+    }
+    for (int t = 0; t <= LI + LJ; ++t) cur[t] = nx[t];
+  }
+  for (int t = 0; t <= LI + LJ; ++t) E[t] = cur[t];
+}
+
+template <int LA, int LB, int LC>
+__global__ __launch_bounds__(64) void int3c_kernel(const DevShell* __restrict__ shells,
+                                                   const DevShell* __restrict__ ashells,
+                                                   const double* __restrict__ prim,
+                                                   const double* __restrict__ trm,
+                                                   const int2* __restrict__ pairs, int npairs,
+                                                   const int* __restrict__ pshell, int np,
+                                                   double* __restrict__ out, long ld,
+                                                   long slab) {
+  constexpr int NA = nc_of(LA), NB = nc_of(LB), NC = nc_of(LC);
+  constexpr int L = LA + LB + LC, D = L + 1;
+  __shared__ double Rb[2][D * D * D];
+  __shared__ double Fn[L + 1];
+  __shared__ double cart[NA * NB * NC];
+  const long wg = blockIdx.x;
+  if (wg >= (long)npairs * np) return;
+  const int2 pr = pairs[wg / np];
+  const DevShell A = shells[pr.x], B = shells[pr.y];
+  const DevShell P = ashells[pshell[wg % np]];
+  const int tid = threadIdx.x;
+  const bool act = tid < NA * NB;
+  int ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
+  if (act) {
+    comp_of(LA, tid / NB, ea);
+    comp_of(LB, tid % NB, eb);
+  }
+  double acc[NC];
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+  for (int i = 0; i < A.nprim; ++i)
+    for (int j = 0; j < B.nprim; ++j) {
+      const double a = prim[2 * (A.prim + i)], ca = prim[2 * (A.prim + i) + 1];
+      const double b = prim[2 * (B.prim + j)], cb = prim[2 * (B.prim + j) + 1];
+      const double p = a + b;
+      const double Px = (a * A.x + b * B.x) / p, Py = (a * A.y + b * B.y) / p,
+                   Pz = (a * A.z + b * B.z) / p;
+      double Ex[LA + LB + 1], Ey[LA + LB + 1], Ez[LA + LB + 1];
+      if (act) {
+        herm1<LA, LB>(ea[0], eb[0], a, b, AB[0], Ex);
+        herm1<LA, LB>(ea[1], eb[1], a, b, AB[1], Ey);
+        herm1<LA, LB>(ea[2], eb[2], a, b, AB[2], Ez);
+      }
+      for (int k = 0; k < P.nprim; ++k) {
+        const double c = prim[2 * (P.prim + k)], cc = prim[2 * (P.prim + k) + 1];
+        const double alpha = p * c / (p + c);
+        const double X = Px - P.x, Y = Py - P.y, Z = Pz - P.z;
+        __syncthreads();  // the previous triple's R is no longer read
+        if (tid == 0) boys_dev(L, alpha * (X * X + Y * Y + Z * Z), Fn);
+        __syncthreads();
+        // levels n = L .. 0 of R^(n)_tuv, t + u + v <= L - n, ping-pong buffers
+        for (int n = L; n >= 0; --n) {
+          double* cur = Rb[n & 1];
+          const double* prv = Rb[(n + 1) & 1];
+          const int m = L - n;
+          const int cnt = (m + 1) * (m + 2) * (m + 3) / 6;
+          for (int e = tid; e < cnt; e += 64) {
+            // e-th (t, u, v) with t + u + v <= m
+            int t = 0, u = 0, v = 0, r = e;
+            for (t = 0; t <= m; ++t) {
+              const int k2 = (m - t + 1) * (m - t + 2) / 2;
+              if (r < k2) break;
+              r -= k2;
+            }
+            for (u = 0; u <= m - t; ++u) {
+              const int k1 = m - t - u + 1;
+              if (r < k1) break;
+              r -= k1;
+            }
+            v = r;
+            double val;
+            if (t + u + v == 0) {
+              val = Fn[n];
+              for (int q = 0; q < n; ++q) val *= -2.0 * alpha;
+            } else if (t > 0) {
+              val = X * prv[((t - 1) * D + u) * D + v];
+              if (t > 1) val += (t - 1) * prv[((t - 2) * D + u) * D + v];
+            } else if (u > 0) {
+              val = Y * prv[(t * D + u - 1) * D + v];
+              if (u > 1) val += (u - 1) * prv[(t * D + u - 2) * D + v];
+            } else {
+              val = Z * prv[(t * D + u) * D + v - 1];
+              if (v > 1) val += (v - 1) * prv[(t * D + u) * D + v - 2];
+            }
+            cur[(t * D + u) * D + v] = val;
+          }
+          __syncthreads();
+        }
+        if (act) {
+          const double* R = Rb[0];
+          const double f = 2.0 * pow(M_PI, 2.5) / (p * c * sqrt(p + c)) * ca * cb * cc;
+          for (int kc = 0; kc < NC; ++kc) {
+            int ec[3];
+            comp_of(LC, kc, ec);
+            double Fx[LC + 1], Fy[LC + 1], Fz[LC + 1];
+            herm1<LC, 0>(ec[0], 0, c, 0.0, 0.0, Fx);
+            herm1<LC, 0>(ec[1], 0, c, 0.0, 0.0, Fy);
+            herm1<LC, 0>(ec[2], 0, c, 0.0, 0.0, Fz);
+            double s = 0.0;
+            for (int t = 0; t <= ea[0] + eb[0]; ++t)
+              for (int u = 0; u <= ea[1] + eb[1]; ++u)
+                for (int v = 0; v <= ea[2] + eb[2]; ++v) {
+                  const double e = Ex[t] * Ey[u] * Ez[v];
+                  double g = 0.0;
+                  for (int tt = 0; tt <= ec[0]; ++tt)
+                    for (int uu = 0; uu <= ec[1]; ++uu)
+                      for (int vv = 0; vv <= ec[2]; ++vv) {
+                        const double w = Fx[tt] * Fy[uu] * Fz[vv];
+                        const double r = R[((t + tt) * D + u + uu) * D + v + vv];
+                        g += ((tt + uu + vv) & 1) ? -w * r : w * r;
+                      }
+                  s += e * g;
+                }
+            acc[kc] += f * s;
+          }
+        }
+      }
+    }
+  if (act)
+    for (int kc = 0; kc < NC; ++kc) cart[tid * NC + kc] = acc[kc];
+  __syncthreads();
+  const int nfa = A.nfunc, nfb = B.nfunc, nfc = P.nfunc;
+  const bool diag = pr.x == pr.y;
+  for (int o = tid; o < nfa * nfb * nfc; o += 64) {
+    const int fa = o / (nfb * nfc), fb = (o / nfc) % nfb, fc = o % nfc;
+    double s = 0.0;
+    for (int x = 0; x < NA; ++x) {
+      const double ta = trm[A.tr + fa * NA + x];
+      if (ta == 0.0) continue;
+      for (int y = 0; y < NB; ++y) {
+        const double tb = trm[B.tr + fb * NB + y];
+        if (tb == 0.0) continue;
+        double q = 0.0;
+        for (int z = 0; z < NC; ++z) q += trm[P.tr + fc * NC + z] * cart[(x * NB + y) * NC + z];
+        s += ta * tb * q;
+      }
+    }
+    const long m = A.func + fa, n = B.func + fb, Q = P.func + fc;
+    out[m + ld * n + slab * Q] = s;
+    if (!diag) out[n + ld * m + slab * Q] = s;
+  }
+}
+
+template <int LA, int LB, int LC>
+int launch_int3c(hipStream_t st, const DevShell* sh, const DevShell* ash, const double* prim,
+                 const double* trm, const int2* pairs, int npairs, const int* ps, int np,
+                 double* out, long ld, long slab) {
+  const long total = (long)npairs * np;
+  if (total <= 0) return DN_OK;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "three-centre integral grid too large");
+  hipLaunchKernelGGL((int3c_kernel<LA, LB, LC>), dim3((unsigned)total), dim3(64), 0, st, sh, ash,
+                     prim, trm, pairs, npairs, ps, np, out, ld, slab);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+template <int LA, int LB>
+int dispatch_lc(int lc, hipStream_t st, const DevShell* sh, const DevShell* ash, const double* prim,
+                const double* trm, const int2* pairs, int npairs, const int* ps, int np,
+                double* out, long ld, long slab) {
+  switch (lc) {
+    case 0: return launch_int3c<LA, LB, 0>(st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+    case 1: return launch_int3c<LA, LB, 1>(st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+    case 2: return launch_int3c<LA, LB, 2>(st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+    case 3: return launch_int3c<LA, LB, 3>(st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+    case 4: return launch_int3c<LA, LB, 4>(st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+  }
+  return fail(DN_ERR_ARG, "fitting shell angular momentum above 4");
+}
+
+int dispatch_int3c(int la, int lb, int lc, hipStream_t st, const DevShell* sh,
+                   const DevShell* ash, const double* prim, const double* trm,
+                   const int2* pairs, int npairs, const int* ps, int np, double* out, long ld,
+                   long slab) {
+#define DN_CASE(A_, B_)                                                                   \
+  if (la == A_ && lb == B_)                                                               \
+    return dispatch_lc<A_, B_>(lc, st, sh, ash, prim, trm, pairs, npairs, ps, np, out, ld, slab);
+  DN_CASE(0, 0) DN_CASE(1, 0) DN_CASE(0, 1) DN_CASE(1, 1) DN_CASE(2, 0) DN_CASE(0, 2)
+  DN_CASE(2, 1) DN_CASE(1, 2) DN_CASE(2, 2)
+#undef DN_CASE
+  return fail(DN_ERR_ARG, "orbital shell angular momentum above 2 on the device path");
+}
+
 }  // namespace
+
+// (mn|P) on the device into dI (device, column-major with leading dimension ld per slab
+// column and slab stride ld * nbf); the caller clears dI's pad rows.  Orbital shells up to
+// d, fitting shells up to g.
+int int3c2e_device(hipStream_t st, const dn_basis* b, const dn_basis* aux, double* dI, long ld) {
+  if (!b || !aux || !dI) return fail(DN_ERR_ARG, "int3c2e_device: null argument");
+  for (const auto& s : b->shells)
+    if (s.l > kDevMaxLab) return fail(DN_ERR_ARG, "device three-centre integrals: orbital l > 2");
+  for (const auto& s : aux->shells)
+    if (s.l > kDevMaxLc) return fail(DN_ERR_ARG, "device three-centre integrals: fitting l > 4");
+  std::vector<DevShell> hs, ha;
+  std::vector<double> prim, trm;
+  auto add = [&](const dn_basis* bs, std::vector<DevShell>& out) {
+    for (const auto& s : bs->shells) {
+      DevShell d;
+      d.x = s.ctr[0];
+      d.y = s.ctr[1];
+      d.z = s.ctr[2];
+      d.l = s.l;
+      d.nprim = s.nprim;
+      d.prim = (int)(prim.size() / 2);
+      for (int i = 0; i < s.nprim; ++i) {
+        prim.push_back(s.exps[i]);
+        prim.push_back(s.coef[i]);
+      }
+      d.func = s.offset;
+      d.nfunc = s.nfunc;
+      d.tr = (int)trm.size();
+      trm.insert(trm.end(), s.tr.begin(), s.tr.end());
+      out.push_back(d);
+    }
+  };
+  add(b, hs);
+  add(aux, ha);
+  // pairs (i >= j) by (la, lb), fitting shells by lc
+  std::vector<int2> pairs[kDevMaxLab + 1][kDevMaxLab + 1];
+  for (int i = 0; i < (int)hs.size(); ++i)
+    for (int j = 0; j <= i; ++j) pairs[hs[i].l][hs[j].l].push_back(make_int2(i, j));
+  std::vector<int> ps[kDevMaxLc + 1];
+  for (int q = 0; q < (int)ha.size(); ++q) ps[ha[q].l].push_back(q);
+  std::vector<int2> allp;
+  std::vector<int> allq;
+  size_t poff[kDevMaxLab + 1][kDevMaxLab + 1], qoff[kDevMaxLc + 1];
+  for (int la = 0; la <= kDevMaxLab; ++la)
+    for (int lb = 0; lb <= kDevMaxLab; ++lb) {
+      poff[la][lb] = allp.size();
+      allp.insert(allp.end(), pairs[la][lb].begin(), pairs[la][lb].end());
+    }
+  for (int lc = 0; lc <= kDevMaxLc; ++lc) {
+    qoff[lc] = allq.size();
+    allq.insert(allq.end(), ps[lc].begin(), ps[lc].end());
+  }
+  const size_t bytes = hs.size() * sizeof(DevShell) + ha.size() * sizeof(DevShell) +
+                       prim.size() * 8 + trm.size() * 8 + allp.size() * sizeof(int2) +
+                       allq.size() * sizeof(int) + 64 * 6;
+  char* w = nullptr;
+  DN_HIP(hipMallocAsync(reinterpret_cast<void**>(&w), bytes, st));
+  char* q = w;
+  auto put = [&](const void* src, size_t n) -> void* {
+    void* dst = q;
+    if (n) (void)hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st);
+    q += (n + 63) / 64 * 64;
+    return dst;
+  };
+  auto* dsh = static_cast<const DevShell*>(put(hs.data(), hs.size() * sizeof(DevShell)));
+  auto* dash = static_cast<const DevShell*>(put(ha.data(), ha.size() * sizeof(DevShell)));
+  auto* dprim = static_cast<const double*>(put(prim.data(), prim.size() * 8));
+  auto* dtrm = static_cast<const double*>(put(trm.data(), trm.size() * 8));
+  auto* dpairs = static_cast<const int2*>(put(allp.data(), allp.size() * sizeof(int2)));
+  auto* dq = static_cast<const int*>(put(allq.data(), allq.size() * sizeof(int)));
+  int rc = DN_OK;
+  const long slab = ld * b->nbf;
+  for (int la = 0; la <= kDevMaxLab && rc == DN_OK; ++la)
+    for (int lb = 0; lb <= kDevMaxLab && rc == DN_OK; ++lb)
+      for (int lc = 0; lc <= kDevMaxLc && rc == DN_OK; ++lc) {
+        const int npr = (int)pairs[la][lb].size(), nq = (int)ps[lc].size();
+        if (!npr || !nq) continue;
+        rc = dispatch_int3c(la, lb, lc, st, dsh, dash, dprim, dtrm, dpairs + poff[la][lb], npr,
+                            dq + qoff[lc], nq, dI, ld, slab);
+      }
+  // the host vectors are read by the copies: wait for them before they are freed
+  const hipError_t e1 = hipStreamSynchronize(st);
+  (void)hipFreeAsync(w, st);
+  const hipError_t e2 = hipStreamSynchronize(st);
+  if (rc != DN_OK) return rc;
+  if (e1 != hipSuccess || e2 != hipSuccess)
+    return fail(DN_ERR_HIP, std::string("int3c2e_device: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+  return DN_OK;
+}
+
 }  // namespace dn
 
 using namespace dn;

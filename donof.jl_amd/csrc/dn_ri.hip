@@ -220,9 +220,10 @@ int syevd(hipStream_t st, double* dG, int naux, double* lam, double* E, int* inf
 }  // namespace
 }  // namespace dn
 
-extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
-  DN_TRY(check_sys(s));
-  if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
+// The metric solve of dn_sys_ri_metric with the three-centre integrals written into the
+// device workspace by fill_I (rows ldb per column, slab stride ldb nbf; pad rows zero).
+static int ri_metric_impl(dn_sys* s, const double* G,
+                          const std::function<int(double*)>& fill_I) {
   const int naux = s->d.nbfaux;
   // rows of b in its resident layout: ldb per column of each slab, the pad rows zero (the
   // solve maps zero rows of I3c to zero rows of b)
@@ -253,9 +254,7 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
   int rc = DN_OK, host_flag = 0, host_info = 0;
   auto run = [&]() -> int {
     if (s->ldb != s->d.nbf) DN_HIP(hipMemsetAsync(dI, 0, nI * sizeof(double), s->stream));
-    const size_t nbf = s->d.nbf;
-    DN_HIP(hipMemcpy2DAsync(dI, s->ldb * sizeof(double), I3c, nbf * sizeof(double),
-                            nbf * sizeof(double), nbf * naux, hipMemcpyHostToDevice, s->stream));
+    DN_TRY(fill_I(dI));
     DN_TRY(h2d(s, dG, G, nG));
     DN_HIP(hipMemsetAsync(flag, 0, sizeof(int), s->stream));
     DN_TRY(cholesky_f64(s->stream, dG, naux, Dinv, T, flag));
@@ -320,4 +319,60 @@ extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
     s->riw_transposed = !s->eig_route;  // L^{-T} (Cholesky) or G^{-1/2} itself
   }
   return rc;
+}
+
+extern "C" int dn_sys_ri_metric(dn_sys* s, const double* G, const double* I3c) {
+  DN_TRY(check_sys(s));
+  if (!G || !I3c) return fail(DN_ERR_ARG, "null host pointer");
+  return ri_metric_impl(s, G, [&](double* dI) -> int {
+    const size_t nbf = s->d.nbf;
+    DN_HIP(hipMemcpy2DAsync(dI, s->ldb * sizeof(double), I3c, nbf * sizeof(double),
+                            nbf * sizeof(double), nbf * s->d.nbfaux, hipMemcpyHostToDevice,
+                            s->stream));
+    return DN_OK;
+  });
+}
+
+extern "C" int dn_sys_ri_from_basis(dn_sys* s, const dn_basis* basis, const dn_basis* aux) {
+  DN_TRY(check_sys(s));
+  if (!basis || !aux) return fail(DN_ERR_ARG, "null basis");
+  int nbf = 0, naux = 0;
+  DN_TRY(dn_basis_nbf(basis, &nbf));
+  DN_TRY(dn_basis_nbf(aux, &naux));
+  if (nbf != s->d.nbf || naux != s->d.nbfaux)
+    return fail(DN_ERR_ARG, "dn_sys_ri_from_basis: basis sizes differ from the system's");
+  std::vector<double> G((size_t)naux * naux);
+  DN_TRY(dn_int2c2e(aux, G.data()));
+  for (int i = 0; i < naux; ++i)  // symmetrised as integrals.jl:51 does
+    for (int j = 0; j < i; ++j) {
+      const double v = 0.5 * (G[i + (size_t)naux * j] + G[j + (size_t)naux * i]);
+      G[i + (size_t)naux * j] = G[j + (size_t)naux * i] = v;
+    }
+  return ri_metric_impl(s, G.data(), [&](double* dI) {
+    return int3c2e_device(s->stream, basis, aux, dI, s->ldb);
+  });
+}
+
+extern "C" int dn_int3c2e_device(const dn_basis* basis, const dn_basis* aux, double* I3c) {
+  if (!basis || !aux || !I3c) return fail(DN_ERR_ARG, "dn_int3c2e_device: null argument");
+  int nbf = 0, naux = 0;
+  DN_TRY(dn_basis_nbf(basis, &nbf));
+  DN_TRY(dn_basis_nbf(aux, &naux));
+  const size_t n = (size_t)nbf * nbf * naux;
+  hipStream_t st = nullptr;
+  DN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  double* d = nullptr;
+  int rc = DN_OK;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(double), st);
+  if (e == hipSuccess) {
+    rc = int3c2e_device(st, basis, aux, d, nbf);
+    if (rc == DN_OK) e = hipMemcpyAsync(I3c, d, n * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFreeAsync(d, st);
+    (void)hipStreamSynchronize(st);
+  }
+  (void)hipStreamDestroy(st);
+  if (rc != DN_OK) return rc;
+  if (e != hipSuccess) return fail(DN_ERR_HIP, std::string("dn_int3c2e_device: ") + hipGetErrorString(e));
+  return DN_OK;
 }

@@ -19,8 +19,9 @@ BOHR_PER_ANGSTROM_E_NUC = 1.88973  # utils.jl:306
 
 def compute_integrals(bset, p, aux=None, ctx=None, precision="fp64", log=print):
     """integrals.jl:1-65, RI branch: S, T, V (each symmetrised as (X + X^T)/2), H = T + V;
-    the fitting set (`aux`, default `basis.aux_basis(bset)`), G = (P|Q) symmetrised and
-    I3c = (mn|P); b = Iaux(G, I3c) on the device.  Sets p.nbfaux and returns
+    the fitting set (`aux`, default `bset.fitting_set()`), G = (P|Q) symmetrised and
+    I3c = (mn|P) (on the device for orbital shells up to d and fitting shells up to g,
+    else on the host); b = Iaux(G, I3c) on the device.  Sets p.nbfaux and returns
     (S, T, V, H, I) with I the device-resident System."""
     if not p.RI:
         raise NotImplementedError("the four-index (p.RI = false) path is out of scope; "
@@ -40,11 +41,14 @@ def compute_integrals(bset, p, aux=None, ctx=None, precision="fp64", log=print):
         log("Gaussian Type                                        = %s"
             % ("Spherical" if p.spherical else "Cartesian"))
     ha = aux.handle()
-    G = ha.int2c2e()
-    G = (G + G.T) / 2
-    I3c = h.int3c2e(ha)
     p.nbfaux = ha.nbf
-    I = api.Iaux(G, I3c, p, ctx=ctx, precision=precision)
+    if max(s.l for s in bset.basis) <= 2 and max(s.l for s in aux.basis) <= 4:
+        # (mn|P) on the device straight into the metric solve; (P|Q) on the host
+        I = api.System(p, ctx, precision).ri_from_basis(h, ha)
+    else:
+        G = ha.int2c2e()
+        G = (G + G.T) / 2
+        I = api.Iaux(G, h.int3c2e(ha), p, ctx=ctx, precision=precision)
     return S, T, V, H, I
 
 

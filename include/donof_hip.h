@@ -357,6 +357,14 @@ int dn_int2c2e(const dn_basis* aux, double* G);
 int dn_int3c2e(const dn_basis* basis, const dn_basis* aux, double* I3c);
 /* (mn|ls), nbf^4 (ERI_2e4c, integrals.jl:16; the non-RI integrals) */
 int dn_int4c2e(const dn_basis* basis, double* eri);
+/* (mn|P) computed on the device (orbital shells up to d, fitting shells up to g), copied
+ * to the host array I3c (nbf x nbf x naux); the same integrals as dn_int3c2e. */
+int dn_int3c2e_device(const dn_basis* basis, const dn_basis* aux, double* I3c);
+/* The RI setup on the device from a basis and a fitting basis: G = (P|Q) on the host
+ * (symmetrised), (mn|P) on the device straight into the metric solve's workspace, then
+ * b = I3c G^{-1/2} as dn_sys_ri_metric (integrals.jl:47-53, 68-91).  The system's nbf and
+ * nbfaux must be the two bases' sizes. */
+int dn_sys_ri_from_basis(dn_sys* sys, const dn_basis* basis, const dn_basis* aux);
 /* Derivatives for the nuclear gradients (GaussianBasis' ∇overlap, ∇kinetic, ∇nuclear,
  * ∇ERI_2e2c, ∇ERI_2e3c as grads.jl:13-19,88-89 calls them): d/dR of each integral for
  * the position R (bohr) of atom `atom`, the Cartesian component last (nbf x nbf x 3,
