@@ -395,19 +395,24 @@ def grouped_leg(api, synthetic, steps, warmup, seed, nsys=64, nbf_range=(48, 400
 
 
 def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8,
-                modes=("grouped", "streams")):
+                modes=("grouped", "streams", "grouped-split")):
     """Served solves: Poisson arrivals at `rate`/s of GNOF orbital optimisations (ADAM, at
     most 200 iterations, stop when 2 max|g| < 1e-4), nbf log-uniform in [50, 600],
     naux = 4 nbf, nbf5 = ceil(nbf/3); admission by the device's HBM budget.  The lock-step
     grouped engines are the served path; one stream per solve is timed beside it.
     Latency is arrival to exit (converged or at the 200-iteration cap); the converged
-    count is reported (the random synthetic Hamiltonians rarely reach 1e-4 in 200)."""
+    count is reported (the random synthetic Hamiltonians rarely reach 1e-4 in 200).
+    Mode "grouped-split" is the grouped server with the single-system solves (nbf >= 160)
+    at split precision; its energies are compared with the FP64 grouped run's."""
     out = {"requests": n, "arrival_rate_per_s": rate, "nbf_range": [50, 600],
            "max_live": max_live, "max_iter": 200, "gtol": 1e-4}
+    energies = {}
     for mode in modes:
         reqs = serve.synthetic_requests(n, rate, seed=seed, nbf_min=50, nbf_max=600,
                                         max_iter=200, gtol=1e-4)
-        srv = serve.Server(hbm_budget=0, max_live=max_live, mode=mode)
+        base, _, prec = mode.partition("-")
+        srv = serve.Server(hbm_budget=0, max_live=max_live, mode=base,
+                           precision=prec or "fp64")
         out["hbm_budget_bytes"] = srv.ctx.hbm_budget
         t0 = time.perf_counter()
         res = srv.run(reqs, log=lambda m: log("serving (%s): %s" % (mode, m)))
@@ -418,6 +423,11 @@ def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8,
             % (mode, st["solves"], wall, st["p50_s"] or 0.0, st["p99_s"] or 0.0))
         errs = [r.error for r in res if r.error]
         out[mode] = {"wall_s": wall, "errors": len(errs), **st}
+        energies[mode] = {r.rid: (r.nbf, r.E) for r in res if not r.error}
+        if prec == "split":
+            out[mode]["split_min_nbf"] = srv.split_min_nbf
+            out[mode]["split_solves"] = sum(srv.precision_of(r.nbf) == serve.DN_SPLIT_F16
+                                            for r in res)
         if errs:
             out[mode]["first_error"] = errs[0]
             log("serving (%s): %d errors, first: %s" % (mode, len(errs), errs[0]))
@@ -427,6 +437,14 @@ def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8,
             out[mode]["phase_s"] = dict(srv.phases)
             log("serving (grouped) phases: %s" % ", ".join(
                 "%s %.2f s" % kv for kv in sorted(srv.phases.items())))
+    if "grouped" in energies and "grouped-split" in energies:
+        # the same requests at FP64 and split precision: 200 ADAM steps each, so the
+        # difference holds the split Lagrangian's error carried along two trajectories
+        a, b = energies["grouped"], energies["grouped-split"]
+        d = [abs(a[k][1] - b[k][1]) for k in a if k in b and a[k][0] >= 160]
+        if d:
+            out["grouped-split"]["max_abs_dE_vs_fp64_Ha"] = max(d)
+            out["grouped-split"]["median_abs_dE_vs_fp64_Ha"] = float(np.median(d))
     return out
 
 
@@ -517,7 +535,7 @@ def main():
                     help="run only the served-solves leg and print its JSON (the main run "
                          "starts this in a child process under a time limit)")
     ap.add_argument("--served-timeout", type=float, default=300.0)
-    ap.add_argument("--served-modes", default="grouped,streams",
+    ap.add_argument("--served-modes", default="grouped,streams,grouped-split",
                     help="modes of the served-solves leg (--served-only)")
     ap.add_argument("--served-requests", type=int, default=100)
     args = ap.parse_args()

@@ -17,6 +17,12 @@ join at the next step (the group is rebuilt).  Size classes keep a small solve f
 advancing at the pace of a large one.  mode "streams": one host thread per live solve,
 each running the whole loop on its own HIP stream (dn_orbopt_resident).
 
+precision "split": solves of nbf >= split_min_nbf that run the single-system loop (every
+solve in mode "streams"; in mode "grouped" those of the classes capped at one) hold b as
+FP16 hi/lo planes only and run both transforms of every Lagrangian on FP16 MFMA (the
+split-precision Lagrangian), with rotation, ADAM step and energy sum in FP64; lock-step
+groups stay FP64 (their steps are launch-bound, not transform-bound).
+
 Every kernel of a solve is deterministic, reads only its own system's buffers, and each
 GEMM of a system uses one tile family whose summation order does not depend on the tile
 shape (split-K boundaries depend only on the system's sizes), so a result does not depend
@@ -35,7 +41,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import api, synthetic
-from ._lib import DnError
+from ._lib import DN_FP64, DN_SPLIT_F16, DnError
 
 
 @dataclass
@@ -90,12 +96,13 @@ def synthetic_requests(n, rate, seed=0, nbf_min=50, nbf_max=600, max_iter=200, g
 _STRUCT_LOCK = threading.Lock()
 
 
-def solve_one(req, ctx):
-    """Run one request on its own resident system; returns (E, nit, success)."""
+def solve_one(req, ctx, precision=DN_FP64):
+    """Run one request on its own resident system (FP64 or split precision); returns
+    (E, nit, success)."""
     sysd = synthetic.SyntheticSystem(req.nbf, req.naux, req.nbf5, req.seed)
     p = sysd.p
     p.maxloop, p.alpha, p.threshgorb = req.max_iter, req.alpha, req.gtol
-    dev = api.System(p, ctx=ctx)
+    dev = api.System(p, ctx=ctx, precision=precision)
     try:
         dev.generate_b(req.seed)
         E, C, nit, ok = api.orbopt("ADAM", sysd.gamma, sysd.C, sysd.H, dev, p)
@@ -248,9 +255,14 @@ class Server:
     kernels (mode "grouped") or by one thread and stream each (mode "streams")."""
 
     def __init__(self, hbm_budget=0, max_live=8, device=0, mode="grouped",
-                 size_classes=(80, 120, 160, 320), group_caps=(8, 8, 8, 1, 1), preparers=4):
+                 size_classes=(80, 120, 160, 320), group_caps=(8, 8, 8, 1, 1), preparers=4,
+                 precision="fp64", split_min_nbf=160):
         if mode not in ("grouped", "streams"):
             raise ValueError("mode must be 'grouped' or 'streams'")
+        if precision not in ("fp64", "split"):
+            raise ValueError("precision must be 'fp64' or 'split'")
+        self.precision = precision
+        self.split_min_nbf = int(split_min_nbf)
         # nbf edges of the engines: a lock-step group advances at the pace of its largest
         # member, so the classes below 160 are narrow (100 requests at 2/s, nbf in
         # [50, 600]: p50 0.63 s with one class below 160, 0.55 s with two, 0.53 s with
@@ -275,6 +287,21 @@ class Server:
         self._lock = threading.Lock()
         self._live = 0
         self._reserved = 0
+
+    def precision_of(self, nbf):
+        """The precision a solve of this size runs at (see the module docstring)."""
+        if self.precision != "split" or nbf < self.split_min_nbf:
+            return DN_FP64
+        if self.mode == "grouped":
+            k = sum(1 for edge in self.size_classes if nbf >= edge)
+            if self.group_caps[k] != 1:
+                return DN_FP64          # a lock-step group (FP64)
+        return DN_SPLIT_F16
+
+    def footprint(self, req):
+        """HBM bytes the request's system reserves at its precision."""
+        return api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5),
+                             self.precision_of(req.nbf))
 
     def _fits(self, bytes_needed):
         # every live solve holds its full footprint until it finishes
@@ -345,7 +372,7 @@ class Server:
             err = ""
             E, nit, ok = float("nan"), 0, False
             try:
-                E, nit, ok = solve_one(req, self.ctx)
+                E, nit, ok = solve_one(req, self.ctx, self.precision_of(req.nbf))
             except Exception as e:  # noqa: BLE001 - reported as the request's error
                 err = "%s: %s" % (type(e).__name__, e)
             t = time.perf_counter() - t0
@@ -393,7 +420,7 @@ class Server:
                     waiting.append(pending.pop(0))
                 while waiting:   # FIFO admission while slots and memory allow
                     req = waiting[0]
-                    need = api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5))
+                    need = self.footprint(req)
                     if need > self.ctx.hbm_budget:
                         waiting.pop(0)
                         with lock:
@@ -447,7 +474,7 @@ class Server:
             err = ""
             E, nit, ok = float("nan"), 0, False
             try:
-                E, nit, ok = solve_one(req, self.ctx)
+                E, nit, ok = solve_one(req, self.ctx, self.precision_of(req.nbf))
             except DnError as e:
                 err = str(e)
             t = time.perf_counter() - t0
@@ -464,7 +491,7 @@ class Server:
             # FIFO admission while slots and memory allow
             while waiting:
                 req = waiting[0]
-                need = api.footprint(synthetic.synthetic_param(req.nbf, req.naux, req.nbf5))
+                need = self.footprint(req)
                 with self._lock:
                     ok = self._live < self.max_live and self._fits(need)
                     if ok:

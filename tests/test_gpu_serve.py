@@ -82,6 +82,35 @@ def test_alone_equals_in_the_mix(mode):
 
 
 @pytest.mark.parametrize("mode", ["grouped", "streams"])
+def test_split_served_alone_equals_in_the_mix(mode):
+    """precision "split": the single-system solves (here nbf >= 60; in mode "grouped" the
+    class from 60 up is capped at one) hold FP16 planes and run split Lagrangians; each
+    result is bitwise the same alone and in the mix, and close to the FP64 solve's."""
+    reqs = requests()
+    kw = dict(precision="split", split_min_nbf=60, size_classes=(60,), group_caps=(8, 1))
+    alone = {}
+    for r in reqs:
+        (a,) = _serve([r], mode, max_live=1, **kw)
+        assert a.error == "", a.error
+        alone[r.rid] = (a.E, a.nit, a.success)
+    res = _serve(reqs, mode, max_live=4, **kw)
+    for r in res:
+        assert r.error == "", (r.rid, r.error)
+        assert (r.E, r.nit, r.success) == alone[r.rid]       # bitwise the same
+    fp64 = {r.rid: r.E for r in _serve(reqs, mode, max_live=4)}
+    n_split = 0
+    for r in res:
+        d = abs(r.E - fp64[r.rid])
+        print("split served rid %d nbf %d E %.10f |dE| vs fp64 %.2e" % (r.rid, r.nbf, r.E, d))
+        if r.nbf >= 60:
+            n_split += 1
+            assert d <= 1e-5 * max(1.0, abs(fp64[r.rid]))
+        else:
+            assert d == 0.0                                  # FP64 either way
+    assert n_split >= 1
+
+
+@pytest.mark.parametrize("mode", ["grouped", "streams"])
 def test_admission_by_hbm_budget(mode):
     reqs = requests(5, rate=0.0)             # all arrive at t = 0
     fp = {r.rid: api.footprint(synthetic.synthetic_param(r.nbf, r.naux, r.nbf5)) for r in reqs}

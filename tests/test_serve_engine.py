@@ -67,17 +67,19 @@ def fake_device(monkeypatch):
     monkeypatch.setattr(serve, "start_solve", lambda req, ctx: (FakeDev(req), object()))
     monkeypatch.setattr(serve, "solve_one", _fake_solo)
     monkeypatch.setattr(serve.api, "OGroup", FakeGroup)
-    monkeypatch.setattr(serve.api, "footprint", lambda p: 10 ** 9)
+    monkeypatch.setattr(serve.api, "footprint", lambda p, precision=0: 10 ** 9)
     FakeGroup.fail_rids = set()
     FakeGroup.sizes = []
 
 
 SOLOS = []
+SOLO_PRECISION = {}
 
 
-def _fake_solo(req, ctx):
+def _fake_solo(req, ctx, precision=serve.DN_FP64):
     """A solve of a class with cap one (the single-system loop): max_iter iterations."""
     SOLOS.append(req.nbf)
+    SOLO_PRECISION[req.rid] = precision
     if req.rid in FakeGroup.fail_rids:
         raise DnError(6, "non-finite energy or orbital gradient")
     return -1.0 * req.nbf, req.max_iter, False
@@ -88,6 +90,8 @@ def _server(max_live=4, classes=(160, 320), caps=(8, 2, 2)):
     srv.ctx = FakeCtx()
     srv.max_live = max_live
     srv.mode = "grouped"
+    srv.precision = "fp64"
+    srv.split_min_nbf = 160
     srv.size_classes = tuple(classes)
     srv.group_caps = tuple(caps)
     srv.results = []
@@ -156,3 +160,33 @@ def test_cap_one_runs_the_single_system_loop(fake_device):
     assert all(not r.error and r.nit == 5 for r in res)
     assert sorted(SOLOS) == sorted(n for n in (r.nbf for r in res) if n >= 160)
     assert all(max(g) < 160 for g in FakeGroup.sizes)
+
+
+def test_split_precision_runs_the_single_system_solves(fake_device):
+    """precision "split": solves of nbf >= split_min_nbf that run the single-system loop
+    (classes capped at one) get split precision; lock-step groups and smaller solves stay
+    FP64."""
+    SOLOS.clear()
+    SOLO_PRECISION.clear()
+    srv = _server(max_live=4, caps=(8, 1, 1))
+    srv.precision, srv.split_min_nbf = "split", 300
+    res = srv.run(_requests(10))
+    assert all(not r.error and r.nit == 5 for r in res)
+    by_rid = {r.rid: r.nbf for r in res}
+    assert SOLO_PRECISION and all(
+        prec == (serve.DN_SPLIT_F16 if by_rid[rid] >= 300 else serve.DN_FP64)
+        for rid, prec in SOLO_PRECISION.items())
+    assert any(prec == serve.DN_SPLIT_F16 for prec in SOLO_PRECISION.values())
+    # a class with a lock-step cap above one stays FP64 whatever its size
+    srv2 = _server(caps=(8, 8, 1))
+    srv2.precision, srv2.split_min_nbf = "split", 100
+    assert srv2.precision_of(250) == serve.DN_FP64          # class [160, 320), cap 8
+    assert srv2.precision_of(400) == serve.DN_SPLIT_F16     # class >= 320, cap 1
+    assert srv2.precision_of(90) == serve.DN_FP64           # below split_min_nbf
+    srv2.mode = "streams"
+    assert srv2.precision_of(250) == serve.DN_SPLIT_F16     # every solve is single-system
+
+
+def test_server_rejects_unknown_precision():
+    with pytest.raises(ValueError):
+        serve.Server.__init__(serve.Server.__new__(serve.Server), precision="bf16")
