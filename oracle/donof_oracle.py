@@ -20,13 +20,15 @@ How each part is pinned (tests/test_oracle_*.py):
                     functional to the Loewdin-Shull two-electron energy; the
                     cumulant sum rule sum_q Delta_qp = n_p (1 - n_p); GNOF with the
                     dynamic factor quenched equals PNOF7; the dynamic factor of a pair
-                    depends on that pair only; symmetry.  der_* checked by finite
-                    differences of cj12/ck12 in gamma through the occupations; PNOF4's
-                    by finite differences of the factors of Pi combined with the
-                    denominator pnof.jl:255 writes.  Parity unpinned: the signs of
-                    GNOF's dynamic beta-cwo / cwo-cwo terms and the block pattern of
-                    its ista = 0 static term (pnof.jl:540-578) are pinned only by this
-                    transcription and by finite-difference consistency.
+                    depends on that pair only; GNOF's dynamic terms are -(c c^T +
+                    n_d n_d^T) on the strong-weak and weak-weak interpair blocks with c
+                    the Loewdin-Shull-signed dynamic amplitudes (one sign vector for
+                    both blocks) and lower the energy; symmetry.  der_* checked by
+                    finite differences of cj12/ck12 in gamma through the occupations;
+                    PNOF4's by finite differences of the factors of Pi combined with
+                    the denominator pnof.jl:255 writes.  Parity unpinned: the block
+                    pattern of GNOF's ista = 0 static term (pnof.jl:540-578) is pinned
+                    only by this transcription and by finite-difference consistency.
   ocupacion_*       pair sums == 1 (softmax), finite-difference Jacobians.
   calcoccg          finite differences of calcocce.
   orbopt_*          deterministic recurrence; replayed step by step.

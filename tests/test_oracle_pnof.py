@@ -191,6 +191,49 @@ def test_gnof_quenched_dynamics_is_pnof7(oracle, ista):
     assert np.abs(ck8 - ck7).max() < 1e-15
 
 
+@pytest.mark.parametrize("seed", [2, 9])
+def test_gnof_dynamic_terms_carry_the_pairing_signs(oracle, seed):
+    """The dynamic part of GNOF's exchange-type coefficients (GNOF minus PNOF7 at the same
+    occupations, ista 1, where the quenched GNOF is PNOF7 exactly) acts only between a
+    strong orbital and the weak orbitals of another pair and between weak orbitals of
+    different pairs, and there equals -(c c^T + n_d n_d^T) with c the dynamic amplitudes
+    signed as the two-electron (Loewdin-Shull) amplitudes: c = +sqrt(n_d) on strong and
+    -sqrt(n_d) on weak orbitals; n_d = n exp(-((1 - n_g) / h_cut)^2) with n_g the strong
+    occupation of the orbital's pair (pnof.jl:506-519).  One sign vector for both blocks
+    fixes their relative sign; the energy it adds with the molecule's K >= 0 is negative
+    (dynamic correlation lowers the energy), which fixes the absolute one."""
+    s, p = system(8, 1, nbf=18, naux=40, nbf5=12, ndoc=3, seed=seed)
+    J, K, Hc = jk(oracle, s)
+    rng = np.random.default_rng(seed)
+    # strong occupations within a few h_cut of 1, where the dynamic factor is of order one
+    n, _ = oracle.ocupacion(0.3 * rng.standard_normal(p.nv) - 5.0, p)
+    _, ck8 = oracle.pnof_selector(n, p)
+    p.ipnof, p.ista = 7, 0
+    cj7, ck7 = oracle.pnof_selector(n, p)
+    p.ipnof, p.ista = 8, 1
+    D = ck8 - ck7
+    pair = np.full(p.nbf5, -1)
+    strong = np.zeros(p.nbf5, dtype=bool)
+    for l in range(1, p.ndoc + 1):
+        g = p.no1 + l - 1
+        pair[g] = l
+        strong[g] = True
+        pair[oracle.pair_weak(p, l)] = l
+    F = np.exp(-((1.0 - n[p.no1:p.no1 + p.ndoc]) / p.h_cut) ** 2)
+    assert F.min() > 0.05
+    nd = n * F[pair - 1]
+    c = np.where(strong, 1.0, -1.0) * np.sqrt(nd)
+    inter = pair[:, None] != pair[None, :]
+    dyn = inter & ~(strong[:, None] & strong[None, :])
+    expect = np.where(dyn, -(np.outer(c, c) + np.outer(nd, nd)), 0.0)
+    assert np.abs(D - expect).max() < 1e-15
+    # the energy the dynamic terms add: negative for the molecule-like K >= 0
+    assert K.min() >= 0.0
+    E8 = oracle.calce(n, cj7, ck8, J, K, Hc, p)
+    E7 = oracle.calce(n, cj7, ck7, J, K, Hc, p)
+    assert E8 - E7 < -1e-4
+
+
 @pytest.mark.parametrize("ipnof", [8, 9])
 @pytest.mark.parametrize("moved", [1, 2, 3])
 def test_dynamic_factor_is_local_to_its_pair(oracle, ipnof, moved):
