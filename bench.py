@@ -555,12 +555,14 @@ def main():
     ap.add_argument("--served-modes", default="grouped,streams,grouped-split",
                     help="modes of the served-solves leg (--served-only)")
     ap.add_argument("--served-requests", type=int, default=100)
+    ap.add_argument("--served-rate", type=float, default=2.0,
+                    help="Poisson arrival rate of the served-solves leg (requests per second)")
     args = ap.parse_args()
     if args.served_only:
         import __graft_entry__
         __graft_entry__.load_package()
         from donof_amd import api, serve
-        print(json.dumps(serving_leg(api, serve, n=args.served_requests,
+        print(json.dumps(serving_leg(api, serve, n=args.served_requests, rate=args.served_rate,
                                      modes=tuple(args.served_modes.split(",")))))
         return 0
     if args.gpus != 1:
