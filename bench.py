@@ -205,8 +205,9 @@ def extra_legs(api, synthetic, steps, warmup, seed):
         "gnof_energy_abs_err_Ha": float(abs(E16 - E64)),
     }
     # the Lagrangian (utils.jl:1-59) at the same orbitals: FP64, and split precision (both
-    # transforms on FP16 planes; C^J/C^K terms and the contraction FP64), host boundary
-    # included (C, H, n, cj12, ck12 up; elag, H_mat down)
+    # transforms and the contraction Z = sum bpn bmat on FP16 planes; the C^J/C^K terms,
+    # elag += C^T Z and the energy FP64), host boundary included (C, H, n, cj12, ck12 up;
+    # elag, H_mat down)
     out["lagrangian"] = {
         "fp64_ms": lag_ms64, "split_ms": lag_ms16, "split_speedup": lag_ms64 / lag_ms16,
         "elag_max_rel_err": float(np.abs(el16 - el64).max()) / scale,

@@ -24,6 +24,8 @@ int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int na
           const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w);
 int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat);
+int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
+                 const double* ck12, const double* w, double* bmat, unsigned* maxbits);
 int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nbf5,
              double* elag);
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
@@ -47,6 +49,10 @@ int split_sym_planes(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, int K
 int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float16* chi,
              const _Float16* clo, const float* cscale_dev, const float* bpn, float* bpq,
              const float* qmax, float* qs, float* qinv);
+size_t split_contract_bytes(int nbf, int nbf5, int naux);
+int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
+                   const double* bmat, const float* qmax, float* zscale, void* work,
+                   size_t work_bytes, double* Z);
 int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
                     double* part, size_t part_elems, double* part2, size_t part2_elems,
                     double* J, double* K);
@@ -80,8 +86,8 @@ struct Layout {
   static size_t f8(size_t floats) { return (floats + 1) / 2; }   // floats -> doubles
   size_t split_doubles() const {
     if (!bplane) return 0;  // FP64 system
-    // + the two doubles of the C5 scale pair (cscale)
-    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 2;
+    // + two doubles each for the C5 scale pair (cscale) and the contraction scales (zscale)
+    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 4;
   }
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 3 + n + big + bpn + cp + bmat + 2 * xw +
@@ -135,6 +141,8 @@ Layout layout_of(const dn_dims& d, int precision) {
     L.f_big = std::max(nbf5 * nbf, (size_t)round_up(d.nbf5, 128) * round_up(d.nbf, 32)) * naux;
     L.f_bpq = nbf5 * nbf5 * naux;
     L.f_naux = naux;
+    // the contraction's planes and FP32 partials live where the FP64 path keeps bpn
+    L.bpn = std::max(L.bpn, (split_contract_bytes(d.nbf, d.nbf5, d.nbfaux) + 7) / 8);
   }
   return L;
 }
@@ -449,9 +457,8 @@ int lagrange_split_transforms(dn_sys* s) {
                   s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
   DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
                   s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
+  // b_pq in FP64 for the C^J/C^K terms; the contraction reads the FP32 bpn itself
   const long nn = (long)d.nbf5 * d.nbf5;
-  DN_TRY(transpose_f32_f64(s->stream, s->bpn32, d.nbf5, d.nbf, d.nbf5, (long)d.nbf5 * d.nbf,
-                           s->bpn, s->ldb, (long)s->ldb * d.nbf5, d.nbfaux));
   return transpose_f32_f64(s->stream, s->bpq32, d.nbf5, d.nbf5, d.nbf5, nn, s->bt, d.nbf5, nn,
                            d.nbfaux);
 }
@@ -472,15 +479,26 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   DN_TRY(hc_products(s, true, false));
   DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
                s->X, s->w));
-  DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
+  if (s->precision == DN_SPLIT_F16) {
+    DN_TRY(lag_bmat_max(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w,
+                        s->bmat, reinterpret_cast<unsigned*>(s->zscale) + 3));
+  } else {
+    DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
+  }
   DN_TRY(lag_init(s->stream, s->Hmat, s->n, d.nbf, d.nbf5, s->elag));
   // Z[n,b] = sum_Q sum_q bpn[n,q,Q] bmat[q,b,Q], then elag += C^T Z
-  const Gemm g = lag_contract_gemm(s);
-  const int used = (g.nbatch + g.bps - 1) / g.bps;
-  if ((size_t)used * d.nbf * d.nbf5 > s->part_elems) return fail(DN_ERR_ARG, "partials overflow");
-  DN_TRY(gemm_f64(s->stream, g));
-  DN_TRY(reduce_splits(s->stream, s->part, (long)d.nbf * d.nbf5, used, d.nbf, d.nbf5, d.nbf,
-                       s->Z, d.nbf, 0.0));
+  if (s->precision == DN_SPLIT_F16) {
+    DN_TRY(split_contract(s->stream, d.nbf, d.nbf5, d.nbfaux, s->bpn32, s->bmat, s->qmax,
+                          s->zscale, s->bpn, s->bpn_elems * sizeof(double), s->Z));
+  } else {
+    const Gemm g = lag_contract_gemm(s);
+    const int used = (g.nbatch + g.bps - 1) / g.bps;
+    if ((size_t)used * d.nbf * d.nbf5 > s->part_elems)
+      return fail(DN_ERR_ARG, "partials overflow");
+    DN_TRY(gemm_f64(s->stream, g));
+    DN_TRY(reduce_splits(s->stream, s->part, (long)d.nbf * d.nbf5, used, d.nbf, d.nbf5, d.nbf,
+                         s->Z, d.nbf, 0.0));
+  }
   DN_TRY(gemm_f64(s->stream, lag_final_gemm(s)));
   DN_TRY(energy_elec(s->stream, s->elag, s->Hmat, s->n, d.nbf, d.nbf5, s->scal));
   return DN_OK;
@@ -743,6 +761,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
   A(L.nbf_nbf5, &s->HC);
   A(L.nbf_nbf5, &s->Z);
   A(L.bpn, &s->bpn);
+  s->bpn_elems = L.bpn;
   A(L.big, &s->bt);
   A(L.bmat, &s->bmat);
   A(L.xw, &s->X);
@@ -793,6 +812,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     A(Layout::f8(L.f_naux), &q); s->qinv = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->lhmax = reinterpret_cast<float*>(q);
     A(2, &q); s->cscale = reinterpret_cast<float*>(q);  // scale, 1/scale, max bits
+    A(2, &q); s->zscale = reinterpret_cast<float*>(q);
     if (rc == DN_OK) {
       // plane padding must read as zero
       hipError_t e1 = hipMemsetAsync(s->bhi, 0, Layout::d8(L.bplane) * 8, s->stream);

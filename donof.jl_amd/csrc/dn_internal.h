@@ -105,6 +105,9 @@ struct SplitGemm {
   long out_ld, out_sb;
   int out_kt;
   const float* out_scale;
+  // batch-reduce kernels (the Lagrangian contraction): output z sums batches
+  // [z bps, min((z + 1) bps, nbatch)), A with batch stride sA (halves)
+  int bps;
 };
 
 // One problem of a grouped split-precision launch (offsets as for GroupedGemm; nsplit is
@@ -393,6 +396,7 @@ struct dn_sys {
                             //                 pad row zero
   double* bt = nullptr;     // nbf5*nbf5*naux: b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q) (general
                             //   transform, the Lagrangian) or W^T (symmetric J/K build)
+  size_t bpn_elems = 0;     // doubles of bpn (split systems keep the contraction planes there)
   double* bmat = nullptr;   // nbf5*nbf5*naux: [q,b,Q] at b + nbf5 (q + nbf5 Q), the C^K/C^J
                             //   right operand of the Lagrangian GEMM
   double* X = nullptr;      // nbf5*naux     : b_pp diagonal per Q
@@ -438,6 +442,9 @@ struct dn_sys {
   float* qmax = nullptr;    // naux
   float* qs = nullptr;      // naux
   float* qinv = nullptr;    // naux
+  // split Lagrangian contraction: (scale of bpn, 1 / (scale of bpn * scale of bmat), scale
+  // of bmat, max |bmat| as float bits); its planes and FP32 partials live in bpn's space
+  float* zscale = nullptr;
   bool b_ready = false;
   // an ADAM-family run advanced step by step (dn_orbopt_start / dn_ogroup_step)
   struct OptRun {

@@ -108,6 +108,33 @@ __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict_
     bmat[e] = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
 }
 
+// lag_bmat_kernel that also leaves max |bmat| (as FP32) in *maxbits, which
+// lag_bmat_max clears first: the scale of the split-precision contraction's bmat planes
+__global__ __launch_bounds__(256) void lag_bmat_max_kernel(const double* __restrict__ bt,
+                                                           long slice, int nbf, int nbf5,
+                                                           int naux,
+                                                           const double* __restrict__ ck12,
+                                                           const double* __restrict__ w,
+                                                           double* __restrict__ bmat,
+                                                           unsigned* __restrict__ maxbits) {
+  __shared__ float red[4];
+  const long total = (long)nbf5 * nbf5 * naux;
+  float m = 0.f;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const double v = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
+    bmat[e] = v;
+    m = fmaxf(m, (float)fabs(v));  // the scale takes 1.0001 times it: rounding is covered
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(maxbits, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+__global__ void clear_u32_kernel(unsigned* p) { *p = 0u; }
+
 __global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
                                 int nbf, int nbf5, double* __restrict__ elag) {
   const long total = (long)nbf * nbf;
@@ -213,6 +240,16 @@ int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int
   const long blocks = ((long)nbf5 * nbf5 * naux + 255) / 256;
   hipLaunchKernelGGL(lag_bmat_kernel, dim3((unsigned)std::min(blocks, 8192L)), dim3(256), 0, s,
                      bt, slice, nbf, nbf5, naux, ck12, w, bmat);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
+                 const double* ck12, const double* w, double* bmat, unsigned* maxbits) {
+  hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(1), 0, s, maxbits);
+  const long blocks = ((long)nbf5 * nbf5 * naux + 255) / 256;
+  hipLaunchKernelGGL(lag_bmat_max_kernel, dim3((unsigned)std::min(blocks, 8192L)), dim3(256), 0,
+                     s, bt, slice, nbf, nbf5, naux, ck12, w, bmat, maxbits);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

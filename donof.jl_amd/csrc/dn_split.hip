@@ -19,6 +19,7 @@
 // per-k-tile stride R * 32 is the operand's lda / ldb; b planes hold one such block per
 // Q (batch stride sB = nbf * Kpad), with k padded to Kpad = 32 * ceil(nbf / 32) by zeros.
 #include <cmath>
+#include <type_traits>
 
 #include "dn_internal.h"
 #include "dn_sched.h"
@@ -575,7 +576,8 @@ __device__ __forceinline__ void split_epilogue_planes(const SplitGemm& g, v4f (&
     }
 }
 
-template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false>
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false,
+          bool BR = false>
 __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int nt, int z,
                                                unsigned char* smem) {
   constexpr int NW = WGM * WGN;
@@ -588,12 +590,16 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
   static_assert(NST >= 2 && NST <= 4 && (NST - 2) * LW < 64, "pipeline depth");
   static_assert(FM >= 1 && FN >= 1, "wave tile");
   static_assert(split_lds_bytes<BM, BN, NST>() == NST * STAGE, "LDS size");
+  static_assert(!BR || (!TRIB && !OUTP), "batch-reduce tiles: plain products, FP32 output");
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
   const int m0 = mt * BM, n0 = nt * BN;
   const int kt0 = TRIB ? n0 / SB_K : 0;
   const int nkt = (g.K + SB_K - 1) / SB_K - kt0;
+  // BR: output z sums batches q0 .. q1 - 1, the k tiles of each in turn (T steps in all)
+  const int q0 = BR ? z * g.bps : 0, q1 = BR ? min(q0 + g.bps, g.nbatch) : 0;
+  const int T = BR ? (q1 > q0 ? (q1 - q0) * nkt : 0) : nkt;
 
   // this wavefront's load instructions: q = wave + NW i; A for q < NA, else B; each moves
   // the 16-row block grp of one plane
@@ -612,12 +618,17 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
       src[i] = (p ? g.alo : g.ahi) + (long)(m0 + r) * 32 + 8 * c;
     } else {
       const int col = min(n0 + r, g.N - 1);
-      src[i] = (p ? g.blo : g.bhi) + (long)z * g.sB + (long)col * 32 + 8 * c;
+      src[i] = (p ? g.blo : g.bhi) + (BR ? 0L : (long)z * g.sB) + (long)col * 32 + 8 * c;
     }
     dst[i] = (isA ? 0 : A_BYTES) + p * rows * 64 + grp * 1024;
   }
   auto issue = [&](int t) {
-    const long kofsA = (long)(kt0 + t) * g.lda, kofsB = (long)(kt0 + t) * g.ldb;
+    long kofsA = (long)(kt0 + t) * g.lda, kofsB = (long)(kt0 + t) * g.ldb;
+    if constexpr (BR) {
+      const int Q = q0 + t / nkt, kt = t % nkt;
+      kofsA = (long)Q * g.sA + (long)kt * g.lda;
+      kofsB = (long)Q * g.sB + (long)kt * g.ldb;
+    }
     unsigned char* base = smem + (t % NST) * STAGE;
 #if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: the host pass only needs the stub
 #pragma unroll
@@ -639,11 +650,11 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
 
 #pragma unroll
   for (int s = 0; s < NST - 1; ++s)
-    if (s < nkt) issue(s);
+    if (s < T) issue(s);
   const int fr = lane & 15, fc = lane >> 4;
-  for (int t = 0; t < nkt; ++t) {
+  for (int t = 0; t < T; ++t) {
     // this wavefront's loads of tile t have landed once at most `ahead` tiles' remain
-    const int ahead = min(NST - 2, nkt - 1 - t);
+    const int ahead = min(NST - 2, T - 1 - t);
     if (ahead >= 2) wait_vmcnt<2 * LW>();
     else if (ahead == 1) wait_vmcnt<LW>();
     else wait_vmcnt<0>();
@@ -655,7 +666,7 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    if (t + NST - 1 < nkt) issue(t + NST - 1);
+    if (t + NST - 1 < T) issue(t + NST - 1);
     const unsigned char* A = smem + (t % NST) * STAGE;
     const unsigned char* B = A + A_BYTES;
     v8h bh[FN], bl[FN], ah[FM], al[FM];
@@ -725,17 +736,19 @@ __device__ __forceinline__ void split_tile_lds(const SplitGemm& g, int mt, int n
   }
 }
 
-template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false>
+template <int BM, int BN, int WGM, int WGN, int NST, bool TRIB, bool OUTP = false,
+          bool BR = false>
 __global__ __launch_bounds__(64 * WGM * WGN) void split_lds_kernel(SplitGemm g, int tiles_m,
                                                                   int tiles_n) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[split_lds_bytes<BM, BN, NST>()];
-  const unsigned total = (unsigned)tiles_m * tiles_n * g.nbatch;
+  const int nz = BR ? (g.nbatch + g.bps - 1) / g.bps : g.nbatch;
+  const unsigned total = (unsigned)tiles_m * tiles_n * nz;
   if (blockIdx.x >= total) return;
   const unsigned lid = xcd_remap16(blockIdx.x, total);
   const int mt = lid % tiles_m;
   const int nt = (lid / tiles_m) % tiles_n;
   const int z = lid / (tiles_m * tiles_n);
-  split_tile_lds<BM, BN, WGM, WGN, NST, TRIB, OUTP>(g, mt, nt, z, smem);
+  split_tile_lds<BM, BN, WGM, WGN, NST, TRIB, OUTP, BR>(g, mt, nt, z, smem);
 }
 
 // Persistent grouped split GEMMs (the schedule of dn_sched.h).
@@ -886,6 +899,168 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
   const unsigned grid = std::max(grid_of((long)Mpad * Kpad), (unsigned)((naux + 255) / 256));
   hipLaunchKernelGGL(c_planes_kernel, dim3(grid), dim3(256), 0, s, C, nbf, nbf5, Mpad, Kpad,
                      cscale, hi, lo, lhmax, naux, qs, qinv);
+  DN_HIP(hipGetLastError());
+  return DN_OK;
+}
+
+// ---- split-precision Lagrangian contraction ----------------------------------------
+//
+// Z[n,b] = sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q] (utils.jl:44-48: the exchange-type term of the
+// orbital Lagrangian, the Coulomb term sitting on bmat's diagonal) on FP16 hi/lo MFMA:
+// both operands become tile-major planes with q as the contraction index k (bpn from the
+// FP32 first transform, bmat from FP64), each under one power-of-two scale taken from its
+// largest element over every slab, so a slab's rounding stays below 2^-22 of the largest
+// slab's terms, as its share of the sum does; bps slabs accumulate in FP32 per output tile
+// and the partials are summed in FP64 in batch order (deterministic).
+
+// Planes of src, element (row r, k) of slab Q at src[Q sS + r rs + k ks] for r < R, k < K,
+// zero for r in [R, Rp) and k in [K, Kq): (r, k) of slab Q at
+// Q Rp Kq + (k / 32) 32 Rp + 32 r + k % 32.  Work item = (Q, k tile, r, 8-wide chunk c);
+// KC (k contiguous in src) puts c fastest, else r (consecutive rows are adjacent).
+template <typename T, bool KC>
+__global__ __launch_bounds__(256) void kplanes_kernel(const T* __restrict__ src, long rs, long ks,
+                                                      long sS, int R, int Rp, int K, int Kq,
+                                                      int naux, const float* __restrict__ scale,
+                                                      _Float16* __restrict__ hi,
+                                                      _Float16* __restrict__ lo) {
+  const int nkt = Kq / 32;
+  const long total = (long)naux * nkt * Rp * 4;
+  const float sc = *scale;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    int r, c;
+    long qk;  // Q nkt + k tile
+    if (KC) {
+      c = (int)(idx & 3);
+      r = (int)((idx >> 2) % Rp);
+      qk = (idx >> 2) / Rp;
+    } else {
+      r = (int)(idx % Rp);
+      c = (int)((idx / Rp) & 3);
+      qk = (idx / Rp) >> 2;
+    }
+    const long Q = qk / nkt;
+    const int kt = (int)(qk - Q * nkt);
+    v8h h8, l8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = kt * 32 + 8 * c + e;
+      _Float16 h = 0, l = 0;
+      if (r < R && k < K) {
+        const T x = src[Q * sS + (long)r * rs + (long)k * ks];
+        if constexpr (std::is_same<T, double>::value) split_f16_d(x * (double)sc, &h, &l);
+        else split_f16((float)x * sc, &h, &l);
+      }
+      h8[e] = h;
+      l8[e] = l;
+    }
+    const long o = Q * (long)Rp * Kq + (long)kt * 32 * Rp + (long)r * 32 + 8 * c;
+    *reinterpret_cast<v8h*>(hi + o) = h8;
+    *reinterpret_cast<v8h*>(lo + o) = l8;
+  }
+}
+
+// zs = (scale of bpn, 1 / (scale of bpn * scale of bmat), scale of bmat, -): the bpn scale
+// from the first transform's per-Q maxima, the bmat scale from the maximum lag_bmat_max
+// left as float bits in zs[3]
+__global__ __launch_bounds__(256) void contract_scales_kernel(const float* __restrict__ qmax,
+                                                              int naux, float* __restrict__ zs) {
+  __shared__ float red[4];
+  float m = 0.f;
+  for (int Q = threadIdx.x; Q < naux; Q += blockDim.x) m = fmaxf(m, qmax[Q]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float ma = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    const float mb = __uint_as_float(reinterpret_cast<const unsigned*>(zs)[3]);
+    const float sa = pow2_scale(ma * 1.0001f), sb = pow2_scale(mb * 1.0001f);
+    zs[0] = sa;
+    zs[1] = 1.0f / (sa * sb);  // powers of two: exact
+    zs[2] = sb;
+  }
+}
+
+// out[e] = sum_z part[z n + e] in FP64, z in order
+__global__ __launch_bounds__(256) void reduce_f32_partials_kernel(const float* __restrict__ part,
+                                                                  long n, int nz,
+                                                                  double* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int z = 0; z < nz; ++z) acc += (double)part[(long)z * n + e];
+    out[e] = acc;
+  }
+}
+
+namespace {
+struct ContractPlan {
+  int Rp, Kq, tm, tn, bps, nz;
+  long planeA, planeB, partial;  // halves per plane of each operand; partial floats
+};
+
+ContractPlan contract_plan(int nbf, int nbf5, int naux) {
+  ContractPlan c{};
+  c.Rp = (nbf + 127) / 128 * 128;
+  c.Kq = (nbf5 + 31) / 32 * 32;
+  c.tm = c.Rp / 128;
+  c.tn = (nbf5 + 127) / 128;
+  // about 2048 workgroups, 4 to 16 slabs per FP32 partial: at least four keeps the
+  // partials (and the workspace) small against the planes, at most sixteen bounds the FP32
+  // accumulation to 16 Kq terms per output element
+  const long want = std::max(1L, 2048L / (long)(c.tm * c.tn));
+  c.bps = (int)std::min(16L, std::max(4L, (naux + want - 1) / want));
+  c.bps = std::min(c.bps, naux);
+  c.nz = (naux + c.bps - 1) / c.bps;
+  c.planeA = (long)c.Rp * c.Kq * naux;
+  c.planeB = (long)nbf5 * c.Kq * naux;
+  c.partial = (long)c.nz * nbf * nbf5;
+  return c;
+}
+}  // namespace
+
+size_t split_contract_bytes(int nbf, int nbf5, int naux) {
+  const ContractPlan c = contract_plan(nbf, nbf5, naux);
+  return (size_t)(2 * c.planeA + 2 * c.planeB) * sizeof(_Float16) +
+         (size_t)c.partial * sizeof(float) + 256;
+}
+
+int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
+                   const double* bmat, const float* qmax, float* zscale, void* work,
+                   size_t work_bytes, double* Z) {
+  const ContractPlan c = contract_plan(nbf, nbf5, naux);
+  if (split_contract_bytes(nbf, nbf5, naux) > work_bytes)
+    return fail(DN_ERR_ARG, "split contraction workspace too small");
+  _Float16* ahi = static_cast<_Float16*>(work);
+  _Float16* alo = ahi + c.planeA;
+  _Float16* bhi = alo + c.planeA;
+  _Float16* blo = bhi + c.planeB;
+  float* part = reinterpret_cast<float*>(blo + c.planeB);  // 64-byte aligned: Kq % 32 == 0
+  hipLaunchKernelGGL(contract_scales_kernel, dim3(1), dim3(256), 0, s, qmax, naux, zscale);
+  // bpn[n,q,Q] at q + nbf5 n + nbf5 nbf Q (rows n, k = q contiguous)
+  const long itA = (long)naux * (c.Kq / 32) * c.Rp * 4, itB = (long)naux * (c.Kq / 32) * nbf5 * 4;
+  hipLaunchKernelGGL((kplanes_kernel<float, true>), dim3(grid_of(itA)), dim3(256), 0, s, bpn,
+                     (long)nbf5, 1L, (long)nbf5 * nbf, nbf, c.Rp, nbf5, c.Kq, naux, zscale, ahi,
+                     alo);
+  // bmat[q,b,Q] at b + nbf5 q + nbf5^2 Q (rows b, k = q)
+  hipLaunchKernelGGL((kplanes_kernel<double, false>), dim3(grid_of(itB)), dim3(256), 0, s, bmat,
+                     1L, (long)nbf5, (long)nbf5 * nbf5, nbf5, nbf5, nbf5, c.Kq, naux, zscale + 2,
+                     bhi, blo);
+  SplitGemm g{};
+  g.M = nbf; g.N = nbf5; g.K = nbf5;
+  g.ahi = ahi; g.alo = alo; g.lda = (long)c.Rp * 32; g.sA = (long)c.Rp * c.Kq;
+  g.bhi = bhi; g.blo = blo; g.ldb = (long)nbf5 * 32; g.sB = (long)nbf5 * c.Kq;
+  g.C = part; g.ldc = nbf; g.sC = (long)nbf * nbf5;
+  g.cs = zscale;  // cs[1] = 1 / (scale of bpn * scale of bmat)
+  g.col_div = 0;
+  g.nbatch = naux; g.bps = c.bps;
+  const long total = (long)c.tm * c.tn * c.nz;
+  if (total > 0x7fffffffL) return fail(DN_ERR_ARG, "split contraction grid too large");
+  hipLaunchKernelGGL((split_lds_kernel<128, 128, 2, 4, 2, false, false, true>),
+                     dim3((unsigned)total), dim3(512), 0, s, g, c.tm, c.tn);
+  const long n = (long)nbf * nbf5;
+  hipLaunchKernelGGL(reduce_f32_partials_kernel, dim3(grid_of(n)), dim3(256), 0, s, part, n,
+                     c.nz, Z);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
