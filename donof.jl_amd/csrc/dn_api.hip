@@ -24,8 +24,9 @@ int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int na
           const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w);
 int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat);
+int lag_bmat_max_blocks(int nbf5, int naux);
 int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
-                 const double* ck12, const double* w, double* bmat, unsigned* maxbits);
+                 const double* ck12, const double* w, double* bmat, float* bmax);
 int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nbf5,
              double* elag);
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
@@ -51,7 +52,7 @@ int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float1
              const float* qmax, float* qs, float* qinv);
 size_t split_contract_bytes(int nbf, int nbf5, int naux);
 int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
-                   const double* bmat, const float* qmax, float* zscale, void* work,
+                   const double* bmat, const float* qmax, int nbmax, float* zscale, void* work,
                    size_t work_bytes, double* Z);
 int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
                     double* part, size_t part_elems, double* part2, size_t part2_elems,
@@ -86,8 +87,9 @@ struct Layout {
   static size_t f8(size_t floats) { return (floats + 1) / 2; }   // floats -> doubles
   size_t split_doubles() const {
     if (!bplane) return 0;  // FP64 system
-    // + two doubles each for the C5 scale pair (cscale) and the contraction scales (zscale)
-    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 4;
+    // + the C5 scale pair (cscale) and the contraction's scales and block maxima (zscale)
+    return 2 * d8(bplane) + 2 * d8(cplane) + f8(f_big) + f8(f_bpq) + 6 * f8(f_naux) + 2 +
+           f8(4 + 8192);
   }
   size_t total_bytes() const {
     return 8 * (b + C + H + n + 4 * nn + nbf2 + nbf_nbf5 * 3 + n + big + bpn + cp + bmat + 2 * xw +
@@ -481,7 +483,7 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
                s->X, s->w));
   if (s->precision == DN_SPLIT_F16) {
     DN_TRY(lag_bmat_max(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w,
-                        s->bmat, reinterpret_cast<unsigned*>(s->zscale) + 3));
+                        s->bmat, s->zscale + 4));
   } else {
     DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
   }
@@ -489,7 +491,8 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
   // Z[n,b] = sum_Q sum_q bpn[n,q,Q] bmat[q,b,Q], then elag += C^T Z
   if (s->precision == DN_SPLIT_F16) {
     DN_TRY(split_contract(s->stream, d.nbf, d.nbf5, d.nbfaux, s->bpn32, s->bmat, s->qmax,
-                          s->zscale, s->bpn, s->bpn_elems * sizeof(double), s->Z));
+                          lag_bmat_max_blocks(d.nbf5, d.nbfaux), s->zscale, s->bpn,
+                          s->bpn_elems * sizeof(double), s->Z));
   } else {
     const Gemm g = lag_contract_gemm(s);
     const int used = (g.nbatch + g.bps - 1) / g.bps;
@@ -812,7 +815,7 @@ int dn_sys_create(dn_ctx* ctx, const dn_dims* dims, int precision, dn_sys** out)
     A(Layout::f8(L.f_naux), &q); s->qinv = reinterpret_cast<float*>(q);
     A(Layout::f8(L.f_naux), &q); s->lhmax = reinterpret_cast<float*>(q);
     A(2, &q); s->cscale = reinterpret_cast<float*>(q);  // scale, 1/scale, max bits
-    A(2, &q); s->zscale = reinterpret_cast<float*>(q);
+    A(Layout::f8(4 + 8192), &q); s->zscale = reinterpret_cast<float*>(q);
     if (rc == DN_OK) {
       // plane padding must read as zero
       hipError_t e1 = hipMemsetAsync(s->bhi, 0, Layout::d8(L.bplane) * 8, s->stream);

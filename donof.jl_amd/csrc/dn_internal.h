@@ -443,7 +443,8 @@ struct dn_sys {
   float* qs = nullptr;      // naux
   float* qinv = nullptr;    // naux
   // split Lagrangian contraction: (scale of bpn, 1 / (scale of bpn * scale of bmat), scale
-  // of bmat, max |bmat| as float bits); its planes and FP32 partials live in bpn's space
+  // of bmat, -) and up to 8192 block maxima of |bmat|; its planes and FP32 partials live
+  // in bpn's space
   float* zscale = nullptr;
   bool b_ready = false;
   // an ADAM-family run advanced step by step (dn_orbopt_start / dn_ogroup_step)

@@ -108,15 +108,15 @@ __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict_
     bmat[e] = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
 }
 
-// lag_bmat_kernel that also leaves max |bmat| (as FP32) in *maxbits, which
-// lag_bmat_max clears first: the scale of the split-precision contraction's bmat planes
+// lag_bmat_kernel that also leaves each block's max |bmat| (as FP32) in bmax[blockIdx.x]
+// (the scale of the split-precision contraction's bmat planes; no atomics)
 __global__ __launch_bounds__(256) void lag_bmat_max_kernel(const double* __restrict__ bt,
                                                            long slice, int nbf, int nbf5,
                                                            int naux,
                                                            const double* __restrict__ ck12,
                                                            const double* __restrict__ w,
                                                            double* __restrict__ bmat,
-                                                           unsigned* __restrict__ maxbits) {
+                                                           float* __restrict__ bmax) {
   __shared__ float red[4];
   const long total = (long)nbf5 * nbf5 * naux;
   float m = 0.f;
@@ -129,11 +129,8 @@ __global__ __launch_bounds__(256) void lag_bmat_max_kernel(const double* __restr
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0)
-    atomicMax(maxbits, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+  if (threadIdx.x == 0) bmax[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
-
-__global__ void clear_u32_kernel(unsigned* p) { *p = 0u; }
 
 __global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
                                 int nbf, int nbf5, double* __restrict__ elag) {
@@ -244,12 +241,14 @@ int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int
   return DN_OK;
 }
 
+int lag_bmat_max_blocks(int nbf5, int naux) {
+  return (int)std::min(((long)nbf5 * nbf5 * naux + 255) / 256, 8192L);
+}
+
 int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
-                 const double* ck12, const double* w, double* bmat, unsigned* maxbits) {
-  hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(1), 0, s, maxbits);
-  const long blocks = ((long)nbf5 * nbf5 * naux + 255) / 256;
-  hipLaunchKernelGGL(lag_bmat_max_kernel, dim3((unsigned)std::min(blocks, 8192L)), dim3(256), 0,
-                     s, bt, slice, nbf, nbf5, naux, ck12, w, bmat, maxbits);
+                 const double* ck12, const double* w, double* bmat, float* bmax) {
+  hipLaunchKernelGGL(lag_bmat_max_kernel, dim3((unsigned)lag_bmat_max_blocks(nbf5, naux)),
+                     dim3(256), 0, s, bt, slice, nbf, nbf5, naux, ck12, w, bmat, bmax);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

@@ -961,20 +961,28 @@ __global__ __launch_bounds__(256) void kplanes_kernel(const T* __restrict__ src,
 }
 
 // zs = (scale of bpn, 1 / (scale of bpn * scale of bmat), scale of bmat, -): the bpn scale
-// from the first transform's per-Q maxima, the bmat scale from the maximum lag_bmat_max
-// left as float bits in zs[3]
+// from the first transform's per-Q maxima, the bmat scale from the nb block maxima that
+// lag_bmat_max left in zs[4 ..]
 __global__ __launch_bounds__(256) void contract_scales_kernel(const float* __restrict__ qmax,
-                                                              int naux, float* __restrict__ zs) {
-  __shared__ float red[4];
-  float m = 0.f;
-  for (int Q = threadIdx.x; Q < naux; Q += blockDim.x) m = fmaxf(m, qmax[Q]);
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+                                                              int naux, int nb,
+                                                              float* __restrict__ zs) {
+  __shared__ float red[2][4];
+  float ma = 0.f, mb = 0.f;
+  for (int Q = threadIdx.x; Q < naux; Q += blockDim.x) ma = fmaxf(ma, qmax[Q]);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) mb = fmaxf(mb, zs[4 + i]);
+  for (int off = 32; off > 0; off >>= 1) {
+    ma = fmaxf(ma, __shfl_down(ma, off, 64));
+    mb = fmaxf(mb, __shfl_down(mb, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ma;
+    red[1][threadIdx.x >> 6] = mb;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const float ma = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    const float mb = __uint_as_float(reinterpret_cast<const unsigned*>(zs)[3]);
-    const float sa = pow2_scale(ma * 1.0001f), sb = pow2_scale(mb * 1.0001f);
+    const float a = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
+    const float b = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
+    const float sa = pow2_scale(a * 1.0001f), sb = pow2_scale(b * 1.0001f);
     zs[0] = sa;
     zs[1] = 1.0f / (sa * sb);  // powers of two: exact
     zs[2] = sb;
@@ -988,7 +996,15 @@ __global__ __launch_bounds__(256) void reduce_f32_partials_kernel(const float* _
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x) {
     double acc = 0.0;
-    for (int z = 0; z < nz; ++z) acc += (double)part[(long)z * n + e];
+    int z = 0;
+    for (; z + 8 <= nz; z += 8) {  // eight loads in flight, added in z order
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(long)(z + u) * n + e];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += (double)v[u];
+    }
+    for (; z < nz; ++z) acc += (double)part[(long)z * n + e];
     out[e] = acc;
   }
 }
@@ -1026,7 +1042,7 @@ size_t split_contract_bytes(int nbf, int nbf5, int naux) {
 }
 
 int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
-                   const double* bmat, const float* qmax, float* zscale, void* work,
+                   const double* bmat, const float* qmax, int nbmax, float* zscale, void* work,
                    size_t work_bytes, double* Z) {
   const ContractPlan c = contract_plan(nbf, nbf5, naux);
   if (split_contract_bytes(nbf, nbf5, naux) > work_bytes)
@@ -1036,7 +1052,8 @@ int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
   _Float16* bhi = alo + c.planeA;
   _Float16* blo = bhi + c.planeB;
   float* part = reinterpret_cast<float*>(blo + c.planeB);  // 64-byte aligned: Kq % 32 == 0
-  hipLaunchKernelGGL(contract_scales_kernel, dim3(1), dim3(256), 0, s, qmax, naux, zscale);
+  hipLaunchKernelGGL(contract_scales_kernel, dim3(1), dim3(256), 0, s, qmax, naux, nbmax,
+                     zscale);
   // bpn[n,q,Q] at q + nbf5 n + nbf5 nbf Q (rows n, k = q contiguous)
   const long itA = (long)naux * (c.Kq / 32) * c.Rp * 4, itB = (long)naux * (c.Kq / 32) * nbf5 * 4;
   hipLaunchKernelGGL((kplanes_kernel<float, true>), dim3(grid_of(itA)), dim3(256), 0, s, bpn,
