@@ -446,6 +446,10 @@ def serving_leg(api, serve, n=100, rate=2.0, seed=7, max_live=8,
         if d:
             out["grouped-split"]["max_abs_dE_vs_fp64_Ha"] = max(d)
             out["grouped-split"]["median_abs_dE_vs_fp64_Ha"] = float(np.median(d))
+            out["grouped-split"]["dE_note"] = (
+                "end energies of 200 ADAM steps from the same start: one Lagrangian "
+                "evaluation agrees to ~1e-7 Ha, but ADAM's normalised step lets that "
+                "difference change the path of an unconverged solve")
     return out
 
 
