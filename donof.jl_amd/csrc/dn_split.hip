@@ -915,46 +915,75 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
 
 // Planes of src, element (row r, k) of slab Q at src[Q sS + r rs + k ks] for r < R, k < K,
 // zero for r in [R, Rp) and k in [K, Kq): (r, k) of slab Q at
-// Q Rp Kq + (k / 32) 32 Rp + 32 r + k % 32.  Work item = (Q, k tile, r, 8-wide chunk c);
-// KC (k contiguous in src) puts c fastest, else r (consecutive rows are adjacent).
-template <typename T, bool KC>
-__global__ __launch_bounds__(256) void kplanes_kernel(const T* __restrict__ src, long rs, long ks,
+// Q Rp Kq + (k / 32) 32 Rp + 32 r + k % 32.  Work block = (Q, k tile, 64-row block), thread
+// = (row, 8-wide chunk c), adjacent rows adjacent in src (row stride 1); blocks taken
+// grid-stride.
+template <typename T>
+__global__ __launch_bounds__(256) void kplanes_kernel(const T* __restrict__ src, long ks,
                                                       long sS, int R, int Rp, int K, int Kq,
-                                                      int naux, const float* __restrict__ scale,
+                                                      long nblk,
+                                                      const float* __restrict__ scale,
                                                       _Float16* __restrict__ hi,
                                                       _Float16* __restrict__ lo) {
-  const int nkt = Kq / 32;
-  const long total = (long)naux * nkt * Rp * 4;
+  const int nkt = Kq / 32, nrb = (Rp + 63) / 64;
   const float sc = *scale;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    int r, c;
-    long qk;  // Q nkt + k tile
-    if (KC) {
-      c = (int)(idx & 3);
-      r = (int)((idx >> 2) % Rp);
-      qk = (idx >> 2) / Rp;
-    } else {
-      r = (int)(idx % Rp);
-      c = (int)((idx / Rp) & 3);
-      qk = (idx / Rp) >> 2;
-    }
+  for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const long qk = b / nrb;
+    const int rb = (int)(b - qk * nrb);
     const long Q = qk / nkt;
     const int kt = (int)(qk - Q * nkt);
+    const int c = (int)(threadIdx.x >> 6), r = rb * 64 + (int)(threadIdx.x & 63);
+    if (r >= Rp) continue;
+    const T* row = src + Q * sS + r;
     v8h h8, l8;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = kt * 32 + 8 * c + e;
       _Float16 h = 0, l = 0;
       if (r < R && k < K) {
-        const T x = src[Q * sS + (long)r * rs + (long)k * ks];
+        const T x = row[(long)k * ks];
         if constexpr (std::is_same<T, double>::value) split_f16_d(x * (double)sc, &h, &l);
         else split_f16((float)x * sc, &h, &l);
       }
       h8[e] = h;
       l8[e] = l;
     }
-    const long o = Q * (long)Rp * Kq + (long)kt * 32 * Rp + (long)r * 32 + 8 * c;
+    const long o = Q * Rp * Kq + (long)kt * 32 * Rp + (long)r * 32 + 8 * c;
+    *reinterpret_cast<v8h*>(hi + o) = h8;
+    *reinterpret_cast<v8h*>(lo + o) = l8;
+  }
+}
+
+// kplanes_kernel for a k-contiguous FP32 source: thread = (row, 8-wide chunk c) with c
+// fastest (four threads read a row's 128-byte run of one k tile); work block = (Q, k tile,
+// 64-row block), taken grid-stride.  (One element per thread, 32 threads per run, was
+// slower: 289 against 206 us on average over a served mix, its 2-byte stores.)
+__global__ __launch_bounds__(256) void kplanes_kc_f32_kernel(const float* __restrict__ src,
+                                                             long rs, long sS, int R, int Rp,
+                                                             int K, int Kq, long nblk,
+                                                             const float* __restrict__ scale,
+                                                             _Float16* __restrict__ hi,
+                                                             _Float16* __restrict__ lo) {
+  const int nkt = Kq / 32, nrb = (Rp + 63) / 64;
+  const float sc = *scale;
+  for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const long qk = b / nrb;
+    const int rb = (int)(b - qk * nrb);
+    const long Q = qk / nkt;
+    const int kt = (int)(qk - Q * nkt);
+    const int c = (int)(threadIdx.x & 3), r = rb * 64 + (int)(threadIdx.x >> 2);
+    if (r >= Rp) continue;
+    const float* row = src + Q * sS + (long)r * rs;
+    v8h h8, l8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = kt * 32 + 8 * c + e;
+      _Float16 h = 0, l = 0;
+      if (r < R && k < K) split_f16(row[k] * sc, &h, &l);
+      h8[e] = h;
+      l8[e] = l;
+    }
+    const long o = Q * Rp * Kq + (long)kt * 32 * Rp + (long)r * 32 + 8 * c;
     *reinterpret_cast<v8h*>(hi + o) = h8;
     *reinterpret_cast<v8h*>(lo + o) = l8;
   }
@@ -1055,14 +1084,15 @@ int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
   hipLaunchKernelGGL(contract_scales_kernel, dim3(1), dim3(256), 0, s, qmax, naux, nbmax,
                      zscale);
   // bpn[n,q,Q] at q + nbf5 n + nbf5 nbf Q (rows n, k = q contiguous)
-  const long itA = (long)naux * (c.Kq / 32) * c.Rp * 4, itB = (long)naux * (c.Kq / 32) * nbf5 * 4;
-  hipLaunchKernelGGL((kplanes_kernel<float, true>), dim3(grid_of(itA)), dim3(256), 0, s, bpn,
-                     (long)nbf5, 1L, (long)nbf5 * nbf, nbf, c.Rp, nbf5, c.Kq, naux, zscale, ahi,
-                     alo);
+  const long blkA = (long)naux * (c.Kq / 32) * ((c.Rp + 63) / 64);
+  hipLaunchKernelGGL(kplanes_kc_f32_kernel, dim3((unsigned)std::min(blkA, 1L << 20)), dim3(256),
+                     0, s, bpn, (long)nbf5, (long)nbf5 * nbf, nbf, c.Rp, nbf5, c.Kq, blkA, zscale,
+                     ahi, alo);
   // bmat[q,b,Q] at b + nbf5 q + nbf5^2 Q (rows b, k = q)
-  hipLaunchKernelGGL((kplanes_kernel<double, false>), dim3(grid_of(itB)), dim3(256), 0, s, bmat,
-                     1L, (long)nbf5, (long)nbf5 * nbf5, nbf5, nbf5, nbf5, c.Kq, naux, zscale + 2,
-                     bhi, blo);
+  const long blkB = (long)naux * (c.Kq / 32) * ((nbf5 + 63) / 64);
+  hipLaunchKernelGGL(kplanes_kernel<double>, dim3((unsigned)std::min(blkB, 1L << 20)), dim3(256),
+                     0, s, bmat, (long)nbf5, (long)nbf5 * nbf5, nbf5, nbf5, nbf5, c.Kq, blkB,
+                     zscale + 2, bhi, blo);
   SplitGemm g{};
   g.M = nbf; g.N = nbf5; g.K = nbf5;
   g.ahi = ahi; g.alo = alo; g.lda = (long)c.Rp * 32; g.sA = (long)c.Rp * c.Kq;
