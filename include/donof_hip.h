@@ -34,10 +34,12 @@ enum {
   DN_ERR_STATE = 6      /* call not valid in the current state (e.g. b not uploaded) */
 };
 
-/* Arithmetic of the RI transform and J/K contractions of the J/K-MO build.
+/* Arithmetic of the RI transforms and contractions of a system.
  * DN_SPLIT_F16: b and C5 as FP16 hi+lo pairs with power-of-two scales, three MFMA
- * products accumulated in FP32, J/K reduced in FP64; H_core, the Lagrangian, the
- * rotation and the occupation functional stay FP64. */
+ * products accumulated in FP32: the J/K build's two transforms (J/K reduced in FP64) and
+ * the Lagrangian's two transforms and contraction sum_Q bpn bmat (FP32 partials summed in
+ * FP64); H_core, the Lagrangian's one-electron and C^J/C^K terms, the rotation and the
+ * occupation functional stay FP64.  b is not kept in FP64. */
 enum { DN_FP64 = 0, DN_SPLIT_F16 = 1 };
 
 /* Orbital optimiser of the ADAM family: p.orb_method "ADAM", "ADABelief", "YOGI",
