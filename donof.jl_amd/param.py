@@ -48,10 +48,35 @@ class Param:
     nv: int = 0
     nvar: int = 0
     noptorb: int = 0
+    closed: bool = True
+    # fields of the reference's Param that this path accepts and does not read, with the
+    # reference's defaults (param.jl:129-164): the iterative-diagonalisation ("ID") orbital
+    # method and its DIIS, the F-matrix thresholds of the no1 > 0 branch, and the CUDA
+    # device switches (here the device is always the GPU, and FP64 or split precision is
+    # chosen per System)
+    thresheid: float = 1e-6
+    maxitid: int = 30
+    threshl: float = 1e-4
+    threshe: float = 1e-5
+    threshec: float = 1e-8
+    scaling: bool = True
+    nzeros: int = 0
+    nzerosm: int = 5
+    nzerosr: int = 2
+    itziter: int = 10
+    diis: bool = True
+    thdiis: float = 1e-3
+    ndiis: int = 5
+    perdiis: bool = True
+    gpu: bool = False
+    gpu_bits: int = 64
 
     def __post_init__(self):
         self.nalpha = (self.ne + self.mul - 1) // 2
         self.nbeta = (self.ne - self.mul + 1) // 2
+        # param.jl:120: both spins paired as in a closed shell
+        self.closed = (self.nbeta == (self.ne + self.mul - 1) // 2
+                       and self.nalpha == (self.ne - self.mul + 1) // 2)
         self.ndoc = self.nbeta - self.no1
         self.nsoc = self.nalpha - self.nbeta
         self.ndns = self.ndoc + self.nsoc

@@ -162,3 +162,19 @@ def test_param_sizes(dn):
     # H2O / cc-pVDZ sizes of the reference's precompile workload: nbf=24, ne=10
     p = dn.Param(nbf=24, nbfaux=116, ne=10)
     assert (p.ndoc, p.ncwo, p.nbf5, p.nv, p.nvar) == (5, 3, 20, 15, 270)
+
+
+def test_param_has_every_reference_field(dn):
+    """Every field of the reference's Param (param.jl:3-57) exists, so code that sets or
+    reads any of them runs unchanged; closed follows param.jl:120."""
+    import dataclasses
+    fields = {f.name for f in dataclasses.fields(dn.Param)}
+    reference = (
+        "title mol natoms spherical nbf nbfaux nalpha nbeta ne mul ndoc nsoc ndns nvir closed "
+        "ipnof ista maxit maxitid maxloop threshl threshe threshec threshen threshgorb "
+        "threshgocc thresheid scaling nzeros nzerosm nzerosr itziter diis thdiis ndiis "
+        "perdiis ncwo noptorb nac nbf5 no0 no1 nv nvar HighSpin MSpin alpha occ_method "
+        "orb_method RI gpu gpu_bits h_cut").split()
+    assert not [f for f in reference if f not in fields]
+    assert dn.Param(nbf=24, nbfaux=116, ne=10).closed
+    assert not dn.Param(nbf=24, nbfaux=116, ne=9, mul=2).closed
