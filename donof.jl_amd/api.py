@@ -667,6 +667,20 @@ def ocupacion(gamma, p, I):
     return n, dn
 
 
+def ocupacion_softmax(gamma, p, I):
+    """pnof.jl:1122-1163: ocupacion with the softmax parametrisation."""
+    q = p.copy()
+    q.occ_method = "Softmax"
+    return ocupacion(gamma, q, I)
+
+
+def ocupacion_trigonometric(gamma, p, I):
+    """pnof.jl:1047-1120: ocupacion with the trigonometric parametrisation."""
+    q = p.copy()
+    q.occ_method = "Trigonometric"
+    return ocupacion(gamma, q, I)
+
+
 def PNOFi_selector(n, p, I):
     """(cj12, ck12) with zero diagonal (pnof.jl:1-53)."""
     s = _require_system(I)
@@ -795,6 +809,27 @@ def calcorbg(y, n, cj12, ck12, C, H, I, pa):
     Cnew = rotate_orbital(y, C, pa, I)
     elag, _ = compute_Lagrange2(Cnew, n, H, I, cj12, ck12, pa.nalpha, pa.nbeta)
     return pack_grad(elag, pa.nbf5)
+
+
+def calcorbe(y, n, cj12, ck12, C, H, I, p):
+    """pnof.jl:1379-1387: the energy at C exp(Y(y)) from J/K-MO of the rotated orbitals."""
+    Cnew = rotate_orbital(y, C, p, I)
+    J, K, Hc = computeJKH_MO(Cnew, H, I, p)
+    return calce(n, cj12, ck12, J, K, Hc, p, I)
+
+
+def calcorbeg(F, G, y, n, cj12, ck12, C, H, I, pa):
+    """pnof.jl:1409-1433: one Lagrangian at C exp(Y(y)); with G (an nvar array) the packed
+    orbital gradient is written into it, with F not None the energy
+    computeE_elec(Hmat, n, elag) is returned."""
+    Cnew = rotate_orbital(y, C, pa, I)
+    elag, Hmat = compute_Lagrange2(Cnew, n, H, I, cj12, ck12, pa.nalpha, pa.nbeta)
+    if G is not None:
+        G[:] = pack_grad(elag, pa.nbf5)
+    if F is not None:
+        E = float(np.trace(elag[:pa.nbf5, :pa.nbf5]))
+        return E + float(np.dot(np.asarray(n)[:pa.nbf5], np.diag(Hmat[:pa.nbf5, :pa.nbf5])))
+    return None
 
 
 def orbopt(method, gamma, C, H, I, p):

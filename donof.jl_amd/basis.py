@@ -329,6 +329,20 @@ def aux_basis(bset, beta=2.0):
     return BasisSet(bset.name + "-autofit", bset.atoms, shells, bset.spherical)
 
 
+def read_xyz_from_file(filename):
+    """io.jl:3-22: the atom lines of an .xyz file (count on the first line, a comment line,
+    then one line per atom) joined by newlines, for molecule()."""
+    with open(filename) as f:
+        lines = f.read().split("\n")
+    natoms = int(lines[0].strip())
+    return "\n".join(lines[2:2 + natoms])
+
+
+def Z_to_symbol(Z):
+    """utils.jl:338-: the element symbol of atomic number Z."""
+    return SYMBOLS[int(Z)]
+
+
 def molecule(mol, basis, spherical=False):
     """io.jl:25-57: (bset, p) for a molecule string and a basis name; p is the Param of
     param.jl:59-222 (no1 = 0; set_no1 sets the frozen core)."""

@@ -41,6 +41,26 @@ def computeE_elec_hc(Hmat, n, elag, p):
     return E
 
 
+def computeE_elec(*args):
+    """The two methods of utils.jl:191-227: computeE_elec(H, C, n, elag, p) (one-electron
+    energies (C^T H C)_ii, half of them for the singly occupied orbitals in high spin) and
+    computeE_elec(Hmat, n, elag, p) (sum n_i Hmat_ii over i < nbf5), each plus tr elag."""
+    if len(args) == 5:
+        H, C, n, elag, p = args
+        C5 = np.asarray(C)[:, :p.nbf5]
+        hd = np.sum(C5 * (np.asarray(H) @ C5), axis=0)
+        return computeE_elec_hc(np.diag(hd), n, elag, p)
+    Hmat, n, elag, p = args
+    E = float(np.trace(np.asarray(elag)[:p.nbf5, :p.nbf5]))
+    return E + float(np.dot(np.asarray(n)[:p.nbf5], np.diag(np.asarray(Hmat)[:p.nbf5, :p.nbf5])))
+
+
+def computeLagrangeConvergency(elag):
+    """utils.jl:229-236: (sum and max of |elag - elag^T|)."""
+    d = np.abs(np.asarray(elag) - np.asarray(elag).T)
+    return float(d.sum()), float(d.max())
+
+
 def ENERGY1r(C, n, H, I, cj12, ck12, p, want_jkh=False):
     """(E, elag, sumdiff, maxdiff) of utils.jl:117-136 (no1 = 0 branch).  With want_jkh
     also (J_MO, K_MO, H_core) at the same orbitals, taken from the same transform of b."""

@@ -99,6 +99,33 @@ class Param:
         self.nv = self.ncwo * self.ndoc
         self.nvar = round(self.nbf * (self.nbf - 1) / 2 - self.no0 * (self.no0 - 1) / 2)
 
+    def set_no1(self, bset=None, no1=-1):
+        """param.jl:268-311: the frozen (fully occupied) orbitals, by default one per
+        first-row atom, five per second-row atom ... from the atoms of bset; resizes the
+        subspaces.  The energy of a solve with no1 > 0 needs the reference's F-matrix branch
+        of ENERGY1r, which this path does not have (energy() refuses it)."""
+        if no1 == -1:
+            no1 = 0
+            for a in (bset.atoms if bset is not None else []):
+                Z = a.Z
+                for lo, hi, k in ((1, 2, 0), (3, 10, 1), (11, 18, 5), (19, 36, 9),
+                                  (37, 49, 18), (50, 54, 23), (55, 71, 27), (72, 81, 30),
+                                  (82, 86, 39), (87, 109, 43)):
+                    if lo <= Z <= hi:
+                        no1 += k
+                        break
+        self.no1 = no1
+        self.ndoc = self.nbeta - self.no1
+        self.nsoc = self.nalpha - self.nbeta
+        self.ndns = self.ndoc + self.nsoc
+        self.nvir = self.nbf - self.nalpha
+        # ncwo is kept (param.jl:300-310)
+        self.nac = self.ndoc * (1 + self.ncwo)
+        self.nbf5 = self.no1 + self.nac + self.nsoc
+        self.no0 = self.nbf - self.nbf5
+        self.nv = self.ncwo * self.ndoc
+        self.nvar = round(self.nbf * (self.nbf - 1) / 2 - self.no0 * (self.no0 - 1) / 2)
+
     def copy(self):
         import copy
         return copy.copy(self)
