@@ -156,7 +156,10 @@ def energy(*args, **kwargs):
 
 def energy_bset(bset, p, C=None, n=None, aux=None, freeze_occ=False, printmode=True,
                 do_fchk=True, checkpoint=True, ctx=None, precision="fp64", log=print,
-                return_history=False, do_gradients=False):
+                return_history=False, do_gradients=False, fmiug0=None, do_hfidr=False,
+                do_nofmp2=False, nofmp2strategy="numerical", tolnofmp2=1e-10, do_ekt=False,
+                do_mulliken_pop=False, do_lowdin_pop=True, do_m_diagnostic=True,
+                do_mbpt=False, do_translate_to_donofsw=False, do_erpa=False):
     """energy.jl:3-320 for the RI path: compute_integrals (the integral engine and the
     device RI metric), E_nuc, the guess C = eigen(H, S) unless C is given, check_ortho,
     the outer loop (energy_resident), and the FCHK file `<p.title>.fchk`
@@ -165,9 +168,26 @@ def energy_bset(bset, p, C=None, n=None, aux=None, freeze_occ=False, printmode=T
     the per-iteration history with return_history.  do_gradients returns
     (grad, E_t, C, n) as energy.jl:309-314 does: the RI nuclear gradient (natoms x 3,
     Hartree/bohr) of the final state on the device (api.compute_geom_gradients, with the
-    derivative integrals of the package's integral engine)."""
+    derivative integrals of the package's integral engine).
+
+    The reference's other keywords (energy.jl:3-23) are accepted with its defaults: the
+    post-loop analyses on by default there (Lowdin populations, M diagnostic) are skipped
+    with a note, and asking for any other (an HF/ID guess or fmiug0, NOF-MP2, EKT, MBPT,
+    ERPA, Mulliken populations, the DoNOFsw export) raises NotImplementedError: they are
+    outside the orbital-optimisation path this package covers."""
     from . import integrals as ints
     from . import output
+    unsupported = [k for k, v in (("fmiug0", fmiug0 is not None), ("do_hfidr", do_hfidr),
+                                  ("do_nofmp2", do_nofmp2), ("do_ekt", do_ekt),
+                                  ("do_mulliken_pop", do_mulliken_pop), ("do_mbpt", do_mbpt),
+                                  ("do_translate_to_donofsw", do_translate_to_donofsw),
+                                  ("do_erpa", do_erpa)) if v]
+    if unsupported:
+        raise NotImplementedError("energy(): %s not supported (post-SCF methods, the ID "
+                                  "orbital method and DoNOFsw export are out of scope)"
+                                  % ", ".join(unsupported))
+    if printmode and (do_lowdin_pop or do_m_diagnostic):
+        log("(Lowdin populations and the M diagnostic are not computed here)")
     S, T, V, H, I = ints.compute_integrals(bset, p, aux=aux, ctx=ctx, precision=precision,
                                            log=log if printmode else None)
     try:

@@ -43,3 +43,15 @@ def test_computeE_elec_methods_agree():
     assert abs(E2 - expect) < 1e-12
     s, m = energy.computeLagrangeConvergency(elag)
     assert abs(s - np.abs(elag - elag.T).sum()) < 1e-12 and m == np.abs(elag - elag.T).max()
+
+
+def test_energy_accepts_the_reference_keywords():
+    """energy(bset, p; ...) takes every keyword of energy.jl:3-23; requests outside the
+    orbital-optimisation path fail loudly before any device work."""
+    import pytest
+    bset, p = basis.molecule("0 1\n H 0 0 0\n H 0 0 0.74", "sto-3g", spherical=True)
+    for kw in ({"do_nofmp2": True}, {"do_hfidr": True}, {"do_ekt": True}, {"do_erpa": True},
+               {"do_mbpt": True}, {"do_mulliken_pop": True}, {"fmiug0": np.zeros((2, 2))},
+               {"do_translate_to_donofsw": True}):
+        with pytest.raises(NotImplementedError):
+            energy.energy(bset, p, printmode=False, **kw)
