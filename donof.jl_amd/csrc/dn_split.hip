@@ -974,12 +974,27 @@ __global__ __launch_bounds__(256) void kplanes_kc_f32_kernel(const float* __rest
     const int c = (int)(threadIdx.x & 3), r = rb * 64 + (int)(threadIdx.x >> 2);
     if (r >= Rp) continue;
     const float* row = src + Q * sS + (long)r * rs;
+    const int k0 = kt * 32 + 8 * c;
+    float x[8];
+    if (r < R && k0 + 8 <= K && ((uintptr_t)(row + k0) & 15) == 0) {
+      // the whole chunk in range and 16-byte aligned: two vector loads
+      typedef float v4 __attribute__((ext_vector_type(4)));
+      const v4 a = *reinterpret_cast<const v4*>(row + k0);
+      const v4 b = *reinterpret_cast<const v4*>(row + k0 + 4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        x[e] = a[e];
+        x[4 + e] = b[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e] = (r < R && k0 + e < K) ? row[k0 + e] : 0.f;
+    }
     v8h h8, l8;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int k = kt * 32 + 8 * c + e;
-      _Float16 h = 0, l = 0;
-      if (r < R && k < K) split_f16(row[k] * sc, &h, &l);
+      _Float16 h, l;
+      split_f16(x[e] * sc, &h, &l);
       h8[e] = h;
       l8[e] = l;
     }
