@@ -24,9 +24,8 @@ int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int na
           const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w);
 int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
              const double* ck12, const double* w, double* bmat);
-int lag_bmat_max_blocks(int nbf5, int naux);
-int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
-                 const double* ck12, const double* w, double* bmat, float* bmax);
+int lag_w_split(hipStream_t s, const float* bpq, int nbf5, int naux, const double* cj12,
+                const double* n, int nalpha, int nbeta, double* X, double* w);
 int lag_init(hipStream_t s, const double* Hmat, const double* n, int nbf, int nbf5,
              double* elag);
 int energy_elec(hipStream_t s, const double* elag, const double* Hmat, const double* n,
@@ -52,8 +51,8 @@ int split_k2(hipStream_t s, int nbf, int nbf5, int naux, int Mpad, const _Float1
              const float* qmax, float* qs, float* qinv);
 size_t split_contract_bytes(int nbf, int nbf5, int naux);
 int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
-                   const double* bmat, const float* qmax, int nbmax, float* zscale, void* work,
-                   size_t work_bytes, double* Z);
+                   const float* bpq, const double* ck12, const double* w, const float* qmax,
+                   float* zscale, void* work, size_t work_bytes, double* Z);
 int jk_from_bpq_f32(hipStream_t s, const float* bpq, long slice, int nbf5, int naux, double* X,
                     double* part, size_t part_elems, double* part2, size_t part2_elems,
                     double* J, double* K);
@@ -122,10 +121,10 @@ Layout layout_of(const dn_dims& d, int precision) {
   L.nn = nbf5 * nbf5;
   L.nbf2 = nbf * nbf;
   L.nbf_nbf5 = nbf * nbf5;
-  L.big = nbf5 * nbf5 * naux;
+  L.big = precision == DN_SPLIT_F16 ? 0 : nbf5 * nbf5 * naux;  // split: b_pq in FP32 only
   L.bpn = ld * nbf5 * naux;
   L.cp = ld != nbf ? ld * nbf : 0;
-  L.bmat = nbf5 * nbf5 * naux;
+  L.bmat = precision == DN_SPLIT_F16 ? 0 : nbf5 * nbf5 * naux;  // split: never stored
   L.xw = nbf5 * naux;
   const size_t jk_part = 2 * 64 * L.nn;
   const size_t lag_part = lag_splits(d) * L.nbf_nbf5;
@@ -448,21 +447,16 @@ int jkh_resident(dn_sys* s) {
 
 // The Lagrangian's two transforms in split precision (FP16 hi+lo planes of b and C5, FP32
 // accumulation): T[p,n,Q] = sum_m C5[m,p] b[m,n,Q] (the general first transform) and
-// b_pq[p,q,Q] = sum_n T[p,n,Q] C5[n,q], then both transposed into the FP64 layouts of
-// the FP64 path (bpn[n,p,Q], b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)) for its C^J/C^K terms
-// and contraction.
+// b_pq[p,q,Q] = sum_n T[p,n,Q] C5[n,q], both left in FP32 (bpn32, bpq32) for the C^J/C^K
+// terms and the contraction, which read them as they are.
 int lagrange_split_transforms(dn_sys* s) {
   const dn_dims& d = s->d;
   DN_TRY(split_prepare_c(s->stream, s->C, d.nbf, d.nbf5, s->Mpad, s->Kpad, s->cscale, s->chi,
                          s->clo, nullptr, d.nbfaux, s->qs, s->qinv));
   DN_TRY(split_k1(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->Kpad, s->bhi, s->blo,
                   s->binv, s->chi, s->clo, s->cscale, s->bpn32, s->qmax));
-  DN_TRY(split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
-                  s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv));
-  // b_pq in FP64 for the C^J/C^K terms; the contraction reads the FP32 bpn itself
-  const long nn = (long)d.nbf5 * d.nbf5;
-  return transpose_f32_f64(s->stream, s->bpq32, d.nbf5, d.nbf5, d.nbf5, nn, s->bt, d.nbf5, nn,
-                           d.nbfaux);
+  return split_k2(s->stream, d.nbf, d.nbf5, d.nbfaux, s->Mpad, s->chi, s->clo, s->cscale,
+                  s->bpn32, s->bpq32, s->qmax, s->qs, s->qinv);
 }
 
 int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
@@ -479,20 +473,20 @@ int lagrange_resident(dn_sys* s, int nalpha, int nbeta) {
     DN_TRY(gemm_f64(s->stream, t2_flat_gemm(s)));  // b_pq[q,p,Q] at p + nbf5 (q + nbf5 Q)
   }
   DN_TRY(hc_products(s, true, false));
-  DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
-               s->X, s->w));
-  if (s->precision == DN_SPLIT_F16) {
-    DN_TRY(lag_bmat_max(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w,
-                        s->bmat, s->zscale + 4));
+  if (s->precision == DN_SPLIT_F16) {  // b_pq in FP32 (bpq32, stored transposed)
+    DN_TRY(lag_w_split(s->stream, s->bpq32, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha, nbeta,
+                       s->X, s->w));
   } else {
+    DN_TRY(lag_w(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->cj12, s->n, nalpha,
+                 nbeta, s->X, s->w));
     DN_TRY(lag_bmat(s->stream, s->bt, slice, d.nbf5, d.nbf5, d.nbfaux, s->ck12, s->w, s->bmat));
   }
   DN_TRY(lag_init(s->stream, s->Hmat, s->n, d.nbf, d.nbf5, s->elag));
   // Z[n,b] = sum_Q sum_q bpn[n,q,Q] bmat[q,b,Q], then elag += C^T Z
   if (s->precision == DN_SPLIT_F16) {
-    DN_TRY(split_contract(s->stream, d.nbf, d.nbf5, d.nbfaux, s->bpn32, s->bmat, s->qmax,
-                          lag_bmat_max_blocks(d.nbf5, d.nbfaux), s->zscale, s->bpn,
-                          s->bpn_elems * sizeof(double), s->Z));
+    DN_TRY(split_contract(s->stream, d.nbf, d.nbf5, d.nbfaux, s->bpn32, s->bpq32, s->ck12,
+                          s->w, s->qmax, s->zscale, s->bpn, s->bpn_elems * sizeof(double),
+                          s->Z));
   } else {
     const Gemm g = lag_contract_gemm(s);
     const int used = (g.nbatch + g.bps - 1) / g.bps;
@@ -1023,8 +1017,12 @@ int dn_lagrange2_jkh(dn_sys* s, const double* C, const double* n, const double* 
   DN_TRY(lagrange_resident(s, nalpha, nbeta));
   // b_pq[q,p,Q] of the Lagrangian's second transform gives K and X = diag(b_pq), and
   // J = X X^T; H_core is the diagonal of Hmat's leading nbf5 block (C5^T H C5)
-  DN_TRY(jk_from_bpq(s->stream, s->bt, (long)nn, d.nbf5, d.nbfaux, s->X, s->part,
-                     s->part_elems, s->part2, s->part2_elems, s->J, s->K));
+  if (s->precision == DN_SPLIT_F16)  // FP32 b_pq (K comes out as the transpose: b_pq = b_qp)
+    DN_TRY(jk_from_bpq_f32(s->stream, s->bpq32, (long)nn, d.nbf5, d.nbfaux, s->X, s->part,
+                           s->part_elems, s->part2, s->part2_elems, s->J, s->K));
+  else
+    DN_TRY(jk_from_bpq(s->stream, s->bt, (long)nn, d.nbf5, d.nbfaux, s->X, s->part,
+                       s->part_elems, s->part2, s->part2_elems, s->J, s->K));
   DN_TRY(hcore_from_hmat(s->stream, s->Hmat, d.nbf, d.nbf5, s->Hcore));
   std::vector<double> hm;
   double* hm_out = Hmat;

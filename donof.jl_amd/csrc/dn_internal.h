@@ -161,9 +161,6 @@ int split_group_prep(hipStream_t s, const SplitPrepDesc* dev, int n, long plane_
 int split_group_scales(hipStream_t s, const SplitPrepDesc* dev, int n, long q_blocks);
 int split_k1_begin(hipStream_t s, float* qmax, int naux);
 int split_k2_begin(hipStream_t s, const float* qmax, int naux, float* qs, float* qinv);
-// dst[c + ldd r] = src[r + lds c] (FP32 -> FP64) for r < R, c < Cc, per slab
-int transpose_f32_f64(hipStream_t s, const float* src, int R, int Cc, long lds, long sbs,
-                      double* dst, long ldd, long sbd, int nbatch);
 long split_grouped_plan(GroupedSplit* desc, int n, int config);
 int split_grouped(hipStream_t s, const GroupedSplit* desc_host, const GroupedSplit* desc_dev,
                   int n, long total, int config, int kind, int grid);

@@ -22,16 +22,24 @@ namespace dn {
 
 // ---- per-block / per-element bodies (single-system and grouped kernels alike) -------------
 
+// bt[q,a,Q] (a < nbf5 here): at a + nbf q + slice Q, or, TR (the split second transform's
+// FP32 output, b_pq stored transposed), at q + nbf5 a + slice Q
+template <typename T, bool TR>
+__device__ __forceinline__ double bt_at(const T* __restrict__ bt, long slice, int nbf, int nbf5,
+                                        int a, int q, long Q) {
+  return TR ? (double)bt[q + (long)nbf5 * a + slice * Q] : (double)bt[a + (long)nbf * q + slice * Q];
+}
+
 // X[:,Q] and w[:,Q] for all b < nbf5 (one workgroup per Q); xs: nbf5 doubles of LDS.
-// bt[q,q,Q] is at q (nbf + 1) + slice Q.
-__device__ __forceinline__ void lag_w_block(int Q, const double* __restrict__ bt, long slice,
+// bt[q,q,Q] is at q (nbf + 1) + slice Q (q (nbf5 + 1) + slice Q for TR).
+template <typename T = double, bool TR = false>
+__device__ __forceinline__ void lag_w_block(int Q, const T* __restrict__ bt, long slice,
                                             int nbf, int nbf5, const double* __restrict__ cj12,
                                             const double* __restrict__ n, int nalpha, int nbeta,
                                             double* __restrict__ X, double* __restrict__ w,
                                             double* xs) {
-  const double* s = bt + (long)Q * slice;
   for (int q = threadIdx.x; q < nbf5; q += blockDim.x) {
-    const double v = s[(long)q * (nbf + 1)];
+    const double v = bt_at<T, TR>(bt, slice, nbf, nbf5, q, q, Q);
     xs[q] = v;
     X[q + (long)Q * nbf5] = v;
   }
@@ -46,7 +54,8 @@ __device__ __forceinline__ void lag_w_block(int Q, const double* __restrict__ bt
 }
 
 // bmat[q,b,Q] = -ck12[b,q] bt[q,b,Q] + [q==b] w[b,Q] for element e = b + nbf5 q + nbf5^2 Q
-__device__ __forceinline__ double lag_bmat_element(long e, const double* __restrict__ bt,
+template <typename T = double, bool TR = false>
+__device__ __forceinline__ double lag_bmat_element(long e, const T* __restrict__ bt,
                                                    long slice, int nbf, int nbf5,
                                                    const double* __restrict__ ck12,
                                                    const double* __restrict__ w) {
@@ -54,7 +63,7 @@ __device__ __forceinline__ double lag_bmat_element(long e, const double* __restr
   const long Q = e / nn;
   const int r = (int)(e - Q * nn);
   const int b = r % nbf5, q = r / nbf5;
-  double v = -ck12[r] * bt[b + (long)nbf * q + slice * Q];
+  double v = -ck12[r] * bt_at<T, TR>(bt, slice, nbf, nbf5, b, q, Q);
   if (q == b) v += w[b + (long)nbf5 * Q];
   return v;
 }
@@ -87,14 +96,15 @@ __device__ __forceinline__ void energy_block(const double* __restrict__ elag,
 
 // ---- single-system kernels --------------------------------------------------------------
 
-__global__ void lag_w_kernel(const double* __restrict__ bt, long slice, int nbf, int nbf5,
+template <typename T = double, bool TR = false>
+__global__ void lag_w_kernel(const T* __restrict__ bt, long slice, int nbf, int nbf5,
                              int naux, const double* __restrict__ cj12,
                              const double* __restrict__ n, int nalpha, int nbeta,
                              double* __restrict__ X, double* __restrict__ w) {
   extern __shared__ double xs[];  // nbf5 diagonal entries of slice Q
   const int Q = blockIdx.x;
   if (Q >= naux) return;
-  lag_w_block(Q, bt, slice, nbf, nbf5, cj12, n, nalpha, nbeta, X, w, xs);
+  lag_w_block<T, TR>(Q, bt, slice, nbf, nbf5, cj12, n, nalpha, nbeta, X, w, xs);
 }
 
 __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict__ bt, long slice,
@@ -106,30 +116,6 @@ __global__ __launch_bounds__(256) void lag_bmat_kernel(const double* __restrict_
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
        e += (long)gridDim.x * blockDim.x)
     bmat[e] = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
-}
-
-// lag_bmat_kernel that also leaves each block's max |bmat| (as FP32) in bmax[blockIdx.x]
-// (the scale of the split-precision contraction's bmat planes; no atomics)
-__global__ __launch_bounds__(256) void lag_bmat_max_kernel(const double* __restrict__ bt,
-                                                           long slice, int nbf, int nbf5,
-                                                           int naux,
-                                                           const double* __restrict__ ck12,
-                                                           const double* __restrict__ w,
-                                                           double* __restrict__ bmat,
-                                                           float* __restrict__ bmax) {
-  __shared__ float red[4];
-  const long total = (long)nbf5 * nbf5 * naux;
-  float m = 0.f;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-       e += (long)gridDim.x * blockDim.x) {
-    const double v = lag_bmat_element(e, bt, slice, nbf, nbf5, ck12, w);
-    bmat[e] = v;
-    m = fmaxf(m, (float)fabs(v));  // the scale takes 1.0001 times it: rounding is covered
-  }
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) bmax[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 __global__ void lag_init_kernel(const double* __restrict__ Hmat, const double* __restrict__ n,
@@ -226,8 +212,8 @@ __global__ __launch_bounds__(256) void group_pack_grad_kernel(const LagSys* __re
 
 int lag_w(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
           const double* cj12, const double* n, int nalpha, int nbeta, double* X, double* w) {
-  hipLaunchKernelGGL(lag_w_kernel, dim3(naux), dim3(128), nbf5 * sizeof(double), s, bt, slice,
-                     nbf, nbf5, naux, cj12, n, nalpha, nbeta, X, w);
+  hipLaunchKernelGGL((lag_w_kernel<double, false>), dim3(naux), dim3(128), nbf5 * sizeof(double),
+                     s, bt, slice, nbf, nbf5, naux, cj12, n, nalpha, nbeta, X, w);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }
@@ -241,14 +227,12 @@ int lag_bmat(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int
   return DN_OK;
 }
 
-int lag_bmat_max_blocks(int nbf5, int naux) {
-  return (int)std::min(((long)nbf5 * nbf5 * naux + 255) / 256, 8192L);
-}
-
-int lag_bmat_max(hipStream_t s, const double* bt, long slice, int nbf, int nbf5, int naux,
-                 const double* ck12, const double* w, double* bmat, float* bmax) {
-  hipLaunchKernelGGL(lag_bmat_max_kernel, dim3((unsigned)lag_bmat_max_blocks(nbf5, naux)),
-                     dim3(256), 0, s, bt, slice, nbf, nbf5, naux, ck12, w, bmat, bmax);
+// X and w of the split path straight from the second transform's FP32 output (b_pq
+// stored transposed, slice nbf5^2)
+int lag_w_split(hipStream_t s, const float* bpq, int nbf5, int naux, const double* cj12,
+                const double* n, int nalpha, int nbeta, double* X, double* w) {
+  hipLaunchKernelGGL((lag_w_kernel<float, true>), dim3(naux), dim3(128), nbf5 * sizeof(double), s,
+                     bpq, (long)nbf5 * nbf5, nbf5, nbf5, naux, cj12, n, nalpha, nbeta, X, w);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

@@ -908,55 +908,20 @@ int split_prepare_c(hipStream_t s, const double* C, int nbf, int nbf5, int Mpad,
 // Z[n,b] = sum_{q,Q} bpn[n,q,Q] bmat[q,b,Q] (utils.jl:44-48: the exchange-type term of the
 // orbital Lagrangian, the Coulomb term sitting on bmat's diagonal) on FP16 hi/lo MFMA:
 // both operands become tile-major planes with q as the contraction index k (bpn from the
-// FP32 first transform, bmat from FP64), each under one power-of-two scale taken from its
+// FP32 first transform, bmat evaluated in FP64 from the FP32 second transform and never
+// stored), each under one power-of-two scale taken from its
 // largest element over every slab, so a slab's rounding stays below 2^-22 of the largest
 // slab's terms, as its share of the sum does; bps slabs accumulate in FP32 per output tile
 // and the partials are summed in FP64 in batch order (deterministic).
 
-// Planes of src, element (row r, k) of slab Q at src[Q sS + r rs + k ks] for r < R, k < K,
-// zero for r in [R, Rp) and k in [K, Kq): (r, k) of slab Q at
-// Q Rp Kq + (k / 32) 32 Rp + 32 r + k % 32.  Work block = (Q, k tile, 64-row block), thread
-// = (row, 8-wide chunk c), adjacent rows adjacent in src (row stride 1); blocks taken
-// grid-stride.
-template <typename T>
-__global__ __launch_bounds__(256) void kplanes_kernel(const T* __restrict__ src, long ks,
-                                                      long sS, int R, int Rp, int K, int Kq,
-                                                      long nblk,
-                                                      const float* __restrict__ scale,
-                                                      _Float16* __restrict__ hi,
-                                                      _Float16* __restrict__ lo) {
-  const int nkt = Kq / 32, nrb = (Rp + 63) / 64;
-  const float sc = *scale;
-  for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const long qk = b / nrb;
-    const int rb = (int)(b - qk * nrb);
-    const long Q = qk / nkt;
-    const int kt = (int)(qk - Q * nkt);
-    const int c = (int)(threadIdx.x >> 6), r = rb * 64 + (int)(threadIdx.x & 63);
-    if (r >= Rp) continue;
-    const T* row = src + Q * sS + r;
-    v8h h8, l8;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = kt * 32 + 8 * c + e;
-      _Float16 h = 0, l = 0;
-      if (r < R && k < K) {
-        const T x = row[(long)k * ks];
-        if constexpr (std::is_same<T, double>::value) split_f16_d(x * (double)sc, &h, &l);
-        else split_f16((float)x * sc, &h, &l);
-      }
-      h8[e] = h;
-      l8[e] = l;
-    }
-    const long o = Q * Rp * Kq + (long)kt * 32 * Rp + (long)r * 32 + 8 * c;
-    *reinterpret_cast<v8h*>(hi + o) = h8;
-    *reinterpret_cast<v8h*>(lo + o) = l8;
-  }
-}
+// Operand planes of the contraction (tile-major, q as the contraction index k): element
+// (row r, k) of slab Q at Q Rp Kq + (k / 32) 32 Rp + 32 r + k % 32, zero for k in [K, Kq)
+// (and for rows in [R, Rp)).
 
-// kplanes_kernel for a k-contiguous FP32 source: thread = (row, 8-wide chunk c) with c
-// fastest (four threads read a row's 128-byte run of one k tile); work block = (Q, k tile,
-// 64-row block), taken grid-stride.  (One element per thread, 32 threads per run, was
+// Planes of a k-contiguous FP32 source, element (row r, k) of slab Q at
+// src[Q sS + r rs + k] for r < R, k < K: thread = (row, 8-wide chunk c) with c fastest (four
+// threads read a row's 128-byte run of one k tile); work block = (Q, k tile, 64-row block),
+// taken grid-stride.  (One element per thread, 32 threads per run, was
 // slower: 289 against 206 us on average over a served mix, its 2-byte stores.)
 __global__ __launch_bounds__(256) void kplanes_kc_f32_kernel(const float* __restrict__ src,
                                                              long rs, long sS, int R, int Rp,
@@ -1004,9 +969,95 @@ __global__ __launch_bounds__(256) void kplanes_kc_f32_kernel(const float* __rest
   }
 }
 
+// The bmat operand of the split contraction without an FP64 bmat (utils.jl:44-48):
+//   bmat[q,b,Q] = -ck12[b + nbf5 q] b_pq[b,q,Q] + [q == b] w[b + nbf5 Q],
+// evaluated in FP64 from the split second transform's FP32 output (b_pq[b,q,Q] at
+// q + nbf5 b + nbf5^2 Q: row b's k = q run is contiguous).  PLANES = false leaves each
+// workgroup's max |bmat| (FP32) in bmax[blockIdx.x]; PLANES = true writes the hi/lo planes of
+// bmat * scale[0] in the operand layout above (rows b, k = q, Rp = nbf5).  Thread = (row b,
+// 8-wide chunk c of one k tile), c fastest; work block = (k tile, 64-row block, range of
+// qper slabs), nblk of them taken grid-stride, so a thread's eight ck12 values stay in
+// registers over a range.  Both passes evaluate
+// the same expression, so the scale bounds exactly the values that are split.
+template <bool PLANES>
+__global__ __launch_bounds__(256) void bmat_kc_kernel(const float* __restrict__ bpq,
+                                                      const double* __restrict__ ck12,
+                                                      const double* __restrict__ w, int nbf5,
+                                                      int naux, int Kq, int qper, long nblk,
+                                                      const float* __restrict__ scale,
+                                                      float* __restrict__ bmax,
+                                                      _Float16* __restrict__ hi,
+                                                      _Float16* __restrict__ lo) {
+  __shared__ float red[4];
+  const int nkt = Kq / 32, nrb = (nbf5 + 63) / 64;
+  const long pairs = (long)nkt * nrb;
+  const double sc = PLANES ? (double)*scale : 0.0;
+  float m = 0.f;
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int pair = (int)(blk % pairs);
+    const int Q0 = (int)(blk / pairs) * qper;
+    const int Q1 = min(Q0 + qper, naux);
+    const int kt = pair / nrb, rb = pair - kt * nrb;
+    const int c = (int)(threadIdx.x & 3), r = rb * 64 + (int)(threadIdx.x >> 2);
+    const int k0 = kt * 32 + 8 * c;
+    if (r >= nbf5) continue;
+    double ck[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ck[e] = (k0 + e < nbf5) ? -ck12[r + (long)nbf5 * (k0 + e)] : 0.0;
+    const int diag = r - k0;  // chunk position of q == b, if in [0, 8)
+    for (int Q = Q0; Q < Q1; ++Q) {
+      const float* row = bpq + (long)nbf5 * nbf5 * Q + (long)nbf5 * r;
+      float v[8];
+      if (k0 + 8 <= nbf5 && ((uintptr_t)(row + k0) & 15) == 0) {
+        typedef float v4 __attribute__((ext_vector_type(4)));
+        const v4 a = *reinterpret_cast<const v4*>(row + k0);
+        const v4 b = *reinterpret_cast<const v4*>(row + k0 + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[e] = a[e];
+          v[4 + e] = b[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (k0 + e < nbf5) ? row[k0 + e] : 0.f;
+      }
+      double x[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        double y = ck[e] * (double)v[e];
+        if (e == diag) y += w[r + (long)nbf5 * Q];
+        x[e] = y;
+      }
+      if constexpr (PLANES) {
+        v8h h8, l8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          _Float16 h, l;
+          split_f16_d(x[e] * sc, &h, &l);
+          h8[e] = h;
+          l8[e] = l;
+        }
+        const long o = (long)Q * nbf5 * Kq + (long)kt * 32 * nbf5 + (long)r * 32 + 8 * c;
+        *reinterpret_cast<v8h*>(hi + o) = h8;
+        *reinterpret_cast<v8h*>(lo + o) = l8;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) m = fmaxf(m, (float)fabs(x[e]));  // the scale takes 1.0001x
+      }
+    }
+  }
+  if constexpr (!PLANES) {
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      bmax[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  }
+}
+
 // zs = (scale of bpn, 1 / (scale of bpn * scale of bmat), scale of bmat, -): the bpn scale
 // from the first transform's per-Q maxima, the bmat scale from the nb block maxima that
-// lag_bmat_max left in zs[4 ..]
+// bmat_kc_kernel<false> left in zs[4 ..]
 __global__ __launch_bounds__(256) void contract_scales_kernel(const float* __restrict__ qmax,
                                                               int naux, int nb,
                                                               float* __restrict__ zs) {
@@ -1086,16 +1137,26 @@ size_t split_contract_bytes(int nbf, int nbf5, int naux) {
 }
 
 int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
-                   const double* bmat, const float* qmax, int nbmax, float* zscale, void* work,
-                   size_t work_bytes, double* Z) {
+                   const float* bpq, const double* ck12, const double* w, const float* qmax,
+                   float* zscale, void* work, size_t work_bytes, double* Z) {
   const ContractPlan c = contract_plan(nbf, nbf5, naux);
   if (split_contract_bytes(nbf, nbf5, naux) > work_bytes)
     return fail(DN_ERR_ARG, "split contraction workspace too small");
+  // bmat work blocks: (k tile, 64-row block) pairs times ranges of qper slabs, about 4096
+  // of them; at most 8192 workgroups (the max pass leaves one maximum per workgroup in
+  // zscale[4 ..])
+  const long pairs = (long)(c.Kq / 32) * ((nbf5 + 63) / 64);
+  const long ranges = std::max(1L, std::min((long)naux, 4096 / pairs));
+  const int qper = (int)((naux + ranges - 1) / ranges);
+  const long nblkB = pairs * ((naux + qper - 1) / qper);
+  const int nbmax = (int)std::min(nblkB, 8192L);
   _Float16* ahi = static_cast<_Float16*>(work);
   _Float16* alo = ahi + c.planeA;
   _Float16* bhi = alo + c.planeA;
   _Float16* blo = bhi + c.planeB;
   float* part = reinterpret_cast<float*>(blo + c.planeB);  // 64-byte aligned: Kq % 32 == 0
+  hipLaunchKernelGGL(bmat_kc_kernel<false>, dim3((unsigned)nbmax), dim3(256), 0, s, bpq, ck12,
+                     w, nbf5, naux, c.Kq, qper, nblkB, nullptr, zscale + 4, nullptr, nullptr);
   hipLaunchKernelGGL(contract_scales_kernel, dim3(1), dim3(256), 0, s, qmax, naux, nbmax,
                      zscale);
   // bpn[n,q,Q] at q + nbf5 n + nbf5 nbf Q (rows n, k = q contiguous)
@@ -1103,11 +1164,9 @@ int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
   hipLaunchKernelGGL(kplanes_kc_f32_kernel, dim3((unsigned)std::min(blkA, 1L << 20)), dim3(256),
                      0, s, bpn, (long)nbf5, (long)nbf5 * nbf, nbf, c.Rp, nbf5, c.Kq, blkA, zscale,
                      ahi, alo);
-  // bmat[q,b,Q] at b + nbf5 q + nbf5^2 Q (rows b, k = q)
-  const long blkB = (long)naux * (c.Kq / 32) * ((nbf5 + 63) / 64);
-  hipLaunchKernelGGL(kplanes_kernel<double>, dim3((unsigned)std::min(blkB, 1L << 20)), dim3(256),
-                     0, s, bmat, (long)nbf5, (long)nbf5 * nbf5, nbf5, nbf5, nbf5, c.Kq, blkB,
-                     zscale + 2, bhi, blo);
+  // bmat (rows b, k = q) evaluated again, now into planes under its scale
+  hipLaunchKernelGGL(bmat_kc_kernel<true>, dim3((unsigned)nbmax), dim3(256), 0, s, bpq, ck12, w,
+                     nbf5, naux, c.Kq, qper, nblkB, zscale + 2, nullptr, bhi, blo);
   SplitGemm g{};
   g.M = nbf; g.N = nbf5; g.K = nbf5;
   g.ahi = ahi; g.alo = alo; g.lda = (long)c.Rp * 32; g.sA = (long)c.Rp * c.Kq;
@@ -1123,45 +1182,6 @@ int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
   const long n = (long)nbf * nbf5;
   hipLaunchKernelGGL(reduce_f32_partials_kernel, dim3(grid_of(n)), dim3(256), 0, s, part, n,
                      c.nz, Z);
-  DN_HIP(hipGetLastError());
-  return DN_OK;
-}
-
-// dst[c + ldd r] = (double) src[r + lds c] for r < R, c < Cc in each of nbatch slabs: the
-// FP32 products of the split transforms, transposed into the FP64 layouts the Lagrangian
-// reads.  32 x 32 tiles through LDS (coalesced reads and writes); one block per tile.
-__global__ __launch_bounds__(256) void transpose_f32_f64_kernel(const float* __restrict__ src,
-                                                                int R, int Cc, long lds, long sbs,
-                                                                double* __restrict__ dst,
-                                                                long ldd, long sbd, int tiles_r,
-                                                                int tiles_c) {
-  __shared__ float t[32][33];
-  const long per = (long)tiles_r * tiles_c;
-  const long z = blockIdx.x / per;
-  const int rem = (int)(blockIdx.x - z * per);
-  const int r0 = 32 * (rem % tiles_r), c0 = 32 * (rem / tiles_r);
-  const float* S = src + z * sbs;
-  double* D = dst + z * sbd;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int j = ty; j < 32; j += 8) {
-    const int r = r0 + tx, c = c0 + j;
-    if (r < R && c < Cc) t[j][tx] = S[r + lds * c];
-  }
-  __syncthreads();
-  for (int j = ty; j < 32; j += 8) {
-    const int c = c0 + tx, r = r0 + j;
-    if (r < R && c < Cc) D[c + ldd * r] = (double)t[tx][j];
-  }
-}
-
-int transpose_f32_f64(hipStream_t s, const float* src, int R, int Cc, long lds, long sbs,
-                      double* dst, long ldd, long sbd, int nbatch) {
-  const int tr = (R + 31) / 32, tc = (Cc + 31) / 32;
-  const long blocks = (long)tr * tc * nbatch;
-  if (blocks <= 0) return DN_OK;
-  if (blocks > 0x7fffffffL) return fail(DN_ERR_ARG, "transpose grid too large");
-  hipLaunchKernelGGL(transpose_f32_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, R,
-                     Cc, lds, sbs, dst, ldd, sbd, tr, tc);
   DN_HIP(hipGetLastError());
   return DN_OK;
 }

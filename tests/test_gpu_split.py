@@ -85,3 +85,27 @@ def test_split_lagrangian_against_oracle(oracle, nbf, naux, nbf5):
     assert np.abs(J - Jr).max() <= 5e-6 * np.abs(Jr).max()
     assert np.abs(K - Kr).max() <= 5e-6 * np.abs(Kr).max()
     dev.close()
+
+
+@pytest.mark.parametrize("nbf,naux,nbf5", [(130, 300, 45), (37, 70, 10)])
+def test_split_lagrange2_jkh_against_oracle(oracle, nbf, naux, nbf5):
+    """dn_lagrange2_jkh on a split system: the Lagrangian and J/K/H_core of the same
+    orbitals from the one FP32 b_pq (utils.jl:1-59, integrals.jl:144-172), to the split
+    bounds; H_core and Hmat stay FP64."""
+    print("case split lag+jkh", nbf, naux, nbf5, flush=True)
+    s = synthetic.SyntheticSystem(nbf, naux, nbf5, 9 + nbf, ndoc=5)
+    p = s.p
+    b = synthetic.make_b(nbf, naux, s.seed)
+    dev = api.System(p, precision=DN_SPLIT_F16).upload_b(b)
+    cj, ck = oracle.pnof_selector(s.n, p)
+    el, Hm, E, J, K, Hc = api.compute_Lagrange2_JKH(s.C, s.n, s.H, dev, cj, ck, p.nalpha,
+                                                    p.nbeta)
+    elr, Hmr = oracle.compute_lagrange2(s.C, s.n, s.H, b, cj, ck, p.nalpha, p.nbeta)
+    Jr, Kr, Hcr = oracle.jkh_mo(s.C, s.H, b, nbf5)
+    assert np.abs(el - elr).max() <= 5e-6 * np.abs(elr).max()
+    assert np.abs(Hm - Hmr).max() <= 1e-10 * max(1.0, np.abs(Hmr).max())
+    assert abs(E - oracle.computeE_elec_hmat(Hmr, s.n, elr, nbf5)) <= 1e-6
+    assert np.abs(J - Jr).max() <= 5e-6 * np.abs(Jr).max()
+    assert np.abs(K - Kr).max() <= 5e-6 * np.abs(Kr).max()
+    assert np.abs(Hc - Hcr).max() <= 1e-10 * max(1.0, np.abs(Hcr).max())
+    dev.close()
