@@ -38,8 +38,9 @@ enum {
  * DN_SPLIT_F16: b and C5 as FP16 hi+lo pairs with power-of-two scales, three MFMA
  * products accumulated in FP32: the J/K build's two transforms (J/K reduced in FP64) and
  * the Lagrangian's two transforms and contraction sum_Q bpn bmat (FP32 partials summed in
- * FP64); H_core, the Lagrangian's one-electron and C^J/C^K terms, the rotation and the
- * occupation functional stay FP64.  b is not kept in FP64. */
+ * FP64); H_core, the Lagrangian's one-electron and C^J/C^K terms (evaluated in FP64 from
+ * the FP32 transforms), the rotation and the occupation functional stay FP64.  b, b_pq
+ * and the contraction's right operand are not kept in FP64. */
 enum { DN_FP64 = 0, DN_SPLIT_F16 = 1 };
 
 /* Orbital optimiser of the ADAM family: p.orb_method "ADAM", "ADABelief", "YOGI",
