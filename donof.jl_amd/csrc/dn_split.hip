@@ -1142,11 +1142,11 @@ int split_contract(hipStream_t s, int nbf, int nbf5, int naux, const float* bpn,
   const ContractPlan c = contract_plan(nbf, nbf5, naux);
   if (split_contract_bytes(nbf, nbf5, naux) > work_bytes)
     return fail(DN_ERR_ARG, "split contraction workspace too small");
-  // bmat work blocks: (k tile, 64-row block) pairs times ranges of qper slabs, about 4096
+  // bmat work blocks: (k tile, 64-row block) pairs times ranges of qper slabs, about 8192
   // of them; at most 8192 workgroups (the max pass leaves one maximum per workgroup in
   // zscale[4 ..])
   const long pairs = (long)(c.Kq / 32) * ((nbf5 + 63) / 64);
-  const long ranges = std::max(1L, std::min((long)naux, 4096 / pairs));
+  const long ranges = std::max(1L, std::min((long)naux, 8192 / pairs));
   const int qper = (int)((naux + ranges - 1) / ranges);
   const long nblkB = pairs * ((naux + qper - 1) / qper);
   const int nbmax = (int)std::min(nblkB, 8192L);
